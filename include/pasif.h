@@ -1,0 +1,150 @@
+/*
+ * pasif.h -- C ABI of libpasif_hip.so, the MI355X (gfx950) implementation of
+ * PASIf's batched RK4 forward integrator and its checkpointed adjoint.
+ *
+ * The entry points replace, one for one, the methods of the reference's
+ * pybind11 class `__GpuDriver` (src/PASIfgpu.cpp:22-104, declared in
+ * src/__GpuDriver.cuh:18-83).  Differences at the boundary:
+ *   - every function returns an int status (PASIF_OK == 0); the message of
+ *     the last failure is available from pasif_last_error().  The reference
+ *     calls std::exit(0) on CUDA errors (src/helpers.cuh:43-51);
+ *   - inputs are caller-owned buffers, copied before the call returns (the
+ *     reference copies through pybind11's STL casters);
+ *   - results are written into caller-owned buffers; pass out == NULL to
+ *     query the length first.
+ * Entry points marked "extension" have no counterpart in the reference.
+ *
+ * Indices of tensors are interleaved exactly as in the reference:
+ * entry t of a 3-D tensor has indices idx[3t], idx[3t+1], idx[3t+2]
+ * (output, source, set point; src/helpers.cu:277-282).
+ */
+#ifndef PASIF_H
+#define PASIF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pasif_driver pasif_driver;
+
+enum {
+  PASIF_OK = 0,
+  PASIF_ERR_ARGUMENT = 1,   /* invalid shape / value / order of calls      */
+  PASIF_ERR_DEVICE = 2,     /* HIP runtime failure                          */
+  PASIF_ERR_UNSUPPORTED = 3,/* valid input outside what the kernels handle  */
+  PASIF_ERR_MEMORY = 4,     /* device memory / HBM budget exhausted         */
+  PASIF_ERR_STATE = 5       /* call made before its prerequisites           */
+};
+
+enum { PASIF_SOLVER_PERSISTENT = 0, PASIF_SOLVER_PLAIN = 1 };
+
+/* __GpuDriver(excitationSet_, sampleRate_, numsteps_, dCompute_, dSystem_,
+ * dSolver_, GPUId_)  (src/__GpuDriver.cu:25-157).  excitations is
+ * [n_exc][exc_len] row-major.  numsteps == 0 means exc_len (PASIf.py:156-157
+ * applies that default on the Python side). */
+int pasif_create(pasif_driver** out, const double* excitations, uint32_t n_exc,
+                 uint32_t exc_len, uint32_t sample_rate, uint32_t numsteps,
+                 int d_compute, int d_system, int d_solver, int gpu_id);
+/* ~__GpuDriver (src/__GpuDriver.cu:159-193) */
+int pasif_destroy(pasif_driver* d);
+/* message of the last failure on this handle ("" if none) */
+const char* pasif_last_error(const pasif_driver* d);
+
+/* _setFwdK / _setBwdK (src/__GpuDriver.cu:199-209, 272-282): CSR of the
+ * scaled B+K, n = {rows, cols}, nnz values, nnz column indices, rows+1 ptr. */
+int pasif_set_fwd_K(pasif_driver* d, const uint32_t n[2], const double* values,
+                    const uint32_t* indices, const uint32_t* indptr, size_t nnz);
+int pasif_set_bwd_K(pasif_driver* d, const uint32_t n[2], const double* values,
+                    const uint32_t* indices, const uint32_t* indptr, size_t nnz);
+/* _setFwdGamma / _setFwdLambda / _setFwdPsi and the Bwd versions
+ * (src/__GpuDriver.cu:211-239, 284-312): order-3/4/5 COO tensors. */
+int pasif_set_fwd_gamma(pasif_driver* d, const uint32_t n[3], const double* values,
+                        const uint32_t* indices, size_t nnz);
+int pasif_set_fwd_lambda(pasif_driver* d, const uint32_t n[4], const double* values,
+                         const uint32_t* indices, size_t nnz);
+int pasif_set_fwd_psi(pasif_driver* d, const uint32_t n[5], const double* values,
+                      const uint32_t* indices, size_t nnz);
+int pasif_set_bwd_gamma(pasif_driver* d, const uint32_t n[3], const double* values,
+                        const uint32_t* indices, size_t nnz);
+int pasif_set_bwd_lambda(pasif_driver* d, const uint32_t n[4], const double* values,
+                         const uint32_t* indices, size_t nnz);
+int pasif_set_bwd_psi(pasif_driver* d, const uint32_t n[5], const double* values,
+                      const uint32_t* indices, size_t nnz);
+/* _setFwdForcePattern / _setFwdInitialConditions and Bwd versions
+ * (src/__GpuDriver.cu:242-254, 314-326).  Force entries with |v| <= eps of
+ * the working precision are dropped (src/helpers.cu:674-680). */
+int pasif_set_fwd_force_pattern(pasif_driver* d, const double* force, size_t n);
+int pasif_set_fwd_initial_conditions(pasif_driver* d, const double* q0, size_t n);
+int pasif_set_bwd_force_pattern(pasif_driver* d, const double* force, size_t n);
+int pasif_set_bwd_initial_conditions(pasif_driver* d, const double* lam_T, size_t n);
+/* _allocateSystemOnDevice / _allocateAdjointOnDevice
+ * (src/__GpuDriver.cu:256-265, 328-337): validate, split the block-diagonal
+ * global system into its independent systems, upload. */
+int pasif_allocate_system(pasif_driver* d);
+int pasif_allocate_adjoint(pasif_driver* d);
+
+/* _loadExcitationsSet (src/__GpuDriver.cu:343-381) */
+int pasif_load_excitations(pasif_driver* d, const double* excitations, uint32_t n_exc,
+                           uint32_t exc_len, uint32_t sample_rate);
+/* _setInterpolationMatrix (src/__GpuDriver.cu:383-398): [rows][window]
+ * row-major, rows >= 2 (the kernels read rows 0 and 1).  size == 0 returns
+ * to zero-order hold.  Any window >= 1 is accepted (the reference needs a
+ * power of two). */
+int pasif_set_interpolation_matrix(pasif_driver* d, const double* matrix, size_t size,
+                                   uint32_t window);
+/* _setModulationBuffer (src/__GpuDriver.cu:400-410): validated and stored;
+ * no kernel reads it, as in the reference. */
+int pasif_set_modulation_buffer(pasif_driver* d, const double* buffer, size_t n);
+
+/* _getAmplitudes (src/__GpuDriver.cu:416-461): final state of every DOF of
+ * every replica, [replica][dof].  *len receives the element count. */
+int pasif_get_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* len);
+/* _getTrajectory (src/__GpuDriver.cu:463-500): floor(T/save_steps) slots of
+ * the post-step state after each step t with t % save_steps == 0, laid out
+ * [slot][replica][dof]. */
+int pasif_get_trajectory(pasif_driver* d, uint32_t save_steps, double* out, size_t cap,
+                         size_t* len);
+/* _getGradient (src/__GpuDriver.cu:502-587): lambda_0 of the adjoint
+ * system, [replica][adjoint dof], over the full schedule T..1.  `save` is
+ * accepted for signature compatibility; nonzero values are rejected. */
+int pasif_get_gradient(pasif_driver* d, uint32_t save, double* out, size_t cap, size_t* len);
+
+/* ---------------- extensions (no counterpart in the reference) ---------- */
+/* Per-system structure and scheduling.  sizes_fwd[s] / sizes_adj[s] give the
+ * DOF count of system s in the block-diagonal global system (NULL:
+ * detected from the sparsity).  exc_of_system[s] (NULL: every system runs
+ * against every excitation, the reference's Cartesian product) maps each
+ * system to one excitation.  steps_of_system[s] (NULL: numsteps) gives each
+ * system its own step count. */
+int pasif_set_systems_layout(pasif_driver* d, uint32_t n_systems, const uint32_t* sizes_fwd,
+                             const uint32_t* sizes_adj, const int32_t* exc_of_system,
+                             const uint32_t* steps_of_system);
+/* 64 (default) or 32 */
+int pasif_set_precision(pasif_driver* d, int bits);
+/* PASIF_SOLVER_PERSISTENT (default): all steps in one launch.
+ * PASIF_SOLVER_PLAIN: one launch per RK4 step, state through HBM. */
+int pasif_set_solver(pasif_driver* d, int solver);
+/* tile lanes (64..1024, multiple of 64; 0 = automatic), slot-width
+ * configuration (0 = automatic, 1 = narrow, 2 = wide) and the adjoint
+ * checkpoint interval in steps (0 = automatic). */
+int pasif_set_tuning(pasif_driver* d, uint32_t tile_lanes, uint32_t slot_config,
+                     uint32_t checkpoint_every);
+/* fused reduction: max |q_m| over the post-step states m in (T-window, T] of
+ * every DOF; 0 disables.  Read with pasif_get_max_amplitudes after a solve. */
+int pasif_set_amplitude_window(pasif_driver* d, uint32_t window);
+int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* len);
+/* duration of the last solve's kernel launches (HIP events on the library's
+ * stream), number of launches and number of tiles, lanes per tile */
+int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launches,
+                      uint32_t* tiles, uint32_t* lanes);
+/* library build string */
+const char* pasif_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PASIF_H */

@@ -1,0 +1,1257 @@
+// Host runtime of libpasif_hip.so: the C ABI declared in include/pasif.h.
+//
+// It replaces the reference's __GpuDriver (src/__GpuDriver.cu, bound by
+// src/PASIfgpu.cpp): it owns the excitation set, the forward and adjoint
+// systems as the Python layer assembles them (scaled CSR of B+K, COO tensors,
+// force pattern, initial / terminal conditions), splits the block-diagonal
+// global system back into its independent systems, packs (system,
+// excitation) instances into tiles (pasif_tiles.hpp) and runs the tile kernel
+// (pasif_kernels.hip) on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/pasif.h"
+#include "pasif_tiles.hpp"
+
+using namespace pasif;
+
+namespace {
+
+struct Fail {
+  int code;
+};
+
+// ------------------------------------------------------------------ device
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// ------------------------------------------------------------------ inputs
+struct CsrIn {
+  bool set = false;
+  uint32_t n0 = 0, n1 = 0;
+  std::vector<double> val;
+  std::vector<uint32_t> idx, ptr;
+};
+struct CooIn {
+  bool set = false;
+  int order = 0;
+  std::array<uint32_t, 5> dims{};
+  std::vector<double> val;
+  std::vector<uint32_t> idx;  // interleaved
+};
+struct SideIn {
+  CsrIn K;
+  CooIn G, L, P;
+  bool force_set = false, init_set = false;
+  std::vector<double> force, init;
+  SideIn() {
+    G.order = 3;
+    L.order = 4;
+    P.order = 5;
+  }
+};
+
+// Per-row term lists of one side (forward or adjoint) in global indices:
+// kind k terms of row r are [ptr[k][r], ptr[k][r+1]) of val[k] / idx[k]
+// (idx[k] holds k index words per term: S, X1, X2, X3), in the order the
+// reference applies them (CSR order, then COO order).
+struct RowTerms {
+  uint32_t n = 0;      // rows
+  uint32_t nset = 0;   // size of the set-point index space
+  std::array<std::vector<uint32_t>, 4> ptr;
+  std::array<std::vector<double>, 4> val;
+  std::array<std::vector<uint32_t>, 4> idx;
+  std::vector<double> force;  // dense, eps-dropped at pack time
+  std::vector<double> init;
+  std::vector<uint32_t> off, size;   // system partition (rows)
+  std::vector<uint32_t> xoff;        // adjoint: forward offset of each system
+};
+
+// one (system, excitation) pair
+struct Instance {
+  uint32_t sys;
+  int32_t exc;         // -1: unforced
+  uint32_t steps;
+  uint32_t out_f;      // output base of forward rows
+  uint32_t out_a;      // output base of adjoint rows
+  uint32_t lanes;      // lanes the instance occupies in a tile
+};
+
+struct Packed {
+  bool valid = false;
+  int precision = 0;
+  bool adjoint = false;
+  uint32_t lanes = 0, ntiles = 0, slot_cfg = 0, tile_target = 0;
+  uint64_t key = 0;
+  uint32_t out_len_f = 0, out_len_a = 0, tmax = 0;
+  DevBuf hdr;
+  DevBuf f_info, f_out, f_src, f_steps, f_exc, f_force, f_val, f_idx, f_init;
+  DevBuf a_info, a_out, a_src, a_steps, a_exc, a_force, a_val, a_idx, a_init;
+  size_t program_bytes = 0;
+  void release() {
+    DevBuf* all[] = {&hdr,    &f_info,  &f_out, &f_src, &f_steps, &f_exc,   &f_force,
+                     &f_val,  &f_idx,   &f_init, &a_info, &a_out,  &a_src,  &a_steps,
+                     &a_exc,  &a_force, &a_val, &a_idx, &a_init};
+    for (DevBuf* b : all) b->release();
+    valid = false;
+  }
+};
+
+}  // namespace
+
+struct pasif_driver {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  bool d_compute = false, d_system = false, d_solver = false;
+
+  // excitations
+  std::vector<double> exc;
+  uint32_t n_exc = 0, exc_len = 0, sample_rate = 1, numsteps = 0;
+  std::vector<double> interp;  // rows 0 and 1, [2][W]
+  uint32_t interp_w = 0;
+  std::vector<double> modulation;
+
+  SideIn fin, bin;
+  bool sys_ready = false, adj_ready = false;
+  RowTerms fwd, adj;
+
+  // layout extension
+  std::vector<uint32_t> hint_f, hint_a;
+  std::vector<int32_t> map_exc;
+  std::vector<uint32_t> map_steps;
+  bool mapped = false;
+  uint32_t layout_systems = 0;
+
+  int precision = 64;
+  int solver = PASIF_SOLVER_PERSISTENT;
+  uint32_t tile_lanes = 0, slot_cfg = 0, ck_every = 0, window = 0;
+
+  Packed pk;
+  // device copies of excitations / interpolation for the packed precision
+  DevBuf d_exc, d_interp;
+  int d_exc_prec = 0;
+  bool d_exc_dirty = true;
+  // run buffers
+  DevBuf qstate, lstate, mxstate, progress, out_q, out_max, traj, out_grad, ckpt, segbuf;
+  // results of the last solve (host, double)
+  std::vector<double> res_q, res_max;
+  bool have_max = false;
+  double last_ms = 0.0;
+  uint32_t last_launches = 0;
+};
+
+namespace {
+
+void set_err(pasif_driver* d, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  d->err = buf;
+}
+
+#define PASIF_HIP(call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      set_err(d, "HIP error %s (%d) at %s:%d: %s", hipGetErrorName(e_), (int)e_, __FILE__, \
+              __LINE__, #call);                                                          \
+      throw Fail{PASIF_ERR_DEVICE};                                                      \
+    }                                                                                    \
+  } while (0)
+
+#define PASIF_REQUIRE(cond, code, ...) \
+  do {                                 \
+    if (!(cond)) {                     \
+      set_err(d, __VA_ARGS__);         \
+      throw Fail{code};                \
+    }                                  \
+  } while (0)
+
+template <class F>
+int guarded(pasif_driver* d, F&& f) {
+  if (!d) return PASIF_ERR_ARGUMENT;
+  try {
+    d->err.clear();
+    f();
+    return PASIF_OK;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    d->err = "host memory exhausted";
+    return PASIF_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    d->err = e.what();
+    return PASIF_ERR_ARGUMENT;
+  }
+}
+
+void dev_alloc(pasif_driver* d, DevBuf& b, size_t bytes) {
+  b.release();
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    set_err(d, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorName(e));
+    throw Fail{PASIF_ERR_MEMORY};
+  }
+  b.bytes = bytes;
+}
+
+template <typename T>
+void upload(pasif_driver* d, DevBuf& b, const std::vector<T>& h) {
+  if (d->dev < 0) {  // host-only handle: packing and validation without a GPU
+    b.release();
+    b.bytes = h.size() * sizeof(T);
+    return;
+  }
+  dev_alloc(d, b, h.size() * sizeof(T));
+  if (!h.empty())
+    PASIF_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice,
+                             d->stream));
+}
+
+void check_rows(pasif_driver* d, const CsrIn& K, const char* what) {
+  PASIF_REQUIRE(K.set, PASIF_ERR_STATE, "%s: K (CSR of B+K) not set", what);
+  PASIF_REQUIRE(K.n0 == K.n1, PASIF_ERR_ARGUMENT, "%s: K must be square (%u x %u)", what, K.n0,
+                K.n1);
+  PASIF_REQUIRE(K.ptr.size() == size_t(K.n0) + 1, PASIF_ERR_ARGUMENT, "%s: indptr length", what);
+  PASIF_REQUIRE(K.ptr[0] == 0 && K.ptr.back() == K.val.size(), PASIF_ERR_ARGUMENT,
+                "%s: indptr does not span the values", what);
+  for (uint32_t r = 0; r < K.n0; ++r)
+    PASIF_REQUIRE(K.ptr[r] <= K.ptr[r + 1], PASIF_ERR_ARGUMENT, "%s: indptr not monotone", what);
+  for (uint32_t c : K.idx)
+    PASIF_REQUIRE(c < K.n1, PASIF_ERR_ARGUMENT, "%s: column index %u out of range", what, c);
+}
+
+// Build per-row term lists (global indices) of one side.
+void build_rows(pasif_driver* d, const SideIn& in, uint32_t nset, bool adjoint, RowTerms& out,
+                const char* what) {
+  check_rows(d, in.K, what);
+  const uint32_t n = in.K.n0;
+  PASIF_REQUIRE(in.init_set && in.init.size() == n, PASIF_ERR_ARGUMENT,
+                "%s: %s has %zu entries, the system has %u DOFs", what,
+                adjoint ? "terminal condition" : "initial condition", in.init.size(), n);
+  PASIF_REQUIRE(!in.force_set || in.force.size() == n, PASIF_ERR_ARGUMENT,
+                "%s: force pattern has %zu entries, the system has %u DOFs", what, in.force.size(),
+                n);
+  out = RowTerms{};
+  out.n = n;
+  out.nset = nset;
+  out.init = in.init;
+  out.force = in.force_set ? in.force : std::vector<double>(n, 0.0);
+  // kind 1: CSR rows as given (the reference's SpMM order)
+  out.ptr[0] = in.K.ptr;
+  out.val[0] = in.K.val;
+  out.idx[0] = in.K.idx;
+  const CooIn* tens[3] = {&in.G, &in.L, &in.P};
+  for (int k = 1; k < 4; ++k) {
+    const CooIn& t = *tens[k - 1];
+    const int order = k + 2;
+    const size_t nnz = t.set ? t.val.size() : 0;
+    if (t.set) {
+      PASIF_REQUIRE(t.idx.size() == nnz * order, PASIF_ERR_ARGUMENT,
+                    "%s: tensor of order %d has %zu indices for %zu values", what, order,
+                    t.idx.size(), nnz);
+      if (nnz) {
+        PASIF_REQUIRE(t.dims[0] == n && t.dims[1] == n, PASIF_ERR_ARGUMENT,
+                      "%s: tensor of order %d has leading dims %u x %u, expected %u", what, order,
+                      t.dims[0], t.dims[1], n);
+        for (int a = 2; a < order; ++a)
+          PASIF_REQUIRE(t.dims[a] == nset, PASIF_ERR_ARGUMENT,
+                        "%s: tensor of order %d axis %d has size %u, expected %u", what, order,
+                        a, t.dims[a], nset);
+      }
+    }
+    std::vector<uint32_t>& ptr = out.ptr[k];
+    ptr.assign(n + 1, 0);
+    for (size_t e = 0; e < nnz; ++e) {
+      const uint32_t o = t.idx[e * order];
+      PASIF_REQUIRE(o < n, PASIF_ERR_ARGUMENT, "%s: tensor output index %u out of range", what, o);
+      for (int a = 1; a < order; ++a) {
+        const uint32_t lim = a == 1 ? n : nset;
+        PASIF_REQUIRE(t.idx[e * order + a] < lim, PASIF_ERR_ARGUMENT,
+                      "%s: tensor index %u on axis %d out of range", what, t.idx[e * order + a], a);
+      }
+      ptr[o + 1]++;
+    }
+    for (uint32_t r = 0; r < n; ++r) ptr[r + 1] += ptr[r];
+    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+    const int nf = k + 1;  // state factors of this kind
+    out.val[k].assign(nnz, 0.0);
+    out.idx[k].assign(nnz * nf, 0);
+    for (size_t e = 0; e < nnz; ++e) {  // stable: keeps COO order within a row
+      const uint32_t o = t.idx[e * order];
+      const uint32_t pos = fill[o]++;
+      out.val[k][pos] = t.val[e];
+      for (int a = 0; a < nf; ++a) out.idx[k][size_t(pos) * nf + a] = t.idx[e * order + 1 + a];
+    }
+  }
+}
+
+// Finest partition of [0, n) into contiguous blocks closed under every
+// term's indices (the block-diagonal structure PASIf.py:475-543 builds).
+std::vector<uint32_t> detect_blocks(const RowTerms& rt, bool use_x) {
+  const uint32_t n = rt.n;
+  std::vector<uint32_t> reach(n);
+  std::iota(reach.begin(), reach.end(), 0u);
+  for (int k = 0; k < 4; ++k) {
+    const int nidx = k + 1;
+    for (uint32_t r = 0; r < n; ++r)
+      for (uint32_t e = rt.ptr[k][r]; e < rt.ptr[k][r + 1]; ++e) {
+        uint32_t lo = r, hi = r;
+        const int lim = use_x ? nidx : 1;
+        for (int a = 0; a < lim; ++a) {
+          const uint32_t v = rt.idx[k][size_t(e) * nidx + a];
+          lo = std::min(lo, v);
+          hi = std::max(hi, v);
+        }
+        reach[lo] = std::max(reach[lo], hi);
+      }
+  }
+  std::vector<uint32_t> sizes;
+  uint32_t start = 0, far = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    far = std::max(far, reach[i]);
+    if (far == i) {
+      sizes.push_back(i + 1 - start);
+      start = i + 1;
+    }
+  }
+  return sizes;
+}
+
+void apply_partition(pasif_driver* d, RowTerms& rt, const std::vector<uint32_t>& sizes,
+                     const char* what) {
+  uint64_t tot = 0;
+  for (uint32_t s : sizes) {
+    PASIF_REQUIRE(s > 0, PASIF_ERR_ARGUMENT, "%s: empty system in the layout", what);
+    tot += s;
+  }
+  PASIF_REQUIRE(tot == rt.n, PASIF_ERR_ARGUMENT, "%s: system sizes add up to %llu, not %u", what,
+                (unsigned long long)tot, rt.n);
+  rt.size = sizes;
+  rt.off.assign(sizes.size(), 0);
+  for (size_t s = 1; s < sizes.size(); ++s) rt.off[s] = rt.off[s - 1] + sizes[s - 1];
+}
+
+// every term of system s must stay inside its block (S) and inside the
+// matching set-point block (X)
+void validate_blocks(pasif_driver* d, const RowTerms& rt, const RowTerms* xside, const char* what) {
+  for (size_t s = 0; s < rt.size.size(); ++s) {
+    const uint32_t o0 = rt.off[s], o1 = o0 + rt.size[s];
+    const uint32_t x0 = xside ? xside->off[s] : o0;
+    const uint32_t x1 = xside ? x0 + xside->size[s] : o1;
+    for (uint32_t r = o0; r < o1; ++r)
+      for (int k = 0; k < 4; ++k) {
+        const int nidx = k + 1;
+        for (uint32_t e = rt.ptr[k][r]; e < rt.ptr[k][r + 1]; ++e)
+          for (int a = 0; a < nidx; ++a) {
+            const uint32_t v = rt.idx[k][size_t(e) * nidx + a];
+            const bool ok = a == 0 ? (v >= o0 && v < o1) : (v >= x0 && v < x1);
+            PASIF_REQUIRE(ok, PASIF_ERR_ARGUMENT,
+                          "%s: row %u couples system %zu to index %u outside its block", what, r,
+                          s, v);
+          }
+      }
+  }
+}
+
+uint32_t seg_count(const RowTerms& rt, uint32_t r, const SlotConfig& c) {
+  const int cap[4] = {c.c1, c.c2, c.c3, c.c4};
+  uint32_t ns = 1;
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t cnt = rt.ptr[k][r + 1] - rt.ptr[k][r];
+    if (cnt) {
+      if (cap[k] == 0) return 0;
+      ns = std::max<uint32_t>(ns, (cnt + cap[k] - 1) / cap[k]);
+    }
+  }
+  return ns;
+}
+
+uint32_t system_lanes(const RowTerms& rt, uint32_t s, const SlotConfig& c) {
+  uint32_t l = 0;
+  for (uint32_t r = rt.off[s]; r < rt.off[s] + rt.size[s]; ++r) {
+    const uint32_t ns = seg_count(rt, r, c);
+    if (ns == 0 || ns > 1023) return 0;
+    l += ns;
+  }
+  return l;
+}
+
+template <typename R>
+struct HostProgram {
+  std::vector<uint32_t> info, out, src, steps;
+  std::vector<int32_t> exc;
+  std::vector<R> force, val;
+  std::vector<uint32_t> idx;
+  // Padding slots (value 0) point every factor at the zero row L.
+  void resize(size_t ntiles, uint32_t L, const SlotConfig& c, uint32_t spad, uint32_t xpad) {
+    const size_t nl = ntiles * L;
+    info.assign(nl, make_lane(kRowIdle, 0, false));
+    out.assign(nl, 0);
+    src.assign(nl, 0);
+    steps.assign(nl, 0);
+    exc.assign(nl, -1);
+    force.assign(nl, R(0));
+    val.assign(nl * slot_values(c), R(0));
+    const int CW = slot_words(c);
+    idx.resize(nl * CW);
+    for (size_t t = 0; t < ntiles; ++t)
+      for (int w = 0; w < CW; ++w) {
+        const int rel = w - (c.c1 + c.c2);
+        const bool second = rel >= 0 && (rel % 2) == 1;
+        const uint32_t pad = second ? (xpad | (xpad << 16)) : (spad | (xpad << 16));
+        std::fill_n(idx.begin() + (t * CW + w) * L, L, pad);
+      }
+  }
+};
+
+// Place one instance's rows of one side into tile `t` starting at lane `lo`
+// and tile row `ro`.  X indices are offset by `xro` (the instance's forward
+// rows in the tile).  Indices are stored as LDS byte offsets: rows of the
+// stage buffer are `sstride` bytes apart, rows of the set-point buffer
+// `xstride`.  Returns lanes used; updates the tile's trip counts.
+template <typename R>
+uint32_t place(const RowTerms& rt, uint32_t sys, const Instance& in, bool adjoint,
+               HostProgram<R>& hp, size_t t, uint32_t L, uint32_t lo, uint32_t ro, uint32_t xro,
+               const SlotConfig& c, uint32_t cnt[4], bool& split, uint32_t xoff,
+               uint32_t sstride, uint32_t xstride) {
+  const int cap[4] = {c.c1, c.c2, c.c3, c.c4};
+  int vbase[4], wbase[4];
+  vbase[0] = 0;
+  vbase[1] = c.c1;
+  vbase[2] = c.c1 + c.c2;
+  vbase[3] = c.c1 + c.c2 + c.c3;
+  wbase[0] = 0;
+  wbase[1] = c.c1;
+  wbase[2] = c.c1 + c.c2;
+  wbase[3] = c.c1 + c.c2 + 2 * c.c3;
+  const int CT = slot_values(c), CW = slot_words(c);
+  const uint32_t off = rt.off[sys], n = rt.size[sys];
+  const double eps = std::numeric_limits<R>::epsilon();
+  uint32_t lane = lo;
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t g = off + r;
+    const uint32_t ns = seg_count(rt, g, c);
+    if (ns > 1) split = true;
+    const size_t owner = t * L + lane;
+    hp.info[owner] = make_lane(ro + r, ns, true);
+    for (uint32_t sgm = 1; sgm < ns; ++sgm) hp.info[owner + sgm] = make_lane(ro + r, 0, false);
+    hp.src[owner] = g;
+    hp.steps[owner] = in.steps;
+    if (!adjoint) {
+      hp.out[owner] = in.out_f + r;
+      const double f = rt.force[g];
+      if (std::fabs(f) > eps && in.exc >= 0) {  // COOVector eps drop (helpers.cu:676)
+        hp.force[owner] = R(f);
+        hp.exc[owner] = in.exc;
+      }
+    } else {
+      hp.out[owner] = in.out_a + r;
+    }
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t b = rt.ptr[k][g], e = rt.ptr[k][g + 1];
+      const int nidx = k + 1;
+      for (uint32_t i = 0; i < e - b; ++i) {
+        const uint32_t sgm = i / cap[k], slot = i % cap[k];
+        cnt[k] = std::max(cnt[k], slot + 1);
+        const size_t ln = lane + sgm;
+        const uint32_t* ix = &rt.idx[k][size_t(b + i) * nidx];
+        uint32_t loc[4] = {L * xstride, L * xstride, L * xstride, L * xstride};
+        loc[0] = (ro + (ix[0] - off)) * sstride;
+        for (int a = 1; a < nidx; ++a) loc[a] = (xro + (ix[a] - xoff)) * xstride;
+        hp.val[(t * CT + vbase[k] + slot) * L + ln] = R(rt.val[k][b + i]);
+        if (k < 2) {
+          hp.idx[(t * CW + wbase[k] + slot) * L + ln] = loc[0] | (loc[1] << 16);
+        } else {
+          hp.idx[(t * CW + wbase[k] + 2 * slot) * L + ln] = loc[0] | (loc[1] << 16);
+          hp.idx[(t * CW + wbase[k] + 2 * slot + 1) * L + ln] = loc[2] | (loc[3] << 16);
+        }
+      }
+    }
+    lane += ns;
+  }
+  return lane - lo;
+}
+
+SlotConfig slot_of(uint32_t cfg) { return cfg == 2 ? kSlotWide : kSlotNarrow; }
+
+// Check, before anything reaches the GPU, that every index the kernel will
+// follow stays inside the buffer it addresses: LDS byte offsets inside the
+// stage / set-point buffers, rows inside the tile, segments inside the
+// workgroup, initial-condition and output indices inside their arrays.
+template <typename R>
+void validate_program(pasif_driver* d, const HostProgram<R>& hp, const std::vector<TileHeader>& hdr,
+                      bool adjoint_side, uint32_t L, const SlotConfig& c, uint32_t ss,
+                      uint32_t xs, size_t n_init, size_t out_len, uint32_t n_exc) {
+  const int CW = slot_words(c);
+  const uint32_t smax = L * ss, xmax = L * xs;
+  const size_t ntiles = hdr.size();
+  for (size_t t = 0; t < ntiles; ++t) {
+    const uint32_t rows = adjoint_side ? hdr[t].rows_a : hdr[t].rows_f;
+    PASIF_REQUIRE(rows <= L, PASIF_ERR_UNSUPPORTED, "tile %zu: %u rows > %u lanes", t, rows, L);
+    for (uint32_t l = 0; l < L; ++l) {
+      const size_t i = t * L + l;
+      const uint32_t w = hp.info[i];
+      const uint32_t row = lane_row(w);
+      if (row == kRowIdle) continue;
+      PASIF_REQUIRE(row < rows, PASIF_ERR_UNSUPPORTED, "tile %zu lane %u: row %u of %u", t, l,
+                    row, rows);
+      if (lane_owner(w)) {
+        PASIF_REQUIRE(lane_nseg(w) >= 1 && l + lane_nseg(w) <= L, PASIF_ERR_UNSUPPORTED,
+                      "tile %zu lane %u: %u segments overflow the tile", t, l, lane_nseg(w));
+        PASIF_REQUIRE(hp.src[i] < n_init, PASIF_ERR_UNSUPPORTED, "initial index %u of %zu",
+                      hp.src[i], n_init);
+        PASIF_REQUIRE(hp.out[i] < out_len, PASIF_ERR_UNSUPPORTED, "output index %u of %zu",
+                      hp.out[i], out_len);
+        PASIF_REQUIRE(hp.exc[i] < int32_t(n_exc), PASIF_ERR_UNSUPPORTED, "excitation %d of %u",
+                      hp.exc[i], n_exc);
+      }
+    }
+    for (int wd = 0; wd < CW; ++wd) {
+      const int rel = wd - (c.c1 + c.c2);
+      const bool second = rel >= 0 && (rel % 2) == 1;
+      for (uint32_t l = 0; l < L; ++l) {
+        const uint32_t w = hp.idx[(t * CW + wd) * L + l];
+        const uint32_t lo = w & 0xFFFFu, hi = w >> 16;
+        const bool ok = second ? (lo <= xmax && hi <= xmax) : (lo <= smax && hi <= xmax);
+        PASIF_REQUIRE(ok && (lo % sizeof(R)) == 0 && (hi % sizeof(R)) == 0, PASIF_ERR_UNSUPPORTED,
+                      "tile %zu word %d lane %u: LDS offsets %u/%u beyond %u/%u", t, wd, l, lo, hi,
+                      smax, xmax);
+      }
+    }
+  }
+}
+
+std::vector<Instance> make_instances(pasif_driver* d, bool adjoint) {
+  std::vector<Instance> v;
+  const uint32_t S = d->fwd.size.size();
+  const uint32_t N = d->fwd.n, Na = adjoint ? d->adj.n : 0;
+  if (d->mapped) {
+    v.reserve(S);
+    for (uint32_t s = 0; s < S; ++s) {
+      Instance in{};
+      in.sys = s;
+      in.exc = d->map_exc.empty() ? 0 : d->map_exc[s];
+      PASIF_REQUIRE(in.exc < (int32_t)d->n_exc, PASIF_ERR_ARGUMENT,
+                    "system %u maps to excitation %d of %u", s, in.exc, d->n_exc);
+      in.steps = d->map_steps.empty() ? d->numsteps : d->map_steps[s];
+      in.out_f = d->fwd.off[s];
+      in.out_a = adjoint ? d->adj.off[s] : 0;
+      v.push_back(in);
+    }
+  } else {
+    v.reserve(size_t(S) * d->n_exc);
+    for (uint32_t p = 0; p < d->n_exc; ++p)
+      for (uint32_t s = 0; s < S; ++s) {
+        Instance in{};
+        in.sys = s;
+        in.exc = int32_t(p);
+        in.steps = d->map_steps.empty() ? d->numsteps : d->map_steps[s];
+        in.out_f = p * N + d->fwd.off[s];
+        in.out_a = adjoint ? p * Na + d->adj.off[s] : 0;
+        v.push_back(in);
+      }
+  }
+  return v;
+}
+
+template <typename R>
+void pack_impl(pasif_driver* d, bool adjoint) {
+  Packed& pk = d->pk;
+  const uint32_t S = d->fwd.size.size();
+  uint32_t cfg = d->slot_cfg ? d->slot_cfg : 1;
+  SlotConfig c = slot_of(cfg);
+  // lanes per system
+  std::vector<uint32_t> lanes_f(S), lanes_a(S, 0);
+  uint32_t maxl = 0;
+  for (uint32_t s = 0; s < S; ++s) {
+    lanes_f[s] = system_lanes(d->fwd, s, c);
+    if (adjoint) lanes_a[s] = system_lanes(d->adj, s, c);
+    PASIF_REQUIRE(lanes_f[s] > 0 && (!adjoint || lanes_a[s] > 0), PASIF_ERR_UNSUPPORTED,
+                  "system %u has a term kind the slot configuration cannot hold", s);
+    maxl = std::max(maxl, std::max(lanes_f[s], lanes_a[s]));
+  }
+  PASIF_REQUIRE(maxl <= kMaxLanes, PASIF_ERR_UNSUPPORTED,
+                "a system needs %u lanes; one tile holds at most %u (system too large)", maxl,
+                kMaxLanes);
+  if (maxl > 256 && cfg == 2) {
+    cfg = 1;
+    c = slot_of(cfg);
+    for (uint32_t s = 0; s < S; ++s) {
+      lanes_f[s] = system_lanes(d->fwd, s, c);
+      if (adjoint) lanes_a[s] = system_lanes(d->adj, s, c);
+    }
+  }
+  std::vector<Instance> inst = make_instances(d, adjoint);
+  uint64_t total_lanes = 0;
+  for (Instance& in : inst) {
+    in.lanes = std::max(lanes_f[in.sys], lanes_a[in.sys]);
+    total_lanes += in.lanes;
+  }
+  uint32_t L = d->tile_lanes;
+  if (L == 0) {
+    const uint64_t per = (total_lanes + 511) / 512;  // aim for >= 512 tiles
+    L = uint32_t(std::min<uint64_t>(256, std::max<uint64_t>(64, ((per + 63) / 64) * 64)));
+  }
+  L = std::max(L, ((maxl + 63) / 64) * 64);
+  PASIF_REQUIRE(L <= kMaxLanes, PASIF_ERR_UNSUPPORTED, "tile of %u lanes", L);
+  // longest instances first (their tiles start first), stable otherwise
+  std::stable_sort(inst.begin(), inst.end(),
+                   [](const Instance& a, const Instance& b) { return a.steps > b.steps; });
+  // greedy fill
+  std::vector<uint32_t> tile_of(inst.size());
+  uint32_t ntiles = 0, used = L;
+  for (size_t i = 0; i < inst.size(); ++i) {
+    if (used + inst[i].lanes > L) {
+      ++ntiles;
+      used = 0;
+    }
+    tile_of[i] = ntiles - 1;
+    used += inst[i].lanes;
+  }
+  // LDS row strides: stage buffer [row][2], adjoint set points [row][4]
+  const uint32_t ss = 2 * sizeof(R), xs_f = 2 * sizeof(R), xs_a = 4 * sizeof(R);
+  HostProgram<R> hf, ha;
+  hf.resize(ntiles, L, c, L * ss, L * xs_f);
+  if (adjoint) ha.resize(ntiles, L, c, L * ss, L * xs_a);
+  std::vector<TileHeader> hdr(ntiles);
+  std::memset(hdr.data(), 0, hdr.size() * sizeof(TileHeader));
+  std::vector<uint32_t> lane_used(ntiles, 0), rf(ntiles, 0), ra(ntiles, 0);
+  std::vector<bool> split_f(ntiles, false), split_a(ntiles, false);
+  uint32_t tmax = 0;
+  for (size_t i = 0; i < inst.size(); ++i) {
+    const Instance& in = inst[i];
+    const uint32_t t = tile_of[i];
+    TileHeader& H = hdr[t];
+    bool sf = split_f[t];
+    place(d->fwd, in.sys, in, false, hf, t, L, lane_used[t], rf[t], rf[t], c, H.cnt_f, sf,
+          d->fwd.off[in.sys], ss, xs_f);
+    split_f[t] = sf;
+    if (adjoint) {
+      bool sa = split_a[t];
+      place(d->adj, in.sys, in, true, ha, t, L, lane_used[t], ra[t], rf[t], c, H.cnt_a, sa,
+            d->adj.xoff[in.sys], ss, xs_a);
+      split_a[t] = sa;
+      ra[t] += d->adj.size[in.sys];
+    }
+    rf[t] += d->fwd.size[in.sys];
+    lane_used[t] += in.lanes;
+    H.steps = std::max(H.steps, in.steps);
+    tmax = std::max(tmax, in.steps);
+  }
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    hdr[t].rows_f = rf[t];
+    hdr[t].rows_a = ra[t];
+    hdr[t].split_f = split_f[t] ? 1 : 0;
+    hdr[t].split_a = split_a[t] ? 1 : 0;
+  }
+  const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
+  const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
+  validate_program(d, hf, hdr, false, L, c, ss, xs_f, d->fwd.init.size(), out_len_f, d->n_exc);
+  if (adjoint)
+    validate_program(d, ha, hdr, true, L, c, ss, xs_a, d->adj.init.size(), out_len_a, d->n_exc);
+  // initial conditions in the working precision
+  std::vector<R> initf(d->fwd.init.begin(), d->fwd.init.end());
+  upload(d, pk.hdr, hdr);
+  upload(d, pk.f_info, hf.info);
+  upload(d, pk.f_out, hf.out);
+  upload(d, pk.f_src, hf.src);
+  upload(d, pk.f_steps, hf.steps);
+  upload(d, pk.f_exc, hf.exc);
+  upload(d, pk.f_force, hf.force);
+  upload(d, pk.f_val, hf.val);
+  upload(d, pk.f_idx, hf.idx);
+  upload(d, pk.f_init, initf);
+  pk.program_bytes = pk.f_info.bytes * 4 + pk.f_force.bytes * 2 + pk.f_val.bytes + pk.f_idx.bytes;
+  if (adjoint) {
+    std::vector<R> inita(d->adj.init.begin(), d->adj.init.end());
+    upload(d, pk.a_info, ha.info);
+    upload(d, pk.a_out, ha.out);
+    upload(d, pk.a_src, ha.src);
+    upload(d, pk.a_steps, ha.steps);
+    upload(d, pk.a_exc, ha.exc);
+    upload(d, pk.a_force, ha.force);
+    upload(d, pk.a_val, ha.val);
+    upload(d, pk.a_idx, ha.idx);
+    upload(d, pk.a_init, inita);
+    pk.program_bytes += pk.a_info.bytes * 4 + pk.a_force.bytes * 2 + pk.a_val.bytes +
+                        pk.a_idx.bytes;
+  }
+  if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
+  pk.lanes = L;
+  pk.ntiles = ntiles;
+  pk.slot_cfg = cfg;
+  pk.adjoint = adjoint;
+  pk.precision = int(sizeof(R) * 8);
+  pk.tmax = tmax;
+  pk.out_len_f = out_len_f;
+  pk.out_len_a = out_len_a;
+  pk.valid = true;
+  if (d->d_compute)
+    printf("[pasif] %zu instances in %u tiles of %u lanes (%s slots), T max %u, %.1f MB program\n",
+           inst.size(), ntiles, L, cfg == 2 ? "wide" : "narrow", tmax,
+           pk.program_bytes / 1048576.0);
+}
+
+uint64_t pack_key(pasif_driver* d, bool adjoint) {
+  return (uint64_t(d->precision) << 1) ^ (uint64_t(adjoint) << 8) ^ (uint64_t(d->tile_lanes) << 10) ^
+         (uint64_t(d->slot_cfg) << 24);
+}
+
+void require_device(pasif_driver* d) {
+  PASIF_REQUIRE(d->dev >= 0, PASIF_ERR_STATE,
+                "this handle was created without a GPU (gpu_id < 0): host-side checks only");
+}
+
+void ensure_packed(pasif_driver* d, bool adjoint) {
+  PASIF_REQUIRE(d->sys_ready, PASIF_ERR_STATE, "the system must be allocated before solving");
+  if (adjoint)
+    PASIF_REQUIRE(d->adj_ready, PASIF_ERR_STATE, "the adjoint must be allocated before solving");
+  const uint64_t key = pack_key(d, adjoint);
+  if (d->pk.valid && d->pk.key == key) return;
+  d->pk.valid = false;
+  if (d->precision == 64)
+    pack_impl<double>(d, adjoint);
+  else
+    pack_impl<float>(d, adjoint);
+  d->pk.key = key;
+}
+
+template <typename R>
+void ensure_excitations(pasif_driver* d) {
+  if (!d->d_exc_dirty && d->d_exc_prec == d->precision) return;
+  std::vector<R> e(d->exc.begin(), d->exc.end());
+  upload(d, d->d_exc, e);
+  std::vector<R> m(d->interp.begin(), d->interp.end());
+  upload(d, d->d_interp, m);
+  PASIF_HIP(hipStreamSynchronize(d->stream));
+  d->d_exc_dirty = false;
+  d->d_exc_prec = d->precision;
+}
+
+template <typename R>
+void fill_program(Program<R>& p, const DevBuf& info, const DevBuf& out, const DevBuf& src,
+                  const DevBuf& steps, const DevBuf& exc, const DevBuf& force, const DevBuf& val,
+                  const DevBuf& idx, const DevBuf& init) {
+  p.info = info.as<uint32_t>();
+  p.out = out.as<uint32_t>();
+  p.src = src.as<uint32_t>();
+  p.steps = steps.as<uint32_t>();
+  p.exc = exc.as<int32_t>();
+  p.force = force.as<R>();
+  p.val = val.as<R>();
+  p.idx = idx.as<uint32_t>();
+  p.init = init.as<R>();
+}
+
+// Run the packed tiles to completion.  save_steps > 0 records a trajectory.
+template <typename R>
+void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
+                std::vector<double>* grad_out) {
+  ensure_packed(d, adjoint);
+  ensure_excitations<R>(d);
+  Packed& pk = d->pk;
+  const uint32_t L = pk.lanes, nt = pk.ntiles;
+  const size_t rows = size_t(nt) * L;
+  TileArgs<R> a{};
+  a.hdr = pk.hdr.as<TileHeader>();
+  a.lanes = L;
+  fill_program(a.fwd, pk.f_info, pk.f_out, pk.f_src, pk.f_steps, pk.f_exc, pk.f_force, pk.f_val,
+               pk.f_idx, pk.f_init);
+  if (adjoint)
+    fill_program(a.adj, pk.a_info, pk.a_out, pk.a_src, pk.a_steps, pk.a_exc, pk.a_force, pk.a_val,
+                 pk.a_idx, pk.a_init);
+  a.excit = d->d_exc.as<R>();
+  a.exc_len = d->exc_len;
+  a.interp = d->interp_w ? d->d_interp.as<R>() : nullptr;
+  a.interp_w = d->interp_w;
+  // h = 1/sampleRate in the working type, h2 and h6 from that h in double
+  // (src/__GpuDriver.cu:632-636)
+  const R h = R(1.0 / double(d->sample_rate));
+  a.h = h;
+  a.h2 = R(double(h) / 2.0);
+  a.h6 = R(double(h) / 6.0);
+  dev_alloc(d, d->qstate, rows * sizeof(R));
+  dev_alloc(d, d->progress, size_t(nt) * 4 * sizeof(uint32_t));
+  PASIF_HIP(hipMemsetAsync(d->progress.p, 0, d->progress.bytes, d->stream));
+  a.qstate = d->qstate.as<R>();
+  a.progress = d->progress.as<uint32_t>();
+  a.window = d->window;
+  if (d->window) {
+    dev_alloc(d, d->mxstate, rows * sizeof(R));
+    dev_alloc(d, d->out_max, size_t(pk.out_len_f) * sizeof(R));
+    PASIF_HIP(hipMemsetAsync(d->out_max.p, 0, d->out_max.bytes, d->stream));
+    a.mxstate = d->mxstate.as<R>();
+    a.out_max = d->out_max.as<R>();
+  }
+  dev_alloc(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
+  a.out_q = d->out_q.as<R>();
+  a.out_len = pk.out_len_f;
+  uint32_t nslots = 0;
+  if (save_steps) {
+    nslots = pk.tmax / save_steps;
+    dev_alloc(d, d->traj, size_t(nslots) * pk.out_len_f * sizeof(R));
+    PASIF_HIP(hipMemsetAsync(d->traj.p, 0, d->traj.bytes, d->stream));
+    a.traj = d->traj.as<R>();
+    a.save_every = save_steps;
+    a.nslots = nslots;
+  }
+  uint64_t work = pk.tmax;
+  if (adjoint) {
+    uint32_t K = d->ck_every ? d->ck_every : 256;
+    K = std::max<uint32_t>(1, std::min<uint32_t>(K, std::max<uint32_t>(pk.tmax, 1)));
+    const uint32_t nck = std::max<uint32_t>(1, (pk.tmax + K - 1) / K);
+    dev_alloc(d, d->lstate, rows * sizeof(R));
+    dev_alloc(d, d->ckpt, rows * nck * sizeof(R));
+    dev_alloc(d, d->segbuf, rows * (K + 1) * sizeof(R));
+    dev_alloc(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
+    a.lstate = d->lstate.as<R>();
+    a.ckpt = d->ckpt.as<R>();
+    a.segbuf = d->segbuf.as<R>();
+    a.out_grad = d->out_grad.as<R>();
+    a.ck_every = K;
+    a.nck = nck;
+    a.adjoint = 1;
+    work = 3ull * pk.tmax;
+  }
+  const bool plain = d->solver == PASIF_SOLVER_PLAIN;
+  a.budget = plain ? 1u : 0xFFFFFFFFu;
+  uint64_t launches = plain ? std::max<uint64_t>(work, 1) : 1;
+  PASIF_HIP(hipEventRecord(d->ev0, d->stream));
+  for (uint64_t i = 0; i < launches; ++i) {
+    hipError_t e = launch_tiles(a, nt, int(pk.slot_cfg), d->stream);
+    if (e != hipSuccess) {
+      set_err(d, "tile kernel launch failed (%u tiles of %u lanes): %s", nt, L, hipGetErrorName(e));
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+  }
+  PASIF_HIP(hipEventRecord(d->ev1, d->stream));
+  PASIF_HIP(hipEventSynchronize(d->ev1));
+  float ms = 0.f;
+  PASIF_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  d->last_ms = ms;
+  d->last_launches = uint32_t(launches);
+  // results
+  std::vector<R> hq(pk.out_len_f);
+  PASIF_HIP(hipMemcpy(hq.data(), d->out_q.p, hq.size() * sizeof(R), hipMemcpyDeviceToHost));
+  d->res_q.assign(hq.begin(), hq.end());
+  d->have_max = d->window != 0;
+  if (d->window) {
+    std::vector<R> hm(pk.out_len_f);
+    PASIF_HIP(hipMemcpy(hm.data(), d->out_max.p, hm.size() * sizeof(R), hipMemcpyDeviceToHost));
+    d->res_max.assign(hm.begin(), hm.end());
+  }
+  if (traj_out) {
+    std::vector<R> ht(size_t(nslots) * pk.out_len_f);
+    if (!ht.empty())
+      PASIF_HIP(hipMemcpy(ht.data(), d->traj.p, ht.size() * sizeof(R), hipMemcpyDeviceToHost));
+    traj_out->assign(ht.begin(), ht.end());
+    d->traj.release();
+  }
+  if (grad_out) {
+    std::vector<R> hg(pk.out_len_a);
+    PASIF_HIP(hipMemcpy(hg.data(), d->out_grad.p, hg.size() * sizeof(R), hipMemcpyDeviceToHost));
+    grad_out->assign(hg.begin(), hg.end());
+    d->ckpt.release();
+    d->segbuf.release();
+  }
+}
+
+void solve(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj,
+           std::vector<double>* grad) {
+  require_device(d);
+  PASIF_HIP(hipSetDevice(d->dev));
+  if (d->precision == 64)
+    solve_impl<double>(d, adjoint, save_steps, traj, grad);
+  else
+    solve_impl<float>(d, adjoint, save_steps, traj, grad);
+}
+
+void copy_out(pasif_driver* d, const std::vector<double>& v, double* out, size_t cap, size_t* len) {
+  if (len) *len = v.size();
+  if (!out) return;
+  PASIF_REQUIRE(cap >= v.size(), PASIF_ERR_ARGUMENT, "output buffer holds %zu, need %zu", cap,
+                v.size());
+  std::copy(v.begin(), v.end(), out);
+}
+
+void load_exc(pasif_driver* d, const double* e, uint32_t n_exc, uint32_t len, uint32_t sr) {
+  PASIF_REQUIRE(sr > 0, PASIF_ERR_ARGUMENT, "sample rate must be positive");
+  PASIF_REQUIRE(n_exc > 0 && len > 0, PASIF_ERR_ARGUMENT, "empty excitation set");
+  PASIF_REQUIRE(e != nullptr, PASIF_ERR_ARGUMENT, "null excitation data");
+  d->exc.assign(e, e + size_t(n_exc) * len);
+  d->n_exc = n_exc;
+  d->exc_len = len;
+  d->sample_rate = sr;
+  d->d_exc_dirty = true;
+  d->pk.valid = false;
+}
+
+void set_csr(pasif_driver* d, CsrIn& K, const uint32_t n[2], const double* v, const uint32_t* ix,
+             const uint32_t* ptr, size_t nnz) {
+  PASIF_REQUIRE(n != nullptr && ptr != nullptr, PASIF_ERR_ARGUMENT, "null CSR argument");
+  PASIF_REQUIRE(nnz == 0 || (v && ix), PASIF_ERR_ARGUMENT, "null CSR values");
+  K.n0 = n[0];
+  K.n1 = n[1];
+  K.val.assign(v, v + nnz);
+  K.idx.assign(ix, ix + nnz);
+  K.ptr.assign(ptr, ptr + size_t(n[0]) + 1);
+  K.set = true;
+}
+
+void set_coo(pasif_driver* d, CooIn& t, int order, const uint32_t* n, const double* v,
+             const uint32_t* ix, size_t nnz) {
+  PASIF_REQUIRE(n != nullptr, PASIF_ERR_ARGUMENT, "null tensor dims");
+  PASIF_REQUIRE(nnz == 0 || (v && ix), PASIF_ERR_ARGUMENT, "null tensor data");
+  t.order = order;
+  for (int a = 0; a < order; ++a) t.dims[a] = n[a];
+  t.val.assign(v, v + nnz);
+  t.idx.assign(ix, ix + nnz * order);
+  t.set = true;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* pasif_version(void) { return "pasif_hip 0.1 gfx950"; }
+
+int pasif_create(pasif_driver** out, const double* excitations, uint32_t n_exc, uint32_t exc_len,
+                 uint32_t sample_rate, uint32_t numsteps, int d_compute, int d_system,
+                 int d_solver, int gpu_id) {
+  if (!out) return PASIF_ERR_ARGUMENT;
+  *out = nullptr;
+  std::unique_ptr<pasif_driver> holder(new (std::nothrow) pasif_driver());
+  if (!holder) return PASIF_ERR_MEMORY;
+  pasif_driver* d = holder.get();
+  const int rc = guarded(d, [&] {
+    d->dev = gpu_id;
+    if (gpu_id >= 0) {
+      int ndev = 0;
+      PASIF_HIP(hipGetDeviceCount(&ndev));
+      PASIF_REQUIRE(gpu_id < ndev, PASIF_ERR_ARGUMENT, "GPU %d of %d", gpu_id, ndev);
+      PASIF_HIP(hipSetDevice(gpu_id));
+      PASIF_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+      PASIF_HIP(hipEventCreate(&d->ev0));
+      PASIF_HIP(hipEventCreate(&d->ev1));
+    }
+    d->d_compute = d_compute != 0;
+    d->d_system = d_system != 0;
+    d->d_solver = d_solver != 0;
+    load_exc(d, excitations, n_exc, exc_len, sample_rate);
+    d->numsteps = numsteps ? numsteps : exc_len;
+  });
+  // on failure the handle is still returned so the caller can read the message
+  *out = holder.release();
+  return rc;
+}
+
+int pasif_destroy(pasif_driver* d) {
+  if (!d) return PASIF_OK;
+  if (d->dev >= 0) (void)hipSetDevice(d->dev);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  d->pk.release();
+  if (d->ev0) (void)hipEventDestroy(d->ev0);
+  if (d->ev1) (void)hipEventDestroy(d->ev1);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
+  delete d;
+  return PASIF_OK;
+}
+
+const char* pasif_last_error(const pasif_driver* d) { return d ? d->err.c_str() : "null handle"; }
+
+int pasif_set_fwd_K(pasif_driver* d, const uint32_t n[2], const double* values,
+                    const uint32_t* indices, const uint32_t* indptr, size_t nnz) {
+  return guarded(d, [&] {
+    set_csr(d, d->fin.K, n, values, indices, indptr, nnz);
+    d->sys_ready = false;
+  });
+}
+int pasif_set_bwd_K(pasif_driver* d, const uint32_t n[2], const double* values,
+                    const uint32_t* indices, const uint32_t* indptr, size_t nnz) {
+  return guarded(d, [&] {
+    set_csr(d, d->bin.K, n, values, indices, indptr, nnz);
+    d->adj_ready = false;
+  });
+}
+int pasif_set_fwd_gamma(pasif_driver* d, const uint32_t n[3], const double* v, const uint32_t* ix,
+                        size_t nnz) {
+  return guarded(d, [&] { set_coo(d, d->fin.G, 3, n, v, ix, nnz); d->sys_ready = false; });
+}
+int pasif_set_fwd_lambda(pasif_driver* d, const uint32_t n[4], const double* v, const uint32_t* ix,
+                         size_t nnz) {
+  return guarded(d, [&] { set_coo(d, d->fin.L, 4, n, v, ix, nnz); d->sys_ready = false; });
+}
+int pasif_set_fwd_psi(pasif_driver* d, const uint32_t n[5], const double* v, const uint32_t* ix,
+                      size_t nnz) {
+  return guarded(d, [&] { set_coo(d, d->fin.P, 5, n, v, ix, nnz); d->sys_ready = false; });
+}
+int pasif_set_bwd_gamma(pasif_driver* d, const uint32_t n[3], const double* v, const uint32_t* ix,
+                        size_t nnz) {
+  return guarded(d, [&] { set_coo(d, d->bin.G, 3, n, v, ix, nnz); d->adj_ready = false; });
+}
+int pasif_set_bwd_lambda(pasif_driver* d, const uint32_t n[4], const double* v, const uint32_t* ix,
+                         size_t nnz) {
+  return guarded(d, [&] { set_coo(d, d->bin.L, 4, n, v, ix, nnz); d->adj_ready = false; });
+}
+int pasif_set_bwd_psi(pasif_driver* d, const uint32_t n[5], const double* v, const uint32_t* ix,
+                      size_t nnz) {
+  return guarded(d, [&] { set_coo(d, d->bin.P, 5, n, v, ix, nnz); d->adj_ready = false; });
+}
+int pasif_set_fwd_force_pattern(pasif_driver* d, const double* f, size_t n) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(n == 0 || f, PASIF_ERR_ARGUMENT, "null force pattern");
+    d->fin.force.assign(f, f + n);
+    d->fin.force_set = true;
+    d->sys_ready = false;
+  });
+}
+int pasif_set_fwd_initial_conditions(pasif_driver* d, const double* q0, size_t n) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(n == 0 || q0, PASIF_ERR_ARGUMENT, "null initial conditions");
+    d->fin.init.assign(q0, q0 + n);
+    d->fin.init_set = true;
+    d->sys_ready = false;
+  });
+}
+int pasif_set_bwd_force_pattern(pasif_driver* d, const double* f, size_t n) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(n == 0 || f, PASIF_ERR_ARGUMENT, "null force pattern");
+    d->bin.force.assign(f, f + n);
+    d->bin.force_set = true;
+    d->adj_ready = false;
+  });
+}
+int pasif_set_bwd_initial_conditions(pasif_driver* d, const double* l, size_t n) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(n == 0 || l, PASIF_ERR_ARGUMENT, "null terminal condition");
+    d->bin.init.assign(l, l + n);
+    d->bin.init_set = true;
+    d->adj_ready = false;
+  });
+}
+
+int pasif_allocate_system(pasif_driver* d) {
+  return guarded(d, [&] {
+    d->sys_ready = false;
+    d->adj_ready = false;
+    d->pk.valid = false;
+    build_rows(d, d->fin, d->fin.K.n0, false, d->fwd, "system");
+    std::vector<uint32_t> sizes = d->hint_f.empty() ? detect_blocks(d->fwd, true) : d->hint_f;
+    apply_partition(d, d->fwd, sizes, "system");
+    validate_blocks(d, d->fwd, nullptr, "system");
+    if (d->mapped || !d->map_steps.empty()) {
+      const size_t S = d->fwd.size.size();
+      PASIF_REQUIRE(d->layout_systems == S, PASIF_ERR_ARGUMENT,
+                    "the layout describes %u systems, the system has %zu", d->layout_systems, S);
+    }
+    d->sys_ready = true;
+    if (d->d_system)
+      printf("[pasif] system: %u DOFs in %zu independent systems\n", d->fwd.n,
+             d->fwd.size.size());
+  });
+}
+
+int pasif_allocate_adjoint(pasif_driver* d) {
+  return guarded(d, [&] {
+    d->adj_ready = false;
+    d->pk.valid = false;
+    PASIF_REQUIRE(d->sys_ready, PASIF_ERR_STATE, "the system must be set before the jacobian");
+    build_rows(d, d->bin, d->fwd.n, true, d->adj, "adjoint");
+    std::vector<uint32_t> sizes = d->hint_a.empty() ? detect_blocks(d->adj, false) : d->hint_a;
+    PASIF_REQUIRE(sizes.size() == d->fwd.size.size(), PASIF_ERR_ARGUMENT,
+                  "the adjoint splits into %zu blocks but the system into %zu; pass the "
+                  "per-system adjoint sizes",
+                  sizes.size(), d->fwd.size.size());
+    apply_partition(d, d->adj, sizes, "adjoint");
+    d->adj.xoff = d->fwd.off;
+    RowTerms xs;
+    xs.off = d->fwd.off;
+    xs.size = d->fwd.size;
+    validate_blocks(d, d->adj, &xs, "adjoint");
+    d->adj_ready = true;
+  });
+}
+
+int pasif_load_excitations(pasif_driver* d, const double* e, uint32_t n_exc, uint32_t len,
+                           uint32_t sr) {
+  return guarded(d, [&] { load_exc(d, e, n_exc, len, sr); });
+}
+
+int pasif_set_interpolation_matrix(pasif_driver* d, const double* m, size_t size, uint32_t w) {
+  return guarded(d, [&] {
+    if (size == 0) {
+      d->interp.clear();
+      d->interp_w = 0;
+    } else {
+      PASIF_REQUIRE(m != nullptr, PASIF_ERR_ARGUMENT, "null interpolation matrix");
+      PASIF_REQUIRE(w >= 2, PASIF_ERR_ARGUMENT, "interpolation window must be >= 2 (got %u)", w);
+      PASIF_REQUIRE(w <= kMaxInterpWindow, PASIF_ERR_UNSUPPORTED,
+                    "interpolation window %u exceeds %u", w, kMaxInterpWindow);
+      PASIF_REQUIRE(size % w == 0, PASIF_ERR_ARGUMENT,
+                    "interpolation matrix size %zu is not a multiple of the window %u", size, w);
+      PASIF_REQUIRE(size / w >= 2, PASIF_ERR_ARGUMENT,
+                    "interpolation matrix needs at least 2 rows (rows 0 and 1 are read)");
+      d->interp.assign(m, m + 2 * size_t(w));
+      d->interp_w = w;
+    }
+    d->d_exc_dirty = true;
+  });
+}
+
+int pasif_set_modulation_buffer(pasif_driver* d, const double* b, size_t n) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(n > 0 && b, PASIF_ERR_ARGUMENT, "The modulation buffer must be non-empty.");
+    PASIF_REQUIRE(n <= 32000, PASIF_ERR_ARGUMENT,
+                  "The modulation buffer don't fit in the GPU cst memory.");
+    d->modulation.assign(b, b + n);
+  });
+}
+
+int pasif_get_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* len) {
+  return guarded(d, [&] {
+    if (!out) {  // length query without solving
+      ensure_packed(d, false);
+      if (len) *len = d->pk.out_len_f;
+      return;
+    }
+    solve(d, false, 0, nullptr, nullptr);
+    copy_out(d, d->res_q, out, cap, len);
+  });
+}
+
+int pasif_get_trajectory(pasif_driver* d, uint32_t save_steps, double* out, size_t cap,
+                         size_t* len) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(save_steps > 0, PASIF_ERR_ARGUMENT, "saveSteps must be positive");
+    if (!out) {
+      ensure_packed(d, false);
+      if (len) *len = size_t(d->pk.tmax / save_steps) * d->pk.out_len_f;
+      return;
+    }
+    std::vector<double> traj;
+    solve(d, false, save_steps, &traj, nullptr);
+    copy_out(d, traj, out, cap, len);
+  });
+}
+
+int pasif_get_gradient(pasif_driver* d, uint32_t save, double* out, size_t cap, size_t* len) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(save == 0, PASIF_ERR_UNSUPPORTED,
+                  "partial gradients (save != 0) are not supported; the full schedule runs");
+    if (!out) {
+      ensure_packed(d, true);
+      if (len) *len = d->pk.out_len_a;
+      return;
+    }
+    std::vector<double> g;
+    solve(d, true, 0, nullptr, &g);
+    copy_out(d, g, out, cap, len);
+  });
+}
+
+int pasif_set_systems_layout(pasif_driver* d, uint32_t n_systems, const uint32_t* sizes_fwd,
+                             const uint32_t* sizes_adj, const int32_t* exc_of_system,
+                             const uint32_t* steps_of_system) {
+  return guarded(d, [&] {
+    d->hint_f.clear();
+    d->hint_a.clear();
+    d->map_exc.clear();
+    d->map_steps.clear();
+    d->mapped = false;
+    d->layout_systems = n_systems;
+    if (sizes_fwd) d->hint_f.assign(sizes_fwd, sizes_fwd + n_systems);
+    if (sizes_adj) d->hint_a.assign(sizes_adj, sizes_adj + n_systems);
+    if (exc_of_system) {
+      d->map_exc.assign(exc_of_system, exc_of_system + n_systems);
+      d->mapped = true;
+    }
+    if (steps_of_system) d->map_steps.assign(steps_of_system, steps_of_system + n_systems);
+    d->pk.valid = false;
+    if (d->sys_ready && !d->hint_f.empty()) {
+      apply_partition(d, d->fwd, d->hint_f, "system");
+      validate_blocks(d, d->fwd, nullptr, "system");
+    }
+  });
+}
+
+int pasif_set_precision(pasif_driver* d, int bits) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(bits == 64 || bits == 32, PASIF_ERR_ARGUMENT, "precision must be 32 or 64");
+    d->precision = bits;
+  });
+}
+
+int pasif_set_solver(pasif_driver* d, int solver) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(solver == PASIF_SOLVER_PERSISTENT || solver == PASIF_SOLVER_PLAIN,
+                  PASIF_ERR_ARGUMENT, "unknown solver %d", solver);
+    d->solver = solver;
+  });
+}
+
+int pasif_set_tuning(pasif_driver* d, uint32_t tile_lanes, uint32_t slot_config,
+                     uint32_t checkpoint_every) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(tile_lanes == 0 || (tile_lanes % 64 == 0 && tile_lanes <= kMaxLanes),
+                  PASIF_ERR_ARGUMENT, "tile lanes must be a multiple of 64 up to %u", kMaxLanes);
+    PASIF_REQUIRE(slot_config <= 2, PASIF_ERR_ARGUMENT, "slot configuration 0, 1 or 2");
+    d->tile_lanes = tile_lanes;
+    d->slot_cfg = slot_config;
+    d->ck_every = checkpoint_every;
+  });
+}
+
+int pasif_set_amplitude_window(pasif_driver* d, uint32_t window) {
+  return guarded(d, [&] { d->window = window; });
+}
+
+int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* len) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(d->have_max, PASIF_ERR_STATE, "no windowed solve has run");
+    copy_out(d, d->res_max, out, cap, len);
+  });
+}
+
+int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launches,
+                      uint32_t* tiles, uint32_t* lanes) {
+  if (!d) return PASIF_ERR_ARGUMENT;
+  if (kernel_ms) *kernel_ms = d->last_ms;
+  if (launches) *launches = d->last_launches;
+  if (tiles) *tiles = d->pk.ntiles;
+  if (lanes) *lanes = d->pk.lanes;
+  return PASIF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Tile program format shared by the host runtime (pasif_driver.cpp) and the
+// gfx950 kernels (pasif_kernels.hip).
+//
+// A tile is one workgroup's worth of independent systems ("instances": a
+// system paired with the excitation that drives it).  Every row (DOF) of an
+// instance is owned by one lane; a row with more terms than the lane's slot
+// budget is split over several consecutive lanes ("segments") whose partial
+// sums the owner adds in lane order.  Each lane holds the coefficients and
+// packed LDS indices of its segment's terms in registers for the whole solve;
+// the stage state y lives in LDS, one double per row, double-buffered across
+// the four RK4 stages.
+//
+// Term kinds by number of state factors:
+//   kind 1: A[o,j] S_j                       (CSR B+K)
+//   kind 2: G[o,j,k] S_j X_k                 (Gamma)
+//   kind 3: L[o,j,k,l] S_j X_k X_l           (Lambda)
+//   kind 4: P[o,j,k,l,m] S_j X_k X_l X_m     (Psi)
+// S is the integrated state, X the set point (S itself in the forward pass,
+// the stored forward trajectory in the adjoint pass).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pasif {
+
+constexpr uint32_t kRowIdle = 0xFFFFu;   // lane without a segment
+constexpr uint32_t kMaxLanes = 1024;
+constexpr size_t kMaxLdsBytes = 160 * 1024;   // LDS per CU on gfx950
+constexpr uint32_t kMaxInterpWindow = 2048;
+
+// per-lane word: [0,16) tile-local row, [16,26) segment count (owner lanes
+// only), bit 31 owner flag
+__host__ __device__ inline uint32_t lane_row(uint32_t w) { return w & 0xFFFFu; }
+__host__ __device__ inline uint32_t lane_nseg(uint32_t w) { return (w >> 16) & 0x3FFu; }
+__host__ __device__ inline bool lane_owner(uint32_t w) { return (w >> 31) != 0u; }
+__host__ __device__ inline uint32_t make_lane(uint32_t row, uint32_t nseg, bool owner) {
+  return (row & 0xFFFFu) | ((nseg & 0x3FFu) << 16) | (owner ? 0x80000000u : 0u);
+}
+
+// slot-width configurations (slots per lane for kinds 1..4)
+struct SlotConfig { int c1, c2, c3, c4; };
+constexpr SlotConfig kSlotNarrow{4, 4, 2, 2};
+constexpr SlotConfig kSlotWide{8, 8, 4, 2};
+
+__host__ __device__ constexpr int slot_values(SlotConfig c) { return c.c1 + c.c2 + c.c3 + c.c4; }
+// index words hold LDS byte offsets: kind 1 -> 1 (S), kind 2 -> 1
+// (S | X<<16), kind 3 -> 2 (S | X1<<16, X2), kind 4 -> 2 (S | X1<<16,
+// X2 | X3<<16).  S offsets address the stage double buffer [row][2]; X
+// offsets address it too in the forward program and the set-point buffer
+// [row][4] in the adjoint program.
+__host__ __device__ constexpr int slot_words(SlotConfig c) { return c.c1 + c.c2 + 2 * c.c3 + 2 * c.c4; }
+
+struct alignas(16) TileHeader {
+  uint32_t rows_f;     // forward rows in the tile
+  uint32_t rows_a;     // adjoint rows
+  uint32_t cnt_f[4];   // slot trip counts of the forward program
+  uint32_t cnt_a[4];   // slot trip counts of the adjoint program
+  uint32_t split_f;    // some forward row spans several lanes
+  uint32_t split_a;
+  uint32_t steps;      // largest step count among the tile's instances
+  uint32_t pad;
+};
+static_assert(sizeof(TileHeader) == 64, "tile header layout");
+
+// Per-tile program of one pass (forward or adjoint).  Arrays are
+// [tile][...][lanes]; `lanes` is the launch's workgroup size.
+template <typename R>
+struct Program {
+  const uint32_t* info;   // [tile][lanes] lane word
+  const uint32_t* out;    // [tile][lanes] owner: output index
+  const uint32_t* src;    // [tile][lanes] owner: initial-condition index
+  const uint32_t* steps;  // [tile][lanes] owner: step count of the instance
+  const int32_t* exc;     // [tile][lanes] owner: excitation id or -1
+  const R* force;         // [tile][lanes] owner: force pattern value
+  const R* val;           // [tile][slot_values][lanes]
+  const uint32_t* idx;    // [tile][slot_words][lanes]
+  const R* init;          // initial state (forward q0 / adjoint lambda_T)
+};
+
+// adjoint-pass phases of a tile (stored per tile in `phase`)
+enum : uint32_t { kPhaseForward = 0, kPhaseRecompute = 1, kPhaseBackward = 2, kPhaseDone = 3 };
+
+template <typename R>
+struct TileArgs {
+  const TileHeader* hdr;
+  uint32_t lanes;          // workgroup size == program stride
+  Program<R> fwd;
+  Program<R> adj;
+  // forcing
+  const R* excit;          // [n_exc][exc_len]
+  uint32_t exc_len;
+  const R* interp;         // [2][interp_w] rows 0 and 1, or null (zero-order hold)
+  uint32_t interp_w;
+  R h, h2, h6;
+  // resumable state, [tile][lanes] by row
+  R* qstate;
+  R* lstate;               // adjoint lambda
+  R* mxstate;
+  uint32_t* progress;      // [tile][4]: step, phase, segment, spare
+  uint32_t budget;         // steps of work per launch
+  // outputs
+  R* out_q;
+  R* out_max;
+  uint32_t window;
+  R* traj;
+  uint32_t out_len;
+  uint32_t save_every;
+  uint32_t nslots;
+  R* out_grad;
+  // adjoint scratch, [tile][...][lanes] by forward row
+  R* ckpt;                 // [tile][nck][lanes]
+  R* segbuf;               // [tile][ck_every+1][lanes]
+  uint32_t ck_every;
+  uint32_t nck;
+  uint32_t adjoint;        // run the adjoint program after the forward pass
+};
+
+}  // namespace pasif
+
+namespace pasif {
+// kernel launchers (pasif_kernels.hip); slot_cfg 1 = narrow, 2 = wide
+hipError_t launch_tiles(const TileArgs<double>& a, uint32_t ntiles, int slot_cfg, hipStream_t s);
+hipError_t launch_tiles(const TileArgs<float>& a, uint32_t ntiles, int slot_cfg, hipStream_t s);
+}  // namespace pasif
