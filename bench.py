@@ -1,0 +1,206 @@
+"""Benchmark of the batched RK4 hot path on one MI355X.
+
+``python bench.py [--gpus 1] [--steps K] [--warmup W]`` prints ONE JSON line.
+
+Workload (one bench "step" = one full solve): 4,096 two-DOF systems
+``[x0, x1, v0, v1]`` with ``B = 0.05 I``, ``K = [[2,-1],[-1,2]] s``,
+3 Gamma and 4 Lambda couplings, sinusoidal forcing on DOF 0 (one excitation
+per system), ``dt = 5e-4``, 50,000 RK4 steps, FP64 -- 2.048e8 system-steps
+per solve, inputs resident in HBM before the timed region.
+
+``value`` is system-steps/s of the persistent solver (all steps in one
+launch).  Also reported: the plain per-step kernel on the same GPU, the FP32
+rate, the roofline of the tile kernel (algorithmic FP64 flops per launch over
+the HIP-event duration of the launch), the FP64 error against the NumPy
+oracle on the CPU sample, and the oracle's own rate on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_per_step(p) -> float:
+    """Algorithmic FP64 flops of one RK4 step of every system (FMA = 2):
+    ``4 (2 nnzA + 3 nnzG + 4 nnzL + 5 nnzP + 2 nnzF) + 12 n`` summed over
+    systems; ``nnzF`` counts forced DOFs."""
+    A, G, L, f, q0 = p.getMatrices()
+    nnzA = A.tocsr().nnz
+    nnzG = len(np.asarray(G.val).reshape(-1))
+    nnzL = len(np.asarray(L.val).reshape(-1))
+    nnzF = int(np.count_nonzero(np.abs(f) > np.finfo(np.float64).eps))
+    n = len(q0)
+    return 4.0 * (2 * nnzA + 3 * nnzG + 4 * nnzL + 2 * nnzF) + 12.0 * n
+
+
+def bytes_per_launch(p, w) -> float:
+    """Algorithmic HBM bytes of one persistent launch: coefficients and
+    indices (8+4 B per term), initial state, final state, and the excitation
+    samples read (two per forced system per step)."""
+    A, G, L, f, q0 = p.getMatrices()
+    nnz = A.tocsr().nnz + len(np.asarray(G.val).reshape(-1)) + len(np.asarray(L.val).reshape(-1))
+    n = len(q0)
+    forced = int(np.count_nonzero(np.abs(f) > np.finfo(np.float64).eps))
+    return 12.0 * nnz + 8.0 * n + 8.0 * n + 8.0 * 2 * forced * w.numsteps
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), if the profile summary is
+    committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(w, n_sample: int):
+    """The FP64 NumPy oracle on the host: the first ``n_sample`` systems over
+    all steps (vectorised over systems, one thread)."""
+    import oracle as O
+    from pasif_amd.workloads import Workload
+    sub = Workload(w.name, w.vecM[:n_sample], w.vecB[:n_sample], w.vecK[:n_sample],
+                   w.vecGamma[:n_sample], w.vecLambda[:n_sample], w.vecForce[:n_sample],
+                   w.vecQ0[:n_sample], w.excitations[:n_sample], w.sample_rate, w.numsteps,
+                   exc_of_system=np.arange(n_sample), interp=w.interp)
+    asm = O.assemble_forward(sub.vecM, sub.vecB, sub.vecK, sub.vecGamma, sub.vecLambda,
+                             sub.vecForce, sub.vecQ0)
+    lay = O.mapped_layout(asm.sizes, sub.exc_of_system, np.full(n_sample, w.numsteps))
+    t0 = time.perf_counter()
+    res = O.rk4_forward(asm, O.Forcing(sub.excitations), w.sample_rate, lay)
+    dt = time.perf_counter() - t0
+    return res.q.reshape(-1), n_sample * w.numsteps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--systems", type=int, default=4096)
+    ap.add_argument("--rk-steps", type=int, default=50_000)
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: systems are independent and the path needs no "
+              f"communication; --gpus {args.gpus} is not supported (each GPU would run its "
+              f"own share of systems).", file=sys.stderr)
+        sys.exit(2)
+
+    from pasif_amd import workloads as W
+
+    w = W.two_dof(n_systems=args.systems, steps=args.rk_steps)
+    p = W.make_solver(w)
+    sys_steps = w.n_systems * w.numsteps
+
+    for _ in range(args.warmup):
+        p.getAmplitudes()
+    t0 = time.perf_counter()
+    kernel_ms = []
+    for _ in range(args.steps):
+        q = p.getAmplitudes()
+        kernel_ms.append(p._lastTiming()["kernel_ms"])
+    wall = (time.perf_counter() - t0) / args.steps
+    timing = p._lastTiming()
+    value = sys_steps / wall
+    kms = float(np.mean(kernel_ms))
+
+    flops = flops_per_step(p) * w.numsteps
+    achieved_tf = flops / (kms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "pipe": "FP64 VALU (no matrix op on this path; MI355X FP64 vector peak)",
+        "achieved": round(achieved_tf, 4),
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 5),
+        "traffic": read_traffic("tile_kernel_two_dof_4096x50000_traffic.txt"),
+        "algorithmic_bytes": bytes_per_launch(p, w),
+        "flops_per_launch": flops,
+        "kernel_ms": round(kms, 4),
+        "tiles": timing["tiles"],
+        "lanes_per_tile": timing["lanes"],
+    }
+
+    out = {
+        "metric": "system-steps/s fwd & fwd+adjoint vs plain at rel err<=1e-10; serving p99 ms",
+        "value": value,
+        "unit": "system-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"two_dof {w.n_systems} systems x {w.numsteps} RK4 steps, "
+                               f"dt 5e-4, sinusoidal forcing, forward, FP64",
+                   "systems": w.n_systems, "rk_steps": w.numsteps, "solver": "persistent"},
+        "roofline": roofline,
+    }
+
+    # plain per-step kernel (same GPU, same kernel, one launch per RK4 step)
+    if not args.no_plain:
+        p.setSolver("plain")
+        t0 = time.perf_counter()
+        qp = p.getAmplitudes()
+        pw = time.perf_counter() - t0
+        pt = p._lastTiming()
+        p.setSolver("persistent")
+        out["plain"] = {"value": sys_steps / pw, "kernel_ms": pt["kernel_ms"],
+                        "launches": pt["launches"],
+                        "bitwise_equal_to_persistent": bool(np.array_equal(qp, q))}
+        out["persistent_vs_plain"] = (sys_steps / wall) / (sys_steps / pw)
+
+    # FP32 rate
+    p.setPrecision(32)
+    p.getAmplitudes()
+    t0 = time.perf_counter()
+    q32 = p.getAmplitudes()
+    w32 = time.perf_counter() - t0
+    out["fp32"] = {"value": sys_steps / w32, "kernel_ms": p._lastTiming()["kernel_ms"]}
+    p.setPrecision(64)
+
+    if not args.no_cpu:
+        ref, cpu_rate, cpu_s = cpu_baseline(w, args.cpu_sample)
+        n_out = ref.size
+        g = q[:n_out]
+        floor = 1e-8 * np.abs(ref).max()
+        err = float(np.max(np.abs(g - ref) / np.maximum(np.abs(ref), floor)))
+        g32 = q32[:n_out]
+        err32 = float(np.max(np.abs(g32 - ref) / np.maximum(np.abs(ref), floor)))
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "system-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy oracle, first {args.cpu_sample} of {w.n_systems} systems "
+                      f"x all {w.numsteps} steps ({cpu_s:.1f} s), host cpu_count="
+                      f"{os.cpu_count()}"}
+        out["parity"] = {"max_rel_err_fp64_vs_oracle": err, "max_rel_err_fp32_vs_fp64_oracle": err32,
+                         "sample_systems": args.cpu_sample, "tolerance_fp64": 1e-10}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -128,8 +128,9 @@ int pasif_set_precision(pasif_driver* d, int bits);
  * PASIF_SOLVER_PLAIN: one launch per RK4 step, state through HBM. */
 int pasif_set_solver(pasif_driver* d, int solver);
 /* tile lanes (64..1024, multiple of 64; 0 = automatic), slot-width
- * configuration (0 = automatic, 1 = narrow, 2 = wide) and the adjoint
- * checkpoint interval in steps (0 = automatic). */
+ * configuration (0 = automatic, 1 = narrow {4,4,2,2}, 2 = wide {8,8,4,2},
+ * 3 = small {4,2,2,1} slots per lane for terms with 1..4 state factors) and
+ * the adjoint checkpoint interval in steps (0 = automatic). */
 int pasif_set_tuning(pasif_driver* d, uint32_t tile_lanes, uint32_t slot_config,
                      uint32_t checkpoint_every);
 /* fused reduction: max |q_m| over the post-step states m in (T-window, T] of

@@ -109,13 +109,14 @@ struct Packed {
   uint64_t key = 0;
   uint32_t out_len_f = 0, out_len_a = 0, tmax = 0;
   DevBuf hdr;
-  DevBuf f_info, f_out, f_src, f_steps, f_exc, f_force, f_val, f_idx, f_init;
-  DevBuf a_info, a_out, a_src, a_steps, a_exc, a_force, a_val, a_idx, a_init;
+  DevBuf f_info, f_out, f_src, f_steps, f_exc, f_fslot, f_force, f_val, f_idx, f_init;
+  DevBuf a_info, a_out, a_src, a_steps, a_exc, a_fslot, a_force, a_val, a_idx, a_init;
+  uint32_t max_forced = 0;
   size_t program_bytes = 0;
   void release() {
-    DevBuf* all[] = {&hdr,    &f_info,  &f_out, &f_src, &f_steps, &f_exc,   &f_force,
-                     &f_val,  &f_idx,   &f_init, &a_info, &a_out,  &a_src,  &a_steps,
-                     &a_exc,  &a_force, &a_val, &a_idx, &a_init};
+    DevBuf* all[] = {&hdr,     &f_info,  &f_out,  &f_src,   &f_steps, &f_exc, &f_fslot,
+                     &f_force, &f_val,   &f_idx,  &f_init,  &a_info,  &a_out, &a_src,
+                     &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init};
     for (DevBuf* b : all) b->release();
     valid = false;
   }
@@ -223,6 +224,13 @@ void dev_alloc(pasif_driver* d, DevBuf& b, size_t bytes) {
     throw Fail{PASIF_ERR_MEMORY};
   }
   b.bytes = bytes;
+}
+
+// keep an existing allocation that is large enough (run buffers are reused
+// across solves so a repeated solve allocates nothing)
+void dev_reserve(pasif_driver* d, DevBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return;
+  dev_alloc(d, b, bytes);
 }
 
 template <typename T>
@@ -409,7 +417,7 @@ uint32_t system_lanes(const RowTerms& rt, uint32_t s, const SlotConfig& c) {
 
 template <typename R>
 struct HostProgram {
-  std::vector<uint32_t> info, out, src, steps;
+  std::vector<uint32_t> info, out, src, steps, fslot;
   std::vector<int32_t> exc;
   std::vector<R> force, val;
   std::vector<uint32_t> idx;
@@ -421,6 +429,7 @@ struct HostProgram {
     src.assign(nl, 0);
     steps.assign(nl, 0);
     exc.assign(nl, -1);
+    fslot.assign(nl, 0);
     force.assign(nl, R(0));
     val.assign(nl * slot_values(c), R(0));
     const int CW = slot_words(c);
@@ -444,7 +453,7 @@ template <typename R>
 uint32_t place(const RowTerms& rt, uint32_t sys, const Instance& in, bool adjoint,
                HostProgram<R>& hp, size_t t, uint32_t L, uint32_t lo, uint32_t ro, uint32_t xro,
                const SlotConfig& c, uint32_t cnt[4], bool& split, uint32_t xoff,
-               uint32_t sstride, uint32_t xstride) {
+               uint32_t sstride, uint32_t xstride, uint32_t& nforced) {
   const int cap[4] = {c.c1, c.c2, c.c3, c.c4};
   int vbase[4], wbase[4];
   vbase[0] = 0;
@@ -474,6 +483,7 @@ uint32_t place(const RowTerms& rt, uint32_t sys, const Instance& in, bool adjoin
       if (std::fabs(f) > eps && in.exc >= 0) {  // COOVector eps drop (helpers.cu:676)
         hp.force[owner] = R(f);
         hp.exc[owner] = in.exc;
+        hp.fslot[owner] = nforced++;
       }
     } else {
       hp.out[owner] = in.out_a + r;
@@ -503,7 +513,9 @@ uint32_t place(const RowTerms& rt, uint32_t sys, const Instance& in, bool adjoin
   return lane - lo;
 }
 
-SlotConfig slot_of(uint32_t cfg) { return cfg == 2 ? kSlotWide : kSlotNarrow; }
+SlotConfig slot_of(uint32_t cfg) {
+  return cfg == 2 ? kSlotWide : (cfg == 3 ? kSlotSmall : kSlotNarrow);
+}
 
 // Check, before anything reaches the GPU, that every index the kernel will
 // follow stays inside the buffer it addresses: LDS byte offsets inside the
@@ -535,6 +547,9 @@ void validate_program(pasif_driver* d, const HostProgram<R>& hp, const std::vect
                       hp.out[i], out_len);
         PASIF_REQUIRE(hp.exc[i] < int32_t(n_exc), PASIF_ERR_UNSUPPORTED, "excitation %d of %u",
                       hp.exc[i], n_exc);
+        PASIF_REQUIRE(hp.exc[i] < 0 || hp.fslot[i] < hdr[t].nforced, PASIF_ERR_UNSUPPORTED,
+                      "tile %zu lane %u: sample slot %u of %u", t, l, hp.fslot[i],
+                      hdr[t].nforced);
       }
     }
     for (int wd = 0; wd < CW; ++wd) {
@@ -589,30 +604,58 @@ template <typename R>
 void pack_impl(pasif_driver* d, bool adjoint) {
   Packed& pk = d->pk;
   const uint32_t S = d->fwd.size.size();
-  uint32_t cfg = d->slot_cfg ? d->slot_cfg : 1;
-  SlotConfig c = slot_of(cfg);
-  // lanes per system
-  std::vector<uint32_t> lanes_f(S), lanes_a(S, 0);
-  uint32_t maxl = 0;
-  for (uint32_t s = 0; s < S; ++s) {
-    lanes_f[s] = system_lanes(d->fwd, s, c);
-    if (adjoint) lanes_a[s] = system_lanes(d->adj, s, c);
-    PASIF_REQUIRE(lanes_f[s] > 0 && (!adjoint || lanes_a[s] > 0), PASIF_ERR_UNSUPPORTED,
-                  "system %u has a term kind the slot configuration cannot hold", s);
-    maxl = std::max(maxl, std::max(lanes_f[s], lanes_a[s]));
+  std::vector<Instance> inst = make_instances(d, adjoint);
+  // per-system instance counts (Cartesian: every excitation)
+  std::vector<uint64_t> mult(S, 0);
+  for (const Instance& in : inst) mult[in.sys]++;
+  // kinds in use (a kind no row uses is skipped by the kernel)
+  bool kind_used[4] = {false, false, false, false};
+  for (int k = 0; k < 4; ++k) {
+    kind_used[k] = !d->fwd.val[k].empty();
+    if (adjoint) kind_used[k] = kind_used[k] || !d->adj.val[k].empty();
   }
-  PASIF_REQUIRE(maxl <= kMaxLanes, PASIF_ERR_UNSUPPORTED,
-                "a system needs %u lanes; one tile holds at most %u (system too large)", maxl,
-                kMaxLanes);
-  if (maxl > 256 && cfg == 2) {
-    cfg = 1;
-    c = slot_of(cfg);
-    for (uint32_t s = 0; s < S; ++s) {
-      lanes_f[s] = system_lanes(d->fwd, s, c);
-      if (adjoint) lanes_a[s] = system_lanes(d->adj, s, c);
+  // Slot configuration: the given one, or the one with the fewest lane-slot
+  // LDS reads per stage (lanes x sum over used kinds of slots x factors).
+  const uint32_t cands[3] = {3, 1, 2};
+  uint32_t cfg = d->slot_cfg;
+  SlotConfig c{};
+  std::vector<uint32_t> lanes_f(S), lanes_a(S, 0);
+  double best = -1.0;
+  for (uint32_t cand : cands) {
+    if (d->slot_cfg && cand != d->slot_cfg) continue;
+    const SlotConfig cc = slot_of(cand);
+    std::vector<uint32_t> lf(S), la(S, 0);
+    uint32_t maxl = 0;
+    uint64_t total = 0;
+    bool ok = true;
+    for (uint32_t s = 0; s < S && ok; ++s) {
+      lf[s] = system_lanes(d->fwd, s, cc);
+      if (adjoint) la[s] = system_lanes(d->adj, s, cc);
+      if (lf[s] == 0 || (adjoint && la[s] == 0)) ok = false;
+      const uint32_t l = std::max(lf[s], la[s]);
+      maxl = std::max(maxl, l);
+      total += uint64_t(l) * mult[s];
+    }
+    if (!ok || maxl > kMaxLanes || (maxl > 256 && cand != 1)) continue;
+    const int cap[4] = {cc.c1, cc.c2, cc.c3, cc.c4};
+    double reads = 0;
+    for (int k = 0; k < 4; ++k)
+      if (kind_used[k]) reads += cap[k] * (k + 1);
+    const double cost = double(total) * reads;
+    if (best < 0 || cost < best) {
+      best = cost;
+      cfg = cand;
+      c = cc;
+      lanes_f = lf;
+      lanes_a = la;
     }
   }
-  std::vector<Instance> inst = make_instances(d, adjoint);
+  PASIF_REQUIRE(best >= 0, PASIF_ERR_UNSUPPORTED,
+                "no slot configuration fits these systems (a system needs more than %u lanes, or "
+                "slot configuration %u cannot hold it)",
+                kMaxLanes, d->slot_cfg);
+  uint32_t maxl = 0;
+  for (uint32_t s = 0; s < S; ++s) maxl = std::max(maxl, std::max(lanes_f[s], lanes_a[s]));
   uint64_t total_lanes = 0;
   for (Instance& in : inst) {
     in.lanes = std::max(lanes_f[in.sys], lanes_a[in.sys]);
@@ -655,12 +698,13 @@ void pack_impl(pasif_driver* d, bool adjoint) {
     TileHeader& H = hdr[t];
     bool sf = split_f[t];
     place(d->fwd, in.sys, in, false, hf, t, L, lane_used[t], rf[t], rf[t], c, H.cnt_f, sf,
-          d->fwd.off[in.sys], ss, xs_f);
+          d->fwd.off[in.sys], ss, xs_f, H.nforced);
     split_f[t] = sf;
     if (adjoint) {
       bool sa = split_a[t];
+      uint32_t no_forcing = 0;
       place(d->adj, in.sys, in, true, ha, t, L, lane_used[t], ra[t], rf[t], c, H.cnt_a, sa,
-            d->adj.xoff[in.sys], ss, xs_a);
+            d->adj.xoff[in.sys], ss, xs_a, no_forcing);
       split_a[t] = sa;
       ra[t] += d->adj.size[in.sys];
     }
@@ -669,7 +713,9 @@ void pack_impl(pasif_driver* d, bool adjoint) {
     H.steps = std::max(H.steps, in.steps);
     tmax = std::max(tmax, in.steps);
   }
+  uint32_t max_forced = 0;
   for (uint32_t t = 0; t < ntiles; ++t) {
+    max_forced = std::max(max_forced, hdr[t].nforced);
     hdr[t].rows_f = rf[t];
     hdr[t].rows_a = ra[t];
     hdr[t].split_f = split_f[t] ? 1 : 0;
@@ -678,6 +724,16 @@ void pack_impl(pasif_driver* d, bool adjoint) {
   const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
   const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
   validate_program(d, hf, hdr, false, L, c, ss, xs_f, d->fwd.init.size(), out_len_f, d->n_exc);
+  {
+    uint32_t mf = 0;
+    for (const TileHeader& h : hdr) mf = std::max(mf, h.nforced);
+    const size_t lds = (2 * (L + 1) + L + (adjoint ? 4 * (L + 1) : 0) + 2 * kMaxInterpWindow +
+                        size_t(mf) * kForceStage) * sizeof(R);
+    PASIF_REQUIRE(lds <= kMaxLdsBytes, PASIF_ERR_UNSUPPORTED,
+                  "tiles of %u lanes with %u forced rows need %zu bytes of LDS (> %zu); use "
+                  "smaller tiles",
+                  L, mf, lds, kMaxLdsBytes);
+  }
   if (adjoint)
     validate_program(d, ha, hdr, true, L, c, ss, xs_a, d->adj.init.size(), out_len_a, d->n_exc);
   // initial conditions in the working precision
@@ -688,6 +744,7 @@ void pack_impl(pasif_driver* d, bool adjoint) {
   upload(d, pk.f_src, hf.src);
   upload(d, pk.f_steps, hf.steps);
   upload(d, pk.f_exc, hf.exc);
+  upload(d, pk.f_fslot, hf.fslot);
   upload(d, pk.f_force, hf.force);
   upload(d, pk.f_val, hf.val);
   upload(d, pk.f_idx, hf.idx);
@@ -700,6 +757,7 @@ void pack_impl(pasif_driver* d, bool adjoint) {
     upload(d, pk.a_src, ha.src);
     upload(d, pk.a_steps, ha.steps);
     upload(d, pk.a_exc, ha.exc);
+    upload(d, pk.a_fslot, ha.fslot);
     upload(d, pk.a_force, ha.force);
     upload(d, pk.a_val, ha.val);
     upload(d, pk.a_idx, ha.idx);
@@ -714,12 +772,14 @@ void pack_impl(pasif_driver* d, bool adjoint) {
   pk.adjoint = adjoint;
   pk.precision = int(sizeof(R) * 8);
   pk.tmax = tmax;
+  pk.max_forced = max_forced;
   pk.out_len_f = out_len_f;
   pk.out_len_a = out_len_a;
   pk.valid = true;
   if (d->d_compute)
-    printf("[pasif] %zu instances in %u tiles of %u lanes (%s slots), T max %u, %.1f MB program\n",
-           inst.size(), ntiles, L, cfg == 2 ? "wide" : "narrow", tmax,
+    printf("[pasif] %zu instances in %u tiles of %u lanes (slot config %u), T max %u, %.1f MB "
+           "program\n",
+           inst.size(), ntiles, L, cfg, tmax,
            pk.program_bytes / 1048576.0);
 }
 
@@ -761,13 +821,14 @@ void ensure_excitations(pasif_driver* d) {
 
 template <typename R>
 void fill_program(Program<R>& p, const DevBuf& info, const DevBuf& out, const DevBuf& src,
-                  const DevBuf& steps, const DevBuf& exc, const DevBuf& force, const DevBuf& val,
-                  const DevBuf& idx, const DevBuf& init) {
+                  const DevBuf& steps, const DevBuf& exc, const DevBuf& fslot,
+                  const DevBuf& force, const DevBuf& val, const DevBuf& idx, const DevBuf& init) {
   p.info = info.as<uint32_t>();
   p.out = out.as<uint32_t>();
   p.src = src.as<uint32_t>();
   p.steps = steps.as<uint32_t>();
   p.exc = exc.as<int32_t>();
+  p.fslot = fslot.as<uint32_t>();
   p.force = force.as<R>();
   p.val = val.as<R>();
   p.idx = idx.as<uint32_t>();
@@ -786,11 +847,12 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   TileArgs<R> a{};
   a.hdr = pk.hdr.as<TileHeader>();
   a.lanes = L;
-  fill_program(a.fwd, pk.f_info, pk.f_out, pk.f_src, pk.f_steps, pk.f_exc, pk.f_force, pk.f_val,
-               pk.f_idx, pk.f_init);
+  fill_program(a.fwd, pk.f_info, pk.f_out, pk.f_src, pk.f_steps, pk.f_exc, pk.f_fslot,
+               pk.f_force, pk.f_val, pk.f_idx, pk.f_init);
   if (adjoint)
-    fill_program(a.adj, pk.a_info, pk.a_out, pk.a_src, pk.a_steps, pk.a_exc, pk.a_force, pk.a_val,
-                 pk.a_idx, pk.a_init);
+    fill_program(a.adj, pk.a_info, pk.a_out, pk.a_src, pk.a_steps, pk.a_exc, pk.a_fslot,
+                 pk.a_force, pk.a_val, pk.a_idx, pk.a_init);
+  a.max_forced = pk.max_forced;
   a.excit = d->d_exc.as<R>();
   a.exc_len = d->exc_len;
   a.interp = d->interp_w ? d->d_interp.as<R>() : nullptr;
@@ -801,27 +863,27 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   a.h = h;
   a.h2 = R(double(h) / 2.0);
   a.h6 = R(double(h) / 6.0);
-  dev_alloc(d, d->qstate, rows * sizeof(R));
-  dev_alloc(d, d->progress, size_t(nt) * 4 * sizeof(uint32_t));
-  PASIF_HIP(hipMemsetAsync(d->progress.p, 0, d->progress.bytes, d->stream));
+  dev_reserve(d, d->qstate, rows * sizeof(R));
+  dev_reserve(d, d->progress, size_t(nt) * 4 * sizeof(uint32_t));
+  PASIF_HIP(hipMemsetAsync(d->progress.p, 0, size_t(nt) * 4 * sizeof(uint32_t), d->stream));
   a.qstate = d->qstate.as<R>();
   a.progress = d->progress.as<uint32_t>();
   a.window = d->window;
   if (d->window) {
-    dev_alloc(d, d->mxstate, rows * sizeof(R));
-    dev_alloc(d, d->out_max, size_t(pk.out_len_f) * sizeof(R));
-    PASIF_HIP(hipMemsetAsync(d->out_max.p, 0, d->out_max.bytes, d->stream));
+    dev_reserve(d, d->mxstate, rows * sizeof(R));
+    dev_reserve(d, d->out_max, size_t(pk.out_len_f) * sizeof(R));
+    PASIF_HIP(hipMemsetAsync(d->out_max.p, 0, size_t(pk.out_len_f) * sizeof(R), d->stream));
     a.mxstate = d->mxstate.as<R>();
     a.out_max = d->out_max.as<R>();
   }
-  dev_alloc(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
+  dev_reserve(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
   a.out_q = d->out_q.as<R>();
   a.out_len = pk.out_len_f;
   uint32_t nslots = 0;
   if (save_steps) {
     nslots = pk.tmax / save_steps;
-    dev_alloc(d, d->traj, size_t(nslots) * pk.out_len_f * sizeof(R));
-    PASIF_HIP(hipMemsetAsync(d->traj.p, 0, d->traj.bytes, d->stream));
+    dev_reserve(d, d->traj, size_t(nslots) * pk.out_len_f * sizeof(R));
+    PASIF_HIP(hipMemsetAsync(d->traj.p, 0, size_t(nslots) * pk.out_len_f * sizeof(R), d->stream));
     a.traj = d->traj.as<R>();
     a.save_every = save_steps;
     a.nslots = nslots;
@@ -831,10 +893,10 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     uint32_t K = d->ck_every ? d->ck_every : 256;
     K = std::max<uint32_t>(1, std::min<uint32_t>(K, std::max<uint32_t>(pk.tmax, 1)));
     const uint32_t nck = std::max<uint32_t>(1, (pk.tmax + K - 1) / K);
-    dev_alloc(d, d->lstate, rows * sizeof(R));
-    dev_alloc(d, d->ckpt, rows * nck * sizeof(R));
-    dev_alloc(d, d->segbuf, rows * (K + 1) * sizeof(R));
-    dev_alloc(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
+    dev_reserve(d, d->lstate, rows * sizeof(R));
+    dev_reserve(d, d->ckpt, rows * nck * sizeof(R));
+    dev_reserve(d, d->segbuf, rows * (K + 1) * sizeof(R));
+    dev_reserve(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
     a.lstate = d->lstate.as<R>();
     a.ckpt = d->ckpt.as<R>();
     a.segbuf = d->segbuf.as<R>();
@@ -877,13 +939,13 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
       PASIF_HIP(hipMemcpy(ht.data(), d->traj.p, ht.size() * sizeof(R), hipMemcpyDeviceToHost));
     traj_out->assign(ht.begin(), ht.end());
     d->traj.release();
+
   }
   if (grad_out) {
     std::vector<R> hg(pk.out_len_a);
     PASIF_HIP(hipMemcpy(hg.data(), d->out_grad.p, hg.size() * sizeof(R), hipMemcpyDeviceToHost));
     grad_out->assign(hg.begin(), hg.end());
-    d->ckpt.release();
-    d->segbuf.release();
+
   }
 }
 
@@ -1226,7 +1288,7 @@ int pasif_set_tuning(pasif_driver* d, uint32_t tile_lanes, uint32_t slot_config,
   return guarded(d, [&] {
     PASIF_REQUIRE(tile_lanes == 0 || (tile_lanes % 64 == 0 && tile_lanes <= kMaxLanes),
                   PASIF_ERR_ARGUMENT, "tile lanes must be a multiple of 64 up to %u", kMaxLanes);
-    PASIF_REQUIRE(slot_config <= 2, PASIF_ERR_ARGUMENT, "slot configuration 0, 1 or 2");
+    PASIF_REQUIRE(slot_config <= 3, PASIF_ERR_ARGUMENT, "slot configuration 0 .. 3");
     d->tile_lanes = tile_lanes;
     d->slot_cfg = slot_config;
     d->ck_every = checkpoint_every;

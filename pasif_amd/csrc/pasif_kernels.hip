@@ -37,6 +37,16 @@ template <typename R> __device__ __forceinline__ R absR(R a);
 template <> __device__ __forceinline__ double absR<double>(double a) { return __builtin_fabs(a); }
 template <> __device__ __forceinline__ float absR<float>(float a) { return __builtin_fabsf(a); }
 
+// Workgroup barrier ordering LDS only.  Global memory stays unordered, so
+// prefetched excitation / trajectory loads in flight are not waited for; the
+// full __syncthreads() is kept where global data written by one lane is read
+// by another (checkpoint and segment buffers across phase changes).
+__device__ __forceinline__ void sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t uni(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
@@ -76,105 +86,128 @@ struct LaneProg {
     asm volatile("" : "+v"(ln));
     const R* V = p.val + tile * CT * L + ln;
     const uint32_t* I = p.idx + tile * CW * L + ln;
+    // A kind that the tile uses is loaded whole: its unused slots hold value 0
+    // and point at the zero row, so evaluating them adds exact zeros.
+    if (n1) {
 #pragma unroll
-    for (int i = 0; i < C1; ++i)
-      if (i < (int)n1) {
+      for (int i = 0; i < C1; ++i) {
         v1[i] = V[i * L];
         w1[i] = I[i * L];
       }
+    }
+    if (n2) {
 #pragma unroll
-    for (int i = 0; i < C2; ++i)
-      if (i < (int)n2) {
+      for (int i = 0; i < C2; ++i) {
         v2[i] = V[(C1 + i) * L];
         w2[i] = I[(C1 + i) * L];
       }
+    }
+    if (n3) {
 #pragma unroll
-    for (int i = 0; i < C3; ++i)
-      if (i < (int)n3) {
+      for (int i = 0; i < C3; ++i) {
         v3[i] = V[(C1 + C2 + i) * L];
         w3a[i] = I[(C1 + C2 + 2 * i) * L];
         w3b[i] = I[(C1 + C2 + 2 * i + 1) * L];
       }
+    }
+    if (n4) {
 #pragma unroll
-    for (int i = 0; i < C4; ++i)
-      if (i < (int)n4) {
+      for (int i = 0; i < C4; ++i) {
         v4[i] = V[(C1 + C2 + C3 + i) * L];
         w4a[i] = I[(C1 + C2 + 2 * C3 + 2 * i) * L];
         w4b[i] = I[(C1 + C2 + 2 * C3 + 2 * i + 1) * L];
       }
+    }
   }
 
   // Partial row sum of this segment.  S factors come from the stage buffer
-  // SB of `S`, set-point factors from element XB of `X`.
+  // SB of `S`, set-point factors from element XB of `X`.  Slots of a kind are
+  // evaluated without per-slot branches so their LDS reads issue together.
   template <int SB, int XB>
   __device__ __forceinline__ R eval(const unsigned char* S, const unsigned char* X) const {
-    R a = R(0);
+    // two partial sums per kind (even / odd slots) shorten the dependent
+    // FMA chains; they are combined as ((k1 + k2) + (k3 + k4))
+    R a0 = R(0), a1 = R(0), b0 = R(0), b1 = R(0), c0 = R(0), c1 = R(0), d0 = R(0), d1 = R(0);
+    if (n1) {
 #pragma unroll
-    for (int i = 0; i < C1; ++i)
-      if (i < (int)n1) a = fmaR(v1[i], lds<R, SB>(S, w1[i] & 0xFFFFu), a);
-#pragma unroll
-    for (int i = 0; i < C2; ++i)
-      if (i < (int)n2) {
-        const R t = v2[i] * lds<R, SB>(S, w2[i] & 0xFFFFu);
-        a = fmaR(t, lds<R, XB>(X, w2[i] >> 16), a);
+      for (int i = 0; i < C1; ++i) {
+        const R y = lds<R, SB>(S, w1[i] & 0xFFFFu);
+        if (i & 1) a1 = fmaR(v1[i], y, a1);
+        else a0 = fmaR(v1[i], y, a0);
       }
+    }
+    if (n2) {
 #pragma unroll
-    for (int i = 0; i < C3; ++i)
-      if (i < (int)n3) {
+      for (int i = 0; i < C2; ++i) {
+        const R t = v2[i] * lds<R, SB>(S, w2[i] & 0xFFFFu);
+        const R x = lds<R, XB>(X, w2[i] >> 16);
+        if (i & 1) b1 = fmaR(t, x, b1);
+        else b0 = fmaR(t, x, b0);
+      }
+    }
+    if (n3) {
+#pragma unroll
+      for (int i = 0; i < C3; ++i) {
         R t = v3[i] * lds<R, SB>(S, w3a[i] & 0xFFFFu);
         t = t * lds<R, XB>(X, w3a[i] >> 16);
-        a = fmaR(t, lds<R, XB>(X, w3b[i] & 0xFFFFu), a);
+        const R x = lds<R, XB>(X, w3b[i] & 0xFFFFu);
+        if (i & 1) c1 = fmaR(t, x, c1);
+        else c0 = fmaR(t, x, c0);
       }
+    }
+    if (n4) {
 #pragma unroll
-    for (int i = 0; i < C4; ++i)
-      if (i < (int)n4) {
+      for (int i = 0; i < C4; ++i) {
         R t = v4[i] * lds<R, SB>(S, w4a[i] & 0xFFFFu);
         t = t * lds<R, XB>(X, w4a[i] >> 16);
         t = t * lds<R, XB>(X, w4b[i] & 0xFFFFu);
-        a = fmaR(t, lds<R, XB>(X, w4b[i] >> 16), a);
+        const R x = lds<R, XB>(X, w4b[i] >> 16);
+        if (i & 1) d1 = fmaR(t, x, d1);
+        else d0 = fmaR(t, x, d0);
       }
-    return a;
+    }
+    return ((a0 + a1) + (b0 + b1)) + ((c0 + c1) + (d0 + d1));
   }
 };
 
-// Excitation samples of one forced row for one step, fetched a step ahead.
+// Forcing samples.  A forced row stages the samples of its excitation that
+// the next kForceChunk steps read into its own LDS slot, so the step loop
+// issues no global loads.  Zero-order hold (src/kernels.cu:83-86) reads
+// e[n] (stages 1-3) and e[n+1] (stage 4), each only when 0 < s < L;
+// interpolation (src/kernels.cu:114-124) on the half-step grid
+// hs = 2n .. 2n+2 reads e[i .. i+1] with i = hs / W (any window W >= 2).
+// Samples at index L and beyond read as 0.  A chunk starting at sample b
+// holds e[b .. b + kForceStage).
 template <typename R>
-struct Samples {
-  R e0, e1, e2;
-};
-
-template <typename R>
-__device__ __forceinline__ Samples<R> fetch_samples(const TileArgs<R>& a, int32_t exc, uint32_t n) {
-  const uint32_t Lx = a.exc_len;
-  const R* E = a.excit + static_cast<size_t>(exc) * Lx;
-  Samples<R> f;
-  if (a.interp_w == 0u) {
-    // zero-order hold (src/kernels.cu:83-86): e[n] for stages 1-3, e[n+1]
-    // for stage 4, each only when 0 < s < L
-    f.e0 = (n > 0u && n < Lx) ? E[n] : R(0);
-    f.e1 = (n + 1u < Lx) ? E[n + 1u] : R(0);
-    f.e2 = R(0);
-  } else {
-    // samples i0, i0+1, i0+2 cover hs = 2n .. 2n+2 for any window >= 2;
-    // e[L] and beyond read as 0
-    const uint32_t i0 = (2u * n) / a.interp_w;
-    f.e0 = i0 < Lx ? E[i0] : R(0);
-    f.e1 = i0 + 1u < Lx ? E[i0 + 1u] : R(0);
-    f.e2 = i0 + 2u < Lx ? E[i0 + 2u] : R(0);
+__device__ __forceinline__ void stage_samples(R* slot, const R* E, uint32_t Lx, uint32_t b) {
+  // loads from clamped indices are unconditional, so each batch of 16 is in
+  // flight together; out-of-range samples are zeroed afterwards
+  constexpr uint32_t B = 16;
+#pragma unroll
+  for (uint32_t t0 = 0; t0 < kForceStage; t0 += B) {
+    R v[B];
+#pragma unroll
+    for (uint32_t u = 0; u < B; ++u) {
+      const uint32_t i = b + t0 + u;
+      v[u] = E[i < Lx ? i : Lx - 1u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < B; ++u)
+      if (t0 + u < kForceStage) slot[t0 + u] = (b + t0 + u < Lx) ? v[u] : R(0);
   }
-  return f;
 }
 
-// forcing of half-step hs from samples starting at i0
-// (src/kernels.cu:114-124: Mi[1][j] e[i] + Mi[0][j] e[i+1], 0 < i < L)
+// Mi[1][j] e[i] + Mi[0][j] e[i+1] for the half step hs = i0*W + j0 + t
+// (t = 0, 1, 2); zero unless 0 < i < L.  `slot` holds samples from index b.
 template <typename R>
-__device__ __forceinline__ R interp_at(const Samples<R>& f, const R* wts, uint32_t W, uint32_t Lx,
-                                       uint32_t i0, uint32_t hs) {
-  const uint32_t i = hs / W;
-  const uint32_t j = hs - i * W;
-  const bool same = i == i0;
-  const R ei = same ? f.e0 : f.e1;
-  const R ei1 = same ? f.e1 : f.e2;
+__device__ __forceinline__ R interp_at(const R* slot, uint32_t b, const R* wts, uint32_t W,
+                                       uint32_t Lx, uint32_t i0, uint32_t j0, uint32_t t) {
+  uint32_t j = j0 + t;
+  const bool next = j >= W;
+  j = next ? j - W : j;
+  const uint32_t i = i0 + (next ? 1u : 0u);
+  const R ei = slot[i - b];
+  const R ei1 = slot[i + 1u - b];
   return (i > 0u && i < Lx) ? fmaR(wts[W + j], ei, wts[j] * ei1) : R(0);
 }
 
@@ -204,7 +237,7 @@ __device__ __forceinline__ void rk4_step(const Prog& P, unsigned char* yb, const
     }
     if (split) {
       part[lane] = p;
-      __syncthreads();
+      sync_lds();
       if (owner)
         for (uint32_t j = 1; j < nseg; ++j) p += part[lane + j];
     }
@@ -229,15 +262,17 @@ __device__ __forceinline__ void rk4_step(const Prog& P, unsigned char* yb, const
       }
       *reinterpret_cast<R*>(yb + yoff + ((s + 1) & 1) * sizeof(R)) = y;
     }
-    __syncthreads();
+    sync_lds();
   }
 }
 
-// Dynamic LDS layout (bytes): yb [L+1][2] R (stage double buffer, row L = 0),
-// part [L] R, xb [L+1][4] R (adjoint set points, row L = 0), wts [2][W] R.
+// Dynamic LDS layout (elements of R): yb [L+1][2] (stage double buffer,
+// row L = 0), part [L], xb [L+1][4] (adjoint set points, row L = 0),
+// wts [2][W] (interpolation rows 0 and 1), eb [max_forced][kForceStage]
+// (staged excitation samples).
 template <typename R>
-__host__ __device__ inline size_t lds_bytes(uint32_t L, bool adjoint, uint32_t W) {
-  return (2 * (L + 1) + L + (adjoint ? 4 * (L + 1) : 0) + 2 * W) * sizeof(R);
+__host__ __device__ inline size_t lds_bytes(uint32_t L, bool adjoint, uint32_t W, uint32_t nf) {
+  return (2 * (L + 1) + L + (adjoint ? 4 * (L + 1) : 0) + 2 * W + nf * kForceStage) * sizeof(R);
 }
 
 template <typename R, int C1, int C2, int C3, int C4, int MAXL, bool ADJ>
@@ -251,6 +286,7 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
   unsigned char* xbb = smem_raw + (3 * L + 2) * sizeof(R);
   R* xb = reinterpret_cast<R*>(xbb);
   R* wts = reinterpret_cast<R*>(xbb + (adjoint ? 4 * (L + 1) * sizeof(R) : 0));
+  R* eb = wts + 2 * a.interp_w;
   const uint32_t lane = threadIdx.x;
   const size_t tile = blockIdx.x;
   const TileHeader* H = a.hdr + tile;
@@ -283,12 +319,14 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
   const uint32_t fyoff = frow * 2 * sizeof(R);
   uint32_t fsteps = 0, fout = 0;
   int32_t fexc = -1;
+  uint32_t fslot = 0;
   R fforce = R(0);
   R q = R(0), mx = R(0);
   if (fown) {
     fsteps = a.fwd.steps[rb + lane];
     fout = a.fwd.out[rb + lane];
     fexc = a.fwd.exc[rb + lane];
+    fslot = a.fwd.fslot[rb + lane];
     fforce = a.fwd.force[rb + lane];
     if (n == 0u && phase == kPhaseForward) {
       q = a.fwd.init[a.fwd.src[rb + lane]];
@@ -333,26 +371,39 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
         a.segbuf[segb * L + frow] = q;
       }
       if (fown) *reinterpret_cast<R*>(yb + fyoff) = q;
-      __syncthreads();
+      sync_lds();
       const uint32_t e = recompute ? min(T, s0 + K) : T;
       const uint32_t nend = (e - n < budget) ? e : n + budget;
       budget -= nend - n;
-      Samples<R> fcur{R(0), R(0), R(0)}, fnext{R(0), R(0), R(0)};
-      if (forced) fcur = fetch_samples(a, fexc, n);
+      // half-step position hs = 2n = hi*W + hj, advanced incrementally
+      uint32_t hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
+      const R* E = a.excit + static_cast<size_t>(forced ? fexc : 0) * a.exc_len;
+      R* eslot = eb + fslot * kForceStage;
+      uint32_t ebase = 0, left = 0;  // first staged sample, steps left in the chunk
       for (; n < nend; ++n) {
         R u0 = R(0), u1 = R(0), u3 = R(0);
+        if (left == 0u) {
+          ebase = W ? hi : n;
+          if (forced) stage_samples(eslot, E, a.exc_len, ebase);
+          left = kForceChunk;
+        }
+        --left;
         if (forced) {
-          fnext = fetch_samples(a, fexc, n + 1u);
           if (W == 0u) {
-            u0 = fcur.e0;
-            u1 = fcur.e0;
-            u3 = fcur.e1;
+            const R e0 = eslot[n - ebase], e1 = eslot[n + 1u - ebase];
+            u0 = (n > 0u && n < a.exc_len) ? e0 : R(0);
+            u1 = u0;
+            u3 = e1;
           } else {
-            const uint32_t i0 = (2u * n) / W;
-            u0 = interp_at(fcur, wts, W, a.exc_len, i0, 2u * n);
-            u1 = interp_at(fcur, wts, W, a.exc_len, i0, 2u * n + 1u);
-            u3 = interp_at(fcur, wts, W, a.exc_len, i0, 2u * n + 2u);
+            u0 = interp_at(eslot, ebase, wts, W, a.exc_len, hi, hj, 0u);
+            u1 = interp_at(eslot, ebase, wts, W, a.exc_len, hi, hj, 1u);
+            u3 = interp_at(eslot, ebase, wts, W, a.exc_len, hi, hj, 2u);
           }
+        }
+        hj += 2u;
+        if (W && hj >= W) {
+          hj -= W;
+          hi += 1u;
         }
         if (adjoint && !recompute && fown && (n % K) == 0u)
           a.ckpt[(tile * a.nck + n / K) * L + frow] = q;
@@ -374,7 +425,6 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
             }
           }
         }
-        fcur = fnext;
       }
       if (n == e) {
         if (recompute) {
@@ -400,7 +450,7 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
         xcur = a.segbuf[(segb + (n - s0)) * L + lane];
         if (n > s0) xprev = a.segbuf[(segb + (n - 1u - s0)) * L + lane];
       }
-      __syncthreads();
+      sync_lds();
       const uint32_t nend = (n - s0 < budget) ? s0 : n - budget;
       budget -= n - nend;
       for (; n > nend; --n) {
@@ -413,7 +463,7 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
           xcur = xprev;
           xprev = xnext;
         }
-        __syncthreads();
+        sync_lds();
         rk4_step<true>(P, yb, xbb, part, lane, asplit, aown, anseg, ayoff, lam, false, R(0), R(0),
                        R(0), R(0), n <= asteps, a.h, a.h2, a.h6);
       }
@@ -445,7 +495,7 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
 
 template <typename R, int C1, int C2, int C3, int C4, int MAXL>
 static hipError_t launch_cfg(const TileArgs<R>& a, uint32_t ntiles, hipStream_t s) {
-  const size_t lds = lds_bytes<R>(a.lanes, a.adjoint != 0u, a.interp_w);
+  const size_t lds = lds_bytes<R>(a.lanes, a.adjoint != 0u, a.interp_w, a.max_forced);
   auto kern = a.adjoint ? tile_kernel<R, C1, C2, C3, C4, MAXL, true>
                         : tile_kernel<R, C1, C2, C3, C4, MAXL, false>;
   if (lds > 65536) {
@@ -461,16 +511,19 @@ template <typename R>
 static hipError_t launch_tiles_impl(const TileArgs<R>& a, uint32_t ntiles, int slot_cfg,
                                     hipStream_t s) {
   if (a.lanes == 0 || a.lanes > kMaxLanes || (a.lanes % 64) != 0) return hipErrorInvalidValue;
-  if (lds_bytes<R>(a.lanes, a.adjoint != 0u, a.interp_w) > kMaxLdsBytes)
+  if (lds_bytes<R>(a.lanes, a.adjoint != 0u, a.interp_w, a.max_forced) > kMaxLdsBytes)
     return hipErrorInvalidValue;
   if (a.lanes <= 256) {
     if (slot_cfg == 2)
       return launch_cfg<R, kSlotWide.c1, kSlotWide.c2, kSlotWide.c3, kSlotWide.c4, 256>(a, ntiles,
                                                                                       s);
+    if (slot_cfg == 3)
+      return launch_cfg<R, kSlotSmall.c1, kSlotSmall.c2, kSlotSmall.c3, kSlotSmall.c4, 256>(
+          a, ntiles, s);
     return launch_cfg<R, kSlotNarrow.c1, kSlotNarrow.c2, kSlotNarrow.c3, kSlotNarrow.c4, 256>(
         a, ntiles, s);
   }
-  if (slot_cfg == 2) return hipErrorInvalidValue;
+  if (slot_cfg != 1) return hipErrorInvalidValue;
   return launch_cfg<R, kSlotNarrow.c1, kSlotNarrow.c2, kSlotNarrow.c3, kSlotNarrow.c4, 1024>(
       a, ntiles, s);
 }

@@ -28,6 +28,9 @@ constexpr uint32_t kRowIdle = 0xFFFFu;   // lane without a segment
 constexpr uint32_t kMaxLanes = 1024;
 constexpr size_t kMaxLdsBytes = 160 * 1024;   // LDS per CU on gfx950
 constexpr uint32_t kMaxInterpWindow = 2048;
+// excitation samples staged in LDS per forced row and per chunk of steps
+constexpr uint32_t kForceChunk = 64;
+constexpr uint32_t kForceStage = kForceChunk + 3;
 
 // per-lane word: [0,16) tile-local row, [16,26) segment count (owner lanes
 // only), bit 31 owner flag
@@ -40,8 +43,10 @@ __host__ __device__ inline uint32_t make_lane(uint32_t row, uint32_t nseg, bool 
 
 // slot-width configurations (slots per lane for kinds 1..4)
 struct SlotConfig { int c1, c2, c3, c4; };
+// slot configuration ids: 1 narrow, 2 wide, 3 small
 constexpr SlotConfig kSlotNarrow{4, 4, 2, 2};
 constexpr SlotConfig kSlotWide{8, 8, 4, 2};
+constexpr SlotConfig kSlotSmall{4, 2, 2, 1};
 
 __host__ __device__ constexpr int slot_values(SlotConfig c) { return c.c1 + c.c2 + c.c3 + c.c4; }
 // index words hold LDS byte offsets: kind 1 -> 1 (S), kind 2 -> 1
@@ -59,7 +64,7 @@ struct alignas(16) TileHeader {
   uint32_t split_f;    // some forward row spans several lanes
   uint32_t split_a;
   uint32_t steps;      // largest step count among the tile's instances
-  uint32_t pad;
+  uint32_t nforced;    // forced rows in the tile (LDS sample slots)
 };
 static_assert(sizeof(TileHeader) == 64, "tile header layout");
 
@@ -72,6 +77,7 @@ struct Program {
   const uint32_t* src;    // [tile][lanes] owner: initial-condition index
   const uint32_t* steps;  // [tile][lanes] owner: step count of the instance
   const int32_t* exc;     // [tile][lanes] owner: excitation id or -1
+  const uint32_t* fslot;  // [tile][lanes] forced owner: LDS sample slot
   const R* force;         // [tile][lanes] owner: force pattern value
   const R* val;           // [tile][slot_values][lanes]
   const uint32_t* idx;    // [tile][slot_words][lanes]
@@ -92,6 +98,7 @@ struct TileArgs {
   uint32_t exc_len;
   const R* interp;         // [2][interp_w] rows 0 and 1, or null (zero-order hold)
   uint32_t interp_w;
+  uint32_t max_forced;     // largest nforced over the tiles (LDS sizing)
   R h, h2, h6;
   // resumable state, [tile][lanes] by row
   R* qstate;
@@ -119,7 +126,7 @@ struct TileArgs {
 }  // namespace pasif
 
 namespace pasif {
-// kernel launchers (pasif_kernels.hip); slot_cfg 1 = narrow, 2 = wide
+// kernel launchers (pasif_kernels.hip); slot_cfg 1 = narrow, 2 = wide, 3 = small
 hipError_t launch_tiles(const TileArgs<double>& a, uint32_t ntiles, int slot_cfg, hipStream_t s);
 hipError_t launch_tiles(const TileArgs<float>& a, uint32_t ntiles, int slot_cfg, hipStream_t s);
 }  // namespace pasif
