@@ -49,12 +49,12 @@ def flops_per_step(p) -> float:
 def bytes_per_launch(p, w) -> float:
     """Algorithmic HBM bytes of one persistent launch: coefficients and
     indices (8+4 B per term), initial state, final state, and the excitation
-    samples read (two per forced system per step)."""
+    samples read (each sample of a forced system once: steps + 1)."""
     A, G, L, f, q0 = p.getMatrices()
     nnz = A.tocsr().nnz + len(np.asarray(G.val).reshape(-1)) + len(np.asarray(L.val).reshape(-1))
     n = len(q0)
     forced = int(np.count_nonzero(np.abs(f) > np.finfo(np.float64).eps))
-    return 12.0 * nnz + 8.0 * n + 8.0 * n + 8.0 * 2 * forced * w.numsteps
+    return 12.0 * nnz + 8.0 * n + 8.0 * n + 8.0 * forced * (w.numsteps + 1)
 
 
 def read_traffic(name: str):
