@@ -39,7 +39,7 @@ enum {
   PASIF_ERR_STATE = 5       /* call made before its prerequisites           */
 };
 
-enum { PASIF_SOLVER_PERSISTENT = 0, PASIF_SOLVER_PLAIN = 1 };
+enum { PASIF_SOLVER_PERSISTENT = 0, PASIF_SOLVER_PLAIN = 1, PASIF_SOLVER_GRAPH = 2 };
 
 /* __GpuDriver(excitationSet_, sampleRate_, numsteps_, dCompute_, dSystem_,
  * dSolver_, GPUId_)  (src/__GpuDriver.cu:25-157).  excitations is
@@ -125,8 +125,14 @@ int pasif_set_systems_layout(pasif_driver* d, uint32_t n_systems, const uint32_t
 /* 64 (default) or 32 */
 int pasif_set_precision(pasif_driver* d, int bits);
 /* PASIF_SOLVER_PERSISTENT (default): all steps in one launch.
- * PASIF_SOLVER_PLAIN: one launch per RK4 step, state through HBM. */
+ * PASIF_SOLVER_PLAIN: one launch per RK4 step, state through HBM.
+ * PASIF_SOLVER_GRAPH: launches of at most `steps_per_launch` steps each,
+ * captured `launches_per_graph` at a time into a hipGraph that is replayed
+ * until every tile is done (bounded kernel durations). */
 int pasif_set_solver(pasif_driver* d, int solver);
+/* slicing of PASIF_SOLVER_GRAPH (defaults 256 steps, 16 launches) */
+int pasif_set_graph_slicing(pasif_driver* d, uint32_t steps_per_launch,
+                            uint32_t launches_per_graph);
 /* tile lanes (64..1024, multiple of 64; 0 = automatic), slot-width
  * configuration (0 = automatic, 1 = narrow {4,4,2,2}, 2 = wide {8,8,4,2},
  * 3 = small {4,2,2,1} slots per lane for terms with 1..4 state factors) and

@@ -357,9 +357,12 @@ class PASIf(_GpuDriver):
     def setPrecision(self, bits: int):
         self._setPrecision(bits)
 
-    def setSolver(self, solver: str):
-        """"persistent" (all steps in one launch) or "plain" (one launch per
-        step)."""
+    def setSolver(self, solver: str, steps_per_launch: int = 256, launches_per_graph: int = 16):
+        """"persistent" (all steps in one launch), "plain" (one launch per
+        step) or "graph" (launches of ``steps_per_launch`` steps replayed
+        from a hipGraph of ``launches_per_graph`` launches)."""
+        if solver == "graph":
+            self._setGraphSlicing(steps_per_launch, launches_per_graph)
         self._setSolver(solver)
 
     def setTuning(self, tile_lanes: int = 0, slot_config: int = 0, checkpoint_every: int = 0):

@@ -178,14 +178,14 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
                           sizes_adj: Optional[Sequence[int]] = None,
                           exc_of_system: Optional[Sequence[int]] = None,
                           steps_of_system: Optional[Sequence[int]] = None):
-        """Per-system sizes (block boundaries), excitation id and step count.
-        ``exc_of_system`` switches from the Cartesian product (every system x
-        every excitation) to one excitation per system."""
+        """Per-system sizes (block boundaries), excitation id and step count;
+        every call sets all four (``None`` clears one).  ``exc_of_system``
+        switches from the Cartesian product (every system x every
+        excitation) to one excitation per system."""
         lay = self._layout
         for k, v in (("sizes_f", sizes_fwd), ("sizes_a", sizes_adj), ("exc", exc_of_system),
                      ("steps", steps_of_system)):
-            if v is not None:
-                lay[k] = np.ascontiguousarray(np.asarray(v).reshape(-1))
+            lay[k] = None if v is None else np.ascontiguousarray(np.asarray(v).reshape(-1))
         ns = [len(v) for v in (lay["sizes_f"], lay["sizes_a"], lay["exc"], lay["steps"])
               if v is not None]
         if ns and any(x != ns[0] for x in ns):
@@ -210,12 +210,17 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
         self._chk(self._pl.pasif_set_precision(self._h, int(bits)))
 
     def _setSolver(self, solver: str):
-        code = {"persistent": _lib.PASIF_SOLVER_PERSISTENT, "plain": _lib.PASIF_SOLVER_PLAIN}[solver]
+        code = {"persistent": _lib.PASIF_SOLVER_PERSISTENT, "plain": _lib.PASIF_SOLVER_PLAIN,
+                "graph": _lib.PASIF_SOLVER_GRAPH}[solver]
         self._chk(self._pl.pasif_set_solver(self._h, code))
 
     def _setTuning(self, tile_lanes: int = 0, slot_config: int = 0, checkpoint_every: int = 0):
         self._chk(self._pl.pasif_set_tuning(self._h, int(tile_lanes), int(slot_config),
                                             int(checkpoint_every)))
+
+    def _setGraphSlicing(self, steps_per_launch: int = 256, launches_per_graph: int = 16):
+        self._chk(self._pl.pasif_set_graph_slicing(self._h, int(steps_per_launch),
+                                                   int(launches_per_graph)))
 
     def _setAmplitudeWindow(self, window: int):
         self._chk(self._pl.pasif_set_amplitude_window(self._h, int(window)))

@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("PASIF_LIB", os.path.join(HERE, "libpasif_hip.so"))
 PASIF_OK = 0
 PASIF_SOLVER_PERSISTENT = 0
 PASIF_SOLVER_PLAIN = 1
+PASIF_SOLVER_GRAPH = 2
 ERROR_NAMES = {1: "argument", 2: "device", 3: "unsupported", 4: "memory", 5: "state"}
 
 
@@ -61,6 +62,7 @@ _SIGS = {
     "pasif_set_solver": (C.c_int, [_h, C.c_int]),
     "pasif_set_tuning": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.c_uint32]),
     "pasif_set_amplitude_window": (C.c_int, [_h, C.c_uint32]),
+    "pasif_set_graph_slicing": (C.c_int, [_h, C.c_uint32, C.c_uint32]),
     "pasif_get_max_amplitudes": (C.c_int, [_h, _f64p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pasif_last_timing": (C.c_int, [_h, _f64p, _u32p, _u32p, _u32p]),
     "pasif_version": (C.c_char_p, []),

@@ -1,0 +1,105 @@
+"""Autotuner of the tile kernel: tile lanes (systems per workgroup), slot
+configuration (segment width per term kind) and, for the adjoint, the
+checkpoint interval.
+
+Each candidate runs the loaded workload with every system's step count
+capped at ``probe_steps`` and is timed with the library's HIP events; the
+fastest wins.  Results depend only on the kernel choice (numerics differ at
+round-off level between slot configurations: a row's terms are summed in
+different segment groupings), so the tuner records the configuration, not
+results.  Choices are cached per workload signature in a JSON file.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+from typing import Iterable, Optional
+
+import numpy as np
+
+CACHE = os.environ.get("PASIF_AUTOTUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "pasif_amd",
+                                    "autotune.json"))
+
+LANES = (64, 128, 256)
+SLOTS = (3, 1, 2)          # small, narrow, wide
+CHECKPOINTS = (64, 128, 256, 512)
+
+
+def signature(p, adjoint: bool) -> str:
+    """Structure of the loaded problem: sizes, nnz per kind, steps, mode."""
+    A, G, L, f, q0 = p.getMatrices()
+    parts = [str(len(q0)), str(A.tocsr().nnz), str(len(np.asarray(G.val).reshape(-1))),
+             str(len(np.asarray(L.val).reshape(-1))), str(p.numberOfSystems),
+             str(int(p._max_steps())), str(adjoint),
+             ",".join(str(int(x)) for x in np.bincount(np.asarray(p.system_sizes)))]
+    return hashlib.sha1("|".join(parts).encode()).hexdigest()[:16]
+
+
+def _load_cache() -> dict:
+    try:
+        with open(CACHE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(c: dict):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE + ".tmp", "w") as fh:
+            json.dump(c, fh, indent=1)
+        os.replace(CACHE + ".tmp", CACHE)
+    except OSError:
+        pass
+
+
+def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
+             lanes: Iterable[int] = LANES, slots: Iterable[int] = SLOTS,
+             checkpoints: Optional[Iterable[int]] = None, use_cache: bool = True,
+             apply: bool = True, verbose: bool = False) -> dict:
+    """Pick the fastest (tile lanes, slot configuration[, checkpoint]) for the
+    problem loaded in PASIf instance ``p``; apply it unless ``apply=False``."""
+    key = signature(p, adjoint)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        best = cache[key]
+    else:
+        steps_full = p._sys_steps
+        exc = p._sys_exc
+        n = p.numberOfSystems
+        cap = np.minimum(steps_full if steps_full is not None else np.full(n, p.numsteps),
+                         probe_steps)
+        p.setSystemExcitations(exc, cap)
+        cks = list(checkpoints) if checkpoints is not None else (list(CHECKPOINTS) if adjoint
+                                                                 else [0])
+        trials = []
+        try:
+            for ln in lanes:
+                for sl in slots:
+                    for ck in cks:
+                        try:
+                            p.setTuning(ln, sl, ck)
+                            (p.getGradient if adjoint else p.getAmplitudes)()
+                            ms = p._lastTiming()["kernel_ms"]
+                        except Exception as e:  # configuration cannot hold the systems
+                            if verbose:
+                                print(f"[autotune] lanes {ln} slots {sl} ck {ck}: {e}")
+                            continue
+                        trials.append((ms, ln, sl, ck))
+                        if verbose:
+                            print(f"[autotune] lanes {ln} slots {sl} ck {ck}: {ms:.3f} ms")
+        finally:
+            p.setSystemExcitations(exc, steps_full)
+        if not trials:
+            raise RuntimeError("no tile configuration fits this problem")
+        ms, ln, sl, ck = min(trials)
+        best = {"tile_lanes": ln, "slot_config": sl, "checkpoint_every": ck, "probe_ms": ms,
+                "probe_steps": probe_steps, "trials": len(trials)}
+        if use_cache:
+            cache[key] = best
+            _save_cache(cache)
+    if apply:
+        p.setTuning(best["tile_lanes"], best["slot_config"], best["checkpoint_every"])
+    return best

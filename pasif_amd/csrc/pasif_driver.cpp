@@ -152,6 +152,7 @@ struct pasif_driver {
   int precision = 64;
   int solver = PASIF_SOLVER_PERSISTENT;
   uint32_t tile_lanes = 0, slot_cfg = 0, ck_every = 0, window = 0;
+  uint32_t graph_slice = 256, graph_launches = 16;
 
   Packed pk;
   // device copies of excitations / interpolation for the packed precision
@@ -907,18 +908,68 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     work = 3ull * pk.tmax;
   }
   const bool plain = d->solver == PASIF_SOLVER_PLAIN;
-  a.budget = plain ? 1u : 0xFFFFFFFFu;
-  uint64_t launches = plain ? std::max<uint64_t>(work, 1) : 1;
-  PASIF_HIP(hipEventRecord(d->ev0, d->stream));
-  for (uint64_t i = 0; i < launches; ++i) {
+  const bool graph = d->solver == PASIF_SOLVER_GRAPH;
+  uint64_t launches = 1;
+  if (plain) {
+    a.budget = 1u;
+    launches = std::max<uint64_t>(work, 1);
+  } else if (graph) {
+    a.budget = std::max<uint32_t>(1, d->graph_slice);
+    launches = std::max<uint64_t>(1, (work + a.budget - 1) / a.budget);
+  } else {
+    a.budget = 0xFFFFFFFFu;
+  }
+  auto launch = [&]() {
     hipError_t e = launch_tiles(a, nt, int(pk.slot_cfg), d->stream);
     if (e != hipSuccess) {
       set_err(d, "tile kernel launch failed (%u tiles of %u lanes): %s", nt, L, hipGetErrorName(e));
       throw Fail{PASIF_ERR_DEVICE};
     }
+  };
+  hipGraphExec_t gexec = nullptr;
+  uint32_t per_graph = 1;
+  if (graph) {
+    // one graph of `per_graph` sliced launches, replayed; a launch on a tile
+    // that has finished returns at once, so extra replays are harmless.  No
+    // memset is captured: progress is cleared above, outside the graph.
+    per_graph = uint32_t(std::min<uint64_t>(launches, d->graph_launches));
+    hipGraph_t g = nullptr;
+    PASIF_HIP(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t i = 0; i < per_graph; ++i) {
+      hipError_t e = launch_tiles(a, nt, int(pk.slot_cfg), d->stream);
+      if (e != hipSuccess) {
+        (void)hipStreamEndCapture(d->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        set_err(d, "tile kernel capture failed: %s", hipGetErrorName(e));
+        throw Fail{PASIF_ERR_DEVICE};
+      }
+    }
+    PASIF_HIP(hipStreamEndCapture(d->stream, &g));
+    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      set_err(d, "graph instantiation failed: %s", hipGetErrorName(e));
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+  }
+  PASIF_HIP(hipEventRecord(d->ev0, d->stream));
+  if (graph) {
+    const uint64_t replays = (launches + per_graph - 1) / per_graph;
+    for (uint64_t i = 0; i < replays; ++i) {
+      hipError_t e = hipGraphLaunch(gexec, d->stream);
+      if (e != hipSuccess) {
+        (void)hipGraphExecDestroy(gexec);
+        set_err(d, "graph launch failed: %s", hipGetErrorName(e));
+        throw Fail{PASIF_ERR_DEVICE};
+      }
+    }
+    launches = replays * per_graph;
+  } else {
+    for (uint64_t i = 0; i < launches; ++i) launch();
   }
   PASIF_HIP(hipEventRecord(d->ev1, d->stream));
   PASIF_HIP(hipEventSynchronize(d->ev1));
+  if (gexec) (void)hipGraphExecDestroy(gexec);
   float ms = 0.f;
   PASIF_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
   d->last_ms = ms;
@@ -1277,7 +1328,8 @@ int pasif_set_precision(pasif_driver* d, int bits) {
 
 int pasif_set_solver(pasif_driver* d, int solver) {
   return guarded(d, [&] {
-    PASIF_REQUIRE(solver == PASIF_SOLVER_PERSISTENT || solver == PASIF_SOLVER_PLAIN,
+    PASIF_REQUIRE(solver == PASIF_SOLVER_PERSISTENT || solver == PASIF_SOLVER_PLAIN ||
+                      solver == PASIF_SOLVER_GRAPH,
                   PASIF_ERR_ARGUMENT, "unknown solver %d", solver);
     d->solver = solver;
   });
@@ -1292,6 +1344,17 @@ int pasif_set_tuning(pasif_driver* d, uint32_t tile_lanes, uint32_t slot_config,
     d->tile_lanes = tile_lanes;
     d->slot_cfg = slot_config;
     d->ck_every = checkpoint_every;
+  });
+}
+
+int pasif_set_graph_slicing(pasif_driver* d, uint32_t steps_per_launch,
+                            uint32_t launches_per_graph) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(steps_per_launch > 0 && launches_per_graph > 0 && launches_per_graph <= 4096,
+                  PASIF_ERR_ARGUMENT, "graph slicing: steps per launch and launches per graph "
+                                      "must be positive (at most 4096 launches)");
+    d->graph_slice = steps_per_launch;
+    d->graph_launches = launches_per_graph;
   });
 }
 
