@@ -147,6 +147,44 @@ int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* l
  * stream), number of launches and number of tiles, lanes per tile */
 int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launches,
                       uint32_t* tiles, uint32_t* lanes);
+/* ---------------- continuous-batching pool (extension) ------------------
+ * A pool keeps many requests ("jobs") resident on one GPU.  Each job is a
+ * driver handle created by pasif_pool_job_create and loaded with the usual
+ * setters (systems, layout, interpolation, adjoint); pasif_pool_submit packs
+ * it into tiles of the pool's shape and admits it if the pool's HBM budget
+ * allows (PASIF_ERR_MEMORY otherwise: retry after collecting finished jobs).
+ * pasif_pool_tick advances every admitted, unfinished job by steps_per_tick
+ * steps of work in ONE launch (wait = 0 returns after enqueueing, so the host
+ * can admit new requests while the GPU runs; pasif_pool_wait joins).  A job's
+ * results do not depend on which other jobs share the launches. */
+typedef struct pasif_pool pasif_pool;
+int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile_lanes,
+                      uint32_t slot_config, uint32_t checkpoint_every, uint32_t steps_per_tick,
+                      double hbm_budget_bytes);
+int pasif_pool_destroy(pasif_pool* pool);
+const char* pasif_pool_last_error(const pasif_pool* pool);
+/* Signal bank shared by jobs created with excitations == NULL: [n_exc][len]
+ * samples, the sample rate, and optionally interpolation rows 0 and 1
+ * ([2][window]).  Uploaded once. */
+int pasif_pool_set_bank(pasif_pool* pool, const double* excitations, uint32_t n_exc,
+                        uint32_t exc_len, uint32_t sample_rate, const double* interp,
+                        uint32_t window);
+/* a job handle (a driver on the pool's device); excitations == NULL uses the
+ * pool's signal bank (excitation ids index the bank) */
+int pasif_pool_job_create(pasif_pool* pool, pasif_driver** job, const double* excitations,
+                          uint32_t n_exc, uint32_t exc_len, uint32_t sample_rate,
+                          uint32_t numsteps);
+int pasif_pool_submit(pasif_pool* pool, pasif_driver* job, int adjoint);
+int pasif_pool_tick(pasif_pool* pool, int wait, uint32_t* active_tiles);
+int pasif_pool_wait(pasif_pool* pool, double* kernel_ms);
+int pasif_pool_job_finished(pasif_pool* pool, const pasif_driver* job, int* finished);
+/* final states [dof] and, for adjoint jobs, lambda_0 [adjoint dof]; frees the
+ * job's device memory and removes it from the pool (then pasif_destroy it) */
+int pasif_pool_collect(pasif_pool* pool, pasif_driver* job, double* q, size_t capq, size_t* lenq,
+                       double* grad, size_t capg, size_t* leng);
+int pasif_pool_stats(const pasif_pool* pool, uint32_t* jobs, double* used_bytes,
+                     double* last_kernel_ms, uint32_t* last_blocks);
+
 /* library build string */
 const char* pasif_version(void);
 

@@ -156,6 +156,40 @@ def _neg_inverse_diag(M: dia_matrix) -> dia_matrix:
     return dia_matrix((-(1.0 / d), [0]), shape=(n, n))
 
 
+def assemble_arrays(vecM, vecB, vecK, vecGamma, vecLambda, vecForce, vecQ0, vecPsi=None,
+                    adjoint=False):
+    """Driver inputs of a list of systems, exactly as ``PASIf.setSystems`` /
+    ``setJacobian`` hand them to the driver: block-diagonal unfold
+    (``PASIf.py:475-605``), ``-M^{-1}`` scaling (``607-633``; the adjoint
+    force pattern unscaled), sorted CSR of ``B + K`` (``250-255``).  Returns a
+    dict with ``csr`` (scipy), ``gamma`` / ``lam`` / ``psi`` (coo_tensor),
+    ``force``, ``q0`` and ``sizes``."""
+    sizes = np.array([m.shape[0] for m in vecM], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    n = int(sizes.sum())
+    diag = np.concatenate([np.asarray(m.diagonal(), dtype=np.float64) for m in vecM])
+    Dm = _neg_inverse_diag(dia_matrix((diag, [0]), shape=(n, n))).tocsc()
+    Bs = coo_matrix(Dm.dot(_cat_matrices(vecB, offs, n)))
+    Ks = coo_matrix(Dm.dot(_cat_matrices(vecK, offs, n)))
+    D = Dm.diagonal()
+    G = _cat_tensors(vecGamma)
+    L = _cat_tensors(vecLambda)
+    G.multiplyByDiagMatrix(D)
+    L.multiplyByDiagMatrix(D)
+    P = None
+    if vecPsi is not None:
+        P = _cat_tensors(vecPsi)
+        P.multiplyByDiagMatrix(D)
+    f = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in vecForce])
+    if not adjoint:
+        f = f * (-1.0 * D)
+    q0 = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in vecQ0])
+    csr = (Bs + Ks).tocsr()
+    csr.sort_indices()
+    return {"csr": csr, "gamma": G, "lam": L, "psi": P, "force": f, "q0": q0, "sizes": sizes,
+            "B": Bs, "K": Ks}
+
+
 class PASIf(_GpuDriver):
     """``PASIf(excitationSet, sampleRate, numsteps_=0, dCompute_=False,
     dSystem_=False, dSolver_=False)`` -- ``PASIf.py:148-194``."""

@@ -166,6 +166,47 @@ struct pasif_driver {
   bool have_max = false;
   double last_ms = 0.0;
   uint32_t last_launches = 0;
+  // a job of a pool: uses the pool's device and upload stream
+  pasif_pool* pool = nullptr;
+  uint32_t pool_ck = 0, pool_nck = 0;
+  // excitations / interpolation rows owned by the pool (shared signal bank)
+  const void* ext_exc = nullptr;
+  const void* ext_interp = nullptr;
+};
+
+// Continuous-batching pool: jobs (each a driver holding one request's
+// systems) are packed into tiles of a common shape; every tick advances all
+// unfinished jobs' tiles by `budget` steps in one launch.
+struct pasif_pool {
+  int dev = 0;
+  hipStream_t stream = nullptr;   // ticks
+  hipStream_t upload = nullptr;   // admission (packing, uploads)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, uploaded = nullptr;
+  std::string err;
+  int precision = 64;
+  uint32_t lanes = 256, slot_cfg = 1, ck_every = 256, budget = 1000;
+  double hbm_budget = 0.0;
+  double used = 0.0;
+  struct Job {
+    pasif_driver* d;
+    bool adjoint;
+    uint64_t work, done;
+    double bytes;
+    bool finished;
+  };
+  std::vector<Job> jobs;
+  // shared signal bank, uploaded once in the pool's precision
+  DevBuf bank, bank_interp;
+  uint32_t bank_n = 0, bank_len = 0, bank_w = 0, bank_rate = 0;
+  std::vector<unsigned char> table_host;
+  std::vector<uint32_t> map_host;
+  void* table_dev = nullptr;
+  size_t table_bytes = 0;
+  void* map_dev = nullptr;
+  size_t map_bytes = 0;
+  bool in_flight = false;
+  double last_ms = 0.0;
+  uint32_t last_blocks = 0;
 };
 
 namespace {
@@ -810,6 +851,7 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
 
 template <typename R>
 void ensure_excitations(pasif_driver* d) {
+  if (d->ext_exc) return;  // the pool's signal bank
   if (!d->d_exc_dirty && d->d_exc_prec == d->precision) return;
   std::vector<R> e(d->exc.begin(), d->exc.end());
   upload(d, d->d_exc, e);
@@ -836,10 +878,12 @@ void fill_program(Program<R>& p, const DevBuf& info, const DevBuf& out, const De
   p.init = init.as<R>();
 }
 
-// Run the packed tiles to completion.  save_steps > 0 records a trajectory.
+// Pack (if needed), allocate the run buffers and clear the tile progress:
+// the arguments of a run of the packed tiles.  *work is the number of steps
+// of work each tile needs (3T with the adjoint: forward, recompute, reverse).
 template <typename R>
-void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
-                std::vector<double>* grad_out) {
+TileArgs<R> prepare_run(pasif_driver* d, bool adjoint, uint32_t save_steps, uint64_t* work_out,
+                        uint32_t* nslots_out) {
   ensure_packed(d, adjoint);
   ensure_excitations<R>(d);
   Packed& pk = d->pk;
@@ -854,9 +898,11 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     fill_program(a.adj, pk.a_info, pk.a_out, pk.a_src, pk.a_steps, pk.a_exc, pk.a_fslot,
                  pk.a_force, pk.a_val, pk.a_idx, pk.a_init);
   a.max_forced = pk.max_forced;
-  a.excit = d->d_exc.as<R>();
+  a.excit = d->ext_exc ? static_cast<const R*>(d->ext_exc) : d->d_exc.as<R>();
   a.exc_len = d->exc_len;
-  a.interp = d->interp_w ? d->d_interp.as<R>() : nullptr;
+  a.interp = d->interp_w ? (d->ext_interp ? static_cast<const R*>(d->ext_interp)
+                                          : d->d_interp.as<R>())
+                         : nullptr;
   a.interp_w = d->interp_w;
   // h = 1/sampleRate in the working type, h2 and h6 from that h in double
   // (src/__GpuDriver.cu:632-636)
@@ -907,6 +953,51 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     a.adjoint = 1;
     work = 3ull * pk.tmax;
   }
+  *work_out = work;
+  *nslots_out = nslots;
+  return a;
+}
+
+// Copy the results of a finished run back to the host.
+template <typename R>
+void finish_run(pasif_driver* d, bool adjoint, uint32_t nslots, std::vector<double>* traj_out,
+                std::vector<double>* grad_out) {
+  Packed& pk = d->pk;
+  std::vector<R> hq(pk.out_len_f);
+  PASIF_HIP(hipMemcpy(hq.data(), d->out_q.p, hq.size() * sizeof(R), hipMemcpyDeviceToHost));
+  d->res_q.assign(hq.begin(), hq.end());
+  d->have_max = d->window != 0;
+  if (d->window) {
+    std::vector<R> hm(pk.out_len_f);
+    PASIF_HIP(hipMemcpy(hm.data(), d->out_max.p, hm.size() * sizeof(R), hipMemcpyDeviceToHost));
+    d->res_max.assign(hm.begin(), hm.end());
+  }
+  if (traj_out) {
+    std::vector<R> ht(size_t(nslots) * pk.out_len_f);
+    if (!ht.empty())
+      PASIF_HIP(hipMemcpy(ht.data(), d->traj.p, ht.size() * sizeof(R), hipMemcpyDeviceToHost));
+    traj_out->assign(ht.begin(), ht.end());
+    d->traj.release();
+
+  }
+  if (grad_out) {
+    std::vector<R> hg(pk.out_len_a);
+    PASIF_HIP(hipMemcpy(hg.data(), d->out_grad.p, hg.size() * sizeof(R), hipMemcpyDeviceToHost));
+    grad_out->assign(hg.begin(), hg.end());
+
+  }
+}
+
+
+// Run the packed tiles to completion.  save_steps > 0 records a trajectory.
+template <typename R>
+void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
+                std::vector<double>* grad_out) {
+  uint64_t work = 0;
+  uint32_t nslots = 0;
+  TileArgs<R> a = prepare_run<R>(d, adjoint, save_steps, &work, &nslots);
+  Packed& pk = d->pk;
+  const uint32_t L = pk.lanes, nt = pk.ntiles;
   const bool plain = d->solver == PASIF_SOLVER_PLAIN;
   const bool graph = d->solver == PASIF_SOLVER_GRAPH;
   uint64_t launches = 1;
@@ -974,30 +1065,7 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   PASIF_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
   d->last_ms = ms;
   d->last_launches = uint32_t(launches);
-  // results
-  std::vector<R> hq(pk.out_len_f);
-  PASIF_HIP(hipMemcpy(hq.data(), d->out_q.p, hq.size() * sizeof(R), hipMemcpyDeviceToHost));
-  d->res_q.assign(hq.begin(), hq.end());
-  d->have_max = d->window != 0;
-  if (d->window) {
-    std::vector<R> hm(pk.out_len_f);
-    PASIF_HIP(hipMemcpy(hm.data(), d->out_max.p, hm.size() * sizeof(R), hipMemcpyDeviceToHost));
-    d->res_max.assign(hm.begin(), hm.end());
-  }
-  if (traj_out) {
-    std::vector<R> ht(size_t(nslots) * pk.out_len_f);
-    if (!ht.empty())
-      PASIF_HIP(hipMemcpy(ht.data(), d->traj.p, ht.size() * sizeof(R), hipMemcpyDeviceToHost));
-    traj_out->assign(ht.begin(), ht.end());
-    d->traj.release();
-
-  }
-  if (grad_out) {
-    std::vector<R> hg(pk.out_len_a);
-    PASIF_HIP(hipMemcpy(hg.data(), d->out_grad.p, hg.size() * sizeof(R), hipMemcpyDeviceToHost));
-    grad_out->assign(hg.begin(), hg.end());
-
-  }
+  finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
 }
 
 void solve(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj,
@@ -1092,12 +1160,19 @@ int pasif_create(pasif_driver** out, const double* excitations, uint32_t n_exc, 
 
 int pasif_destroy(pasif_driver* d) {
   if (!d) return PASIF_OK;
+  if (d->pool) {
+    // still a member of a pool: leave it there (the pool destroys it)
+    for (const auto& j : d->pool->jobs)
+      if (j.d == d) return PASIF_ERR_STATE;
+  }
   if (d->dev >= 0) (void)hipSetDevice(d->dev);
   if (d->stream) (void)hipStreamSynchronize(d->stream);
   d->pk.release();
-  if (d->ev0) (void)hipEventDestroy(d->ev0);
-  if (d->ev1) (void)hipEventDestroy(d->ev1);
-  if (d->stream) (void)hipStreamDestroy(d->stream);
+  if (!d->pool) {
+    if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->ev1) (void)hipEventDestroy(d->ev1);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+  }
   delete d;
   return PASIF_OK;
 }
@@ -1219,11 +1294,16 @@ int pasif_allocate_adjoint(pasif_driver* d) {
 
 int pasif_load_excitations(pasif_driver* d, const double* e, uint32_t n_exc, uint32_t len,
                            uint32_t sr) {
-  return guarded(d, [&] { load_exc(d, e, n_exc, len, sr); });
+  return guarded(d, [&] {
+    PASIF_REQUIRE(!d->ext_exc, PASIF_ERR_STATE, "this job uses its pool's signal bank");
+    load_exc(d, e, n_exc, len, sr);
+  });
 }
 
 int pasif_set_interpolation_matrix(pasif_driver* d, const double* m, size_t size, uint32_t w) {
   return guarded(d, [&] {
+    PASIF_REQUIRE(!d->ext_interp && !d->ext_exc, PASIF_ERR_STATE,
+                  "this job uses its pool's signal bank and interpolation");
     if (size == 0) {
       d->interp.clear();
       d->interp_w = 0;
@@ -1376,6 +1456,445 @@ int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launch
   if (launches) *launches = d->last_launches;
   if (tiles) *tiles = d->pk.ntiles;
   if (lanes) *lanes = d->pk.lanes;
+  return PASIF_OK;
+}
+
+}  // extern "C"
+
+// ================================================================= pool
+namespace {
+
+template <class F>
+int pool_guarded(pasif_pool* P, F&& f) {
+  if (!P) return PASIF_ERR_ARGUMENT;
+  try {
+    P->err.clear();
+    f();
+    return PASIF_OK;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    P->err = "host memory exhausted";
+    return PASIF_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    P->err = e.what();
+    return PASIF_ERR_ARGUMENT;
+  }
+}
+
+#define POOL_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      P->err = std::string("HIP error ") + hipGetErrorName(e_) + " in " + #call;         \
+      throw Fail{PASIF_ERR_DEVICE};                                                      \
+    }                                                                                    \
+  } while (0)
+
+#define POOL_REQUIRE(cond, code, msg) \
+  do {                                \
+    if (!(cond)) {                    \
+      P->err = (msg);                 \
+      throw Fail{code};               \
+    }                                 \
+  } while (0)
+
+double job_bytes(const pasif_driver* d) {
+  const DevBuf* run[] = {&d->qstate, &d->lstate, &d->mxstate, &d->progress, &d->out_q,
+                         &d->out_max, &d->out_grad, &d->ckpt, &d->segbuf, &d->d_exc,
+                         &d->d_interp};
+  double b = double(d->pk.program_bytes) + double(d->pk.hdr.bytes);
+  for (const DevBuf* x : run) b += double(x->bytes);
+  return b;
+}
+
+void release_run(pasif_driver* d) {
+  DevBuf* run[] = {&d->qstate, &d->lstate, &d->mxstate, &d->progress, &d->out_q,
+                   &d->out_max, &d->out_grad, &d->ckpt, &d->segbuf};
+  for (DevBuf* x : run) x->release();
+  d->pk.release();
+}
+
+template <typename R>
+void pool_tick_impl(pasif_pool* P) {
+  std::vector<TileArgs<R>> table;
+  P->map_host.clear();
+  size_t lds = 0;
+  for (auto& j : P->jobs) {
+    if (j.finished) continue;
+    pasif_driver* d = j.d;
+    uint64_t work = 0;
+    uint32_t nslots = 0;
+    // the job's buffers already exist: re-derive its arguments without
+    // clearing the progress (prepare_run only clears at admission)
+    TileArgs<R> a{};
+    {
+      Packed& pk = d->pk;
+      a.hdr = pk.hdr.as<TileHeader>();
+      a.lanes = pk.lanes;
+      fill_program(a.fwd, pk.f_info, pk.f_out, pk.f_src, pk.f_steps, pk.f_exc, pk.f_fslot,
+                   pk.f_force, pk.f_val, pk.f_idx, pk.f_init);
+      if (j.adjoint)
+        fill_program(a.adj, pk.a_info, pk.a_out, pk.a_src, pk.a_steps, pk.a_exc, pk.a_fslot,
+                     pk.a_force, pk.a_val, pk.a_idx, pk.a_init);
+      a.max_forced = pk.max_forced;
+      a.excit = d->ext_exc ? static_cast<const R*>(d->ext_exc) : d->d_exc.as<R>();
+      a.exc_len = d->exc_len;
+      a.interp = d->interp_w ? (d->ext_interp ? static_cast<const R*>(d->ext_interp)
+                                              : d->d_interp.as<R>())
+                             : nullptr;
+      a.interp_w = d->interp_w;
+      const R h = R(1.0 / double(d->sample_rate));
+      a.h = h;
+      a.h2 = R(double(h) / 2.0);
+      a.h6 = R(double(h) / 6.0);
+      a.qstate = d->qstate.as<R>();
+      a.progress = d->progress.as<uint32_t>();
+      a.out_q = d->out_q.as<R>();
+      a.out_len = pk.out_len_f;
+      if (j.adjoint) {
+        a.lstate = d->lstate.as<R>();
+        a.ckpt = d->ckpt.as<R>();
+        a.segbuf = d->segbuf.as<R>();
+        a.out_grad = d->out_grad.as<R>();
+        a.ck_every = d->pool_ck;
+        a.nck = d->pool_nck;
+        a.adjoint = 1;
+      }
+      a.budget = P->budget;
+      (void)work;
+      (void)nslots;
+    }
+    const uint32_t id = uint32_t(table.size());
+    table.push_back(a);
+    for (uint32_t t = 0; t < d->pk.ntiles; ++t) {
+      P->map_host.push_back(id);
+      P->map_host.push_back(t);
+    }
+    lds = std::max(lds, tile_lds_bytes(sizeof(R), P->lanes, true, a.interp_w, a.max_forced));
+  }
+  const uint32_t nblocks = uint32_t(P->map_host.size() / 2);
+  P->last_blocks = nblocks;
+  if (nblocks == 0) return;
+  const size_t tb = table.size() * sizeof(TileArgs<R>);
+  P->table_host.assign(reinterpret_cast<unsigned char*>(table.data()),
+                       reinterpret_cast<unsigned char*>(table.data()) + tb);
+  if (P->table_bytes < tb) {
+    if (P->table_dev) (void)hipFree(P->table_dev);
+    P->table_dev = nullptr;
+    POOL_HIP(hipMalloc(&P->table_dev, tb));
+    P->table_bytes = tb;
+  }
+  const size_t mb = P->map_host.size() * sizeof(uint32_t);
+  if (P->map_bytes < mb) {
+    if (P->map_dev) (void)hipFree(P->map_dev);
+    P->map_dev = nullptr;
+    POOL_HIP(hipMalloc(&P->map_dev, mb));
+    P->map_bytes = mb;
+  }
+  POOL_HIP(hipStreamWaitEvent(P->stream, P->uploaded, 0));
+  POOL_HIP(hipMemcpyAsync(P->table_dev, P->table_host.data(), tb, hipMemcpyHostToDevice,
+                          P->stream));
+  POOL_HIP(hipMemcpyAsync(P->map_dev, P->map_host.data(), mb, hipMemcpyHostToDevice, P->stream));
+  POOL_HIP(hipEventRecord(P->ev0, P->stream));
+  hipError_t e = launch_pool(static_cast<const TileArgs<R>*>(P->table_dev),
+                             static_cast<const uint32_t*>(P->map_dev), nblocks, P->lanes,
+                             int(P->slot_cfg), lds, P->stream);
+  if (e != hipSuccess) {
+    P->err = std::string("pool kernel launch failed: ") + hipGetErrorName(e);
+    throw Fail{PASIF_ERR_DEVICE};
+  }
+  POOL_HIP(hipEventRecord(P->ev1, P->stream));
+  for (auto& j : P->jobs)
+    if (!j.finished) j.done += P->budget;
+  P->in_flight = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile_lanes,
+                      uint32_t slot_config, uint32_t checkpoint_every, uint32_t steps_per_tick,
+                      double hbm_budget_bytes) {
+  if (!out) return PASIF_ERR_ARGUMENT;
+  *out = nullptr;
+  std::unique_ptr<pasif_pool> holder(new (std::nothrow) pasif_pool());
+  if (!holder) return PASIF_ERR_MEMORY;
+  pasif_pool* P = holder.get();
+  const int rc = pool_guarded(P, [&] {
+    POOL_REQUIRE(precision == 64 || precision == 32, PASIF_ERR_ARGUMENT, "precision 32 or 64");
+    POOL_REQUIRE(tile_lanes >= 64 && tile_lanes <= kMaxLanes && tile_lanes % 64 == 0,
+                 PASIF_ERR_ARGUMENT, "tile lanes must be a multiple of 64 up to 1024");
+    POOL_REQUIRE(slot_config >= 1 && slot_config <= 3 && (tile_lanes <= 256 || slot_config == 1),
+                 PASIF_ERR_ARGUMENT, "slot configuration 1..3 (1 above 256 lanes)");
+    POOL_REQUIRE(steps_per_tick > 0, PASIF_ERR_ARGUMENT, "steps per tick must be positive");
+    POOL_REQUIRE(hbm_budget_bytes > 0, PASIF_ERR_ARGUMENT, "HBM budget must be positive");
+    int ndev = 0;
+    POOL_HIP(hipGetDeviceCount(&ndev));
+    POOL_REQUIRE(gpu_id >= 0 && gpu_id < ndev, PASIF_ERR_ARGUMENT, "no such GPU");
+    P->dev = gpu_id;
+    POOL_HIP(hipSetDevice(gpu_id));
+    POOL_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    POOL_HIP(hipStreamCreateWithFlags(&P->upload, hipStreamNonBlocking));
+    POOL_HIP(hipEventCreate(&P->ev0));
+    POOL_HIP(hipEventCreate(&P->ev1));
+    POOL_HIP(hipEventCreateWithFlags(&P->uploaded, hipEventDisableTiming));
+    P->precision = precision;
+    P->lanes = tile_lanes;
+    P->slot_cfg = slot_config;
+    P->ck_every = checkpoint_every ? checkpoint_every : 256;
+    P->budget = steps_per_tick;
+    P->hbm_budget = hbm_budget_bytes;
+  });
+  *out = holder.release();
+  return rc;
+}
+
+const char* pasif_pool_last_error(const pasif_pool* P) { return P ? P->err.c_str() : "null pool"; }
+
+int pasif_pool_job_create(pasif_pool* P, pasif_driver** job, const double* excitations,
+                          uint32_t n_exc, uint32_t exc_len, uint32_t sample_rate,
+                          uint32_t numsteps) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(job != nullptr, PASIF_ERR_ARGUMENT, "null job pointer");
+    std::unique_ptr<pasif_driver> d(new pasif_driver());
+    d->dev = P->dev;
+    d->stream = P->upload;
+    d->pool = P;
+    d->precision = P->precision;
+    d->tile_lanes = P->lanes;
+    d->slot_cfg = P->slot_cfg;
+    d->ck_every = P->ck_every;
+    if (excitations == nullptr) {
+      // use the pool's signal bank: excitation ids index the bank
+      POOL_REQUIRE(P->bank_n > 0, PASIF_ERR_STATE, "no signal bank: pasif_pool_set_bank first");
+      d->ext_exc = P->bank.p;
+      d->ext_interp = P->bank_w ? P->bank_interp.p : nullptr;
+      d->n_exc = P->bank_n;
+      d->exc_len = P->bank_len;
+      d->sample_rate = P->bank_rate;
+      d->interp_w = P->bank_w;
+      d->interp.assign(size_t(2) * P->bank_w, 0.0);
+      d->numsteps = numsteps ? numsteps : P->bank_len;
+      d->d_exc_dirty = false;
+    } else {
+      const int rc = guarded(d.get(), [&] {
+        pasif_driver* dd = d.get();
+        load_exc(dd, excitations, n_exc, exc_len, sample_rate);
+        dd->numsteps = numsteps ? numsteps : exc_len;
+      });
+      POOL_REQUIRE(rc == PASIF_OK, rc, d->err);
+    }
+    *job = d.release();
+  });
+}
+
+int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc,
+                        uint32_t exc_len, uint32_t sample_rate, const double* interp,
+                        uint32_t window) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(P->jobs.empty(), PASIF_ERR_STATE, "the bank can only change on an empty pool");
+    POOL_REQUIRE(excitations && n_exc > 0 && exc_len > 0 && sample_rate > 0, PASIF_ERR_ARGUMENT,
+                 "empty signal bank");
+    POOL_REQUIRE(!interp || (window >= 2 && window <= kMaxInterpWindow), PASIF_ERR_ARGUMENT,
+                 "interpolation window must be in [2, 2048]");
+    POOL_HIP(hipSetDevice(P->dev));
+    const size_t n = size_t(n_exc) * exc_len;
+    const size_t el = P->precision == 64 ? 8 : 4;
+    P->bank.release();
+    P->bank_interp.release();
+    POOL_HIP(hipMalloc(&P->bank.p, n * el));
+    P->bank.bytes = n * el;
+    if (P->precision == 64) {
+      POOL_HIP(hipMemcpy(P->bank.p, excitations, n * el, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f(excitations, excitations + n);
+      POOL_HIP(hipMemcpy(P->bank.p, f.data(), n * el, hipMemcpyHostToDevice));
+    }
+    if (interp) {
+      POOL_HIP(hipMalloc(&P->bank_interp.p, 2 * size_t(window) * el));
+      P->bank_interp.bytes = 2 * size_t(window) * el;
+      if (P->precision == 64) {
+        POOL_HIP(hipMemcpy(P->bank_interp.p, interp, 2 * size_t(window) * el,
+                           hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> f(interp, interp + 2 * size_t(window));
+        POOL_HIP(hipMemcpy(P->bank_interp.p, f.data(), 2 * size_t(window) * el,
+                           hipMemcpyHostToDevice));
+      }
+    }
+    P->bank_n = n_exc;
+    P->bank_len = exc_len;
+    P->bank_rate = sample_rate;
+    P->bank_w = interp ? window : 0;
+  });
+}
+
+int pasif_pool_submit(pasif_pool* P, pasif_driver* d, int adjoint) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(d && d->pool == P, PASIF_ERR_ARGUMENT, "job does not belong to this pool");
+    for (const auto& j : P->jobs)
+      POOL_REQUIRE(j.d != d, PASIF_ERR_STATE, "job already submitted");
+    POOL_HIP(hipSetDevice(P->dev));
+    d->precision = P->precision;
+    d->tile_lanes = P->lanes;
+    d->slot_cfg = P->slot_cfg;
+    d->ck_every = P->ck_every;
+    d->window = 0;
+    uint64_t work = 0;
+    uint32_t nslots = 0;
+    int rc = guarded(d, [&] {
+      if (P->precision == 64) {
+        TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
+        d->pool_ck = a.ck_every;
+        d->pool_nck = a.nck;
+      } else {
+        TileArgs<float> a = prepare_run<float>(d, adjoint != 0, 0, &work, &nslots);
+        d->pool_ck = a.ck_every;
+        d->pool_nck = a.nck;
+      }
+      PASIF_REQUIRE(d->pk.lanes == P->lanes, PASIF_ERR_UNSUPPORTED,
+                    "a system of this request needs %u lanes, the pool runs %u", d->pk.lanes,
+                    P->lanes);
+      PASIF_REQUIRE(d->pk.slot_cfg == P->slot_cfg, PASIF_ERR_UNSUPPORTED,
+                    "slot configuration mismatch");
+    });
+    if (rc != PASIF_OK) {
+      release_run(d);
+      P->err = d->err;
+      throw Fail{rc};
+    }
+    const double bytes = job_bytes(d);
+    if (P->used + bytes > P->hbm_budget) {
+      release_run(d);
+      P->err = "HBM budget exhausted: the request needs " + std::to_string(bytes) +
+               " bytes, " + std::to_string(P->hbm_budget - P->used) + " are free";
+      throw Fail{PASIF_ERR_MEMORY};
+    }
+    POOL_HIP(hipEventRecord(P->uploaded, P->upload));
+    P->used += bytes;
+    P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false});
+  });
+}
+
+int pasif_pool_tick(pasif_pool* P, int wait, uint32_t* active_tiles) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(!P->in_flight, PASIF_ERR_STATE, "a tick is already in flight");
+    POOL_HIP(hipSetDevice(P->dev));
+    if (P->precision == 64)
+      pool_tick_impl<double>(P);
+    else
+      pool_tick_impl<float>(P);
+    if (active_tiles) *active_tiles = P->last_blocks;
+    if (wait && P->in_flight) {
+      POOL_HIP(hipEventSynchronize(P->ev1));
+      float ms = 0.f;
+      POOL_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+      P->last_ms = ms;
+      P->in_flight = false;
+    }
+  });
+}
+
+int pasif_pool_wait(pasif_pool* P, double* kernel_ms) {
+  return pool_guarded(P, [&] {
+    if (P->in_flight) {
+      POOL_HIP(hipEventSynchronize(P->ev1));
+      float ms = 0.f;
+      POOL_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+      P->last_ms = ms;
+      P->in_flight = false;
+    }
+    if (kernel_ms) *kernel_ms = P->last_ms;
+  });
+}
+
+int pasif_pool_job_finished(pasif_pool* P, const pasif_driver* d, int* finished) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(finished != nullptr, PASIF_ERR_ARGUMENT, "null output");
+    *finished = 0;
+    for (auto& j : P->jobs)
+      if (j.d == d) {
+        *finished = (!P->in_flight && j.done >= j.work) ? 1 : 0;
+        return;
+      }
+    POOL_REQUIRE(false, PASIF_ERR_ARGUMENT, "job not in this pool");
+  });
+}
+
+int pasif_pool_collect(pasif_pool* P, pasif_driver* d, double* q, size_t capq, size_t* lenq,
+                       double* grad, size_t capg, size_t* leng) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(!P->in_flight, PASIF_ERR_STATE, "wait for the tick first");
+    size_t k = 0;
+    for (; k < P->jobs.size(); ++k)
+      if (P->jobs[k].d == d) break;
+    POOL_REQUIRE(k < P->jobs.size(), PASIF_ERR_ARGUMENT, "job not in this pool");
+    auto& j = P->jobs[k];
+    POOL_REQUIRE(j.done >= j.work, PASIF_ERR_STATE, "job not finished");
+    POOL_HIP(hipSetDevice(P->dev));
+    // every tile must have reached its last phase
+    std::vector<uint32_t> prog(size_t(d->pk.ntiles) * 4);
+    POOL_HIP(hipMemcpy(prog.data(), d->progress.p, prog.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < d->pk.ntiles; ++t) {
+      const bool done = j.adjoint ? prog[4 * t + 1] == kPhaseDone
+                                  : prog[4 * t] >= d->pk.tmax || prog[4 * t + 1] == kPhaseDone;
+      POOL_REQUIRE(done, PASIF_ERR_STATE, "job tiles did not finish");
+    }
+    std::vector<double> grad_v;
+    const int rc = guarded(d, [&] {
+      if (P->precision == 64)
+        finish_run<double>(d, j.adjoint, 0, nullptr, j.adjoint ? &grad_v : nullptr);
+      else
+        finish_run<float>(d, j.adjoint, 0, nullptr, j.adjoint ? &grad_v : nullptr);
+      copy_out(d, d->res_q, q, capq, lenq);
+      if (j.adjoint) copy_out(d, grad_v, grad, capg, leng);
+      else if (leng) *leng = 0;
+    });
+    if (rc != PASIF_OK) {
+      P->err = d->err;
+      throw Fail{rc};
+    }
+    release_run(d);
+    d->d_exc.release();
+    d->d_interp.release();
+    d->d_exc_dirty = true;
+    P->used -= j.bytes;
+    P->jobs.erase(P->jobs.begin() + long(k));
+  });
+}
+
+int pasif_pool_stats(const pasif_pool* P, uint32_t* jobs, double* used_bytes,
+                     double* last_kernel_ms, uint32_t* last_blocks) {
+  if (!P) return PASIF_ERR_ARGUMENT;
+  if (jobs) *jobs = uint32_t(P->jobs.size());
+  if (used_bytes) *used_bytes = P->used;
+  if (last_kernel_ms) *last_kernel_ms = P->last_ms;
+  if (last_blocks) *last_blocks = P->last_blocks;
+  return PASIF_OK;
+}
+
+int pasif_pool_destroy(pasif_pool* P) {
+  if (!P) return PASIF_OK;
+  (void)hipSetDevice(P->dev);
+  if (P->stream) (void)hipStreamSynchronize(P->stream);
+  if (P->upload) (void)hipStreamSynchronize(P->upload);
+  for (auto& j : P->jobs) {
+    release_run(j.d);
+    j.d->pool = nullptr;
+    j.d->stream = nullptr;
+    delete j.d;
+  }
+  P->jobs.clear();
+  if (P->table_dev) (void)hipFree(P->table_dev);
+  if (P->map_dev) (void)hipFree(P->map_dev);
+  if (P->ev0) (void)hipEventDestroy(P->ev0);
+  if (P->ev1) (void)hipEventDestroy(P->ev1);
+  if (P->uploaded) (void)hipEventDestroy(P->uploaded);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  if (P->upload) (void)hipStreamDestroy(P->upload);
+  delete P;
   return PASIF_OK;
 }
 

@@ -275,12 +275,15 @@ __host__ __device__ inline size_t lds_bytes(uint32_t L, bool adjoint, uint32_t W
   return (2 * (L + 1) + L + (adjoint ? 4 * (L + 1) : 0) + 2 * W + nf * kForceStage) * sizeof(R);
 }
 
-template <typename R, int C1, int C2, int C3, int C4, int MAXL, bool ADJ>
-__global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+// One tile of one job through its phases (see the file comment).  ADJ
+// compiles the adjoint phases in; a job without an adjoint program runs the
+// forward phase only.
+template <typename R, int C1, int C2, int C3, int C4, bool ADJ>
+__device__ __forceinline__ void run_tile(const TileArgs<R>& a, const size_t tile,
+                                         unsigned char* smem_raw) {
   using Prog = LaneProg<R, C1, C2, C3, C4>;
   const uint32_t L = a.lanes;
-  constexpr bool adjoint = ADJ;
+  const bool adjoint = ADJ && a.adjoint != 0u;
   unsigned char* yb = smem_raw;
   R* part = reinterpret_cast<R*>(smem_raw + 2 * (L + 1) * sizeof(R));
   unsigned char* xbb = smem_raw + (3 * L + 2) * sizeof(R);
@@ -288,7 +291,6 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
   R* wts = reinterpret_cast<R*>(xbb + (adjoint ? 4 * (L + 1) * sizeof(R) : 0));
   R* eb = wts + 2 * a.interp_w;
   const uint32_t lane = threadIdx.x;
-  const size_t tile = blockIdx.x;
   const TileHeader* H = a.hdr + tile;
   const uint32_t T = uni(H->steps);
   const uint32_t K = adjoint ? a.ck_every : 1u;
@@ -440,7 +442,7 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
         }
       }
       __syncthreads();
-    } else if constexpr (ADJ) {
+    } else if (ADJ) {
       // ---------------- adjoint (reverse) ----------------
       P.load(a.adj, H->cnt_a, tile, L, lane);
       const uint32_t s0 = seg * K;
@@ -493,6 +495,25 @@ __global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
   }
 }
 
+template <typename R, int C1, int C2, int C3, int C4, int MAXL, bool ADJ>
+__global__ void __launch_bounds__(MAXL) tile_kernel(TileArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  run_tile<R, C1, C2, C3, C4, ADJ>(a, blockIdx.x, smem_raw);
+}
+
+// Tiles of many jobs in one launch: workgroup b runs tile map[2b+1] of job
+// map[2b].  Every job shares the workgroup size, slot configuration and
+// precision; its arguments are read from the job table.
+template <typename R, int C1, int C2, int C3, int C4, int MAXL>
+__global__ void __launch_bounds__(MAXL) pool_kernel(const TileArgs<R>* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const uint32_t job = uni(map[2 * blockIdx.x]);
+  const uint32_t tile = uni(map[2 * blockIdx.x + 1]);
+  const TileArgs<R> a = jobs[job];
+  run_tile<R, C1, C2, C3, C4, true>(a, tile, smem_raw);
+}
+
 template <typename R, int C1, int C2, int C3, int C4, int MAXL>
 static hipError_t launch_cfg(const TileArgs<R>& a, uint32_t ntiles, hipStream_t s) {
   const size_t lds = lds_bytes<R>(a.lanes, a.adjoint != 0u, a.interp_w, a.max_forced);
@@ -526,6 +547,52 @@ static hipError_t launch_tiles_impl(const TileArgs<R>& a, uint32_t ntiles, int s
   if (slot_cfg != 1) return hipErrorInvalidValue;
   return launch_cfg<R, kSlotNarrow.c1, kSlotNarrow.c2, kSlotNarrow.c3, kSlotNarrow.c4, 1024>(
       a, ntiles, s);
+}
+
+template <typename R, int C1, int C2, int C3, int C4, int MAXL>
+static hipError_t launch_pool_cfg(const TileArgs<R>* jobs, const uint32_t* map, uint32_t nblocks,
+                                  uint32_t lanes, size_t lds, hipStream_t s) {
+  auto kern = pool_kernel<R, C1, C2, C3, C4, MAXL>;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(lanes), lds, s, jobs, map);
+  return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_pool_impl(const TileArgs<R>* jobs, const uint32_t* map, uint32_t nblocks,
+                                   uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s) {
+  if (lanes == 0 || lanes > kMaxLanes || (lanes % 64) != 0 || lds > kMaxLdsBytes)
+    return hipErrorInvalidValue;
+  if (lanes <= 256) {
+    if (slot_cfg == 2)
+      return launch_pool_cfg<R, kSlotWide.c1, kSlotWide.c2, kSlotWide.c3, kSlotWide.c4, 256>(
+          jobs, map, nblocks, lanes, lds, s);
+    if (slot_cfg == 3)
+      return launch_pool_cfg<R, kSlotSmall.c1, kSlotSmall.c2, kSlotSmall.c3, kSlotSmall.c4, 256>(
+          jobs, map, nblocks, lanes, lds, s);
+    return launch_pool_cfg<R, kSlotNarrow.c1, kSlotNarrow.c2, kSlotNarrow.c3, kSlotNarrow.c4,
+                           256>(jobs, map, nblocks, lanes, lds, s);
+  }
+  if (slot_cfg != 1) return hipErrorInvalidValue;
+  return launch_pool_cfg<R, kSlotNarrow.c1, kSlotNarrow.c2, kSlotNarrow.c3, kSlotNarrow.c4, 1024>(
+      jobs, map, nblocks, lanes, lds, s);
+}
+
+hipError_t launch_pool(const TileArgs<double>* jobs, const uint32_t* map, uint32_t nblocks,
+                       uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s) {
+  return launch_pool_impl(jobs, map, nblocks, lanes, slot_cfg, lds, s);
+}
+hipError_t launch_pool(const TileArgs<float>* jobs, const uint32_t* map, uint32_t nblocks,
+                       uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s) {
+  return launch_pool_impl(jobs, map, nblocks, lanes, slot_cfg, lds, s);
+}
+size_t tile_lds_bytes(size_t elem, uint32_t lanes, bool adjoint, uint32_t W, uint32_t nforced) {
+  return elem == 8 ? lds_bytes<double>(lanes, adjoint, W, nforced)
+                   : lds_bytes<float>(lanes, adjoint, W, nforced);
 }
 
 hipError_t launch_tiles(const TileArgs<double>& a, uint32_t ntiles, int slot_cfg, hipStream_t s) {

@@ -129,4 +129,10 @@ namespace pasif {
 // kernel launchers (pasif_kernels.hip); slot_cfg 1 = narrow, 2 = wide, 3 = small
 hipError_t launch_tiles(const TileArgs<double>& a, uint32_t ntiles, int slot_cfg, hipStream_t s);
 hipError_t launch_tiles(const TileArgs<float>& a, uint32_t ntiles, int slot_cfg, hipStream_t s);
+// many jobs in one launch: block b runs tile map[2b+1] of job map[2b]
+hipError_t launch_pool(const TileArgs<double>* jobs, const uint32_t* map, uint32_t nblocks,
+                       uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s);
+hipError_t launch_pool(const TileArgs<float>* jobs, const uint32_t* map, uint32_t nblocks,
+                       uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s);
+size_t tile_lds_bytes(size_t elem, uint32_t lanes, bool adjoint, uint32_t W, uint32_t nforced);
 }  // namespace pasif

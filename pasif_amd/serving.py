@@ -1,0 +1,411 @@
+"""Continuous-batching solve server on one GPU.
+
+Requests (batches of 1-256 systems, forward-only or forward+adjoint) are
+admitted into a device-resident pool (``pasif_pool_*`` in the C ABI) as
+soon as the pool's HBM budget allows.  Every tick advances all admitted,
+unfinished requests by a slice of steps in ONE kernel launch; the host admits
+newly arrived requests while that launch runs and collects finished ones
+after it, so requests that arrive while others are on the GPU join them at
+the next tick.  A request's result is bitwise independent of its neighbours:
+each request owns its tiles and no arithmetic crosses systems.
+
+Requests are drawn from the heterogeneous-batch distributions (DOF sizes
+{1,...,64}, log-uniform step counts, chain structure).  To keep admission
+cheap at high request rates, systems are taken from a library of
+pre-assembled chain systems per size (``SystemLibrary``) and forcing signals
+from a bank uploaded once (``pasif_pool_set_bank``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from . import adjoint as ADJ
+from . import workloads as W
+from .PASIf import assemble_arrays
+from .PASIfgpu import __GpuDriver as _Driver
+
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+
+_POOL_SIGS = {
+    "pasif_pool_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint32,
+                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_double]),
+    "pasif_pool_destroy": (C.c_int, [C.c_void_p]),
+    "pasif_pool_last_error": (C.c_char_p, [C.c_void_p]),
+    "pasif_pool_set_bank": (C.c_int, [C.c_void_p, _f64p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      _f64p, C.c_uint32]),
+    "pasif_pool_job_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), _f64p, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pasif_pool_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "pasif_pool_tick": (C.c_int, [C.c_void_p, C.c_int, _u32p]),
+    "pasif_pool_wait": (C.c_int, [C.c_void_p, _f64p]),
+    "pasif_pool_job_finished": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    "pasif_pool_collect": (C.c_int, [C.c_void_p, C.c_void_p, _f64p, C.c_size_t,
+                                     C.POINTER(C.c_size_t), _f64p, C.c_size_t,
+                                     C.POINTER(C.c_size_t)]),
+    "pasif_pool_stats": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _u32p]),
+}
+
+
+def _load():
+    lib = _lib.load()
+    for name, (res, args) in _POOL_SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def pool_symbols():
+    return list(_POOL_SIGS)
+
+
+# --------------------------------------------------------------- payloads
+@dataclass
+class SystemArrays:
+    """Driver arrays of one system (local indices), forward and adjoint."""
+    n: int
+    csr: tuple            # data, indices, indptr
+    gamma: tuple          # (idx [nnz,3], val)
+    lam: tuple
+    force: np.ndarray
+    q0: np.ndarray
+    na: int = 0
+    acsr: Optional[tuple] = None
+    agamma: Optional[tuple] = None
+    alam: Optional[tuple] = None
+    apsi: Optional[tuple] = None
+    lamT: Optional[np.ndarray] = None
+
+
+def _tens(t, order):
+    v = np.asarray(t.val, dtype=np.float64).reshape(-1)
+    ix = (np.asarray(t.indices, dtype=np.int64).reshape(-1, order) if v.size
+          else np.zeros((0, order), dtype=np.int64))
+    return ix, v
+
+
+class SystemLibrary:
+    """Pre-assembled chain systems of each DOF size with the energy-gradient
+    adjoint (dE(T)/dK_ii, dE(T)/dLambda_i), ``variants`` per size."""
+
+    def __init__(self, dofs: Sequence[int] = tuple(W.DOF_CHOICES), variants: int = 8,
+                 seed: int = 4, adjoint: bool = True):
+        rng = np.random.default_rng(seed)
+        self.entries: Dict[int, List[SystemArrays]] = {}
+        for d in dofs:
+            lst = []
+            for _ in range(variants):
+                s = rng.uniform(0.9, 1.1)
+                gv = rng.normal(0.0, 0.05, 2 * d)
+                lv = rng.uniform(0.0, 1.0, d)
+                M, B, K, G, L, f, q0, _, _ = W.chain_system(int(d), s, gv, lv)
+                r = assemble_arrays([M], [B], [K], [G], [L], [f], [q0])
+                csr = r["csr"]
+                e = SystemArrays(n=M.shape[0], csr=(csr.data.copy(), csr.indices.astype(np.int64),
+                                                    csr.indptr.astype(np.int64)),
+                                 gamma=_tens(r["gamma"], 3), lam=_tens(r["lam"], 4),
+                                 force=r["force"], q0=r["q0"])
+                if adjoint:
+                    n = M.shape[0]
+                    params = [("K", d + i, i) for i in range(d)] + \
+                             [("Lambda", t) for t in range(len(np.asarray(L.val).reshape(-1)))]
+                    term = np.zeros(n)
+                    term[-1] = 1.0
+                    Mp, Bp, Kp, Gp, Lp, fp, lT, Pp = ADJ.system_jacobian(M, B, K, G, L, params,
+                                                                         term)
+                    ra = assemble_arrays([Mp], [Bp], [Kp], [Gp], [Lp], [fp], [lT], [Pp],
+                                         adjoint=True)
+                    ac = ra["csr"]
+                    e.na = Mp.shape[0]
+                    e.acsr = (ac.data.copy(), ac.indices.astype(np.int64),
+                              ac.indptr.astype(np.int64))
+                    e.agamma = _tens(ra["gamma"], 3)
+                    e.alam = _tens(ra["lam"], 4)
+                    e.apsi = _tens(ra["psi"], 5)
+                    e.lamT = ra["q0"]
+                lst.append(e)
+            self.entries[int(d)] = lst
+
+
+def _cat_csr(parts, offs):
+    data = np.concatenate([p[0] for p in parts])
+    ind = np.concatenate([p[1] + o for p, o in zip(parts, offs)])
+    nnz = np.cumsum([0] + [len(p[0]) for p in parts])
+    ptr = np.concatenate([[0]] + [p[2][1:] + b for p, b in zip(parts, nnz[:-1])])
+    return data, ind, ptr
+
+
+def _cat_tens(parts, offs_a, offs_b, n_a_axes):
+    """indices: the first n_a_axes axes offset by offs_a, the rest by offs_b"""
+    idx, val = [], []
+    for (ix, v), oa, ob in zip(parts, offs_a, offs_b):
+        o = np.array([oa] * n_a_axes + [ob] * (ix.shape[1] - n_a_axes), dtype=np.int64)
+        idx.append(ix + o[None, :])
+        val.append(v)
+    return np.concatenate(idx, axis=0), np.concatenate(val)
+
+
+@dataclass
+class Request:
+    rid: int
+    systems: List[SystemArrays]
+    exc_ids: np.ndarray
+    steps: np.ndarray
+    adjoint: bool
+    arrival: float = 0.0
+    admitted: float = 0.0
+    done: float = 0.0
+    result: Optional[np.ndarray] = None
+    grad: Optional[np.ndarray] = None
+
+    @property
+    def system_steps(self) -> int:
+        return int(np.sum(self.steps))
+
+
+def draw_request(lib: SystemLibrary, rng, rid: int, n_signals: int, tmin=1e3, tmax=1e5,
+                 max_systems: int = 256, adjoint_frac: float = 0.7) -> Request:
+    n = int(rng.integers(1, max_systems + 1))
+    sizes = np.array([d for d in W.DOF_CHOICES if int(d) in lib.entries])
+    probs = np.array([p for d, p in zip(W.DOF_CHOICES, W.DOF_PROBS) if int(d) in lib.entries])
+    dofs = rng.choice(sizes, size=n, p=probs / probs.sum())
+    systems = [lib.entries[int(d)][int(rng.integers(len(lib.entries[int(d)])))] for d in dofs]
+    steps = np.exp(rng.uniform(np.log(tmin), np.log(tmax), n)).astype(np.int64)
+    return Request(rid, systems, rng.integers(0, n_signals, n), steps,
+                   bool(rng.uniform() < adjoint_frac))
+
+
+# --------------------------------------------------------------- pool
+class Pool:
+    """Device-resident pool of jobs advanced together, one launch per tick."""
+
+    def __init__(self, lanes: int = 512, slot_config: int = 1, checkpoint_every: int = 256,
+                 steps_per_tick: int = 2000, hbm_budget: float = 200e9, precision: int = 64,
+                 gpu: int = 0):
+        self.lib = _load()
+        h = C.c_void_p()
+        rc = self.lib.pasif_pool_create(C.byref(h), gpu, precision, lanes, slot_config,
+                                        checkpoint_every, steps_per_tick, float(hbm_budget))
+        self.h = h
+        if rc:
+            msg = self.lib.pasif_pool_last_error(h)
+            self.lib.pasif_pool_destroy(h)
+            self.h = None
+            raise _lib.PasifError(rc, msg.decode() if msg else "")
+        self.steps_per_tick = steps_per_tick
+
+    def _chk(self, rc):
+        if rc:
+            raise _lib.PasifError(rc, self.lib.pasif_pool_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.pasif_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bank(self, exc: np.ndarray, sample_rate: int, interp: Optional[np.ndarray]):
+        e = np.ascontiguousarray(exc, dtype=np.float64)
+        m = None if interp is None else np.ascontiguousarray(interp[:2], dtype=np.float64)
+        self._chk(self.lib.pasif_pool_set_bank(
+            self.h, e.ctypes.data_as(_f64p), e.shape[0], e.shape[1], int(sample_rate),
+            None if m is None else m.ctypes.data_as(_f64p), 0 if m is None else m.shape[1]))
+
+    def _job(self):
+        j = C.c_void_p()
+        self._chk(self.lib.pasif_pool_job_create(self.h, C.byref(j), None, 0, 0, 0, 0))
+        d = object.__new__(_Driver)
+        d._pl = self.lib
+        d._h = j
+        d._layout = {"n": 0, "sizes_f": None, "sizes_a": None, "exc": None, "steps": None}
+        return d
+
+    def submit(self, req: Request):
+        """Load and admit a request; returns the job or None when the HBM
+        budget is exhausted (retry after collecting)."""
+        d = self._job()
+        sy = req.systems
+        sizes = np.array([s.n for s in sy], dtype=np.int64)
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+        n = int(sizes.sum())
+        data, ind, ptr = _cat_csr([s.csr for s in sy], offs)
+        d._setFwdK((n, n), data, ind, ptr)
+        gi, gv = _cat_tens([s.gamma for s in sy], offs, offs, 3)
+        d._setFwdGamma((n, n, n), gv, gi.reshape(-1))
+        li, lv = _cat_tens([s.lam for s in sy], offs, offs, 4)
+        d._setFwdLambda((n, n, n, n), lv, li.reshape(-1))
+        d._setFwdPsi((1, 1, 1, 1, 1), [], [])
+        d._setFwdForcePattern(np.concatenate([s.force for s in sy]))
+        d._setFwdInitialConditions(np.concatenate([s.q0 for s in sy]))
+        sizes_a = None
+        if req.adjoint:
+            sizes_a = np.array([s.na for s in sy], dtype=np.int64)
+            aoffs = np.concatenate([[0], np.cumsum(sizes_a)[:-1]])
+            na = int(sizes_a.sum())
+            data, ind, ptr = _cat_csr([s.acsr for s in sy], aoffs)
+            d._setBwdK((na, na), data, ind, ptr)
+            gi, gv = _cat_tens([s.agamma for s in sy], aoffs, offs, 2)
+            d._setBwdGamma((na, na, n), gv, gi.reshape(-1))
+            li, lv = _cat_tens([s.alam for s in sy], aoffs, offs, 2)
+            d._setBwdLambda((na, na, n, n), lv, li.reshape(-1))
+            pi, pv = _cat_tens([s.apsi for s in sy], aoffs, offs, 2)
+            d._setBwdPsi((na, na, n, n, n), pv, pi.reshape(-1))
+            d._setBwdForcePattern(np.zeros(na))
+            d._setBwdInitialConditions(np.concatenate([s.lamT for s in sy]))
+        d._setSystemsLayout(sizes_fwd=sizes, sizes_adj=sizes_a, exc_of_system=req.exc_ids,
+                            steps_of_system=req.steps)
+        d._allocateSystemOnDevice()
+        if req.adjoint:
+            d._allocateAdjointOnDevice()
+        rc = self.lib.pasif_pool_submit(self.h, d._h, int(req.adjoint))
+        if rc == 4:          # HBM budget: keep queued
+            self.lib.pasif_destroy(d._h)
+            d._h = None
+            return None
+        self._chk(rc)
+        return d
+
+    def tick(self, wait: bool) -> int:
+        n = C.c_uint32(0)
+        self._chk(self.lib.pasif_pool_tick(self.h, int(wait), C.byref(n)))
+        return n.value
+
+    def wait(self) -> float:
+        ms = C.c_double(0)
+        self._chk(self.lib.pasif_pool_wait(self.h, C.byref(ms)))
+        return ms.value
+
+    def finished(self, d) -> bool:
+        f = C.c_int(0)
+        self._chk(self.lib.pasif_pool_job_finished(self.h, d._h, C.byref(f)))
+        return bool(f.value)
+
+    def collect(self, d, req: "Request"):
+        adjoint = req.adjoint
+        lq, lg = C.c_size_t(0), C.c_size_t(0)
+        q = np.empty(sum(s.n for s in req.systems))
+        g = np.empty(sum(s.na for s in req.systems) if adjoint else 0)
+        self._chk(self.lib.pasif_pool_collect(self.h, d._h, q.ctypes.data_as(_f64p), q.size,
+                                              C.byref(lq), g.ctypes.data_as(_f64p), g.size,
+                                              C.byref(lg)))
+        self.lib.pasif_destroy(d._h)
+        d._h = None
+        return q[:lq.value], (g[:lg.value] if adjoint else None)
+
+    def stats(self) -> dict:
+        j, b, ms, blk = C.c_uint32(0), C.c_double(0), C.c_double(0), C.c_uint32(0)
+        self._chk(self.lib.pasif_pool_stats(self.h, C.byref(j), C.byref(b), C.byref(ms),
+                                            C.byref(blk)))
+        return {"jobs": j.value, "used_bytes": b.value, "kernel_ms": ms.value,
+                "blocks": blk.value}
+
+
+# --------------------------------------------------------------- server
+@dataclass
+class ServeReport:
+    offered_rps: float
+    duration_s: float
+    completed: int
+    admitted: int
+    arrived: int
+    p50_ms: float
+    p99_ms: float
+    mean_ms: float
+    system_steps_per_s: float
+    ticks: int
+    gpu_busy_frac: float
+    latencies_ms: List[float] = field(default_factory=list)
+
+
+class SolveServer:
+    def __init__(self, pool: Pool):
+        self.pool = pool
+        self.inflight = []          # (job, request)
+
+    def run(self, requests: Sequence[Request], horizon_s: Optional[float] = None,
+            drain: bool = True, max_wall_s: float = 600.0) -> ServeReport:
+        """Serve ``requests`` (arrival times relative to start).  Requests that
+        arrive while a tick runs are admitted during it (host work overlaps the
+        launch) and join at the next tick."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        queue = deque()
+        done: List[Request] = []
+        i = 0
+        t0 = time.perf_counter()
+        ticks = 0
+        busy_ms = 0.0
+        admitted = 0
+
+        def admit(now):
+            nonlocal i, admitted
+            while i < len(reqs) and reqs[i].arrival <= now:
+                queue.append(reqs[i])
+                i += 1
+            while queue:
+                r = queue[0]
+                job = self.pool.submit(r)
+                if job is None:
+                    break
+                r.admitted = time.perf_counter() - t0
+                self.inflight.append((job, r))
+                admitted += 1
+                queue.popleft()
+
+        while True:
+            now = time.perf_counter() - t0
+            if now > max_wall_s:
+                break
+            if horizon_s is not None and now > horizon_s and not drain:
+                break
+            admit(now)
+            if self.inflight:
+                self.pool.tick(wait=False)
+                ticks += 1
+                admit(time.perf_counter() - t0)    # overlaps the running launch
+                busy_ms += self.pool.wait()
+                keep = []
+                for job, r in self.inflight:
+                    if self.pool.finished(job):
+                        r.result, r.grad = self.pool.collect(job, r)
+                        r.done = time.perf_counter() - t0
+                        done.append(r)
+                    else:
+                        keep.append((job, r))
+                self.inflight = keep
+            elif i >= len(reqs) and not queue:
+                break
+            else:
+                nxt = reqs[i].arrival if i < len(reqs) else now
+                time.sleep(max(0.0, min(0.01, nxt - now)))
+        wall = time.perf_counter() - t0
+        lat = np.array([(r.done - r.arrival) * 1e3 for r in done]) if done else np.zeros(1)
+        steps = sum(r.system_steps for r in done)
+        return ServeReport(
+            offered_rps=len(reqs) / max(reqs[-1].arrival, 1e-9) if reqs else 0.0,
+            duration_s=wall, completed=len(done), admitted=admitted, arrived=i,
+            p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+            mean_ms=float(lat.mean()), system_steps_per_s=steps / wall, ticks=ticks,
+            gpu_busy_frac=busy_ms / 1e3 / wall, latencies_ms=list(lat))
+
+
+def poisson_arrivals(rng, rate: float, duration: float) -> np.ndarray:
+    n = rng.poisson(rate * duration)
+    return np.sort(rng.uniform(0.0, duration, n))
+
+
+def signal_bank(rng, n_signals: int, tmax: float, steps_per_sample: int = 10):
+    n_samples = int(tmax) // steps_per_sample + 2
+    return np.stack([W.band_limited(rng, n_samples) for _ in range(n_signals)])

@@ -1,0 +1,135 @@
+"""GPU checks of the continuous-batching pool:
+
+* a request's results are bitwise identical alone and in a mix of other
+  requests sharing every launch, and match the oracle (forward 1e-10,
+  adjoint 1e-10);
+* admission respects the HBM budget: requests wait until finished ones free
+  memory;
+* the server drains a small Poisson trace with latencies recorded.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pasif_amd import serving as S
+from tests.test_gpu_forward import TOL64, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_alone(pool, req):
+    job = pool.submit(req)
+    assert job is not None
+    while not pool.finished(job):
+        pool.tick(wait=True)
+    return pool.collect(job, req)
+
+
+def _oracle(req, bank, lib_rate, interp):
+    sy = req.systems
+    sizes = np.array([s.n for s in sy])
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    n = int(sizes.sum())
+    import scipy.sparse as sp
+    data, ind, ptr = S._cat_csr([s.csr for s in sy], offs)
+    A = sp.csr_matrix((data, ind, ptr), shape=(n, n))
+    gi, gv = S._cat_tens([s.gamma for s in sy], offs, offs, 3)
+    li, lv = S._cat_tens([s.lam for s in sy], offs, offs, 4)
+    fwd = O.Assembled(n=n, A=A, gamma_idx=gi, gamma_val=gv, lam_idx=li, lam_val=lv,
+                      psi_idx=np.zeros((0, 5), dtype=np.int64), psi_val=np.zeros(0),
+                      force=np.concatenate([s.force for s in sy]),
+                      q0=np.concatenate([s.q0 for s in sy]), sizes=sizes)
+    lay = O.mapped_layout(sizes, req.exc_ids, req.steps)
+    F = O.Forcing(bank, interp)
+    q = O.rk4_forward(fwd, F, lib_rate, lay).q[0]
+    if not req.adjoint:
+        return q, None
+    sa = np.array([s.na for s in sy])
+    aoffs = np.concatenate([[0], np.cumsum(sa)[:-1]])
+    na = int(sa.sum())
+    data, ind, ptr = S._cat_csr([s.acsr for s in sy], aoffs)
+    Aa = sp.csr_matrix((data, ind, ptr), shape=(na, na))
+    gi, gv = S._cat_tens([s.agamma for s in sy], aoffs, offs, 2)
+    li, lv = S._cat_tens([s.alam for s in sy], aoffs, offs, 2)
+    pi, pv = S._cat_tens([s.apsi for s in sy], aoffs, offs, 2)
+    adj = O.Assembled(n=na, A=Aa, gamma_idx=gi, gamma_val=gv, lam_idx=li, lam_val=lv,
+                      psi_idx=pi, psi_val=pv, force=np.zeros(na),
+                      q0=np.concatenate([s.lamT for s in sy]), n_set=n, sizes=sa)
+    ast = np.repeat(req.steps, sa)[None, :]
+    g = O.adjoint_gradient(fwd, adj, F, lib_rate, lay, ast, checkpoint_every=256)[0]
+    return q, g
+
+
+def test_alone_equals_in_mix_and_matches_oracle():
+    print("[gpu case] serving identity", flush=True)
+    rng = np.random.default_rng(41)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 8), variants=3)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    interp = __import__("pasif_amd.workloads", fromlist=["x"]).linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=256, slot_config=1, steps_per_tick=150, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        return pool
+
+    mk = lambda rid, adj: S.draw_request(lib, rng, rid, 6, tmin=200, tmax=1200,
+                                         max_systems=12, adjoint_frac=1.0 if adj else 0.0)
+    target = mk(0, True)
+    others = [mk(i, i % 2 == 0) for i in range(1, 7)]
+    q1, g1 = _run_alone(new_pool(), target)
+    pool = new_pool()
+    jobs = [(pool.submit(r), r) for r in [others[0], others[1], target] + others[2:]]
+    results = {}
+    while jobs:
+        pool.tick(wait=True)
+        keep = []
+        for j, r in jobs:
+            if pool.finished(j):
+                results[r.rid] = pool.collect(j, r)
+            else:
+                keep.append((j, r))
+        jobs = keep
+    q2, g2 = results[0]
+    np.testing.assert_array_equal(q1, q2)
+    np.testing.assert_array_equal(g1, g2)
+    oq, og = _oracle(target, bank, 1000, interp)
+    assert rel_err(q1, oq) < TOL64
+    assert rel_err(g1, og) < TOL64
+
+
+def test_admission_waits_for_budget():
+    print("[gpu case] serving budget", flush=True)
+    rng = np.random.default_rng(42)
+    lib = S.SystemLibrary(dofs=(2, 4), variants=2, adjoint=False)
+    bank = S.signal_bank(rng, 4, 2000)
+    pool = S.Pool(lanes=256, slot_config=1, steps_per_tick=500, hbm_budget=1.0)
+    pool.set_bank(bank, 1000, None)
+    r = S.draw_request(lib, rng, 0, 4, tmin=100, tmax=300, max_systems=4, adjoint_frac=0.0)
+    assert pool.submit(r) is None          # does not fit in 1 byte
+    pool2 = S.Pool(lanes=256, slot_config=1, steps_per_tick=500, hbm_budget=2e9)
+    pool2.set_bank(bank, 1000, None)
+    reqs = [S.draw_request(lib, rng, i, 4, tmin=100, tmax=300, max_systems=4,
+                           adjoint_frac=0.0) for i in range(5)]
+    for i, r in enumerate(reqs):
+        r.arrival = 0.001 * i
+    rep = S.SolveServer(pool2).run(reqs, max_wall_s=120)
+    assert rep.completed == 5
+
+
+def test_server_drains_poisson_trace():
+    print("[gpu case] serving poisson", flush=True)
+    rng = np.random.default_rng(43)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 8, 16), variants=2)
+    bank = S.signal_bank(rng, 8, 2000, steps_per_sample=10)
+    from pasif_amd.workloads import linear_interp_matrix
+    pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=400, hbm_budget=20e9)
+    pool.set_bank(bank, 1000, linear_interp_matrix(10))
+    arr = S.poisson_arrivals(rng, 40.0, 1.0)
+    reqs = []
+    for i, t in enumerate(arr):
+        r = S.draw_request(lib, rng, i, 8, tmin=100, tmax=1500, max_systems=16)
+        r.arrival = float(t)
+        reqs.append(r)
+    rep = S.SolveServer(pool).run(reqs, max_wall_s=120)
+    assert rep.completed == len(reqs)
+    assert rep.p99_ms >= rep.p50_ms > 0
