@@ -12,7 +12,9 @@ per solve, inputs resident in HBM before the timed region.
 launch).  Also reported: the plain per-step kernel on the same GPU, the FP32
 rate, the roofline of the tile kernel (algorithmic FP64 flops per launch over
 the HIP-event duration of the launch), the FP64 error against the NumPy
-oracle on the CPU sample, and the oracle's own rate on the host cores.
+oracle on the CPU sample, the oracle's own rate on the host cores, and
+(``fwd_adj``) forward + checkpointed adjoint on 8,192 chains of 16 DOF x
+20,000 steps, persistent vs plain.
 """
 from __future__ import annotations
 
@@ -91,6 +93,60 @@ def cpu_baseline(w, n_sample: int):
     return res.q.reshape(-1), n_sample * w.numsteps / dt, dt
 
 
+def adjoint_flops_per_step(p) -> float:
+    """Algorithmic flops of one adjoint step of every system:
+    ``4 (2 nnzA' + 3 nnzG' + 4 nnzL' + 5 nnzPsi') + 12 n_adj + 2 n_fwd``."""
+    import scipy.sparse as sp
+    Ja = (p.jacobian_B + p.jacobian_K).tocsr()
+    nG = len(np.asarray(p.jacobian_Gamma.val).reshape(-1))
+    nL = len(np.asarray(p.jacobian_Lambda.val).reshape(-1))
+    nP = len(np.asarray(p.jacobian_Psi.val).reshape(-1))
+    return 4.0 * (2 * Ja.nnz + 3 * nG + 4 * nL + 5 * nP) + 12.0 * p.globalAdjointSize + \
+        2.0 * p.globalSystemSize
+
+
+def fwd_adj_section(args) -> dict:
+    """Forward + checkpointed adjoint on chains of 16 DOF (energy gradient
+    w.r.t. the K diagonal and the Lambda coefficients), FP64, checkpoint every
+    256 steps: persistent launch vs the plain one-launch-per-step kernel.  A
+    system-step is one forward step plus one adjoint step of one system."""
+    from pasif_amd import adjoint as ADJ
+    from pasif_amd import workloads as W
+    w = W.chains(n_systems=args.adj_systems, d=16, steps=args.adj_steps, steps_per_sample=10)
+    p = W.make_solver(w)
+    p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
+    p.setTuning(checkpoint_every=256)
+    p.getGradient()                                   # warm-up (packing, upload)
+    t0 = time.perf_counter()
+    g = p.getGradient()
+    wall = time.perf_counter() - t0
+    t = p._lastTiming()
+    sys_steps = w.n_systems * w.numsteps
+    # work in the launch: forward twice (checkpoint pass + recompute) and the adjoint
+    flops = (2 * flops_per_step(p) + adjoint_flops_per_step(p)) * w.numsteps
+    achieved = flops / (t["kernel_ms"] * 1e-3) / 1e12
+    sec = {"workload": f"chains {w.n_systems} x 16 DOF x {w.numsteps} steps, dt 1e-3, "
+                       f"interpolated band-limited forcing, gradient of E(T) w.r.t. 16 K_ii + "
+                       f"16 Lambda per system, checkpoint 256, FP64",
+           "value": sys_steps / wall, "unit": "system-steps/s (fwd+adj)",
+           "kernel_ms": t["kernel_ms"], "tiles": t["tiles"], "lanes_per_tile": t["lanes"],
+           "roofline": {"achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS,
+                        "flops_per_launch": flops}}
+    if not args.no_plain:
+        p.setSolver("plain")
+        t0 = time.perf_counter()
+        gp = p.getGradient()
+        pw = time.perf_counter() - t0
+        pt = p._lastTiming()
+        p.setSolver("persistent")
+        sec["plain"] = {"value": sys_steps / pw, "kernel_ms": pt["kernel_ms"],
+                        "launches": pt["launches"],
+                        "bitwise_equal_to_persistent": bool(np.array_equal(gp, g))}
+        sec["persistent_vs_plain"] = (sys_steps / wall) / (sys_steps / pw)
+    return sec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +157,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-systems", type=int, default=8192)
+    ap.add_argument("--adj-steps", type=int, default=20_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: systems are independent and the path needs no "
@@ -183,6 +242,9 @@ def main():
     w32 = time.perf_counter() - t0
     out["fp32"] = {"value": sys_steps / w32, "kernel_ms": p._lastTiming()["kernel_ms"]}
     p.setPrecision(64)
+
+    if not args.no_adjoint:
+        out["fwd_adj"] = fwd_adj_section(args)
 
     if not args.no_cpu:
         ref, cpu_rate, cpu_s = cpu_baseline(w, args.cpu_sample)

@@ -1,0 +1,105 @@
+"""Serving benchmark: Poisson streams of solve requests against the
+continuous-batching pool on one MI355X.
+
+Each request is a batch of 1-256 systems drawn from the heterogeneous
+distributions (DOF {1,..,64}, steps log-uniform on [1e3, 1e5], dt 1e-3,
+chain structure, interpolated band-limited forcing); 30 % forward-only, 70 %
+forward+adjoint (energy gradient), FP64, admitted against a 200 GB HBM
+budget and a resident-tile (slot) cap; library systems are registered with
+the pool once and requests travel as system ids.  For every offered load it prints one JSON line: p50 / p99 / mean
+latency (arrival to results on the host) of completed requests, the backlog
+left at the end with p99 over all arrived requests (waiting time so far for
+the unfinished ones, a lower bound), sustained system-steps/s and GPU busy
+fraction; and once, the plain one-launch-per-step kernel run per request on
+a sample of requests for comparison.
+
+    python bench_serving.py --loads 1000 5000 20000 --duration 60
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--loads", type=float, nargs="+", default=[1000, 5000, 20000])
+    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--max-wall", type=float, default=120.0)
+    ap.add_argument("--tmin", type=float, default=1e3)
+    ap.add_argument("--tmax", type=float, default=1e5)
+    ap.add_argument("--max-systems", type=int, default=256)
+    ap.add_argument("--steps-per-tick", type=int, default=2000)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--lanes", type=int, default=512)
+    ap.add_argument("--slots", type=int, default=1536,
+                    help="resident tile cap (0: HBM budget only)")
+    ap.add_argument("--plain-sample", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=4)
+    a = ap.parse_args()
+
+    from pasif_amd import serving as S
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(a.seed)
+    lib = S.SystemLibrary(variants=4)
+    n_signals = 32
+    bank = S.signal_bank(rng, n_signals, a.tmax, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+
+    def new_pool(steps_per_tick):
+        pool = S.Pool(lanes=a.lanes, slot_config=1, checkpoint_every=256,
+                      steps_per_tick=steps_per_tick, hbm_budget=a.budget_gb * 1e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        pool.set_slots(a.slots)
+        return pool
+
+    for li, load in enumerate(a.loads):
+        arr = S.poisson_arrivals(rng, load, a.duration)
+        reqs = S.LazyRequests(lib, len(arr), a.seed * 1000 + li, n_signals, tmin=a.tmin,
+                              tmax=a.tmax, max_systems=a.max_systems)
+        pool = new_pool(a.steps_per_tick)
+        rep = S.SolveServer(pool).run(reqs, horizon_s=a.duration, drain=False,
+                                      max_wall_s=a.max_wall, arrivals=arr)
+        pool.close()
+        print(json.dumps({"metric": "serving latency", "offered_rps": load,
+                          "duration_s": a.duration, "arrived": rep.arrived,
+                          "admitted": rep.admitted, "completed": rep.completed,
+                          "backlog": rep.backlog,
+                          "p50_ms": rep.p50_ms, "p99_ms": rep.p99_ms, "mean_ms": rep.mean_ms,
+                          "p99_all_ms": rep.p99_all_ms, "admit_ms_mean": rep.admit_ms_mean,
+                          "system_steps_per_s": rep.system_steps_per_s, "ticks": rep.ticks,
+                          "gpu_busy_frac": rep.gpu_busy_frac, "wall_s": rep.duration_s,
+                          "fwd_adj_fraction": 0.7, "dtype": "f64"}), flush=True)
+
+    # the plain kernel run per request: one launch per step, each request alone
+    rates = []
+    for k in range(a.plain_sample):
+        r = S.draw_request(lib, rng, 10**6 + k, n_signals, tmin=a.tmin, tmax=a.tmin * 3,
+                           max_systems=a.max_systems)
+        pool = new_pool(1)
+        job = pool.submit_ids(r)
+        t0 = time.perf_counter()
+        while not pool.finished(job):
+            pool.tick(wait=True)
+        wall = time.perf_counter() - t0
+        pool.collect(job, r)
+        pool.close()
+        rates.append(r.system_steps / wall)
+    print(json.dumps({"metric": "plain kernel per request", "requests": a.plain_sample,
+                      "system_steps_per_s_mean": float(np.mean(rates)),
+                      "note": "one launch per step (steps_per_tick=1), request alone"}),
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()

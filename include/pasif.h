@@ -184,6 +184,27 @@ int pasif_pool_collect(pasif_pool* pool, pasif_driver* job, double* q, size_t ca
                        double* grad, size_t capg, size_t* leng);
 int pasif_pool_stats(const pasif_pool* pool, uint32_t* jobs, double* used_bytes,
                      double* last_kernel_ms, uint32_t* last_blocks);
+/* Slot cap: at most max_tiles tiles resident (0 = no cap; the HBM budget
+ * still applies).  A submit that would exceed it returns PASIF_ERR_MEMORY
+ * like an exhausted budget (retry after collecting).  A job is always
+ * admitted into an empty pool. */
+int pasif_pool_set_slots(pasif_pool* pool, uint32_t max_tiles);
+/* Register every system of `src` (a handle, host-only or not, whose system
+ * -- and, optionally, adjoint -- was allocated) with the pool: ids
+ * first_id .. first_id + count - 1 in the handle's system order.  Requests
+ * then name systems by id and only their excitation ids and step counts
+ * travel per request. */
+int pasif_pool_register(pasif_pool* pool, const pasif_driver* src, uint32_t* first_id,
+                        uint32_t* count);
+/* Create, pack and admit a job of n registered systems in one call: system
+ * i is ids[i] driven by bank excitation exc_of_system[i] (-1: unforced) for
+ * steps_of_system[i] steps.  On PASIF_ERR_MEMORY (no slots / budget) *job is
+ * NULL and nothing is held. */
+int pasif_pool_submit_systems(pasif_pool* pool, uint32_t n, const uint32_t* ids,
+                              const int32_t* exc_of_system, const uint32_t* steps_of_system,
+                              int adjoint, pasif_driver** job);
+/* *running = 1 while the last tick's launch is still executing */
+int pasif_pool_poll(pasif_pool* pool, int* running);
 
 /* library build string */
 const char* pasif_version(void);

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -33,17 +34,87 @@ struct Fail {
 };
 
 // ------------------------------------------------------------------ device
+// Sub-allocator of a pool's device memory: chunks taken with hipMalloc once
+// and carved into 256-byte aligned blocks (first fit over an offset-ordered,
+// coalescing free list).  Admitting or retiring a job then neither allocates
+// nor frees device memory (hipFree synchronises the whole device).  A block
+// is only handed out again after the job that held it left the pool, i.e.
+// after the last launch that could read it completed.
+struct Arena {
+  struct Chunk {
+    char* base;
+    size_t cap;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+  };
+  std::vector<Chunk> chunks;
+  size_t chunk_bytes = size_t(1) << 30;
+  size_t reserved = 0;
+  static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~size_t(255); }
+  void* take(size_t bytes) {
+    const size_t need = round(bytes);
+    for (Chunk& c : chunks)
+      for (auto it = c.free_.begin(); it != c.free_.end(); ++it)
+        if (it->second >= need) {
+          const size_t off = it->first, rest = it->second - need;
+          c.free_.erase(it);
+          if (rest) c.free_.emplace(off + need, rest);
+          return c.base + off;
+        }
+    Chunk c{};
+    c.cap = std::max(chunk_bytes, need);
+    void* base = nullptr;
+    if (hipMalloc(&base, c.cap) != hipSuccess) return nullptr;
+    c.base = static_cast<char*>(base);
+    reserved += c.cap;
+    if (c.cap > need) c.free_.emplace(need, c.cap - need);
+    chunks.push_back(std::move(c));
+    return chunks.back().base;
+  }
+  void give(void* p, size_t bytes) {
+    char* q = static_cast<char*>(p);
+    for (Chunk& c : chunks) {
+      if (q < c.base || q >= c.base + c.cap) continue;
+      size_t off = size_t(q - c.base), len = round(bytes);
+      auto nx = c.free_.lower_bound(off);
+      if (nx != c.free_.end() && off + len == nx->first) {
+        len += nx->second;
+        nx = c.free_.erase(nx);
+      }
+      if (nx != c.free_.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+          off = pv->first;
+          len += pv->second;
+          c.free_.erase(pv);
+        }
+      }
+      c.free_.emplace(off, len);
+      return;
+    }
+  }
+  ~Arena() {
+    for (Chunk& c : chunks) (void)hipFree(c.base);
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;  // block of a pool's arena (else a hipMalloc)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (arena)
+        arena->give(p, bytes);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
+    arena = nullptr;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
@@ -193,6 +264,7 @@ struct pasif_pool {
     uint64_t work, done;
     double bytes;
     bool finished;
+    uint32_t tiles;
   };
   std::vector<Job> jobs;
   // shared signal bank, uploaded once in the pool's precision
@@ -207,6 +279,18 @@ struct pasif_pool {
   bool in_flight = false;
   double last_ms = 0.0;
   uint32_t last_blocks = 0;
+  // slot cap: resident tiles (0 = only the HBM budget limits admission)
+  uint32_t max_tiles = 0, tiles_used = 0;
+  // device memory of the jobs
+  std::unique_ptr<Arena> arena;
+  // registered systems: one forward block each (local indices) and, when
+  // given, its adjoint block (factor 0 local adjoint rows, factors 1..3
+  // local forward rows)
+  struct Entry {
+    RowTerms f, a;
+    bool has_adj = false;
+  };
+  std::vector<Entry> reg;
 };
 
 namespace {
@@ -259,6 +343,16 @@ int guarded(pasif_driver* d, F&& f) {
 void dev_alloc(pasif_driver* d, DevBuf& b, size_t bytes) {
   b.release();
   if (bytes == 0) bytes = 16;
+  if (d->pool && d->pool->arena) {
+    b.p = d->pool->arena->take(bytes);
+    if (!b.p) {
+      set_err(d, "device allocation of %zu bytes for the pool failed", bytes);
+      throw Fail{PASIF_ERR_MEMORY};
+    }
+    b.arena = d->pool->arena.get();
+    b.bytes = bytes;
+    return;
+  }
   hipError_t e = hipMalloc(&b.p, bytes);
   if (e != hipSuccess) {
     b.p = nullptr;
@@ -1515,6 +1609,165 @@ void release_run(pasif_driver* d) {
   d->pk.release();
 }
 
+// One system's block of a side, re-indexed from 0: factor 0 relative to the
+// block's own rows, factors 1..3 relative to its set-point block (the forward
+// rows of the same system for the adjoint, its own rows for the forward).
+RowTerms slice_block(const RowTerms& rt, size_t s, bool adjoint) {
+  RowTerms o;
+  const uint32_t off = rt.off[s], n = rt.size[s];
+  const uint32_t xo = adjoint ? rt.xoff[s] : off;
+  o.n = n;
+  o.nset = n;
+  for (int k = 0; k < 4; ++k) {
+    const int nf = k + 1;
+    const uint32_t b = rt.ptr[k][off], e = rt.ptr[k][off + n];
+    o.ptr[k].resize(n + 1);
+    for (uint32_t r = 0; r <= n; ++r) o.ptr[k][r] = rt.ptr[k][off + r] - b;
+    o.val[k].assign(rt.val[k].begin() + b, rt.val[k].begin() + e);
+    o.idx[k].assign(rt.idx[k].begin() + size_t(b) * nf, rt.idx[k].begin() + size_t(e) * nf);
+    for (size_t t = 0; t < size_t(e - b); ++t) {
+      o.idx[k][t * nf] -= off;
+      for (int a = 1; a < nf; ++a) o.idx[k][t * nf + a] -= xo;
+    }
+  }
+  o.force.assign(rt.force.begin() + off, rt.force.begin() + off + n);
+  o.init.assign(rt.init.begin() + off, rt.init.begin() + off + n);
+  o.size = {n};
+  o.off = {0};
+  if (adjoint) o.xoff = {0};
+  return o;
+}
+
+// Concatenate re-indexed blocks into one side with the given partition; for
+// the adjoint, factors 1..3 of block i are shifted by xoff[i] (the forward
+// offset of system i).
+void concat_blocks(const std::vector<const RowTerms*>& parts, bool adjoint,
+                   const std::vector<uint32_t>& xoff, RowTerms& o) {
+  o = RowTerms{};
+  size_t n = 0, terms[4] = {0, 0, 0, 0};
+  for (const RowTerms* p : parts) {
+    n += p->n;
+    for (int k = 0; k < 4; ++k) terms[k] += p->val[k].size();
+  }
+  o.n = uint32_t(n);
+  o.size.reserve(parts.size());
+  o.off.reserve(parts.size());
+  o.force.reserve(n);
+  o.init.reserve(n);
+  for (int k = 0; k < 4; ++k) {
+    o.ptr[k].reserve(n + 1);
+    o.ptr[k].push_back(0);
+    o.val[k].reserve(terms[k]);
+    o.idx[k].reserve(terms[k] * (k + 1));
+  }
+  uint32_t off = 0;
+  for (size_t i = 0; i < parts.size(); ++i) {
+    const RowTerms& p = *parts[i];
+    const uint32_t xo = adjoint ? xoff[i] : off;
+    for (int k = 0; k < 4; ++k) {
+      const int nf = k + 1;
+      const uint32_t base = o.ptr[k].back();
+      for (uint32_t r = 1; r <= p.n; ++r) o.ptr[k].push_back(base + p.ptr[k][r]);
+      o.val[k].insert(o.val[k].end(), p.val[k].begin(), p.val[k].end());
+      const size_t first = o.idx[k].size();
+      o.idx[k].insert(o.idx[k].end(), p.idx[k].begin(), p.idx[k].end());
+      for (size_t t = first; t < o.idx[k].size(); t += nf) {
+        o.idx[k][t] += off;
+        for (int a = 1; a < nf; ++a) o.idx[k][t + a] += xo;
+      }
+    }
+    o.force.insert(o.force.end(), p.force.begin(), p.force.end());
+    o.init.insert(o.init.end(), p.init.begin(), p.init.end());
+    o.size.push_back(p.n);
+    o.off.push_back(off);
+    off += p.n;
+  }
+  if (adjoint) o.xoff = xoff;
+}
+
+// a job handle reading the pool's signal bank
+pasif_driver* new_bank_job(pasif_pool* P, uint32_t numsteps) {
+  POOL_REQUIRE(P->bank_n > 0, PASIF_ERR_STATE, "no signal bank: pasif_pool_set_bank first");
+  pasif_driver* d = new pasif_driver();
+  d->dev = P->dev;
+  d->stream = P->upload;
+  d->pool = P;
+  d->precision = P->precision;
+  d->tile_lanes = P->lanes;
+  d->slot_cfg = P->slot_cfg;
+  d->ck_every = P->ck_every;
+  d->ext_exc = P->bank.p;
+  d->ext_interp = P->bank_w ? P->bank_interp.p : nullptr;
+  d->n_exc = P->bank_n;
+  d->exc_len = P->bank_len;
+  d->sample_rate = P->bank_rate;
+  d->interp_w = P->bank_w;
+  d->interp.assign(size_t(2) * P->bank_w, 0.0);
+  d->numsteps = numsteps ? numsteps : P->bank_len;
+  d->d_exc_dirty = false;
+  return d;
+}
+
+// Pack a loaded job, allocate its run buffers from the arena and admit it
+// if the slot cap and the HBM budget allow (PASIF_ERR_MEMORY otherwise,
+// leaving the job unadmitted with no device memory).
+void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
+  for (const auto& j : P->jobs)
+    POOL_REQUIRE(j.d != d, PASIF_ERR_STATE, "job already submitted");
+  // full already: refuse before packing
+  POOL_REQUIRE(P->jobs.empty() || P->max_tiles == 0 || P->tiles_used < P->max_tiles,
+               PASIF_ERR_MEMORY, "no free slots: every tile slot of the pool is in use");
+  POOL_REQUIRE(P->jobs.empty() || P->used < P->hbm_budget, PASIF_ERR_MEMORY,
+               "HBM budget exhausted");
+  POOL_HIP(hipSetDevice(P->dev));
+  d->precision = P->precision;
+  d->tile_lanes = P->lanes;
+  d->slot_cfg = P->slot_cfg;
+  d->ck_every = P->ck_every;
+  d->window = 0;
+  uint64_t work = 0;
+  uint32_t nslots = 0;
+  int rc = guarded(d, [&] {
+    if (P->precision == 64) {
+      TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
+      d->pool_ck = a.ck_every;
+      d->pool_nck = a.nck;
+    } else {
+      TileArgs<float> a = prepare_run<float>(d, adjoint != 0, 0, &work, &nslots);
+      d->pool_ck = a.ck_every;
+      d->pool_nck = a.nck;
+    }
+    PASIF_REQUIRE(d->pk.lanes == P->lanes, PASIF_ERR_UNSUPPORTED,
+                  "a system of this request needs %u lanes, the pool runs %u", d->pk.lanes,
+                  P->lanes);
+    PASIF_REQUIRE(d->pk.slot_cfg == P->slot_cfg, PASIF_ERR_UNSUPPORTED,
+                  "slot configuration mismatch");
+  });
+  if (rc != PASIF_OK) {
+    release_run(d);
+    P->err = d->err;
+    throw Fail{rc};
+  }
+  const double bytes = job_bytes(d);
+  const uint32_t tiles = d->pk.ntiles;
+  if (!P->jobs.empty() && P->max_tiles && P->tiles_used + tiles > P->max_tiles) {
+    release_run(d);
+    P->err = "no free slots: the request needs " + std::to_string(tiles) + " tiles, " +
+             std::to_string(P->max_tiles - P->tiles_used) + " are free";
+    throw Fail{PASIF_ERR_MEMORY};
+  }
+  if (P->used + bytes > P->hbm_budget) {
+    release_run(d);
+    P->err = "HBM budget exhausted: the request needs " + std::to_string(bytes) +
+             " bytes, " + std::to_string(P->hbm_budget - P->used) + " are free";
+    throw Fail{PASIF_ERR_MEMORY};
+  }
+  POOL_HIP(hipEventRecord(P->uploaded, P->upload));
+  P->used += bytes;
+  P->tiles_used += tiles;
+  P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false, tiles});
+}
+
 template <typename R>
 void pool_tick_impl(pasif_pool* P) {
   std::vector<TileArgs<R>> table;
@@ -1646,6 +1899,8 @@ int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile
     P->ck_every = checkpoint_every ? checkpoint_every : 256;
     P->budget = steps_per_tick;
     P->hbm_budget = hbm_budget_bytes;
+    P->arena.reset(new Arena());
+    P->arena->chunk_bytes = size_t(std::min(hbm_budget_bytes, double(size_t(1) << 30)));
   });
   *out = holder.release();
   return rc;
@@ -1658,6 +1913,10 @@ int pasif_pool_job_create(pasif_pool* P, pasif_driver** job, const double* excit
                           uint32_t numsteps) {
   return pool_guarded(P, [&] {
     POOL_REQUIRE(job != nullptr, PASIF_ERR_ARGUMENT, "null job pointer");
+    if (excitations == nullptr) {  // the pool's signal bank: excitation ids index it
+      *job = new_bank_job(P, numsteps);
+      return;
+    }
     std::unique_ptr<pasif_driver> d(new pasif_driver());
     d->dev = P->dev;
     d->stream = P->upload;
@@ -1666,19 +1925,7 @@ int pasif_pool_job_create(pasif_pool* P, pasif_driver** job, const double* excit
     d->tile_lanes = P->lanes;
     d->slot_cfg = P->slot_cfg;
     d->ck_every = P->ck_every;
-    if (excitations == nullptr) {
-      // use the pool's signal bank: excitation ids index the bank
-      POOL_REQUIRE(P->bank_n > 0, PASIF_ERR_STATE, "no signal bank: pasif_pool_set_bank first");
-      d->ext_exc = P->bank.p;
-      d->ext_interp = P->bank_w ? P->bank_interp.p : nullptr;
-      d->n_exc = P->bank_n;
-      d->exc_len = P->bank_len;
-      d->sample_rate = P->bank_rate;
-      d->interp_w = P->bank_w;
-      d->interp.assign(size_t(2) * P->bank_w, 0.0);
-      d->numsteps = numsteps ? numsteps : P->bank_len;
-      d->d_exc_dirty = false;
-    } else {
+    {
       const int rc = guarded(d.get(), [&] {
         pasif_driver* dd = d.get();
         load_exc(dd, excitations, n_exc, exc_len, sample_rate);
@@ -1734,47 +1981,90 @@ int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc
 int pasif_pool_submit(pasif_pool* P, pasif_driver* d, int adjoint) {
   return pool_guarded(P, [&] {
     POOL_REQUIRE(d && d->pool == P, PASIF_ERR_ARGUMENT, "job does not belong to this pool");
-    for (const auto& j : P->jobs)
-      POOL_REQUIRE(j.d != d, PASIF_ERR_STATE, "job already submitted");
-    POOL_HIP(hipSetDevice(P->dev));
-    d->precision = P->precision;
-    d->tile_lanes = P->lanes;
-    d->slot_cfg = P->slot_cfg;
-    d->ck_every = P->ck_every;
-    d->window = 0;
-    uint64_t work = 0;
-    uint32_t nslots = 0;
-    int rc = guarded(d, [&] {
-      if (P->precision == 64) {
-        TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
-        d->pool_ck = a.ck_every;
-        d->pool_nck = a.nck;
-      } else {
-        TileArgs<float> a = prepare_run<float>(d, adjoint != 0, 0, &work, &nslots);
-        d->pool_ck = a.ck_every;
-        d->pool_nck = a.nck;
+    admit_job(P, d, adjoint);
+  });
+}
+
+int pasif_pool_set_slots(pasif_pool* P, uint32_t max_tiles) {
+  return pool_guarded(P, [&] { P->max_tiles = max_tiles; });
+}
+
+int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_id,
+                        uint32_t* count) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(src && first_id && count, PASIF_ERR_ARGUMENT, "null argument");
+    POOL_REQUIRE(src->sys_ready, PASIF_ERR_STATE,
+                 "the source handle's system must be allocated (pasif_allocate_system)");
+    const size_t S = src->fwd.size.size();
+    POOL_REQUIRE(P->reg.size() + S < (size_t(1) << 31), PASIF_ERR_ARGUMENT, "registry full");
+    const size_t first = P->reg.size();
+    P->reg.reserve(first + S);
+    for (size_t s = 0; s < S; ++s) {
+      pasif_pool::Entry e;
+      e.f = slice_block(src->fwd, s, false);
+      if (src->adj_ready) {
+        e.a = slice_block(src->adj, s, true);
+        e.a.nset = e.f.n;
+        e.has_adj = true;
       }
-      PASIF_REQUIRE(d->pk.lanes == P->lanes, PASIF_ERR_UNSUPPORTED,
-                    "a system of this request needs %u lanes, the pool runs %u", d->pk.lanes,
-                    P->lanes);
-      PASIF_REQUIRE(d->pk.slot_cfg == P->slot_cfg, PASIF_ERR_UNSUPPORTED,
-                    "slot configuration mismatch");
-    });
-    if (rc != PASIF_OK) {
-      release_run(d);
-      P->err = d->err;
-      throw Fail{rc};
+      P->reg.push_back(std::move(e));
     }
-    const double bytes = job_bytes(d);
-    if (P->used + bytes > P->hbm_budget) {
-      release_run(d);
-      P->err = "HBM budget exhausted: the request needs " + std::to_string(bytes) +
-               " bytes, " + std::to_string(P->hbm_budget - P->used) + " are free";
-      throw Fail{PASIF_ERR_MEMORY};
+    *first_id = uint32_t(first);
+    *count = uint32_t(S);
+  });
+}
+
+int pasif_pool_submit_systems(pasif_pool* P, uint32_t n, const uint32_t* ids,
+                              const int32_t* exc_of_system, const uint32_t* steps_of_system,
+                              int adjoint, pasif_driver** job) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(job != nullptr, PASIF_ERR_ARGUMENT, "null job pointer");
+    *job = nullptr;
+    POOL_REQUIRE(n > 0 && ids && exc_of_system && steps_of_system, PASIF_ERR_ARGUMENT,
+                 "a request needs at least one system with its excitation and step count");
+    POOL_REQUIRE(P->bank_n > 0, PASIF_ERR_STATE, "no signal bank: pasif_pool_set_bank first");
+    for (uint32_t i = 0; i < n; ++i) {
+      POOL_REQUIRE(ids[i] < P->reg.size(), PASIF_ERR_ARGUMENT,
+                   "system id " + std::to_string(ids[i]) + " is not registered");
+      POOL_REQUIRE(!adjoint || P->reg[ids[i]].has_adj, PASIF_ERR_ARGUMENT,
+                   "system id " + std::to_string(ids[i]) + " has no registered adjoint");
+      POOL_REQUIRE(exc_of_system[i] >= -1 && exc_of_system[i] < int32_t(P->bank_n),
+                   PASIF_ERR_ARGUMENT,
+                   "excitation " + std::to_string(exc_of_system[i]) + " not in the bank");
+      POOL_REQUIRE(steps_of_system[i] > 0, PASIF_ERR_ARGUMENT, "step counts must be positive");
     }
-    POOL_HIP(hipEventRecord(P->uploaded, P->upload));
-    P->used += bytes;
-    P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false});
+    std::unique_ptr<pasif_driver> d(new_bank_job(P, 0));
+    std::vector<const RowTerms*> parts(n);
+    for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].f;
+    concat_blocks(parts, false, {}, d->fwd);
+    d->fwd.nset = d->fwd.n;
+    if (adjoint) {
+      for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].a;
+      concat_blocks(parts, true, d->fwd.off, d->adj);
+      d->adj.nset = d->fwd.n;
+    }
+    d->mapped = true;
+    d->map_exc.assign(exc_of_system, exc_of_system + n);
+    d->map_steps.assign(steps_of_system, steps_of_system + n);
+    d->layout_systems = n;
+    d->sys_ready = true;
+    d->adj_ready = adjoint != 0;
+    admit_job(P, d.get(), adjoint);
+    *job = d.release();
+  });
+}
+
+int pasif_pool_poll(pasif_pool* P, int* running) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(running != nullptr, PASIF_ERR_ARGUMENT, "null output");
+    *running = 0;
+    if (!P->in_flight) return;
+    const hipError_t e = hipEventQuery(P->ev1);
+    if (e == hipErrorNotReady) {
+      *running = 1;
+      return;
+    }
+    POOL_HIP(e);
   });
 }
 
@@ -1861,6 +2151,7 @@ int pasif_pool_collect(pasif_pool* P, pasif_driver* d, double* q, size_t capq, s
     d->d_interp.release();
     d->d_exc_dirty = true;
     P->used -= j.bytes;
+    P->tiles_used -= j.tiles;
     P->jobs.erase(P->jobs.begin() + long(k));
   });
 }
@@ -1887,6 +2178,7 @@ int pasif_pool_destroy(pasif_pool* P) {
     delete j.d;
   }
   P->jobs.clear();
+  P->arena.reset();
   if (P->table_dev) (void)hipFree(P->table_dev);
   if (P->map_dev) (void)hipFree(P->map_dev);
   if (P->ev0) (void)hipEventDestroy(P->ev0);

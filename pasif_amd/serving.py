@@ -33,6 +33,7 @@ from .PASIfgpu import __GpuDriver as _Driver
 
 _u32p = C.POINTER(C.c_uint32)
 _f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
 
 _POOL_SIGS = {
     "pasif_pool_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint32,
@@ -51,6 +52,11 @@ _POOL_SIGS = {
                                      C.POINTER(C.c_size_t), _f64p, C.c_size_t,
                                      C.POINTER(C.c_size_t)]),
     "pasif_pool_stats": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _u32p]),
+    "pasif_pool_set_slots": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pasif_pool_register": (C.c_int, [C.c_void_p, C.c_void_p, _u32p, _u32p]),
+    "pasif_pool_submit_systems": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _i32p, _u32p,
+                                            C.c_int, C.POINTER(C.c_void_p)]),
+    "pasif_pool_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
 }
 
 
@@ -83,6 +89,7 @@ class SystemArrays:
     alam: Optional[tuple] = None
     apsi: Optional[tuple] = None
     lamT: Optional[np.ndarray] = None
+    key: int = -1         # position in SystemLibrary.flat
 
 
 def _tens(t, order):
@@ -133,6 +140,10 @@ class SystemLibrary:
                     e.lamT = ra["q0"]
                 lst.append(e)
             self.entries[int(d)] = lst
+        self.flat: List[SystemArrays] = [e for d in sorted(self.entries)
+                                         for e in self.entries[d]]
+        for k, e in enumerate(self.flat):
+            e.key = k
 
 
 def _cat_csr(parts, offs):
@@ -165,6 +176,7 @@ class Request:
     done: float = 0.0
     result: Optional[np.ndarray] = None
     grad: Optional[np.ndarray] = None
+    keys: Optional[np.ndarray] = None     # library positions of the systems
 
     @property
     def system_steps(self) -> int:
@@ -180,7 +192,8 @@ def draw_request(lib: SystemLibrary, rng, rid: int, n_signals: int, tmin=1e3, tm
     systems = [lib.entries[int(d)][int(rng.integers(len(lib.entries[int(d)])))] for d in dofs]
     steps = np.exp(rng.uniform(np.log(tmin), np.log(tmax), n)).astype(np.int64)
     return Request(rid, systems, rng.integers(0, n_signals, n), steps,
-                   bool(rng.uniform() < adjoint_frac))
+                   bool(rng.uniform() < adjoint_frac),
+                   keys=np.array([s.key for s in systems], dtype=np.int64))
 
 
 # --------------------------------------------------------------- pool
@@ -191,6 +204,7 @@ class Pool:
                  steps_per_tick: int = 2000, hbm_budget: float = 200e9, precision: int = 64,
                  gpu: int = 0):
         self.lib = _load()
+        self.ids = None
         h = C.c_void_p()
         rc = self.lib.pasif_pool_create(C.byref(h), gpu, precision, lanes, slot_config,
                                         checkpoint_every, steps_per_tick, float(hbm_budget))
@@ -233,11 +247,9 @@ class Pool:
         d._layout = {"n": 0, "sizes_f": None, "sizes_a": None, "exc": None, "steps": None}
         return d
 
-    def submit(self, req: Request):
-        """Load and admit a request; returns the job or None when the HBM
-        budget is exhausted (retry after collecting)."""
-        d = self._job()
-        sy = req.systems
+    @staticmethod
+    def _load_systems(d, sy: Sequence[SystemArrays], adjoint: bool):
+        """Concatenate systems' arrays into driver handle ``d`` (block-diagonal)."""
         sizes = np.array([s.n for s in sy], dtype=np.int64)
         offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
         n = int(sizes.sum())
@@ -251,7 +263,7 @@ class Pool:
         d._setFwdForcePattern(np.concatenate([s.force for s in sy]))
         d._setFwdInitialConditions(np.concatenate([s.q0 for s in sy]))
         sizes_a = None
-        if req.adjoint:
+        if adjoint:
             sizes_a = np.array([s.na for s in sy], dtype=np.int64)
             aoffs = np.concatenate([[0], np.cumsum(sizes_a)[:-1]])
             na = int(sizes_a.sum())
@@ -265,18 +277,73 @@ class Pool:
             d._setBwdPsi((na, na, n, n, n), pv, pi.reshape(-1))
             d._setBwdForcePattern(np.zeros(na))
             d._setBwdInitialConditions(np.concatenate([s.lamT for s in sy]))
+        return sizes, sizes_a
+
+    def submit(self, req: Request):
+        """Load and admit a request with its full system arrays; returns the
+        job or None when the slots or the HBM budget are exhausted (retry
+        after collecting)."""
+        d = self._job()
+        sizes, sizes_a = self._load_systems(d, req.systems, req.adjoint)
         d._setSystemsLayout(sizes_fwd=sizes, sizes_adj=sizes_a, exc_of_system=req.exc_ids,
                             steps_of_system=req.steps)
         d._allocateSystemOnDevice()
         if req.adjoint:
             d._allocateAdjointOnDevice()
         rc = self.lib.pasif_pool_submit(self.h, d._h, int(req.adjoint))
-        if rc == 4:          # HBM budget: keep queued
+        if rc == 4:          # slots / HBM budget: keep queued
             self.lib.pasif_destroy(d._h)
             d._h = None
             return None
         self._chk(rc)
         return d
+
+    def register_library(self, lib: SystemLibrary):
+        """Register every library system (with its adjoint) with the pool, so
+        requests of library systems travel as ids (``submit_ids``)."""
+        host = _Driver([np.zeros(2)], 1, 1, GPUId_=-1)      # host-only handle
+        sizes, sizes_a = self._load_systems(host, lib.flat, True)
+        host._setSystemsLayout(sizes_fwd=sizes, sizes_adj=sizes_a)
+        host._allocateSystemOnDevice()
+        host._allocateAdjointOnDevice()
+        first, count = C.c_uint32(0), C.c_uint32(0)
+        try:
+            self._chk(self.lib.pasif_pool_register(self.h, host._h, C.byref(first),
+                                                   C.byref(count)))
+        finally:
+            self.lib.pasif_destroy(host._h)
+            host._h = None
+        if count.value != len(lib.flat):
+            raise RuntimeError(f"registered {count.value} systems, the library has "
+                               f"{len(lib.flat)}")
+        self.ids = first.value + np.arange(count.value, dtype=np.uint32)
+
+    def submit_ids(self, req: Request):
+        """Admit a request of registered library systems: one C call carries
+        the system ids, excitation ids and step counts.  None when the slots
+        or the HBM budget are exhausted."""
+        ids = np.ascontiguousarray(self.ids[req.keys], dtype=np.uint32)
+        exc = np.ascontiguousarray(req.exc_ids, dtype=np.int32)
+        steps = np.ascontiguousarray(req.steps, dtype=np.uint32)
+        j = C.c_void_p()
+        rc = self.lib.pasif_pool_submit_systems(
+            self.h, len(ids), ids.ctypes.data_as(_u32p), exc.ctypes.data_as(_i32p),
+            steps.ctypes.data_as(_u32p), int(req.adjoint), C.byref(j))
+        if rc == 4:
+            return None
+        self._chk(rc)
+        d = object.__new__(_Driver)
+        d._pl = self.lib
+        d._h = j
+        return d
+
+    def set_slots(self, max_tiles: int):
+        self._chk(self.lib.pasif_pool_set_slots(self.h, int(max_tiles)))
+
+    def running(self) -> bool:
+        r = C.c_int(0)
+        self._chk(self.lib.pasif_pool_poll(self.h, C.byref(r)))
+        return bool(r.value)
 
     def tick(self, wait: bool) -> int:
         n = C.c_uint32(0)
@@ -328,6 +395,9 @@ class ServeReport:
     ticks: int
     gpu_busy_frac: float
     latencies_ms: List[float] = field(default_factory=list)
+    backlog: int = 0
+    p99_all_ms: float = 0.0
+    admit_ms_mean: float = 0.0    # host time per admitted request (packing, upload)
 
 
 class SolveServer:
@@ -336,33 +406,61 @@ class SolveServer:
         self.inflight = []          # (job, request)
 
     def run(self, requests: Sequence[Request], horizon_s: Optional[float] = None,
-            drain: bool = True, max_wall_s: float = 600.0) -> ServeReport:
+            drain: bool = True, max_wall_s: float = 600.0,
+            arrivals: Optional[np.ndarray] = None) -> ServeReport:
         """Serve ``requests`` (arrival times relative to start).  Requests that
         arrive while a tick runs are admitted during it (host work overlaps the
-        launch) and join at the next tick."""
-        reqs = sorted(requests, key=lambda r: r.arrival)
-        queue = deque()
+        launch) and join at the next tick.
+
+        With ``arrivals`` (sorted arrival times) ``requests`` may be a lazy
+        sequence: ``requests[k]`` is only touched when request ``k`` is
+        submitted, so an overloaded stream does not build requests the pool
+        never reaches.  Latency percentiles cover completed requests; requests
+        still queued or in flight at the end are reported as ``backlog`` with
+        their waiting time so far in ``p99_all_ms`` (a lower bound)."""
+        if arrivals is None:
+            reqs = sorted(requests, key=lambda r: r.arrival)
+            arr = np.array([r.arrival for r in reqs], dtype=np.float64)
+        else:
+            reqs = requests
+            arr = np.asarray(arrivals, dtype=np.float64)
+        n_req = len(arr)
+        queue = deque()             # indices of arrived, not yet admitted requests
         done: List[Request] = []
         i = 0
         t0 = time.perf_counter()
         ticks = 0
         busy_ms = 0.0
         admitted = 0
+        blocked = False             # a submit was refused; retry after a collect
+        admit_s = 0.0
+        by_id = self.pool.ids is not None
 
-        def admit(now):
-            nonlocal i, admitted
-            while i < len(reqs) and reqs[i].arrival <= now:
-                queue.append(reqs[i])
+        def arrivals_until(now):
+            nonlocal i
+            while i < n_req and arr[i] <= now:
+                queue.append(i)
                 i += 1
-            while queue:
-                r = queue[0]
-                job = self.pool.submit(r)
-                if job is None:
-                    break
-                r.admitted = time.perf_counter() - t0
-                self.inflight.append((job, r))
-                admitted += 1
-                queue.popleft()
+
+        def admit_one() -> bool:
+            """Admit the oldest queued request; False if none or refused."""
+            nonlocal admitted, blocked, admit_s
+            if blocked or not queue:
+                return False
+            ta = time.perf_counter()
+            r = reqs[queue[0]]
+            r.arrival = float(arr[queue[0]])
+            job = (self.pool.submit_ids(r) if by_id and r.keys is not None
+                   else self.pool.submit(r))
+            admit_s += time.perf_counter() - ta
+            if job is None:
+                blocked = True
+                return False
+            r.admitted = time.perf_counter() - t0
+            self.inflight.append((job, r))
+            admitted += 1
+            queue.popleft()
+            return True
 
         while True:
             now = time.perf_counter() - t0
@@ -370,11 +468,18 @@ class SolveServer:
                 break
             if horizon_s is not None and now > horizon_s and not drain:
                 break
-            admit(now)
+            arrivals_until(now)
+            if not self.inflight:
+                while admit_one():
+                    pass
             if self.inflight:
                 self.pool.tick(wait=False)
                 ticks += 1
-                admit(time.perf_counter() - t0)    # overlaps the running launch
+                # admit while the launch runs; new jobs join the next tick
+                while self.pool.running():
+                    arrivals_until(time.perf_counter() - t0)
+                    if not admit_one():
+                        break
                 busy_ms += self.pool.wait()
                 keep = []
                 for job, r in self.inflight:
@@ -382,23 +487,50 @@ class SolveServer:
                         r.result, r.grad = self.pool.collect(job, r)
                         r.done = time.perf_counter() - t0
                         done.append(r)
+                        blocked = False
                     else:
                         keep.append((job, r))
                 self.inflight = keep
-            elif i >= len(reqs) and not queue:
+                # fill freed slots before the next launch
+                arrivals_until(time.perf_counter() - t0)
+                while admit_one():
+                    if not queue:
+                        break
+            elif i >= n_req and not queue:
                 break
             else:
-                nxt = reqs[i].arrival if i < len(reqs) else now
+                nxt = arr[i] if i < n_req else now
                 time.sleep(max(0.0, min(0.01, nxt - now)))
         wall = time.perf_counter() - t0
         lat = np.array([(r.done - r.arrival) * 1e3 for r in done]) if done else np.zeros(1)
+        waiting = [(wall - r.arrival) * 1e3 for _, r in self.inflight] + \
+            [(wall - arr[k]) * 1e3 for k in queue]
+        lat_all = np.concatenate([lat, np.array(waiting)]) if waiting else lat
         steps = sum(r.system_steps for r in done)
         return ServeReport(
-            offered_rps=len(reqs) / max(reqs[-1].arrival, 1e-9) if reqs else 0.0,
+            offered_rps=n_req / max(float(arr[-1]), 1e-9) if n_req else 0.0,
             duration_s=wall, completed=len(done), admitted=admitted, arrived=i,
             p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
             mean_ms=float(lat.mean()), system_steps_per_s=steps / wall, ticks=ticks,
-            gpu_busy_frac=busy_ms / 1e3 / wall, latencies_ms=list(lat))
+            gpu_busy_frac=busy_ms / 1e3 / wall, latencies_ms=list(lat),
+            backlog=len(waiting), p99_all_ms=float(np.percentile(lat_all, 99)),
+            admit_ms_mean=admit_s * 1e3 / max(admitted, 1))
+
+
+class LazyRequests:
+    """Indexable stream of random requests, each drawn on first access from
+    its own seed (``seed``, index), so the k-th request does not depend on
+    which others were drawn."""
+
+    def __init__(self, lib: SystemLibrary, n: int, seed: int, n_signals: int, **kw):
+        self.lib, self.n, self.seed, self.n_signals, self.kw = lib, n, seed, n_signals, kw
+
+    def __len__(self):
+        return self.n
+
+    def __getitem__(self, k: int) -> Request:
+        rng = np.random.default_rng((self.seed, int(k)))
+        return draw_request(self.lib, rng, int(k), self.n_signals, **self.kw)
 
 
 def poisson_arrivals(rng, rate: float, duration: float) -> np.ndarray:

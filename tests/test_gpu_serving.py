@@ -133,3 +133,56 @@ def test_server_drains_poisson_trace():
     rep = S.SolveServer(pool).run(reqs, max_wall_s=120)
     assert rep.completed == len(reqs)
     assert rep.p99_ms >= rep.p50_ms > 0
+
+
+def test_registered_ids_equal_full_arrays_under_slot_cap():
+    """Requests submitted as registered system ids give bitwise the results
+    of the same requests submitted with their full arrays; a slot cap refuses
+    admission until jobs are collected, and device blocks freed by collected
+    jobs are reused by later ones without changing results."""
+    print("[gpu case] serving registry + slots", flush=True)
+    rng = np.random.default_rng(44)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 8, 16), variants=2)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    from pasif_amd.workloads import linear_interp_matrix
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=256, slot_config=1, steps_per_tick=300, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        return pool
+
+    reqs = [S.draw_request(lib, rng, i, 6, tmin=200, tmax=1500, max_systems=24)
+            for i in range(8)]
+    ref = {}
+    for r in reqs:
+        ref[r.rid] = _run_alone(new_pool(), r)
+    pool = new_pool()
+    pool.register_library(lib)
+    pool.set_slots(1)                      # one tile: every extra request waits
+    first = pool.submit_ids(reqs[0])
+    assert first is not None               # an empty pool always admits
+    assert pool.submit_ids(reqs[1]) is None
+    jobs = [(first, reqs[0])]
+    pending = list(reqs[1:])
+    got = {}
+    while jobs or pending:
+        while pending:
+            j = pool.submit_ids(pending[0])
+            if j is None:
+                break
+            jobs.append((j, pending.pop(0)))
+        pool.tick(wait=True)
+        keep = []
+        for j, r in jobs:
+            if pool.finished(j):
+                got[r.rid] = pool.collect(j, r)
+            else:
+                keep.append((j, r))
+        jobs = keep
+    for r in reqs:
+        np.testing.assert_array_equal(got[r.rid][0], ref[r.rid][0])
+        if r.adjoint:
+            np.testing.assert_array_equal(got[r.rid][1], ref[r.rid][1])
+    st = pool.stats()
+    assert st["jobs"] == 0 and st["used_bytes"] == 0
