@@ -40,6 +40,7 @@ enum {
 };
 
 enum { PASIF_SOLVER_PERSISTENT = 0, PASIF_SOLVER_PLAIN = 1, PASIF_SOLVER_GRAPH = 2 };
+enum { PASIF_KERNEL_AUTO = 0, PASIF_KERNEL_TILE = 1, PASIF_KERNEL_LANE = 2 };
 
 /* __GpuDriver(excitationSet_, sampleRate_, numsteps_, dCompute_, dSystem_,
  * dSolver_, GPUId_)  (src/__GpuDriver.cu:25-157).  excitations is
@@ -147,6 +148,24 @@ int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* l
  * stream), number of launches and number of tiles, lanes per tile */
 int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launches,
                       uint32_t* tiles, uint32_t* lanes);
+/* Kernel family.  PASIF_KERNEL_TILE: rows of packed systems across the lanes
+ * of a workgroup, stage values exchanged through LDS (any system size).
+ * PASIF_KERNEL_LANE: one lane per (system, excitation) with the state and
+ * coefficients in registers, code generated per sparsity structure and
+ * compiled at the first solve (small systems: PASIF_ERR_UNSUPPORTED when a
+ * system does not fit; at most 8 distinct structures).  PASIF_KERNEL_AUTO
+ * (default): the lane kernel when every system fits it, else tiles. */
+int pasif_set_kernel(pasif_driver* d, int kernel);
+/* what the packed problem runs on: kernel (TILE or LANE), structure classes
+ * (lane kernel; 0 for tiles) and the checkpoint interval the adjoint uses
+ * (the lane kernel shortens it until a segment fits its LDS) */
+int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* classes,
+                      uint32_t* checkpoint_every);
+/* Diagnostics: the generated HIP source of the lane kernel for the loaded
+ * problem (NUL-terminated; out == NULL: length query), and with compile != 0
+ * its hipRTC compilation for gfx950, which needs no GPU. */
+int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_t* len,
+                      int compile);
 /* ---------------- continuous-batching pool (extension) ------------------
  * A pool keeps many requests ("jobs") resident on one GPU.  Each job is a
  * driver handle created by pasif_pool_job_create and loaded with the usual

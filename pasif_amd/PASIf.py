@@ -402,6 +402,16 @@ class PASIf(_GpuDriver):
     def setTuning(self, tile_lanes: int = 0, slot_config: int = 0, checkpoint_every: int = 0):
         self._setTuning(tile_lanes, slot_config, checkpoint_every)
 
+    def setKernel(self, kernel: str = "auto"):
+        """"tile" (any system size), "lane" (one lane per system, state in
+        registers; small systems) or "auto" (lane when every system fits)."""
+        self._setKernel(kernel)
+
+    def kernelInfo(self, adjoint: bool = False) -> dict:
+        """Kernel the loaded problem runs on, its structure classes and the
+        checkpoint interval of the adjoint."""
+        return self._kernelInfo(adjoint)
+
     def getMaxAmplitudes(self, window: int):
         """Final states and ``max |q_m|`` over the post-step states of the last
         ``window`` steps of every DOF, from one fused solve."""

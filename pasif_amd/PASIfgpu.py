@@ -228,6 +228,30 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
     def _getMaxAmplitudes(self) -> np.ndarray:
         return self._get(self._pl.pasif_get_max_amplitudes)
 
+    _KERNELS = {"auto": 0, "tile": 1, "lane": 2}
+
+    def _setKernel(self, kernel: str = "auto"):
+        if kernel not in self._KERNELS:
+            raise ValueError(f"kernel must be one of {sorted(self._KERNELS)}")
+        self._chk(self._pl.pasif_set_kernel(self._h, self._KERNELS[kernel]))
+
+    def _kernelInfo(self, adjoint: bool = False) -> dict:
+        k, ncls, ck = C.c_int(0), C.c_uint32(0), C.c_uint32(0)
+        self._chk(self._pl.pasif_kernel_info(self._h, int(adjoint), C.byref(k), C.byref(ncls),
+                                             C.byref(ck)))
+        name = {1: "tile", 2: "lane"}.get(k.value, "?")
+        return {"kernel": name, "classes": ncls.value, "checkpoint_every": ck.value}
+
+    def _laneSource(self, adjoint: bool = False, compile: bool = False) -> str:
+        """Generated HIP source of the lane kernel (optionally compiled with
+        hipRTC for gfx950, which needs no GPU)."""
+        n = C.c_size_t(0)
+        self._chk(self._pl.pasif_lane_source(self._h, int(adjoint), None, 0, C.byref(n), 0))
+        buf = C.create_string_buffer(n.value)
+        self._chk(self._pl.pasif_lane_source(self._h, int(adjoint), buf, n.value, C.byref(n),
+                                             int(compile)))
+        return buf.value.decode()
+
     def _lastTiming(self) -> dict:
         ms = C.c_double(0)
         launches, tiles, lanes = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)

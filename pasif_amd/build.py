@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpasif_hip.so")
-SOURCES = ["pasif_kernels.hip", "pasif_driver.cpp"]
+SOURCES = ["pasif_kernels.hip", "pasif_driver.cpp", "pasif_lane.cpp"]
 ARCH = os.environ.get("PASIF_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 VALU ops are disabled for all device code.
@@ -35,7 +35,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + srcs + ["-o", tmp]
+    cmd = [hipcc()] + HIPCC_FLAGS + srcs + ["-o", tmp, "-lhiprtc"]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)

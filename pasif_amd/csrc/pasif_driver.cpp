@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -23,6 +24,8 @@
 #include <vector>
 
 #include "../../include/pasif.h"
+#include "pasif_host.hpp"
+#include "pasif_lane.hpp"
 #include "pasif_tiles.hpp"
 
 using namespace pasif;
@@ -146,36 +149,27 @@ struct SideIn {
   }
 };
 
-// Per-row term lists of one side (forward or adjoint) in global indices:
-// kind k terms of row r are [ptr[k][r], ptr[k][r+1]) of val[k] / idx[k]
-// (idx[k] holds k index words per term: S, X1, X2, X3), in the order the
-// reference applies them (CSR order, then COO order).
-struct RowTerms {
-  uint32_t n = 0;      // rows
-  uint32_t nset = 0;   // size of the set-point index space
-  std::array<std::vector<uint32_t>, 4> ptr;
-  std::array<std::vector<double>, 4> val;
-  std::array<std::vector<uint32_t>, 4> idx;
-  std::vector<double> force;  // dense, eps-dropped at pack time
-  std::vector<double> init;
-  std::vector<uint32_t> off, size;   // system partition (rows)
-  std::vector<uint32_t> xoff;        // adjoint: forward offset of each system
-};
-
-// one (system, excitation) pair
-struct Instance {
-  uint32_t sys;
-  int32_t exc;         // -1: unforced
-  uint32_t steps;
-  uint32_t out_f;      // output base of forward rows
-  uint32_t out_a;      // output base of adjoint rows
-  uint32_t lanes;      // lanes the instance occupies in a tile
+// One structure class of the lane kernel: its instances (one per lane,
+// sorted by step count, descending) and their device buffers.
+struct LaneCls {
+  uint32_t count = 0, stride = 0, waves = 0, block_begin = 0, tmax = 0;
+  uint32_t nf = 0, na = 0;
+  DevBuf coef_f, coef_a, init_f, init_a, steps, exc, out_f, out_a;  // program
+  DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;            // run
 };
 
 struct Packed {
   bool valid = false;
   int precision = 0;
   bool adjoint = false;
+  // lane kernel instead of tiles
+  bool lane = false;
+  std::vector<std::unique_ptr<LaneCls>> lcls;
+  std::vector<LaneShape> lshapes;
+  LaneKernel lkern;
+  uint32_t lseg = 1, lblocks = 0;
+  size_t llds = 0;
+  DevBuf largs;
   uint32_t lanes = 0, ntiles = 0, slot_cfg = 0, tile_target = 0;
   uint64_t key = 0;
   uint32_t out_len_f = 0, out_len_a = 0, tmax = 0;
@@ -189,6 +183,10 @@ struct Packed {
                      &f_force, &f_val,   &f_idx,  &f_init,  &a_info,  &a_out, &a_src,
                      &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init};
     for (DevBuf* b : all) b->release();
+    lcls.clear();
+    lshapes.clear();
+    largs.release();
+    lane = false;
     valid = false;
   }
 };
@@ -223,6 +221,7 @@ struct pasif_driver {
   int precision = 64;
   int solver = PASIF_SOLVER_PERSISTENT;
   uint32_t tile_lanes = 0, slot_cfg = 0, ck_every = 0, window = 0;
+  int kernel_path = PASIF_KERNEL_AUTO;
   uint32_t graph_slice = 256, graph_launches = 16;
 
   Packed pk;
@@ -921,7 +920,220 @@ void pack_impl(pasif_driver* d, bool adjoint) {
 
 uint64_t pack_key(pasif_driver* d, bool adjoint) {
   return (uint64_t(d->precision) << 1) ^ (uint64_t(adjoint) << 8) ^ (uint64_t(d->tile_lanes) << 10) ^
-         (uint64_t(d->slot_cfg) << 24);
+         (uint64_t(d->slot_cfg) << 24) ^ (uint64_t(d->kernel_path) << 28) ^
+         (uint64_t(d->ck_every) << 32);
+}
+
+// ------------------------------------------------------------ lane plan
+// Largest state + coefficient count (values of the working type) a class may
+// keep live: 2 VGPRs per FP64 value, leaving room for temporaries in 256.
+constexpr uint32_t kLaneMaxLive = 96;
+uint32_t lane_max_live() {
+  if (const char* e = std::getenv("PASIF_LANE_MAX_LIVE")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v > 0 && v < 100000) return uint32_t(v);
+  }
+  return kLaneMaxLive;
+}
+constexpr uint32_t kLaneMaxClasses = 8;
+constexpr size_t kLaneMaxLds = 64 * 1024;
+
+// Structure of system s: its terms in coefficient order with local factor
+// rows, and (forward) the rows with a force above the working type's eps.
+LaneSide lane_side(const RowTerms& rt, uint32_t s, bool adjoint, double eps) {
+  LaneSide sd;
+  const uint32_t off = rt.off[s], n = rt.size[s];
+  const uint32_t xo = adjoint ? rt.xoff[s] : off;
+  sd.n = n;
+  sd.rows.resize(n);
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t g = off + r;
+    for (int k = 0; k < 4; ++k) {
+      const int nf = k + 1;
+      for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e) {
+        LaneTerm t{};
+        t.kind = uint32_t(k);
+        t.f[0] = rt.idx[k][size_t(e) * nf] - off;
+        for (int a = 1; a < nf; ++a) t.f[a] = rt.idx[k][size_t(e) * nf + a] - xo;
+        sd.rows[r].push_back(t);
+      }
+    }
+    if (!adjoint && std::fabs(rt.force[g]) > eps) sd.forced.push_back(r);
+  }
+  return sd;
+}
+
+void shape_key(const LaneSide& sd, std::vector<uint32_t>& key) {
+  key.push_back(sd.n);
+  for (const auto& row : sd.rows) {
+    key.push_back(uint32_t(row.size()));
+    for (const LaneTerm& t : row) {
+      key.push_back(t.kind);
+      for (uint32_t a = 0; a <= t.kind; ++a) key.push_back(t.f[a]);
+    }
+  }
+  key.push_back(uint32_t(sd.forced.size()));
+  for (uint32_t r : sd.forced) key.push_back(r);
+}
+
+// Coefficients of system s in the class's slot order (terms row by row,
+// kinds ascending, then the force of each forced row).
+void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, std::vector<double>& out) {
+  out.clear();
+  const uint32_t off = rt.off[s];
+  for (uint32_t r = 0; r < sd.n; ++r)
+    for (int k = 0; k < 4; ++k)
+      for (uint32_t e = rt.ptr[k][off + r]; e < rt.ptr[k][off + r + 1]; ++e)
+        out.push_back(rt.val[k][e]);
+  for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
+}
+
+// Plan the lane kernel for the loaded problem: true if every system fits a
+// lane and the systems form at most kLaneMaxClasses structure classes.
+template <typename R>
+bool lane_plan(pasif_driver* d, bool adjoint) {
+  Packed& pk = d->pk;
+  const uint32_t S = uint32_t(d->fwd.size.size());
+  const double eps = std::numeric_limits<R>::epsilon();
+  std::map<std::vector<uint32_t>, uint32_t> cls_of_key;
+  std::vector<uint32_t> cls_of(S);
+  std::vector<LaneShape> shapes;
+  std::vector<uint32_t> key;
+  const char* why = nullptr;
+  for (uint32_t s = 0; s < S && !why; ++s) {
+    LaneShape sh;
+    sh.f = lane_side(d->fwd, s, false, eps);
+    sh.adjoint = adjoint;
+    if (adjoint) sh.a = lane_side(d->adj, s, true, eps);
+    if (sh.live_values() > lane_max_live()) {
+      why = "a system needs more registers than a lane holds";
+      break;
+    }
+    key.clear();
+    shape_key(sh.f, key);
+    if (adjoint) shape_key(sh.a, key);
+    auto it = cls_of_key.find(key);
+    if (it == cls_of_key.end()) {
+      if (shapes.size() == kLaneMaxClasses) {
+        why = "the systems have too many distinct structures";
+        break;
+      }
+      it = cls_of_key.emplace(key, uint32_t(shapes.size())).first;
+      shapes.push_back(std::move(sh));
+    }
+    cls_of[s] = it->second;
+  }
+  if (why) {
+    PASIF_REQUIRE(d->kernel_path != PASIF_KERNEL_LANE, PASIF_ERR_UNSUPPORTED,
+                  "lane kernel: %s", why);
+    return false;
+  }
+  // checkpoint interval: the requested one, shortened until the segment fits LDS
+  uint32_t nf_max = 0;
+  for (const LaneShape& sh : shapes) nf_max = std::max(nf_max, sh.f.n);
+  const size_t el = sizeof(R);
+  uint32_t K = d->ck_every ? d->ck_every : 256;
+  if (adjoint) {
+    const size_t stage = size_t(kLaneStage) * kLaneWave * el;
+    const size_t fit = (kLaneMaxLds - stage) / (size_t(nf_max) * kLaneWave * el);
+    K = uint32_t(std::max<size_t>(1, std::min<size_t>(K, fit > 0 ? fit - 1 : 0)));
+  }
+  std::vector<Instance> inst = make_instances(d, adjoint);
+  std::vector<std::vector<const Instance*>> members(shapes.size());
+  for (const Instance& in : inst) members[cls_of[in.sys]].push_back(&in);
+  pk.release();
+  uint32_t block = 0, tmax = 0;
+  std::vector<double> cv;
+  for (size_t c = 0; c < shapes.size(); ++c) {
+    auto& mem = members[c];
+    std::stable_sort(mem.begin(), mem.end(),
+                     [](const Instance* x, const Instance* y) { return x->steps > y->steps; });
+    std::unique_ptr<LaneCls> L(new LaneCls());
+    const LaneShape& sh = shapes[c];
+    L->count = uint32_t(mem.size());
+    L->stride = std::max<uint32_t>(kLaneWave, (L->count + kLaneWave - 1) / kLaneWave * kLaneWave);
+    L->waves = L->stride / kLaneWave;
+    L->block_begin = block;
+    L->nf = sh.f.n;
+    L->na = adjoint ? sh.a.n : 0;
+    block += L->waves;
+    const uint32_t st = L->stride, TF = sh.f.coefs(), TA = adjoint ? sh.a.coefs() : 0;
+    std::vector<R> cf(size_t(std::max<uint32_t>(TF, 1)) * st, R(0));
+    std::vector<R> ca(size_t(std::max<uint32_t>(TA, 1)) * st, R(0));
+    std::vector<R> qf(size_t(L->nf) * st, R(0));
+    std::vector<R> qa(size_t(std::max<uint32_t>(L->na, 1)) * st, R(0));
+    std::vector<uint32_t> stp(st, 0), of(st, 0), oa(st, 0);
+    std::vector<int32_t> ex(st, -1);
+    for (uint32_t i = 0; i < L->count; ++i) {
+      const Instance& in = *mem[i];
+      lane_coefs(d->fwd, in.sys, sh.f, cv);
+      for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + i] = R(cv[t]);
+      for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
+      if (adjoint) {
+        lane_coefs(d->adj, in.sys, sh.a, cv);
+        for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
+        for (uint32_t r = 0; r < L->na; ++r)
+          qa[size_t(r) * st + i] = R(d->adj.init[d->adj.off[in.sys] + r]);
+      }
+      stp[i] = in.steps;
+      ex[i] = in.exc;
+      of[i] = in.out_f;
+      oa[i] = in.out_a;
+      L->tmax = std::max(L->tmax, in.steps);
+    }
+    tmax = std::max(tmax, L->tmax);
+    upload(d, L->coef_f, cf);
+    upload(d, L->coef_a, ca);
+    upload(d, L->init_f, qf);
+    upload(d, L->init_a, qa);
+    upload(d, L->steps, stp);
+    upload(d, L->exc, ex);
+    upload(d, L->out_f, of);
+    upload(d, L->out_a, oa);
+    pk.lcls.push_back(std::move(L));
+  }
+  // every output index the kernel writes stays inside the output arrays
+  const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
+  const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
+  for (size_t c = 0; c < shapes.size(); ++c)
+    for (const Instance* in : members[c]) {
+      PASIF_REQUIRE(uint64_t(in->out_f) + shapes[c].f.n <= out_len_f, PASIF_ERR_UNSUPPORTED,
+                    "lane plan: output index out of range");
+      PASIF_REQUIRE(!adjoint || uint64_t(in->out_a) + shapes[c].a.n <= out_len_a,
+                    PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
+      PASIF_REQUIRE(in->exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
+                    "lane plan: excitation out of range");
+    }
+  pk.lshapes = shapes;
+  pk.lseg = adjoint ? K : 1;
+  pk.lblocks = block;
+  pk.llds = lane_lds_bytes(shapes, int(sizeof(R) * 8), pk.lseg);
+  PASIF_REQUIRE(pk.llds <= kLaneMaxLds, PASIF_ERR_UNSUPPORTED, "lane kernel LDS %zu bytes",
+                pk.llds);
+  if (d->dev >= 0) {
+    const std::string src = lane_source(shapes, int(sizeof(R) * 8), pk.lseg);
+    std::string err;
+    if (!lane_compile(src, d->dev, &pk.lkern, &err)) {
+      set_err(d, "%s", err.c_str());
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+    PASIF_HIP(hipStreamSynchronize(d->stream));
+  }
+  pk.lane = true;
+  pk.adjoint = adjoint;
+  pk.precision = int(sizeof(R) * 8);
+  pk.tmax = tmax;
+  pk.out_len_f = out_len_f;
+  pk.out_len_a = out_len_a;
+  pk.ntiles = block;
+  pk.lanes = kLaneWave;
+  pk.slot_cfg = 0;
+  pk.valid = true;
+  if (d->d_compute)
+    printf("[pasif] lane kernel: %zu instances in %zu structure class(es), %u waves, T max %u, "
+           "checkpoint %u\n",
+           inst.size(), shapes.size(), block, tmax, pk.lseg);
+  return true;
 }
 
 void require_device(pasif_driver* d) {
@@ -936,10 +1148,16 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
   const uint64_t key = pack_key(d, adjoint);
   if (d->pk.valid && d->pk.key == key) return;
   d->pk.valid = false;
-  if (d->precision == 64)
-    pack_impl<double>(d, adjoint);
-  else
-    pack_impl<float>(d, adjoint);
+  bool lane = false;
+  if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool)
+    lane = d->precision == 64 ? lane_plan<double>(d, adjoint) : lane_plan<float>(d, adjoint);
+  if (!lane) {
+    d->pk.release();
+    if (d->precision == 64)
+      pack_impl<double>(d, adjoint);
+    else
+      pack_impl<float>(d, adjoint);
+  }
   d->pk.key = key;
 }
 
@@ -1083,49 +1301,41 @@ void finish_run(pasif_driver* d, bool adjoint, uint32_t nslots, std::vector<doub
 }
 
 
-// Run the packed tiles to completion.  save_steps > 0 records a trajectory.
-template <typename R>
-void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
-                std::vector<double>* grad_out) {
-  uint64_t work = 0;
-  uint32_t nslots = 0;
-  TileArgs<R> a = prepare_run<R>(d, adjoint, save_steps, &work, &nslots);
-  Packed& pk = d->pk;
-  const uint32_t L = pk.lanes, nt = pk.ntiles;
-  const bool plain = d->solver == PASIF_SOLVER_PLAIN;
-  const bool graph = d->solver == PASIF_SOLVER_GRAPH;
-  uint64_t launches = 1;
-  if (plain) {
-    a.budget = 1u;
-    launches = std::max<uint64_t>(work, 1);
-  } else if (graph) {
-    a.budget = std::max<uint32_t>(1, d->graph_slice);
-    launches = std::max<uint64_t>(1, (work + a.budget - 1) / a.budget);
-  } else {
-    a.budget = 0xFFFFFFFFu;
+// Steps of work per launch for the solver mode, and the launches that cover
+// `work` steps (a launch on finished work returns at once).
+uint32_t launch_budget(const pasif_driver* d, uint64_t work, uint64_t* launches) {
+  if (d->solver == PASIF_SOLVER_PLAIN) {
+    *launches = std::max<uint64_t>(work, 1);
+    return 1u;
   }
-  auto launch = [&]() {
-    hipError_t e = launch_tiles(a, nt, int(pk.slot_cfg), d->stream);
-    if (e != hipSuccess) {
-      set_err(d, "tile kernel launch failed (%u tiles of %u lanes): %s", nt, L, hipGetErrorName(e));
-      throw Fail{PASIF_ERR_DEVICE};
-    }
-  };
+  if (d->solver == PASIF_SOLVER_GRAPH) {
+    const uint32_t b = std::max<uint32_t>(1, d->graph_slice);
+    *launches = std::max<uint64_t>(1, (work + b - 1) / b);
+    return b;
+  }
+  *launches = 1;
+  return 0xFFFFFFFFu;
+}
+
+// Issue `launches` calls of launch() on the driver's stream -- directly, or
+// for the graph solver as replays of a captured graph of up to
+// graph_launches calls -- and time them with HIP events.  No memset is
+// captured: run state is cleared before, outside the graph.
+template <class F>
+void drive(pasif_driver* d, uint64_t launches, F&& launch) {
+  const bool graph = d->solver == PASIF_SOLVER_GRAPH;
   hipGraphExec_t gexec = nullptr;
   uint32_t per_graph = 1;
   if (graph) {
-    // one graph of `per_graph` sliced launches, replayed; a launch on a tile
-    // that has finished returns at once, so extra replays are harmless.  No
-    // memset is captured: progress is cleared above, outside the graph.
     per_graph = uint32_t(std::min<uint64_t>(launches, d->graph_launches));
     hipGraph_t g = nullptr;
     PASIF_HIP(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t i = 0; i < per_graph; ++i) {
-      hipError_t e = launch_tiles(a, nt, int(pk.slot_cfg), d->stream);
+      hipError_t e = launch();
       if (e != hipSuccess) {
         (void)hipStreamEndCapture(d->stream, &g);
         if (g) (void)hipGraphDestroy(g);
-        set_err(d, "tile kernel capture failed: %s", hipGetErrorName(e));
+        set_err(d, "kernel capture failed: %s", hipGetErrorName(e));
         throw Fail{PASIF_ERR_DEVICE};
       }
     }
@@ -1150,7 +1360,13 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     }
     launches = replays * per_graph;
   } else {
-    for (uint64_t i = 0; i < launches; ++i) launch();
+    for (uint64_t i = 0; i < launches; ++i) {
+      hipError_t e = launch();
+      if (e != hipSuccess) {
+        set_err(d, "kernel launch failed: %s", hipGetErrorName(e));
+        throw Fail{PASIF_ERR_DEVICE};
+      }
+    }
   }
   PASIF_HIP(hipEventRecord(d->ev1, d->stream));
   PASIF_HIP(hipEventSynchronize(d->ev1));
@@ -1159,6 +1375,118 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   PASIF_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
   d->last_ms = ms;
   d->last_launches = uint32_t(launches);
+}
+
+// Run the lane kernel of the packed problem to completion.
+template <typename R>
+void solve_lane(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
+                std::vector<double>* grad_out) {
+  ensure_excitations<R>(d);
+  Packed& pk = d->pk;
+  dev_reserve(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
+  if (d->window) {
+    dev_reserve(d, d->out_max, size_t(pk.out_len_f) * sizeof(R));
+    PASIF_HIP(hipMemsetAsync(d->out_max.p, 0, size_t(pk.out_len_f) * sizeof(R), d->stream));
+  }
+  uint32_t nslots = 0;
+  if (save_steps) {
+    nslots = pk.tmax / save_steps;
+    dev_reserve(d, d->traj, size_t(nslots) * pk.out_len_f * sizeof(R));
+    PASIF_HIP(hipMemsetAsync(d->traj.p, 0, size_t(nslots) * pk.out_len_f * sizeof(R), d->stream));
+  }
+  if (adjoint) dev_reserve(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
+  const uint64_t work = adjoint ? 3ull * pk.tmax : pk.tmax;
+  uint64_t launches = 1;
+  const uint32_t budget = launch_budget(d, work, &launches);
+  const uint32_t K = pk.lseg;
+  const R h = R(1.0 / double(d->sample_rate));
+  std::vector<LaneArgs> args(pk.lcls.size());
+  for (size_t c = 0; c < pk.lcls.size(); ++c) {
+    LaneCls& L = *pk.lcls[c];
+    const size_t st = L.stride;
+    dev_reserve(d, L.qstate, size_t(L.nf) * st * sizeof(R));
+    dev_reserve(d, L.progress, size_t(L.waves) * 4 * sizeof(uint32_t));
+    PASIF_HIP(hipMemsetAsync(L.progress.p, 0, size_t(L.waves) * 4 * sizeof(uint32_t), d->stream));
+    if (d->window) dev_reserve(d, L.mxstate, size_t(L.nf) * st * sizeof(R));
+    uint32_t nck = 1;
+    if (adjoint) {
+      nck = std::max<uint32_t>(1, (L.tmax + K - 1) / K);
+      dev_reserve(d, L.lstate, size_t(std::max<uint32_t>(L.na, 1)) * st * sizeof(R));
+      dev_reserve(d, L.ckpt, size_t(nck) * L.nf * st * sizeof(R));
+      dev_reserve(d, L.segbuf, size_t(K + 1) * L.nf * st * sizeof(R));
+    }
+    LaneArgs& a = args[c];
+    std::memset(&a, 0, sizeof a);
+    a.coef_f = L.coef_f.p;
+    a.coef_a = L.coef_a.p;
+    a.init_f = L.init_f.p;
+    a.init_a = L.init_a.p;
+    a.steps = L.steps.p;
+    a.exc = L.exc.p;
+    a.out_f = L.out_f.p;
+    a.out_a = L.out_a.p;
+    a.qstate = L.qstate.p;
+    a.lstate = L.lstate.p;
+    a.mxstate = L.mxstate.p;
+    a.progress = L.progress.p;
+    a.ckpt = L.ckpt.p;
+    a.segbuf = L.segbuf.p;
+    a.out_q = d->out_q.p;
+    a.out_max = d->window ? d->out_max.p : nullptr;
+    a.traj = save_steps ? d->traj.p : nullptr;
+    a.out_grad = adjoint ? d->out_grad.p : nullptr;
+    a.excit = d->d_exc.p;
+    a.interp = d->interp_w ? d->d_interp.p : nullptr;
+    a.stride = st;
+    a.count = L.count;
+    a.block_begin = L.block_begin;
+    a.exc_len = d->exc_len;
+    a.interp_w = d->interp_w;
+    a.budget = budget;
+    a.window = d->window;
+    a.out_len = pk.out_len_f;
+    a.save_every = save_steps;
+    a.nslots = nslots;
+    a.ck_every = K;
+    a.nck = nck;
+    a.adjoint = adjoint ? 1 : 0;
+    // h = 1/sampleRate in the working type, h2 and h6 from that h in double
+    // (src/__GpuDriver.cu:632-636), carried exactly in a double
+    a.h = double(h);
+    a.h2 = double(R(double(h) / 2.0));
+    a.h6 = double(R(double(h) / 6.0));
+  }
+  upload(d, pk.largs, args);
+  const void* dargs = pk.largs.p;
+  uint32_t ncls = uint32_t(args.size());
+  hipFunction_t fn = pk.lkern.fn;
+  const uint32_t blocks = pk.lblocks;
+  const uint32_t lds = uint32_t(pk.llds);
+  hipStream_t stream = d->stream;
+  drive(d, launches, [&]() -> hipError_t {
+    void* params[] = {const_cast<void**>(&dargs), &ncls};
+    return hipModuleLaunchKernel(fn, blocks, 1, 1, kLaneWave, 1, 1, lds, stream, params, nullptr);
+  });
+  finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
+}
+
+// Run the packed problem to completion.  save_steps > 0 records a trajectory.
+template <typename R>
+void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
+                std::vector<double>* grad_out) {
+  ensure_packed(d, adjoint);
+  if (d->pk.lane) {
+    solve_lane<R>(d, adjoint, save_steps, traj_out, grad_out);
+    return;
+  }
+  uint64_t work = 0;
+  uint32_t nslots = 0;
+  TileArgs<R> a = prepare_run<R>(d, adjoint, save_steps, &work, &nslots);
+  Packed& pk = d->pk;
+  const uint32_t nt = pk.ntiles;
+  uint64_t launches = 1;
+  a.budget = launch_budget(d, work, &launches);
+  drive(d, launches, [&]() { return launch_tiles(a, nt, int(pk.slot_cfg), d->stream); });
   finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
 }
 
@@ -1540,6 +1868,66 @@ int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* l
   return guarded(d, [&] {
     PASIF_REQUIRE(d->have_max, PASIF_ERR_STATE, "no windowed solve has run");
     copy_out(d, d->res_max, out, cap, len);
+  });
+}
+
+int pasif_set_kernel(pasif_driver* d, int kernel) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(kernel == PASIF_KERNEL_AUTO || kernel == PASIF_KERNEL_TILE ||
+                      kernel == PASIF_KERNEL_LANE,
+                  PASIF_ERR_ARGUMENT, "unknown kernel %d", kernel);
+    d->kernel_path = kernel;
+  });
+}
+
+int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* classes,
+                      uint32_t* checkpoint_every) {
+  return guarded(d, [&] {
+    ensure_packed(d, adjoint != 0);
+    if (kernel) *kernel = d->pk.lane ? PASIF_KERNEL_LANE : PASIF_KERNEL_TILE;
+    if (classes) *classes = d->pk.lane ? uint32_t(d->pk.lshapes.size()) : 0u;
+    if (checkpoint_every)
+      *checkpoint_every = !adjoint ? 0u
+                          : d->pk.lane ? d->pk.lseg
+                                       : (d->ck_every ? d->ck_every : 256u);
+  });
+}
+
+int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_t* len,
+                      int compile) {
+  return guarded(d, [&] {
+    const int saved = d->kernel_path;
+    d->kernel_path = PASIF_KERNEL_LANE;
+    const int dev = d->dev;
+    d->dev = -1;  // plan on the host only; the compilation below needs no GPU
+    d->pk.valid = false;
+    const int rc = guarded(d, [&] {
+      if (d->precision == 64)
+        (void)lane_plan<double>(d, adjoint != 0);
+      else
+        (void)lane_plan<float>(d, adjoint != 0);
+    });
+    d->dev = dev;
+    d->kernel_path = saved;
+    const std::vector<LaneShape> shapes = d->pk.lshapes;
+    const uint32_t seg = d->pk.lseg;
+    d->pk.release();
+    if (rc != PASIF_OK) throw Fail{rc};
+    const std::string src = lane_source(shapes, d->precision, seg);
+    if (len) *len = src.size() + 1;
+    if (out) {
+      PASIF_REQUIRE(cap > src.size(), PASIF_ERR_ARGUMENT, "source buffer holds %zu, need %zu",
+                    cap, src.size() + 1);
+      std::memcpy(out, src.c_str(), src.size() + 1);
+    }
+    if (compile) {
+      std::string err;
+      size_t bytes = 0;
+      if (!lane_compile_only(src, &bytes, &err)) {
+        set_err(d, "%s", err.c_str());
+        throw Fail{PASIF_ERR_DEVICE};
+      }
+    }
   });
 }
 

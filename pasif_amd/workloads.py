@@ -218,12 +218,15 @@ DOF_PROBS = np.array([0.3, 0.25, 0.15, 0.1, 0.1, 0.06, 0.04])
 
 def heterogeneous(n_systems: int = 100_000, tmin: float = 1e3, tmax: float = 1e5,
                   dt: float = 1e-3, steps_per_sample: int = 10, seed: int = 3,
-                  n_signals: int = 64) -> Workload:
+                  n_signals: int = 64, dof_choices=None, dof_probs=None) -> Workload:
     """Systems of uneven size and step count: DOF from {1,..,64} with the
-    configured probabilities, steps log-uniform on [tmin, tmax], chain
-    structure at each size, one of ``n_signals`` band-limited forcings each."""
+    configured probabilities (or ``dof_choices`` / ``dof_probs``), steps
+    log-uniform on [tmin, tmax], chain structure at each size, one of
+    ``n_signals`` band-limited forcings each."""
     rng = np.random.default_rng(seed)
-    dofs = rng.choice(DOF_CHOICES, size=n_systems, p=DOF_PROBS)
+    choices = DOF_CHOICES if dof_choices is None else np.asarray(dof_choices)
+    probs = DOF_PROBS if dof_probs is None else np.asarray(dof_probs, dtype=np.float64)
+    dofs = rng.choice(choices, size=n_systems, p=probs / np.sum(probs))
     steps = np.exp(rng.uniform(np.log(tmin), np.log(tmax), n_systems)).astype(np.int64)
     sr = int(round(1.0 / dt))
     n_samples = int(steps.max()) // steps_per_sample + 1

@@ -70,6 +70,7 @@ def test_duffing_gradient_and_forward_amplitudes_together():
     terms = [np.array([0.0, 0.0, 1.0])] * w.n_systems
     J = ADJ.jacobian_inputs(w.vecM, w.vecB, w.vecK, w.vecGamma, w.vecLambda, params, terms)
     p = W.make_solver(w)
+    p.setKernel("tile")
     p.setJacobian(*J.as_args())
     g = p.getGradient()
     assert rel_err(g, oracle_gradient(w, J, 256)) < TOL64
@@ -91,6 +92,7 @@ def test_cartesian_random_adjoint_with_psi():
     J = ADJ.JacobianInputs(aM, aB, aK, [as_coo(t) for t in aG], [as_coo(t) for t in aL], aF, aQ,
                            [as_coo(t) for t in aP], sizes=np.array([5, 6]))
     p = W.make_solver(w)
+    p.setKernel("tile")
     p.setJacobian(*J.as_args())
     g = p.getGradient()
     ref = oracle_gradient(w, J, 16)
@@ -103,6 +105,7 @@ def test_heterogeneous_gradient():
     w = W.heterogeneous(n_systems=30, tmin=40, tmax=300, steps_per_sample=5, n_signals=3)
     J = ADJ.energy_gradient_inputs(w)
     p = W.make_solver(w)
+    p.setKernel("tile")
     p.setJacobian(*J.as_args())
     p.setTuning(checkpoint_every=32)
     g = p.getGradient()

@@ -55,6 +55,7 @@ def test_duffing_small_fp64():
     _print_case("duffing 16 x 1500")
     w = W.duffing(n_systems=16, steps=1500)
     p = W.make_solver(w)
+    p.setKernel("tile")
     q, mx = p.getMaxAmplitudes(window=300)
     ref = oracle_run(w, window=300)
     assert rel_err(q, ref.q) < TOL64
@@ -65,6 +66,7 @@ def test_two_dof_fp64_and_plain_matches_persistent():
     _print_case("two_dof 64 x 800")
     w = W.two_dof(n_systems=64, steps=800)
     p = W.make_solver(w)
+    p.setKernel("tile")
     q = p.getAmplitudes()
     ref = oracle_run(w)
     assert rel_err(q, ref.q) < TOL64
@@ -94,6 +96,7 @@ def test_cartesian_product_random_systems_and_trajectory():
     exc = rng.normal(0, 1, (3, 121))
     w = W.Workload("random", vM, vB, vK, vG, vL, vF, vQ, exc, 100, 120)
     p = W.make_solver(w)
+    p.setKernel("tile")
     q = p.getAmplitudes()
     ref = oracle_run(w, save_steps=7)
     assert q.shape == ref.q.reshape(-1).shape
@@ -125,6 +128,7 @@ def test_heterogeneous_steps_and_sizes():
     _print_case("heterogeneous 60 systems")
     w = W.heterogeneous(n_systems=60, tmin=50, tmax=400, steps_per_sample=5, n_signals=4)
     p = W.make_solver(w)
+    p.setKernel("tile")
     q = p.getAmplitudes()
     ref = oracle_run(w)
     assert rel_err(q, ref.q) < TOL64
@@ -134,6 +138,7 @@ def test_fp32_against_fp32_and_fp64_oracle():
     _print_case("two_dof fp32")
     w = W.two_dof(n_systems=32, steps=1000)
     p = W.make_solver(w)
+    p.setKernel("tile")
     p.setPrecision(32)
     q = p.getAmplitudes()
     r32 = oracle_run(w, dtype=np.float32)

@@ -1,0 +1,65 @@
+"""Lane kernel on the host (no GPU): structure classes, kernel choice,
+checkpoint shortening, and hipRTC compilation of the generated source for
+gfx950 (hipRTC needs no device)."""
+import os
+
+import numpy as np
+import pytest
+
+from pasif_amd import _lib
+from pasif_amd import adjoint as ADJ
+from pasif_amd import workloads as W
+
+
+def host_solver(w):
+    os.environ["LOCAL_RANK"] = "-1"
+    try:
+        return W.make_solver(w)
+    finally:
+        del os.environ["LOCAL_RANK"]
+
+
+@pytest.mark.parametrize("maker,adjoint,classes", [
+    (lambda: W.two_dof(16, 50), False, 1),
+    (lambda: W.duffing(16, 50), False, 1),
+    (lambda: W.chains(8, 2, 60, steps_per_sample=10), True, 1),
+    (lambda: W.heterogeneous(40, 30, 300, steps_per_sample=5, n_signals=3,
+                             dof_choices=(1, 2, 4), dof_probs=(1, 1, 1)), False, 3),
+])
+def test_generated_source_compiles_for_gfx950(maker, adjoint, classes):
+    w = maker()
+    p = host_solver(w)
+    if adjoint:
+        p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
+    info = p.kernelInfo(adjoint=adjoint)
+    assert info["kernel"] == "lane" and info["classes"] == classes
+    for bits in (64, 32):
+        p.setPrecision(bits)
+        src = p._laneSource(adjoint, compile=True)
+        assert "pasif_lane_kernel" in src
+        assert ("typedef double R;" in src) == (bits == 64)
+
+
+def test_large_systems_stay_on_tiles_and_lane_refuses_them():
+    w = W.chains(8, 16, 60, steps_per_sample=10)
+    p = host_solver(w)
+    assert p.kernelInfo()["kernel"] == "tile"
+    p.setKernel("lane")
+    with pytest.raises(_lib.PasifError) as e:
+        p.kernelInfo()
+    assert e.value.code == 3 and "registers" in str(e.value)
+    p.setKernel("tile")
+    assert p.kernelInfo()["kernel"] == "tile"
+
+
+def test_checkpoint_interval_shortened_to_fit_lds():
+    w = W.chains(8, 2, 60, steps_per_sample=10)
+    p = host_solver(w)
+    p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
+    p.setTuning(checkpoint_every=256)
+    k = p.kernelInfo(adjoint=True)["checkpoint_every"]
+    nf = 5
+    assert 1 <= k < 256
+    assert (35 * 64 + (k + 1) * nf * 64) * 8 <= 64 * 1024
+    p.setTuning(checkpoint_every=4)
+    assert p.kernelInfo(adjoint=True)["checkpoint_every"] == 4
