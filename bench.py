@@ -184,17 +184,20 @@ def main():
     timing = p._lastTiming()
     value = sys_steps / wall
     kms = float(np.mean(kernel_ms))
+    kinfo = p.kernelInfo()
+    kname = f"{kinfo['kernel']}_kernel"
 
     flops = flops_per_step(p) * w.numsteps
     achieved_tf = flops / (kms * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
         "pipe": "FP64 VALU (no matrix op on this path; MI355X FP64 vector peak)",
+        "kernel": kname,
         "achieved": round(achieved_tf, 4),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 5),
-        "traffic": read_traffic("tile_kernel_two_dof_4096x50000_traffic.txt"),
+        "traffic": read_traffic(f"{kname}_two_dof_4096x50000_traffic.txt"),
         "algorithmic_bytes": bytes_per_launch(p, w),
         "flops_per_launch": flops,
         "kernel_ms": round(kms, 4),
@@ -217,7 +220,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"two_dof {w.n_systems} systems x {w.numsteps} RK4 steps, "
                                f"dt 5e-4, sinusoidal forcing, forward, FP64",
-                   "systems": w.n_systems, "rk_steps": w.numsteps, "solver": "persistent"},
+                   "systems": w.n_systems, "rk_steps": w.numsteps, "solver": "persistent",
+                   "kernel": kinfo["kernel"], "structure_classes": kinfo["classes"]},
         "roofline": roofline,
     }
 
@@ -233,6 +237,18 @@ def main():
                         "launches": pt["launches"],
                         "bitwise_equal_to_persistent": bool(np.array_equal(qp, q))}
         out["persistent_vs_plain"] = (sys_steps / wall) / (sys_steps / pw)
+
+    # the tile kernel (rows across lanes, LDS exchange) on the same problem
+    if kinfo["kernel"] != "tile":
+        p.setKernel("tile")
+        p.getAmplitudes()
+        t0 = time.perf_counter()
+        qt = p.getAmplitudes()
+        tw = time.perf_counter() - t0
+        out["tile_kernel"] = {"value": sys_steps / tw, "kernel_ms": p._lastTiming()["kernel_ms"],
+                              "max_rel_diff_vs_lane": float(np.max(
+                                  np.abs(qt - q) / np.maximum(np.abs(q), 1e-8 * np.abs(q).max())))}
+        p.setKernel("auto")
 
     # FP32 rate
     p.setPrecision(32)

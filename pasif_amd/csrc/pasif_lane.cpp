@@ -39,30 +39,32 @@ namespace {
 // Device-side runtime shared by every class (R, LaneArgs and the class
 // policies are defined before it).
 const char* kLaneRuntime = R"LANE(
+#define GP(T) __attribute__((address_space(1))) T*
 __device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ u32 umin(u32 a, u32 b) { return a < b ? a : b; }
 enum : u32 { kForward = 0, kRecompute = 1, kBackward = 2, kDone = 3 };
+template <bool B> struct Flag { static constexpr bool value = B; };
 
-// e[b .. b + STAGE) of the lane's excitation into its LDS column (0 beyond L)
-__device__ __forceinline__ void stage(R* eb, u32 lane, const R* E, u32 Lx, u32 b) {
+// e[b .. b + STAGE) of one excitation (indices clamped into [0, L))
+__device__ __forceinline__ void fetch(R (&v)[STAGE], GP(const R) E, u32 Lx, u32 b) {
 #pragma unroll
-  for (u32 t0 = 0; t0 < STAGE; t0 += 8) {
-    R v[8];
-#pragma unroll
-    for (u32 u = 0; u < 8; ++u) {
-      const u32 i = b + t0 + u;
-      v[u] = E[i < Lx ? i : Lx - 1u];
-    }
-#pragma unroll
-    for (u32 u = 0; u < 8; ++u)
-      if (t0 + u < STAGE) eb[(t0 + u) * WAVE + lane] = (b + t0 + u < Lx) ? v[u] : R(0);
+  for (u32 u = 0; u < STAGE; ++u) {
+    const u32 i = b + u;
+    v[u] = E[i < Lx ? i : Lx - 1u];
   }
+}
+// into the lane's LDS column; 0 beyond L and for unforced lanes
+__device__ __forceinline__ void put(R* eb, u32 lane, const R (&v)[STAGE], u32 Lx, u32 b,
+                                    bool forced) {
+#pragma unroll
+  for (u32 u = 0; u < STAGE; ++u)
+    eb[u * WAVE + lane] = (forced && b + u < Lx) ? v[u] : R(0);
 }
 
 // Mi[1][j] e[i] + Mi[0][j] e[i+1] on the half step i0*W + j0 + t, zero
 // unless 0 < i < L (src/kernels.cu:114-124)
-__device__ __forceinline__ R interp_at(const R* eb, u32 lane, u32 b, const R* wts, u32 W, u32 Lx,
-                                       u32 i0, u32 j0, u32 t) {
+__device__ __forceinline__ R interp_at(const R* eb, u32 lane, u32 b, GP(const R) wts, u32 W,
+                                       u32 Lx, u32 i0, u32 j0, u32 t) {
   u32 j = j0 + t;
   const bool next = j >= W;
   j = next ? j - W : j;
@@ -72,8 +74,8 @@ __device__ __forceinline__ R interp_at(const R* eb, u32 lane, u32 b, const R* wt
   return (i > 0u && i < Lx) ? fmaR(wts[W + j], ei, wts[j] * ei1) : R(0);
 }
 
-// one RK4 step of N rows; rhs(k, y, stage)
-template <int N, class F>
+// one RK4 step of N rows; rhs(k, y, stage).  ALL: every lane is active.
+template <int N, bool ALL, class F>
 __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
   R y[N], k[N], acc[N];
 #pragma unroll
@@ -95,15 +97,16 @@ __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
       } else {
         acc[r] = acc[r] + k[r];
         const R qn = fmaR(h6, acc[r], q[r]);
-        q[r] = active ? qn : q[r];
+        q[r] = (ALL || active) ? qn : q[r];
       }
     }
   }
 }
 
 // One wave of one class through its phases; same schedule and resumable
-// state as the tile kernel.  LDS: eb [STAGE][WAVE] staged samples, then
-// seg [K+1][NF][WAVE] (adjoint) the recomputed segment.
+// state as the tile kernel.  LDS: eb [STAGE][WAVE] staged samples of the
+// current chunk of steps (the next chunk is prefetched into registers),
+// then seg [K+1][NF][WAVE] (adjoint) the recomputed segment.
 template <class S>
 __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr int NF = S::NF;
@@ -120,25 +123,34 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   const u32 K = adjoint ? (u32)a.ck_every : 1u;
   const u32 W = (u32)a.interp_w;
   const u32 Lx = (u32)a.exc_len;
+  const u32 SE = (u32)a.save_every;
   const R h = R(a.h), h2 = R(a.h2), h6 = R(a.h6);
-  const u32* steps = (const u32*)a.steps;
+  GP(const u32) steps = (GP(const u32))a.steps;
+  // instances are sorted by step count, descending: the wave's first lane
+  // has the most steps, its last valid lane the fewest
   const u32 T = uni(steps[wave * WAVE]);
+  const u32 Tmin = uni(steps[umin(wave * WAVE + WAVE, (u32)a.count) - 1u]);
   const u32 fsteps = steps[i];
-  const i32 exc = ((const i32*)a.exc)[i];
-  const u32 fout = ((const u32*)a.out_f)[i];
-  const u32 aout = ((const u32*)a.out_a)[i];
-  const R* cf = (const R*)a.coef_f + i;
-  const R* ca = (const R*)a.coef_a + i;
-  R* qs = (R*)a.qstate + i;
-  R* ms = (R*)a.mxstate + i;
-  R* ls = (R*)a.lstate + i;
-  R* ck = (R*)a.ckpt + i;
-  R* sb = (R*)a.segbuf + i;
-  R* outq = (R*)a.out_q;
-  R* outm = (R*)a.out_max;
-  R* outg = (R*)a.out_grad;
-  const R* wts = (const R*)a.interp;
-  u32* pst = (u32*)a.progress + 4 * wave;
+  const i32 exc = ((GP(const i32))a.exc)[i];
+  const bool forced = exc >= 0;
+  const u32 fout = ((GP(const u32))a.out_f)[i];
+  const u32 aout = ((GP(const u32))a.out_a)[i];
+  GP(const R) cf = (GP(const R))a.coef_f + i;
+  GP(const R) ca = (GP(const R))a.coef_a + i;
+  GP(const R) qi = (GP(const R))a.init_f + i;
+  GP(const R) li = (GP(const R))a.init_a + i;
+  GP(R) qs = (GP(R))a.qstate + i;
+  GP(R) ms = (GP(R))a.mxstate + i;
+  GP(R) ls = (GP(R))a.lstate + i;
+  GP(R) ck = (GP(R))a.ckpt + i;
+  GP(R) sb = (GP(R))a.segbuf + i;
+  GP(R) outq = (GP(R))a.out_q;
+  GP(R) outm = (GP(R))a.out_max;
+  GP(R) outg = (GP(R))a.out_grad;
+  GP(R) traj = (GP(R))a.traj;
+  GP(const R) wts = (GP(const R))a.interp;
+  GP(const R) E = (GP(const R))a.excit + (u64)(forced ? exc : 0) * Lx;
+  GP(u32) pst = (GP(u32))a.progress + 4 * wave;
   u32 n = uni(pst[0]);
   u32 phase = uni(pst[1]);
   u32 sg = uni(pst[2]);
@@ -151,7 +163,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   for (int r = 0; r < NA; ++r) lam[r] = R(0);
   if (n == 0u && phase == kForward) {
 #pragma unroll
-    for (int r = 0; r < NF; ++r) q[r] = ((const R*)a.init_f)[r * st + i];
+    for (int r = 0; r < NF; ++r) q[r] = qi[r * st];
     if (fsteps == 0u && valid)
 #pragma unroll
       for (int r = 0; r < NF; ++r) outq[fout + r] = q[r];
@@ -195,19 +207,24 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       const u32 e = recompute ? umin(T, s0 + K) : T;
       const u32 nend = (e - n < budget) ? e : n + budget;
       budget -= nend - n;
+      const u32 nfirst = n;
+      // per-step bookkeeping (checkpoints, segment, trajectory, window,
+      // per-lane output) only when a run needs it
+      const bool gen = recompute || adjoint || SE != 0u || a.window != 0ull;
+      u32 next_ck = ((n + K - 1u) / K) * K;
+      u32 next_save = SE ? ((n + SE - 1u) / SE) * SE : 0u;
+      // forcing cursor: half-step position 2n = hi*W + hj, staged chunk
       u32 hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
-      const R* E = (const R*)a.excit + (u64)(exc >= 0 ? exc : 0) * Lx;
-      u32 ebase = 0, left = 0;
-      for (; n < nend; ++n) {
-        if (left == 0u) {
-          ebase = W ? hi : n;
-          if (exc >= 0) stage(eb, lane, E, Lx, ebase);
-          left = CHUNK;
-        }
-        --left;
-        R u0 = R(0), u1 = R(0), u3 = R(0);
-        if (exc >= 0) {
-          if (W == 0u) {
+      u32 ebase = 0, left = 0, pbase = 0;
+      bool have = false;
+      R pre[STAGE];
+      auto steps_to = [&](auto all_t, auto gen_t, auto zoh_t, u32 stop) {
+        constexpr bool ALL = decltype(all_t)::value;
+        constexpr bool GEN = decltype(gen_t)::value;
+        constexpr bool ZOH = decltype(zoh_t)::value;
+        for (; n < stop; ++n) {
+          R u0, u1, u3;
+          if (ZOH) {
             const R e0 = eb[(n - ebase) * WAVE + lane], e1 = eb[(n + 1u - ebase) * WAVE + lane];
             u0 = (n > 0u && n < Lx) ? e0 : R(0);
             u1 = u0;
@@ -216,46 +233,95 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             u0 = interp_at(eb, lane, ebase, wts, W, Lx, hi, hj, 0u);
             u1 = interp_at(eb, lane, ebase, wts, W, Lx, hi, hj, 1u);
             u3 = interp_at(eb, lane, ebase, wts, W, Lx, hi, hj, 2u);
+            hj += 2u;
+            if (hj >= W) {
+              hj -= W;
+              hi += 1u;
+            }
           }
-        }
-        hj += 2u;
-        if (W && hj >= W) {
-          hj -= W;
-          hi += 1u;
-        }
-        if (adjoint && !recompute && (n % K) == 0u)
+          if (GEN && adjoint && !recompute && n == next_ck) {
 #pragma unroll
-          for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * st] = q[r];
-        const bool active = n < fsteps;
-        rk4<NF>(q, [&](R* k, const R* y, int s) {
-          S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
-        }, h, h2, h6, active);
-        if (recompute) {
-          const u32 m = n + 1u - s0;
-#pragma unroll
-          for (int r = 0; r < NF; ++r) {
-            seg[(m * NF + r) * WAVE + lane] = q[r];
-            sb[((u64)m * NF + r) * st] = q[r];
+            for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * st] = q[r];
+            next_ck += K;
           }
-        } else {
-          if (a.save_every && (n % (u32)a.save_every) == 0u) {
-            const u32 slot = n / (u32)a.save_every;
-            if (slot < (u32)a.nslots && valid)
+          rk4<NF, ALL>(q, [&](R* k, const R* y, int s) {
+            S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
+          }, h, h2, h6, n < fsteps);
+          if (GEN) {
+            if (recompute) {
+              const u32 m = n + 1u - s0;
 #pragma unroll
-              for (int r = 0; r < NF; ++r) ((R*)a.traj)[(u64)slot * a.out_len + fout + r] = q[r];
-          }
-          if (a.window && active && n + 1u + (u32)a.window > fsteps)
+              for (int r = 0; r < NF; ++r) {
+                seg[(m * NF + r) * WAVE + lane] = q[r];
+                sb[((u64)m * NF + r) * st] = q[r];
+              }
+            } else {
+              if (SE && n == next_save) {
+                const u32 slot = n / SE;
+                if (slot < (u32)a.nslots && valid)
 #pragma unroll
-            for (int r = 0; r < NF; ++r) mx[r] = fmaxR(mx[r], absR(q[r]));
-          if (n + 1u == fsteps && valid) {
+                  for (int r = 0; r < NF; ++r) traj[(u64)slot * a.out_len + fout + r] = q[r];
+                next_save += SE;
+              }
+              if (a.window && (ALL || n < fsteps) && n + 1u + (u32)a.window > fsteps)
 #pragma unroll
-            for (int r = 0; r < NF; ++r) {
-              outq[fout + r] = q[r];
-              if (a.window) outm[fout + r] = mx[r];
+                for (int r = 0; r < NF; ++r) mx[r] = fmaxR(mx[r], absR(q[r]));
+              if (n + 1u == fsteps && valid) {
+#pragma unroll
+                for (int r = 0; r < NF; ++r) {
+                  outq[fout + r] = q[r];
+                  if (a.window) outm[fout + r] = mx[r];
+                }
+              }
             }
           }
         }
+      };
+      while (n < nend) {
+        if (left == 0u) {
+          // a new chunk: its samples come from the prefetch (or are loaded
+          // now), and the next chunk's are requested
+          const u32 base = W ? hi : n;
+          if (have && pbase == base) {
+            put(eb, lane, pre, Lx, base, forced);
+          } else {
+            R v[STAGE];
+            fetch(v, E, Lx, base);
+            put(eb, lane, v, Lx, base, forced);
+          }
+          ebase = base;
+          left = CHUNK;
+          pbase = W ? (2u * (n + CHUNK)) / W : n + CHUNK;
+          fetch(pre, E, Lx, pbase);
+          have = true;
+        }
+        u32 stop = umin(nend, n + left);
+        if (n < Tmin && Tmin < stop) stop = Tmin;
+        left -= stop - n;
+        const bool all = n < Tmin;
+        if (gen) {
+          if (W == 0u) {
+            if (all) steps_to(Flag<true>(), Flag<true>(), Flag<true>(), stop);
+            else steps_to(Flag<false>(), Flag<true>(), Flag<true>(), stop);
+          } else {
+            if (all) steps_to(Flag<true>(), Flag<true>(), Flag<false>(), stop);
+            else steps_to(Flag<false>(), Flag<true>(), Flag<false>(), stop);
+          }
+        } else {
+          if (W == 0u) {
+            if (all) steps_to(Flag<true>(), Flag<false>(), Flag<true>(), stop);
+            else steps_to(Flag<false>(), Flag<false>(), Flag<true>(), stop);
+          } else {
+            if (all) steps_to(Flag<true>(), Flag<false>(), Flag<false>(), stop);
+            else steps_to(Flag<false>(), Flag<false>(), Flag<false>(), stop);
+          }
+        }
       }
+      // without per-step bookkeeping: a lane whose last step fell in this
+      // range kept its final state since (inactive steps leave it unchanged)
+      if (!gen && valid && fsteps > nfirst && fsteps <= nend)
+#pragma unroll
+        for (int r = 0; r < NF; ++r) outq[fout + r] = q[r];
       if (n == e) {
         if (recompute) {
           phase = kBackward;
@@ -264,12 +330,12 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           n = sg * K;
           phase = kRecompute;
 #pragma unroll
-          for (int r = 0; r < NA; ++r) lam[r] = ((const R*)a.init_a)[r * st + i];
+          for (int r = 0; r < NA; ++r) lam[r] = li[r * st];
         } else {
           phase = kDone;
           if (adjoint && valid)
 #pragma unroll
-            for (int r = 0; r < S::NA; ++r) outg[aout + r] = ((const R*)a.init_a)[r * st + i];
+            for (int r = 0; r < S::NA; ++r) outg[aout + r] = li[r * st];
         }
       }
     } else if (S::ADJ) {
@@ -286,18 +352,24 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       }
       const u32 nend = (n - s0 < budget) ? s0 : n - budget;
       budget -= n - nend;
-      for (; n > nend; --n) {
-        R xc[NF], xb[NF], xp[NF];
+      auto back_to = [&](auto all_t, u32 stop) {
+        constexpr bool ALL = decltype(all_t)::value;
+        for (; n > stop; --n) {
+          R xc[NF], xb[NF], xp[NF];
 #pragma unroll
-        for (int r = 0; r < NF; ++r) {
-          xc[r] = seg[((n - s0) * NF + r) * WAVE + lane];
-          xp[r] = seg[((n - 1u - s0) * NF + r) * WAVE + lane];
-          xb[r] = R(0.5) * (xc[r] + xp[r]);
+          for (int r = 0; r < NF; ++r) {
+            xc[r] = seg[((n - s0) * NF + r) * WAVE + lane];
+            xp[r] = seg[((n - 1u - s0) * NF + r) * WAVE + lane];
+            xb[r] = R(0.5) * (xc[r] + xp[r]);
+          }
+          rk4<NA, ALL>(lam, [&](R* k, const R* y, int s) {
+            S::rhs_a(k, y, s == 0 ? xc : (s == 3 ? xp : xb), c);
+          }, h, h2, h6, n <= fsteps);
         }
-        rk4<NA>(lam, [&](R* k, const R* y, int s) {
-          S::rhs_a(k, y, s == 0 ? xc : (s == 3 ? xp : xb), c);
-        }, h, h2, h6, n <= fsteps);
-      }
+      };
+      // steps n > Tmin leave some lanes inactive; from Tmin down all are active
+      if (n > Tmin) back_to(Flag<false>(), Tmin > nend ? Tmin : nend);
+      if (n > nend) back_to(Flag<true>(), nend);
       if (n == s0) {
         if (sg == 0u) {
           phase = kDone;
@@ -422,7 +494,7 @@ void mkdirs(const std::string& path) {
   }
 }
 
-const char* kJitOptions[] = {"--gpu-architecture=gfx950", "-O3", "-ffp-contract=off",
+const char* kJitOptions[] = {"--gpu-architecture=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
 constexpr int kJitOptionCount = sizeof(kJitOptions) / sizeof(kJitOptions[0]);
 

@@ -89,7 +89,7 @@ struct LaneShape {
 };
 
 // samples of the excitation staged per lane, per chunk of steps
-constexpr uint32_t kLaneChunk = 32;
+constexpr uint32_t kLaneChunk = 16;
 constexpr uint32_t kLaneStage = kLaneChunk + 3;
 constexpr uint32_t kLaneWave = 64;
 

@@ -60,6 +60,6 @@ def test_checkpoint_interval_shortened_to_fit_lds():
     k = p.kernelInfo(adjoint=True)["checkpoint_every"]
     nf = 5
     assert 1 <= k < 256
-    assert (35 * 64 + (k + 1) * nf * 64) * 8 <= 64 * 1024
+    assert (19 * 64 + (k + 1) * nf * 64) * 8 <= 64 * 1024
     p.setTuning(checkpoint_every=4)
     assert p.kernelInfo(adjoint=True)["checkpoint_every"] == 4
