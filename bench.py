@@ -105,46 +105,94 @@ def adjoint_flops_per_step(p) -> float:
         2.0 * p.globalSystemSize
 
 
+def _timed(p, fn):
+    t0 = time.perf_counter()
+    out = fn()
+    return out, time.perf_counter() - t0, p._lastTiming()
+
+
 def fwd_adj_section(args) -> dict:
     """Forward + checkpointed adjoint on chains of 16 DOF (energy gradient
-    w.r.t. the K diagonal and the Lambda coefficients), FP64, checkpoint every
-    256 steps: persistent launch vs the plain one-launch-per-step kernel.  A
-    system-step is one forward step plus one adjoint step of one system."""
+    w.r.t. the K diagonal and the Lambda coefficients), FP64, checkpoint
+    every 256 steps requested: the persistent launch of the kernel the
+    library picks, against the plain one-launch-per-step solver of the same
+    kernel and of the tile kernel.  A system-step is one forward step plus
+    one adjoint step of one system."""
     from pasif_amd import adjoint as ADJ
     from pasif_amd import workloads as W
     w = W.chains(n_systems=args.adj_systems, d=16, steps=args.adj_steps, steps_per_sample=10)
     p = W.make_solver(w)
     p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
     p.setTuning(checkpoint_every=256)
-    p.getGradient()                                   # warm-up (packing, upload)
-    t0 = time.perf_counter()
-    g = p.getGradient()
-    wall = time.perf_counter() - t0
-    t = p._lastTiming()
+    info = p.kernelInfo(adjoint=True)
+    p.getGradient()                                   # warm-up (packing, compilation, upload)
+    g, wall, t = _timed(p, p.getGradient)
     sys_steps = w.n_systems * w.numsteps
     # work in the launch: forward twice (checkpoint pass + recompute) and the adjoint
     flops = (2 * flops_per_step(p) + adjoint_flops_per_step(p)) * w.numsteps
     achieved = flops / (t["kernel_ms"] * 1e-3) / 1e12
     sec = {"workload": f"chains {w.n_systems} x 16 DOF x {w.numsteps} steps, dt 1e-3, "
                        f"interpolated band-limited forcing, gradient of E(T) w.r.t. 16 K_ii + "
-                       f"16 Lambda per system, checkpoint 256, FP64",
+                       f"16 Lambda per system, FP64",
+           "kernel": info["kernel"], "checkpoint_every": info["checkpoint_every"],
            "value": sys_steps / wall, "unit": "system-steps/s (fwd+adj)",
-           "kernel_ms": t["kernel_ms"], "tiles": t["tiles"], "lanes_per_tile": t["lanes"],
+           "kernel_ms": t["kernel_ms"], "workgroups": t["tiles"],
            "roofline": {"achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP64_TFLOPS,
                         "flops_per_launch": flops}}
     if not args.no_plain:
         p.setSolver("plain")
-        t0 = time.perf_counter()
-        gp = p.getGradient()
-        pw = time.perf_counter() - t0
-        pt = p._lastTiming()
-        p.setSolver("persistent")
-        sec["plain"] = {"value": sys_steps / pw, "kernel_ms": pt["kernel_ms"],
-                        "launches": pt["launches"],
+        gp, pw, pt = _timed(p, p.getGradient)
+        sec["plain"] = {"kernel": info["kernel"], "value": sys_steps / pw,
+                        "kernel_ms": pt["kernel_ms"], "launches": pt["launches"],
                         "bitwise_equal_to_persistent": bool(np.array_equal(gp, g))}
-        sec["persistent_vs_plain"] = (sys_steps / wall) / (sys_steps / pw)
+        sec["persistent_vs_plain"] = pw / wall
+        if info["kernel"] != "tile":
+            p.setKernel("tile")
+            p.setSolver("persistent")
+            gt, tw, tt = _timed(p, p.getGradient)
+            sec["tile_persistent"] = {"value": sys_steps / tw, "kernel_ms": tt["kernel_ms"],
+                                      "max_rel_diff": float(np.max(np.abs(gt - g) / np.maximum(
+                                          np.abs(g), 1e-8 * np.abs(g).max())))}
+            p.setSolver("plain")
+            _, tpw, tpt = _timed(p, p.getGradient)
+            sec["plain_tile"] = {"value": sys_steps / tpw, "kernel_ms": tpt["kernel_ms"],
+                                 "launches": tpt["launches"]}
+            sec["persistent_vs_plain_tile"] = tpw / wall
+            p.setKernel("auto")
+        p.setSolver("persistent")
+    if not args.no_cpu:
+        sec["parity"] = fwd_adj_parity(w.numsteps)
     return sec
+
+
+def fwd_adj_parity(steps: int, n: int = 8) -> dict:
+    """FP64 error of the GPU forward states and gradients against the NumPy
+    oracle on ``n`` chains of the same shape over the full step count."""
+    import oracle as O
+    from pasif_amd import adjoint as ADJ
+    from pasif_amd import workloads as W
+    w = W.chains(n_systems=n, d=16, steps=steps, steps_per_sample=10)
+    J = ADJ.energy_gradient_inputs(w)
+    p = W.make_solver(w)
+    p.setJacobian(*J.as_args())
+    p.setTuning(checkpoint_every=256)
+    q, g = p.getAmplitudes(), p.getGradient()
+    t0 = time.perf_counter()
+    fwd = O.assemble_forward(w.vecM, w.vecB, w.vecK, w.vecGamma, w.vecLambda, w.vecForce,
+                             w.vecQ0)
+    adj = O.assemble_adjoint(*J.as_args(), n_fwd=fwd.n)
+    lay = O.mapped_layout(fwd.sizes, w.exc_of_system, np.full(n, steps))
+    F = O.Forcing(w.excitations, w.interp)
+    oq = O.rk4_forward(fwd, F, w.sample_rate, lay).q[0]
+    og = O.adjoint_gradient(fwd, adj, F, w.sample_rate, lay, np.full((1, adj.n), steps),
+                            checkpoint_every=256)[0]
+
+    def rel(a, o):
+        return float(np.max(np.abs(a - o) / np.maximum(np.abs(o), 1e-8 * np.abs(o).max())))
+    return {"sample": f"{n} chains x 16 DOF x {steps} steps (oracle {time.perf_counter() - t0:.1f} s)",
+            "max_rel_err_q": rel(q, oq), "max_rel_err_grad": rel(g, og),
+            "tolerance_fp64": 1e-10}
 
 
 def main():
@@ -233,7 +281,8 @@ def main():
         pw = time.perf_counter() - t0
         pt = p._lastTiming()
         p.setSolver("persistent")
-        out["plain"] = {"value": sys_steps / pw, "kernel_ms": pt["kernel_ms"],
+        out["plain"] = {"kernel": kinfo["kernel"], "value": sys_steps / pw,
+                        "kernel_ms": pt["kernel_ms"],
                         "launches": pt["launches"],
                         "bitwise_equal_to_persistent": bool(np.array_equal(qp, q))}
         out["persistent_vs_plain"] = (sys_steps / wall) / (sys_steps / pw)
@@ -242,12 +291,17 @@ def main():
     if kinfo["kernel"] != "tile":
         p.setKernel("tile")
         p.getAmplitudes()
-        t0 = time.perf_counter()
-        qt = p.getAmplitudes()
-        tw = time.perf_counter() - t0
-        out["tile_kernel"] = {"value": sys_steps / tw, "kernel_ms": p._lastTiming()["kernel_ms"],
+        qt, tw, tt = _timed(p, p.getAmplitudes)
+        out["tile_kernel"] = {"value": sys_steps / tw, "kernel_ms": tt["kernel_ms"],
                               "max_rel_diff_vs_lane": float(np.max(
                                   np.abs(qt - q) / np.maximum(np.abs(q), 1e-8 * np.abs(q).max())))}
+        if not args.no_plain:
+            p.setSolver("plain")
+            _, tpw, tpt = _timed(p, p.getAmplitudes)
+            out["plain_tile"] = {"value": sys_steps / tpw, "kernel_ms": tpt["kernel_ms"],
+                                 "launches": tpt["launches"]}
+            out["persistent_vs_plain_tile"] = tpw / wall
+            p.setSolver("persistent")
         p.setKernel("auto")
 
     # FP32 rate

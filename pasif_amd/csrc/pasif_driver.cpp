@@ -150,7 +150,8 @@ struct SideIn {
 };
 
 // One structure class of the lane kernel: its instances (one per lane,
-// sorted by step count, descending) and their device buffers.
+// sorted by step count, descending) and their device buffers.  `waves`
+// counts the class's workgroups (one progress record each).
 struct LaneCls {
   uint32_t count = 0, stride = 0, waves = 0, block_begin = 0, tmax = 0;
   uint32_t nf = 0, na = 0;
@@ -168,6 +169,7 @@ struct Packed {
   std::vector<LaneShape> lshapes;
   LaneKernel lkern;
   uint32_t lseg = 1, lblocks = 0;
+  LaneLayout llay;
   size_t llds = 0;
   DevBuf largs;
   uint32_t lanes = 0, ntiles = 0, slot_cfg = 0, tile_target = 0;
@@ -936,6 +938,9 @@ uint32_t lane_max_live() {
   return kLaneMaxLive;
 }
 constexpr uint32_t kLaneMaxClasses = 8;
+// multi-wave: waves per workgroup and values one wave's share may keep live
+constexpr uint32_t kLaneMaxWaves = 16;
+constexpr uint32_t kLaneMaxWaveLive = 100;
 constexpr size_t kLaneMaxLds = 64 * 1024;
 
 // Structure of system s: its terms in coefficient order with local factor
@@ -1005,10 +1010,6 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     sh.f = lane_side(d->fwd, s, false, eps);
     sh.adjoint = adjoint;
     if (adjoint) sh.a = lane_side(d->adj, s, true, eps);
-    if (sh.live_values() > lane_max_live()) {
-      why = "a system needs more registers than a lane holds";
-      break;
-    }
     key.clear();
     shape_key(sh.f, key);
     if (adjoint) shape_key(sh.a, key);
@@ -1023,21 +1024,72 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     }
     cls_of[s] = it->second;
   }
+  // Layout: one wave per instance group when every class fits a lane's
+  // registers; otherwise the rows of each instance spread over the fewest
+  // waves (up to kLaneMaxWaves) whose share fits.
+  LaneLayout lay;
+  const size_t el = sizeof(R);
+  const uint32_t Kreq = d->ck_every ? d->ck_every : 256;
+  if (!why) {
+    uint32_t live = 0;
+    for (const LaneShape& sh : shapes) live = std::max(live, sh.live_values());
+    if (live > lane_max_live()) {
+      uint32_t waves = 2;
+      for (LaneShape& sh : shapes) {
+        while (waves <= kLaneMaxWaves && lane_partition(sh, waves) > kLaneMaxWaveLive) ++waves;
+        if (waves > kLaneMaxWaves) {
+          why = "a system needs more registers than a workgroup of lanes holds";
+          break;
+        }
+      }
+      if (!why) {
+        for (LaneShape& sh : shapes) (void)lane_partition(sh, waves);
+        lay.waves = waves;
+        // instances per workgroup: enough workgroups for every CU, then the
+        // longest checkpoint interval (up to the requested one) that fits LDS
+        const uint64_t total = make_instances(d, adjoint).size();
+        lay.lw = total >= 256ull * 64 ? 64 : (total >= 256ull * 32 ? 32 : 16);
+        for (;; lay.lw /= 2) {
+          lay.seg = 1;
+          if (adjoint) {
+            uint32_t lo = 1, hi = Kreq;
+            lay.seg = 0;
+            while (lo <= hi) {
+              const uint32_t mid = lo + (hi - lo) / 2;
+              LaneLayout t = lay;
+              t.seg = mid;
+              if (lane_lds_bytes(shapes, int(el * 8), t) <= kMaxLdsBytes) {
+                lay.seg = mid;
+                lo = mid + 1;
+              } else {
+                hi = mid - 1;
+              }
+            }
+          }
+          const bool fits = lay.seg >= 1 && lane_lds_bytes(shapes, int(el * 8), lay) <= kMaxLdsBytes;
+          if (fits && (!adjoint || lay.seg >= std::min<uint32_t>(Kreq, 8) || lay.lw == 16)) break;
+          if (lay.lw == 16) {
+            if (!fits) why = "the systems' stage buffers do not fit LDS";
+            break;
+          }
+        }
+      }
+    } else if (adjoint) {
+      // checkpoint interval: the requested one, shortened until the segment fits LDS
+      uint32_t nf_max = 0;
+      for (const LaneShape& sh : shapes) nf_max = std::max(nf_max, sh.f.n);
+      const size_t stage = size_t(kLaneStage) * kLaneWave * el;
+      const size_t fit = (kLaneMaxLds - stage) / (size_t(nf_max) * kLaneWave * el);
+      lay.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+    }
+  }
   if (why) {
     PASIF_REQUIRE(d->kernel_path != PASIF_KERNEL_LANE, PASIF_ERR_UNSUPPORTED,
                   "lane kernel: %s", why);
     return false;
   }
-  // checkpoint interval: the requested one, shortened until the segment fits LDS
-  uint32_t nf_max = 0;
-  for (const LaneShape& sh : shapes) nf_max = std::max(nf_max, sh.f.n);
-  const size_t el = sizeof(R);
-  uint32_t K = d->ck_every ? d->ck_every : 256;
-  if (adjoint) {
-    const size_t stage = size_t(kLaneStage) * kLaneWave * el;
-    const size_t fit = (kLaneMaxLds - stage) / (size_t(nf_max) * kLaneWave * el);
-    K = uint32_t(std::max<size_t>(1, std::min<size_t>(K, fit > 0 ? fit - 1 : 0)));
-  }
+  const uint32_t K = adjoint ? lay.seg : 1;
+  const uint32_t unit = lay.waves > 1 ? lay.lw : kLaneWave;  // instances per workgroup
   std::vector<Instance> inst = make_instances(d, adjoint);
   std::vector<std::vector<const Instance*>> members(shapes.size());
   for (const Instance& in : inst) members[cls_of[in.sys]].push_back(&in);
@@ -1052,7 +1104,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     const LaneShape& sh = shapes[c];
     L->count = uint32_t(mem.size());
     L->stride = std::max<uint32_t>(kLaneWave, (L->count + kLaneWave - 1) / kLaneWave * kLaneWave);
-    L->waves = L->stride / kLaneWave;
+    L->waves = L->stride / unit;
     L->block_begin = block;
     L->nf = sh.f.n;
     L->na = adjoint ? sh.a.n : 0;
@@ -1106,12 +1158,14 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     }
   pk.lshapes = shapes;
   pk.lseg = adjoint ? K : 1;
+  pk.llay = lay;
+  pk.llay.seg = pk.lseg;
   pk.lblocks = block;
-  pk.llds = lane_lds_bytes(shapes, int(sizeof(R) * 8), pk.lseg);
-  PASIF_REQUIRE(pk.llds <= kLaneMaxLds, PASIF_ERR_UNSUPPORTED, "lane kernel LDS %zu bytes",
-                pk.llds);
+  pk.llds = lane_lds_bytes(shapes, int(sizeof(R) * 8), pk.llay);
+  PASIF_REQUIRE(pk.llds <= (lay.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
+                "lane kernel LDS %zu bytes", pk.llds);
   if (d->dev >= 0) {
-    const std::string src = lane_source(shapes, int(sizeof(R) * 8), pk.lseg);
+    const std::string src = lane_source(shapes, int(sizeof(R) * 8), pk.llay);
     std::string err;
     if (!lane_compile(src, d->dev, &pk.lkern, &err)) {
       set_err(d, "%s", err.c_str());
@@ -1130,9 +1184,9 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   pk.slot_cfg = 0;
   pk.valid = true;
   if (d->d_compute)
-    printf("[pasif] lane kernel: %zu instances in %zu structure class(es), %u waves, T max %u, "
-           "checkpoint %u\n",
-           inst.size(), shapes.size(), block, tmax, pk.lseg);
+    printf("[pasif] lane kernel: %zu instances in %zu structure class(es), %u workgroups of %u "
+           "wave(s), T max %u, checkpoint %u\n",
+           inst.size(), shapes.size(), block, lay.waves, tmax, pk.lseg);
   return true;
 }
 
@@ -1461,11 +1515,13 @@ void solve_lane(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   uint32_t ncls = uint32_t(args.size());
   hipFunction_t fn = pk.lkern.fn;
   const uint32_t blocks = pk.lblocks;
-  const uint32_t lds = uint32_t(pk.llds);
+  // multi-wave kernels declare their LDS statically
+  const uint32_t lds = pk.llay.waves > 1 ? 0u : uint32_t(pk.llds);
+  const uint32_t bdim = kLaneWave * pk.llay.waves;
   hipStream_t stream = d->stream;
   drive(d, launches, [&]() -> hipError_t {
     void* params[] = {const_cast<void**>(&dargs), &ncls};
-    return hipModuleLaunchKernel(fn, blocks, 1, 1, kLaneWave, 1, 1, lds, stream, params, nullptr);
+    return hipModuleLaunchKernel(fn, blocks, 1, 1, bdim, 1, 1, lds, stream, params, nullptr);
   });
   finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
 }
@@ -1910,10 +1966,10 @@ int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_
     d->dev = dev;
     d->kernel_path = saved;
     const std::vector<LaneShape> shapes = d->pk.lshapes;
-    const uint32_t seg = d->pk.lseg;
+    const LaneLayout lay = d->pk.llay;
     d->pk.release();
     if (rc != PASIF_OK) throw Fail{rc};
-    const std::string src = lane_source(shapes, d->precision, seg);
+    const std::string src = lane_source(shapes, d->precision, lay);
     if (len) *len = src.size() + 1;
     if (out) {
       PASIF_REQUIRE(cap > src.size(), PASIF_ERR_ARGUMENT, "source buffer holds %zu, need %zu",

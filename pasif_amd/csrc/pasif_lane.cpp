@@ -14,6 +14,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -32,6 +33,57 @@ uint32_t LaneShape::live_values() const {
   const uint32_t fwd = 4 * f.n + f.coefs();
   const uint32_t adj = adjoint ? 4 * a.n + 4 * f.n + a.coefs() : 0;
   return fwd > adj ? fwd : adj;
+}
+
+// LDS stage slot of every row that a term of `sd` reads as its own-state
+// factor (and, for the forward side, as a set point); -1 for rows nothing
+// reads (accumulators), which need no slot.
+std::vector<int> read_slots(const LaneSide& sd, bool x_is_self) {
+  std::vector<bool> used(sd.n, false);
+  for (const auto& row : sd.rows)
+    for (const LaneTerm& t : row) {
+      used[t.f[0]] = true;
+      if (x_is_self)
+        for (uint32_t a = 1; a <= t.kind; ++a) used[t.f[a]] = true;
+    }
+  std::vector<int> slot(sd.n, -1);
+  int k = 0;
+  for (uint32_t r = 0; r < sd.n; ++r)
+    if (used[r]) slot[r] = k++;
+  return slot;
+}
+
+uint32_t lane_partition(LaneShape& sh, uint32_t waves) {
+  std::vector<uint32_t> cf(waves, 0), ca(waves, 0), rws(waves, 0);
+  auto part = [&](const LaneSide& sd, std::vector<uint32_t>& owner, std::vector<uint32_t>& coefs) {
+    owner.assign(sd.n, 0);
+    std::vector<uint32_t> cost(sd.n), order(sd.n);
+    std::vector<bool> forced(sd.n, false);
+    for (uint32_t r : sd.forced) forced[r] = true;
+    for (uint32_t r = 0; r < sd.n; ++r) {
+      uint32_t c = 3 + (forced[r] ? 2 : 0);
+      for (const LaneTerm& t : sd.rows[r]) c += t.kind + 2;
+      cost[r] = c;
+      order[r] = r;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+    std::vector<uint64_t> load(waves, 0);
+    for (uint32_t r : order) {
+      uint32_t w = 0;
+      for (uint32_t v = 1; v < waves; ++v)
+        if (load[v] < load[w]) w = v;
+      owner[r] = w;
+      load[w] += cost[r];
+      coefs[w] += uint32_t(sd.rows[r].size()) + (forced[r] ? 1 : 0);
+      rws[w] += 1;
+    }
+  };
+  part(sh.f, sh.fwave, cf);
+  if (sh.adjoint) part(sh.a, sh.awave, ca);
+  uint32_t worst = 0;
+  for (uint32_t w = 0; w < waves; ++w) worst = std::max(worst, std::max(cf[w], ca[w]) + 3 * rws[w]);
+  return worst;
 }
 
 namespace {
@@ -53,24 +105,25 @@ __device__ __forceinline__ void fetch(R (&v)[STAGE], GP(const R) E, u32 Lx, u32 
     v[u] = E[i < Lx ? i : Lx - 1u];
   }
 }
-// into the lane's LDS column; 0 beyond L and for unforced lanes
-__device__ __forceinline__ void put(R* eb, u32 lane, const R (&v)[STAGE], u32 Lx, u32 b,
-                                    bool forced) {
+// into the lane's LDS column (`e` points at the lane's first sample, rows
+// SS apart); 0 beyond L and for unforced lanes
+template <u32 SS>
+__device__ __forceinline__ void put(R* e, const R (&v)[STAGE], u32 Lx, u32 b, bool forced) {
 #pragma unroll
-  for (u32 u = 0; u < STAGE; ++u)
-    eb[u * WAVE + lane] = (forced && b + u < Lx) ? v[u] : R(0);
+  for (u32 u = 0; u < STAGE; ++u) e[u * SS] = (forced && b + u < Lx) ? v[u] : R(0);
 }
 
 // Mi[1][j] e[i] + Mi[0][j] e[i+1] on the half step i0*W + j0 + t, zero
 // unless 0 < i < L (src/kernels.cu:114-124)
-__device__ __forceinline__ R interp_at(const R* eb, u32 lane, u32 b, GP(const R) wts, u32 W,
-                                       u32 Lx, u32 i0, u32 j0, u32 t) {
+template <u32 SS>
+__device__ __forceinline__ R interp_at(const R* e, u32 b, GP(const R) wts, u32 W, u32 Lx, u32 i0,
+                                       u32 j0, u32 t) {
   u32 j = j0 + t;
   const bool next = j >= W;
   j = next ? j - W : j;
   const u32 i = i0 + (next ? 1u : 0u);
-  const R ei = eb[(i - b) * WAVE + lane];
-  const R ei1 = eb[(i + 1u - b) * WAVE + lane];
+  const R ei = e[(i - b) * SS];
+  const R ei1 = e[(i + 1u - b) * SS];
   return (i > 0u && i < Lx) ? fmaR(wts[W + j], ei, wts[j] * ei1) : R(0);
 }
 
@@ -117,7 +170,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   const u64 st = a.stride;
   const u32 i = wave * WAVE + lane;
   const bool valid = i < (u32)a.count;
-  R* eb = smem;
+  R* eb = smem + lane;
   R* seg = smem + STAGE * WAVE;
   const bool adjoint = S::ADJ && a.adjoint != 0ull;
   const u32 K = adjoint ? (u32)a.ck_every : 1u;
@@ -225,14 +278,14 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         for (; n < stop; ++n) {
           R u0, u1, u3;
           if (ZOH) {
-            const R e0 = eb[(n - ebase) * WAVE + lane], e1 = eb[(n + 1u - ebase) * WAVE + lane];
+            const R e0 = eb[(n - ebase) * WAVE], e1 = eb[(n + 1u - ebase) * WAVE];
             u0 = (n > 0u && n < Lx) ? e0 : R(0);
             u1 = u0;
             u3 = e1;
           } else {
-            u0 = interp_at(eb, lane, ebase, wts, W, Lx, hi, hj, 0u);
-            u1 = interp_at(eb, lane, ebase, wts, W, Lx, hi, hj, 1u);
-            u3 = interp_at(eb, lane, ebase, wts, W, Lx, hi, hj, 2u);
+            u0 = interp_at<WAVE>(eb, ebase, wts, W, Lx, hi, hj, 0u);
+            u1 = interp_at<WAVE>(eb, ebase, wts, W, Lx, hi, hj, 1u);
+            u3 = interp_at<WAVE>(eb, ebase, wts, W, Lx, hi, hj, 2u);
             hj += 2u;
             if (hj >= W) {
               hj -= W;
@@ -283,11 +336,11 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           // now), and the next chunk's are requested
           const u32 base = W ? hi : n;
           if (have && pbase == base) {
-            put(eb, lane, pre, Lx, base, forced);
+            put<WAVE>(eb, pre, Lx, base, forced);
           } else {
             R v[STAGE];
             fetch(v, E, Lx, base);
-            put(eb, lane, v, Lx, base, forced);
+            put<WAVE>(eb, v, Lx, base, forced);
           }
           ebase = base;
           left = CHUNK;
@@ -401,6 +454,353 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
     pst[2] = sg;
   }
 }
+// ---------------------------------------------------------------- multi-wave
+// LDS-only workgroup barrier: global memory stays unordered, so loads in
+// flight are not waited for.
+__device__ __forceinline__ void bar() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+#define YSTR (NYMAX * LW)
+// adjoint set point j at stage s: x_n, xbar = (x_n + x_{n-1}) / 2, x_{n-1}
+__device__ __forceinline__ R xs(const R* Xc, const R* Xp, int j, int s) {
+  return s == 0 ? Xc[j * LW] : (s == 3 ? Xp[j * LW] : R(0.5) * (Xc[j * LW] + Xp[j * LW]));
+}
+
+// One RK4 step of the NR rows a wave owns.  Stage values of every row go
+// through the LDS buffers Y[2][NYMAX][LW]: stage s reads buffer s % 2 and
+// writes buffer (s + 1) % 2; stage 4 writes the new state into buffer 0,
+// which the next step's first stage reads.  One barrier per stage.
+template <int NR, bool ALL, class F, class Row>
+__device__ __forceinline__ void rk4_mw(R* q, R* Y, F rhs, Row row, R h, R h2, R h6, bool active) {
+  R k[NR > 0 ? NR : 1], acc[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const R* Yc = Y + (s & 1) * YSTR;
+    R* Yn = Y + ((s + 1) & 1) * YSTR;
+    rhs(k, Yc, s);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      R v;
+      if (s == 0) {
+        acc[r] = k[r];
+        v = fmaR(h2, k[r], q[r]);
+      } else if (s == 1) {
+        acc[r] = fmaR(R(2), k[r], acc[r]);
+        v = fmaR(h2, k[r], q[r]);
+      } else if (s == 2) {
+        acc[r] = fmaR(R(2), k[r], acc[r]);
+        v = fmaR(h, k[r], q[r]);
+      } else {
+        acc[r] = acc[r] + k[r];
+        const R qn = fmaR(h6, acc[r], q[r]);
+        q[r] = (ALL || active) ? qn : q[r];
+        v = q[r];
+      }
+      if (row(r) >= 0) Yn[row(r) * LW] = v;
+    }
+    bar();
+  }
+}
+
+// One wave of a multi-wave workgroup: the rows SW owns of LW instances, same
+// phases, budget and resumable state as run().  Every wave of the workgroup
+// follows the same control flow (it depends only on the workgroup's step
+// counts and progress), so all reach the same barriers.  Lanes >= LW mirror
+// lane % LW: they compute the same values and store them to the same
+// addresses.  LDS: Y [2][NYMAX][LW], seg [SEGK+1][NFMAX][LW] (adjoint), and a
+// [STAGE][LW] sample region per wave that drives forced rows.
+template <class S, class SW>
+__device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
+  constexpr int NF = S::NF;
+  constexpr int RF = SW::RF > 0 ? SW::RF : 1;
+  constexpr int RA = SW::RA > 0 ? SW::RA : 1;
+  constexpr int TFW = SW::TF > 0 ? SW::TF : 1;
+  constexpr int TAW = SW::TA > 0 ? SW::TA : 1;
+  constexpr bool DRIVES = SW::FSLOT >= 0;
+  const u32 le = threadIdx.x & (LW - 1u);
+  const u64 st = a.stride;
+  const u32 i = wg * LW + le;
+  const bool valid = i < (u32)a.count;
+  R* Y = smem + le;
+  R* seg = smem + 2 * YSTR + le;
+  R* eb = smem + 2 * YSTR + SEGN + (DRIVES ? SW::FSLOT : 0) * (STAGE * LW) + le;
+  const bool adjoint = S::ADJ && a.adjoint != 0ull;
+  const u32 K = adjoint ? (u32)a.ck_every : 1u;
+  const u32 W = (u32)a.interp_w;
+  const u32 Lx = (u32)a.exc_len;
+  const u32 SE = (u32)a.save_every;
+  const R h = R(a.h), h2 = R(a.h2), h6 = R(a.h6);
+  GP(const u32) steps = (GP(const u32))a.steps;
+  const u32 T = uni(steps[wg * LW]);
+  const u32 Tmin = uni(steps[umin(wg * LW + LW, (u32)a.count) - 1u]);
+  const u32 fsteps = steps[i];
+  const i32 exc = ((GP(const i32))a.exc)[i];
+  const bool forced = exc >= 0;
+  const u32 fout = ((GP(const u32))a.out_f)[i];
+  const u32 aout = ((GP(const u32))a.out_a)[i];
+  GP(const R) cf = (GP(const R))a.coef_f + i;
+  GP(const R) ca = (GP(const R))a.coef_a + i;
+  GP(const R) qi = (GP(const R))a.init_f + i;
+  GP(const R) li = (GP(const R))a.init_a + i;
+  GP(R) qs = (GP(R))a.qstate + i;
+  GP(R) ms = (GP(R))a.mxstate + i;
+  GP(R) ls = (GP(R))a.lstate + i;
+  GP(R) ck = (GP(R))a.ckpt + i;
+  GP(R) sb = (GP(R))a.segbuf + i;
+  GP(R) outq = (GP(R))a.out_q;
+  GP(R) outm = (GP(R))a.out_max;
+  GP(R) outg = (GP(R))a.out_grad;
+  GP(R) traj = (GP(R))a.traj;
+  GP(const R) wts = (GP(const R))a.interp;
+  GP(const R) E = (GP(const R))a.excit + (u64)(forced ? exc : 0) * Lx;
+  GP(u32) pst = (GP(u32))a.progress + 4 * wg;
+  u32 n = uni(pst[0]);
+  u32 phase = uni(pst[1]);
+  u32 sg = uni(pst[2]);
+  u32 budget = (u32)a.budget;
+  auto slotf = [](int r) { return SW::yf(r); };
+  auto slota = [](int r) { return SW::ya(r); };
+
+  R q[RF], mx[RF], lam[RA];
+#pragma unroll
+  for (int r = 0; r < RF; ++r) mx[r] = R(0);
+#pragma unroll
+  for (int r = 0; r < RA; ++r) lam[r] = R(0);
+  if (n == 0u && phase == kForward) {
+#pragma unroll
+    for (int r = 0; r < SW::RF; ++r) q[r] = qi[SW::rf(r) * st];
+    if (fsteps == 0u && valid)
+#pragma unroll
+      for (int r = 0; r < SW::RF; ++r) outq[fout + SW::rf(r)] = q[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < SW::RF; ++r) {
+      q[r] = qs[SW::rf(r) * st];
+      if (a.window) mx[r] = ms[SW::rf(r) * st];
+    }
+  }
+  if (adjoint && phase != kForward)
+#pragma unroll
+    for (int r = 0; r < SW::RA; ++r) lam[r] = ls[SW::ra(r) * st];
+  bool seg_here = false;  // LDS holds the current segment up to step n
+
+  while (budget > 0u && phase != kDone) {
+    if (phase != kBackward) {
+      const bool recompute = phase == kRecompute;
+      R c[TFW];
+      SW::load_f(c, cf, st);
+      const u32 s0 = sg * K;
+      if (recompute) {
+        if (n == s0) {
+#pragma unroll
+          for (int r = 0; r < SW::RF; ++r) {
+            q[r] = ck[((u64)sg * NF + SW::rf(r)) * st];
+            seg[SW::rf(r) * LW] = q[r];
+            sb[SW::rf(r) * st] = q[r];
+          }
+        } else if (!seg_here) {
+          for (u32 m = 0; m <= n - s0; ++m)
+#pragma unroll
+            for (int r = 0; r < SW::RF; ++r)
+              seg[(m * NFMAX + SW::rf(r)) * LW] = sb[((u64)m * NF + SW::rf(r)) * st];
+        }
+        seg_here = true;
+      }
+      const u32 e = recompute ? umin(T, s0 + K) : T;
+      const u32 nend = (e - n < budget) ? e : n + budget;
+      budget -= nend - n;
+      const u32 nfirst = n;
+      const bool gen = recompute || adjoint || SE != 0u || a.window != 0ull;
+      u32 next_ck = ((n + K - 1u) / K) * K;
+      u32 next_save = SE ? ((n + SE - 1u) / SE) * SE : 0u;
+      u32 hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
+      u32 ebase = 0, left = 0, pbase = 0;
+      bool have = false;
+      R pre[DRIVES ? STAGE : 1];
+      // every row's state is visible before the first stage reads it
+#pragma unroll
+      for (int r = 0; r < SW::RF; ++r)
+        if (SW::yf(r) >= 0) Y[SW::yf(r) * LW] = q[r];
+      bar();
+      auto steps_to = [&](auto all_t, auto gen_t, auto zoh_t, u32 stop) {
+        constexpr bool ALL = decltype(all_t)::value;
+        constexpr bool GEN = decltype(gen_t)::value;
+        constexpr bool ZOH = decltype(zoh_t)::value;
+        for (; n < stop; ++n) {
+          R u0 = R(0), u1 = R(0), u3 = R(0);
+          if constexpr (DRIVES) {
+            if (ZOH) {
+              const R e0 = eb[(n - ebase) * LW], e1 = eb[(n + 1u - ebase) * LW];
+              u0 = (n > 0u && n < Lx) ? e0 : R(0);
+              u1 = u0;
+              u3 = e1;
+            } else {
+              u0 = interp_at<LW>(eb, ebase, wts, W, Lx, hi, hj, 0u);
+              u1 = interp_at<LW>(eb, ebase, wts, W, Lx, hi, hj, 1u);
+              u3 = interp_at<LW>(eb, ebase, wts, W, Lx, hi, hj, 2u);
+            }
+          }
+          if (!ZOH) {
+            hj += 2u;
+            if (hj >= W) {
+              hj -= W;
+              hi += 1u;
+            }
+          }
+          if (GEN && adjoint && !recompute && n == next_ck) {
+#pragma unroll
+            for (int r = 0; r < SW::RF; ++r) ck[((u64)(n / K) * NF + SW::rf(r)) * st] = q[r];
+            next_ck += K;
+          }
+          rk4_mw<SW::RF, ALL>(q, Y, [&](R* k, const R* Yc, int s) {
+            SW::rhs_f(k, Yc, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
+          }, slotf, h, h2, h6, n < fsteps);
+          if (GEN) {
+            if (recompute) {
+              const u32 m = n + 1u - s0;
+#pragma unroll
+              for (int r = 0; r < SW::RF; ++r) {
+                seg[(m * NFMAX + SW::rf(r)) * LW] = q[r];
+                sb[((u64)m * NF + SW::rf(r)) * st] = q[r];
+              }
+            } else {
+              if (SE && n == next_save) {
+                const u32 slot = n / SE;
+                if (slot < (u32)a.nslots && valid)
+#pragma unroll
+                  for (int r = 0; r < SW::RF; ++r)
+                    traj[(u64)slot * a.out_len + fout + SW::rf(r)] = q[r];
+                next_save += SE;
+              }
+              if (a.window && (ALL || n < fsteps) && n + 1u + (u32)a.window > fsteps)
+#pragma unroll
+                for (int r = 0; r < SW::RF; ++r) mx[r] = fmaxR(mx[r], absR(q[r]));
+              if (n + 1u == fsteps && valid) {
+#pragma unroll
+                for (int r = 0; r < SW::RF; ++r) {
+                  outq[fout + SW::rf(r)] = q[r];
+                  if (a.window) outm[fout + SW::rf(r)] = mx[r];
+                }
+              }
+            }
+          }
+        }
+      };
+      while (n < nend) {
+        if (left == 0u) {
+          if constexpr (DRIVES) {
+            const u32 base = W ? hi : n;
+            if (have && pbase == base) {
+              put<LW>(eb, pre, Lx, base, forced);
+            } else {
+              R v[STAGE];
+              fetch(v, E, Lx, base);
+              put<LW>(eb, v, Lx, base, forced);
+            }
+            ebase = base;
+            pbase = W ? (2u * (n + CHUNK)) / W : n + CHUNK;
+            fetch(pre, E, Lx, pbase);
+            have = true;
+          }
+          left = CHUNK;
+        }
+        u32 stop = umin(nend, n + left);
+        if (n < Tmin && Tmin < stop) stop = Tmin;
+        left -= stop - n;
+        const bool all = n < Tmin;
+        // one step is long here: per-step bookkeeping checks cost little
+        if (W == 0u) {
+          if (all) steps_to(Flag<true>(), Flag<true>(), Flag<true>(), stop);
+          else steps_to(Flag<false>(), Flag<true>(), Flag<true>(), stop);
+        } else {
+          if (all) steps_to(Flag<true>(), Flag<true>(), Flag<false>(), stop);
+          else steps_to(Flag<false>(), Flag<true>(), Flag<false>(), stop);
+        }
+      }
+      if (!gen && valid && fsteps > nfirst && fsteps <= nend)
+#pragma unroll
+        for (int r = 0; r < SW::RF; ++r) outq[fout + SW::rf(r)] = q[r];
+      if (n == e) {
+        if (recompute) {
+          phase = kBackward;
+        } else if (adjoint && T > 0u) {
+          sg = (T - 1u) / K;
+          n = sg * K;
+          phase = kRecompute;
+#pragma unroll
+          for (int r = 0; r < SW::RA; ++r) lam[r] = li[SW::ra(r) * st];
+        } else {
+          phase = kDone;
+          if (adjoint && valid)
+#pragma unroll
+            for (int r = 0; r < SW::RA; ++r) outg[aout + SW::ra(r)] = li[SW::ra(r) * st];
+        }
+      }
+      bar();
+    } else if (S::ADJ) {
+      R c[TAW];
+      SW::load_a(c, ca, st);
+      const u32 s0 = sg * K;
+      if (!seg_here) {
+        for (u32 m = 0; m <= n - s0; ++m)
+#pragma unroll
+          for (int r = 0; r < SW::RF; ++r)
+            seg[(m * NFMAX + SW::rf(r)) * LW] = sb[((u64)m * NF + SW::rf(r)) * st];
+        seg_here = true;
+      }
+#pragma unroll
+      for (int r = 0; r < SW::RA; ++r)
+        if (SW::ya(r) >= 0) Y[SW::ya(r) * LW] = lam[r];
+      // every wave's segment rows and adjoint state are visible
+      bar();
+      const u32 nend = (n - s0 < budget) ? s0 : n - budget;
+      budget -= n - nend;
+      auto back_to = [&](auto all_t, u32 stop) {
+        constexpr bool ALL = decltype(all_t)::value;
+        for (; n > stop; --n) {
+          const R* Xc = seg + (n - s0) * (NFMAX * LW);
+          const R* Xp = seg + (n - 1u - s0) * (NFMAX * LW);
+          rk4_mw<SW::RA, ALL>(lam, Y, [&](R* k, const R* Yc, int s) {
+            SW::rhs_a(k, Yc, Xc, Xp, c, s);
+          }, slota, h, h2, h6, n <= fsteps);
+        }
+      };
+      if (n > Tmin) back_to(Flag<false>(), Tmin > nend ? Tmin : nend);
+      if (n > nend) back_to(Flag<true>(), nend);
+      if (n == s0) {
+        if (sg == 0u) {
+          phase = kDone;
+          if (valid)
+#pragma unroll
+            for (int r = 0; r < SW::RA; ++r) outg[aout + SW::ra(r)] = lam[r];
+        } else {
+          sg -= 1u;
+          n = sg * K;
+          phase = kRecompute;
+          seg_here = false;
+        }
+      }
+      bar();
+    } else {
+      phase = kDone;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < SW::RF; ++r) {
+    qs[SW::rf(r) * st] = q[r];
+    if (a.window) ms[SW::rf(r) * st] = mx[r];
+  }
+  if (adjoint)
+#pragma unroll
+    for (int r = 0; r < SW::RA; ++r) ls[SW::ra(r) * st] = lam[r];
+  if (threadIdx.x == 0u) {
+    pst[0] = n;
+    pst[1] = phase;
+    pst[2] = sg;
+  }
+}
 )LANE";
 
 std::string type_text(const char* t) {
@@ -409,62 +809,123 @@ std::string type_text(const char* t) {
   return s;
 }
 
-// Right-hand side of one side of a class: k[r] for every row.  Factor 0 of
-// each term reads `S`, the others `X`.
+using Namer = std::function<std::string(uint32_t)>;
+
+// Global coefficient slots of a side: terms row by row, then the force of
+// every forced row (the order the host packs coefficients in).
+struct Slots {
+  std::vector<uint32_t> base;   // first term slot of each row
+  std::vector<int> force;       // force slot of each row, -1 if unforced
+};
+Slots slots_of(const LaneSide& sd) {
+  Slots sl;
+  sl.base.resize(sd.n);
+  sl.force.assign(sd.n, -1);
+  uint32_t t = 0;
+  for (uint32_t r = 0; r < sd.n; ++r) {
+    sl.base[r] = t;
+    t += uint32_t(sd.rows[r].size());
+  }
+  for (size_t i = 0; i < sd.forced.size(); ++i) sl.force[sd.forced[i]] = int(t + i);
+  return sl;
+}
+
+// Statements computing k[kidx] for row r: each kind summed in term order
+// (first term a product, then fused multiply-adds), the kinds added in
+// order, then the force.  coef(g) names global coefficient slot g; S(j) and
+// X(j) name factor j of the side's own state and of the set point.
+void emit_row(std::ostringstream& o, const LaneSide& sd, const Slots& sl, uint32_t r,
+              uint32_t kidx, bool forced_input, const Namer& coef, const Namer& S,
+              const Namer& X) {
+  std::string sums[4];
+  uint32_t g = sl.base[r];
+  for (int kind = 0; kind < 4; ++kind) {
+    std::string acc;
+    uint32_t gt = sl.base[r];
+    for (const LaneTerm& tm : sd.rows[r]) {
+      const uint32_t slot = gt++;
+      if (int(tm.kind) != kind) continue;
+      std::string prod;
+      if (kind == 0) {
+        prod = acc.empty() ? coef(slot) + " * " + S(tm.f[0])
+                           : "fmaR(" + coef(slot) + ", " + S(tm.f[0]) + ", " + acc + ")";
+      } else {
+        std::string l = coef(slot) + " * " + S(tm.f[0]);
+        for (int f = 1; f < int(kind); ++f) l = "(" + l + ") * " + X(tm.f[f]);
+        const std::string last = X(tm.f[kind]);
+        prod = acc.empty() ? "(" + l + ") * " + last : "fmaR(" + l + ", " + last + ", " + acc + ")";
+      }
+      const std::string name = "s" + std::to_string(kind) + "_" + std::to_string(r);
+      if (acc.empty())
+        o << "    R " << name << " = " << prod << ";\n";
+      else
+        o << "    " << name << " = " << prod << ";\n";
+      acc = name;
+    }
+    sums[kind] = acc;
+  }
+  (void)g;
+  std::string k;
+  for (int kind = 0; kind < 4; ++kind) {
+    if (sums[kind].empty()) continue;
+    k = k.empty() ? sums[kind] : "(" + k + " + " + sums[kind] + ")";
+  }
+  if (k.empty()) k = "R(0)";
+  if (forced_input && sl.force[r] >= 0) k = "fmaR(" + coef(uint32_t(sl.force[r])) + ", u, " + k + ")";
+  o << "    k[" << kidx << "] = " << k << ";\n";
+}
+
+// Single-wave right-hand side: every row, coefficients c[global slot].
 void emit_rhs(std::ostringstream& o, const LaneSide& sd, bool forced_input, const char* S,
               const char* X) {
-  uint32_t t = 0;
-  std::vector<uint32_t> fslot(sd.n, ~0u);
-  {
-    uint32_t nterms = 0;
-    for (const auto& r : sd.rows) nterms += uint32_t(r.size());
-    for (size_t i = 0; i < sd.forced.size(); ++i) fslot[sd.forced[i]] = nterms + uint32_t(i);
-  }
-  for (uint32_t r = 0; r < sd.n; ++r) {
-    const auto& terms = sd.rows[r];
-    std::string sums[4];
-    for (int kind = 0; kind < 4; ++kind) {
-      std::string acc;
-      for (const LaneTerm& tm : terms) {
-        if (int(tm.kind) != kind) continue;
-        std::ostringstream prod;
-        const uint32_t slot = t++;
-        if (kind == 0) {
-          if (acc.empty())
-            prod << "c[" << slot << "] * " << S << "[" << tm.f[0] << "]";
-          else
-            prod << "fmaR(c[" << slot << "], " << S << "[" << tm.f[0] << "], " << acc << ")";
-        } else {
-          std::ostringstream lead;
-          lead << "c[" << slot << "] * " << S << "[" << tm.f[0] << "]";
-          std::string l = lead.str();
-          for (int f = 1; f < int(kind); ++f)
-            l = "(" + l + ") * " + X + "[" + std::to_string(tm.f[f]) + "]";
-          const std::string last = std::string(X) + "[" + std::to_string(tm.f[kind]) + "]";
-          if (acc.empty())
-            prod << "(" << l << ") * " << last;
-          else
-            prod << "fmaR(" << l << ", " << last << ", " << acc << ")";
-        }
-        const std::string name = "s" + std::to_string(kind) + "_" + std::to_string(r);
-        if (acc.empty())
-          o << "    R " << name << " = " << prod.str() << ";\n";
-        else
-          o << "    " << name << " = " << prod.str() << ";\n";
-        acc = name;
-      }
-      sums[kind] = acc;
+  const Slots sl = slots_of(sd);
+  const std::string sn(S), xn(X);
+  for (uint32_t r = 0; r < sd.n; ++r)
+    emit_row(o, sd, sl, r, r, forced_input,
+             [](uint32_t gs) { return "c[" + std::to_string(gs) + "]"; },
+             [&](uint32_t j) { return sn + "[" + std::to_string(j) + "]"; },
+             [&](uint32_t j) { return xn + "[" + std::to_string(j) + "]"; });
+}
+
+// One wave's rows of a side in a multi-wave class: its own coefficient
+// array (local slots, loaded from the global slots listed in *gslots).
+void emit_wave_side(std::ostringstream& o, const LaneSide& sd, const std::vector<uint32_t>& rows,
+                    bool forced_input, bool adjoint_side, std::vector<uint32_t>* gslots) {
+  const Slots sl = slots_of(sd);
+  std::map<uint32_t, uint32_t> local;
+  auto coef = [&](uint32_t gs) {
+    auto it = local.find(gs);
+    if (it == local.end()) {
+      it = local.emplace(gs, uint32_t(gslots->size())).first;
+      gslots->push_back(gs);
     }
-    std::string k;
-    for (int kind = 0; kind < 4; ++kind) {
-      if (sums[kind].empty()) continue;
-      k = k.empty() ? sums[kind] : "(" + k + " + " + sums[kind] + ")";
-    }
-    if (k.empty()) k = "R(0)";
-    if (forced_input && fslot[r] != ~0u)
-      k = "fmaR(c[" + std::to_string(fslot[r]) + "], u, " + k + ")";
-    o << "    k[" << r << "] = " << k << ";\n";
+    return "c[" + std::to_string(it->second) + "]";
+  };
+  const std::vector<int> ys = read_slots(sd, !adjoint_side);
+  auto S = [&](uint32_t j) { return "Y[" + std::to_string(ys[j]) + " * LW]"; };
+  auto X = [&](uint32_t j) {
+    return adjoint_side ? "xs(Xc, Xp, " + std::to_string(j) + ", s)"
+                        : "Y[" + std::to_string(ys[j]) + " * LW]";
+  };
+  for (size_t lr = 0; lr < rows.size(); ++lr) {
+    emit_row(o, sd, sl, rows[lr], uint32_t(lr), forced_input, coef, S, X);
+    // factors are re-read from LDS per row rather than all kept in registers
+    o << "    asm volatile(\"\" ::: \"memory\");\n";
   }
+}
+
+std::string ternary_imap(const std::vector<int>& v) {
+  std::string e = "-1";
+  for (size_t i = v.size(); i-- > 0;)
+    e = "r == " + std::to_string(i) + " ? " + std::to_string(v[i]) + " : (" + e + ")";
+  return e;
+}
+
+std::string ternary_map(const std::vector<uint32_t>& v) {
+  std::string e = "0";
+  for (size_t i = v.size(); i-- > 0;)
+    e = "r == " + std::to_string(i) + " ? " + std::to_string(v[i]) + " : (" + e + ")";
+  return e;
 }
 
 uint64_t fnv1a(const std::string& s) {
@@ -503,10 +964,47 @@ std::map<std::pair<int, uint64_t>, LaneKernel> g_jit;
 
 }  // namespace
 
-std::string lane_source(const std::vector<LaneShape>& shapes, int bits, uint32_t seg_len) {
+namespace {
+
+struct MwDims {
+  uint32_t nymax = 1, nfmax = 1, nfslot = 0;
+  bool adjoint = false;
+};
+
+MwDims mw_dims(const std::vector<LaneShape>& shapes, uint32_t waves) {
+  MwDims d;
+  for (const LaneShape& sh : shapes) {
+    auto count = [](const std::vector<int>& v) {
+      uint32_t k = 0;
+      for (int x : v) k += x >= 0 ? 1 : 0;
+      return k;
+    };
+    d.nymax = std::max(d.nymax, count(read_slots(sh.f, true)));
+    if (sh.adjoint) d.nymax = std::max(d.nymax, count(read_slots(sh.a, false)));
+    d.nfmax = std::max(d.nfmax, sh.f.n);
+    d.adjoint = d.adjoint || sh.adjoint;
+    std::vector<bool> drives(waves, false);
+    for (uint32_t r : sh.f.forced) drives[sh.fwave[r]] = true;
+    uint32_t k = 0;
+    for (bool b : drives) k += b ? 1 : 0;
+    d.nfslot = std::max(d.nfslot, k);
+  }
+  return d;
+}
+
+size_t mw_lds_values(const MwDims& d, const LaneLayout& lay) {
+  return size_t(lay.lw) * (2 * size_t(d.nymax) + (d.adjoint ? size_t(lay.seg + 1) * d.nfmax : 0) +
+                           size_t(d.nfslot) * kLaneStage);
+}
+
+}  // namespace
+
+std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const LaneLayout& lay) {
+  const bool mw = lay.waves > 1;
   std::ostringstream o;
   o << "// generated lane kernel: " << shapes.size() << " structure class(es), FP" << bits
-    << ", segment " << seg_len << "\n";
+    << ", " << lay.waves << " wave(s) per workgroup, " << (mw ? lay.lw : kLaneWave)
+    << " instances per workgroup, segment " << lay.seg << "\n";
   o << "typedef unsigned int u32;\ntypedef int i32;\ntypedef unsigned long long u64;\n";
   o << "typedef " << (bits == 64 ? "double" : "float") << " R;\n";
   o << "struct LaneArgs {\n";
@@ -516,6 +1014,11 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, uint32_t
   o << "};\n";
   o << "#define WAVE " << kLaneWave << "u\n#define CHUNK " << kLaneChunk << "u\n#define STAGE "
     << kLaneStage << "u\n";
+  MwDims dims;
+  if (mw) dims = mw_dims(shapes, lay.waves);
+  o << "#define LW " << (mw ? lay.lw : kLaneWave) << "u\n#define NYMAX " << dims.nymax
+    << "u\n#define NFMAX " << dims.nfmax << "u\n#define SEGK " << lay.seg << "u\n#define SEGN "
+    << (dims.adjoint ? size_t(lay.seg + 1) * dims.nfmax * lay.lw : 0) << "u\n";
   if (bits == 64) {
     o << "__device__ __forceinline__ R fmaR(R a, R b, R c) { return __builtin_fma(a, b, c); }\n"
          "__device__ __forceinline__ R absR(R a) { return __builtin_fabs(a); }\n"
@@ -532,31 +1035,106 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, uint32_t
     o << "  enum { NF = " << sh.f.n << ", NA = " << (sh.adjoint ? sh.a.n : 0)
       << ", TF = " << sh.f.coefs() << ", TA = " << (sh.adjoint ? sh.a.coefs() : 0)
       << ", ADJ = " << (sh.adjoint ? 1 : 0) << " };\n";
-    o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* y, const R* c, R u) {\n";
-    emit_rhs(o, sh.f, true, "y", "y");
-    o << "  }\n";
-    o << "  static __device__ __forceinline__ void rhs_a(R* k, const R* y, const R* x, const R* c) "
-         "{\n";
-    if (sh.adjoint) emit_rhs(o, sh.a, false, "y", "x");
-    o << "  }\n};\n";
+    if (!mw) {
+      o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* y, const R* c, R u) {\n";
+      emit_rhs(o, sh.f, true, "y", "y");
+      o << "  }\n";
+      o << "  static __device__ __forceinline__ void rhs_a(R* k, const R* y, const R* x, "
+           "const R* c) {\n";
+      if (sh.adjoint) emit_rhs(o, sh.a, false, "y", "x");
+      o << "  }\n";
+    }
+    o << "};\n";
+    if (!mw) continue;
+    int fslot = 0;
+    for (uint32_t w = 0; w < lay.waves; ++w) {
+      std::vector<uint32_t> rf, ra;
+      for (uint32_t r = 0; r < sh.f.n; ++r)
+        if (sh.fwave[r] == w) rf.push_back(r);
+      if (sh.adjoint)
+        for (uint32_t r = 0; r < sh.a.n; ++r)
+          if (sh.awave[r] == w) ra.push_back(r);
+      bool drives = false;
+      for (uint32_t r : sh.f.forced) drives = drives || sh.fwave[r] == w;
+      std::ostringstream bf, ba;
+      std::vector<uint32_t> gf, ga;
+      emit_wave_side(bf, sh.f, rf, true, false, &gf);
+      if (sh.adjoint) emit_wave_side(ba, sh.a, ra, false, true, &ga);
+      o << "struct C" << c << "W" << w << " {\n";
+      o << "  enum { RF = " << rf.size() << ", RA = " << ra.size() << ", TF = " << gf.size()
+        << ", TA = " << ga.size() << ", FSLOT = " << (drives ? fslot : -1) << " };\n";
+      if (drives) ++fslot;
+      o << "  static __device__ __forceinline__ constexpr int rf(int r) { return "
+        << ternary_map(rf) << "; }\n";
+      o << "  static __device__ __forceinline__ constexpr int ra(int r) { return "
+        << ternary_map(ra) << "; }\n";
+      {
+        const std::vector<int> yfs = read_slots(sh.f, true);
+        std::vector<int> yf, ya;
+        for (uint32_t r : rf) yf.push_back(yfs[r]);
+        if (sh.adjoint) {
+          const std::vector<int> yas = read_slots(sh.a, false);
+          for (uint32_t r : ra) ya.push_back(yas[r]);
+        }
+        o << "  static __device__ __forceinline__ constexpr int yf(int r) { return "
+          << ternary_imap(yf) << "; }\n";
+        o << "  static __device__ __forceinline__ constexpr int ya(int r) { return "
+          << ternary_imap(ya) << "; }\n";
+      }
+      o << "  static __device__ __forceinline__ void load_f(R* c, GP(const R) cf, u64 st) {\n";
+      for (size_t t = 0; t < gf.size(); ++t)
+        o << "    c[" << t << "] = cf[" << gf[t] << "ull * st];\n";
+      o << "  }\n";
+      o << "  static __device__ __forceinline__ void load_a(R* c, GP(const R) ca, u64 st) {\n";
+      for (size_t t = 0; t < ga.size(); ++t)
+        o << "    c[" << t << "] = ca[" << ga[t] << "ull * st];\n";
+      o << "  }\n";
+      o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* Y, const R* c, R u) {\n"
+        << bf.str() << "  }\n";
+      o << "  static __device__ __forceinline__ void rhs_a(R* k, const R* Y, const R* Xc, "
+           "const R* Xp, const R* c, int s) {\n"
+        << ba.str() << "  }\n";
+      o << "};\n";
+    }
   }
-  o << "extern \"C\" __global__ void __launch_bounds__(" << kLaneWave
-    << ") pasif_lane_kernel(const LaneArgs* __restrict__ cls, u32 ncls) {\n"
-       "  extern __shared__ R smem[];\n"
-       "  const u32 b = blockIdx.x;\n"
-       "  u32 c = 0;\n"
-       "  while (c + 1u < ncls && b >= (u32)cls[c + 1u].block_begin) ++c;\n"
-       "  c = uni(c);\n"
-       "  switch (c) {\n";
-  for (size_t c = 0; c < shapes.size(); ++c)
-    o << "    case " << c << ": run<C" << c << ">(cls[" << c << "], b - (u32)cls[" << c
-      << "].block_begin, smem); break;\n";
-  o << "    default: break;\n  }\n}\n";
-  (void)seg_len;
+  if (!mw) {
+    o << "extern \"C\" __global__ void __launch_bounds__(" << kLaneWave
+      << ") pasif_lane_kernel(const LaneArgs* __restrict__ cls, u32 ncls) {\n"
+         "  extern __shared__ R smem[];\n"
+         "  const u32 b = blockIdx.x;\n"
+         "  u32 c = 0;\n"
+         "  while (c + 1u < ncls && b >= (u32)cls[c + 1u].block_begin) ++c;\n"
+         "  c = uni(c);\n"
+         "  switch (c) {\n";
+    for (size_t c = 0; c < shapes.size(); ++c)
+      o << "    case " << c << ": run<C" << c << ">(cls[" << c << "], b - (u32)cls[" << c
+        << "].block_begin, smem); break;\n";
+    o << "    default: break;\n  }\n}\n";
+  } else {
+    o << "extern \"C\" __global__ void __launch_bounds__(" << kLaneWave * lay.waves
+      << ") pasif_lane_kernel(const LaneArgs* __restrict__ cls, u32 ncls) {\n"
+      << "  __shared__ R smem[" << mw_lds_values(dims, lay) << "];\n"
+      << "  const u32 b = blockIdx.x;\n"
+         "  const u32 wid = uni(threadIdx.x >> 6);\n"
+         "  u32 c = 0;\n"
+         "  while (c + 1u < ncls && b >= (u32)cls[c + 1u].block_begin) ++c;\n"
+         "  c = uni(c);\n"
+         "  switch (c) {\n";
+    for (size_t c = 0; c < shapes.size(); ++c) {
+      o << "    case " << c << ":\n      switch (wid) {\n";
+      for (uint32_t w = 0; w < lay.waves; ++w)
+        o << "        case " << w << ": run_mw<C" << c << ", C" << c << "W" << w << ">(cls[" << c
+          << "], b - (u32)cls[" << c << "].block_begin, smem); break;\n";
+      o << "        default: break;\n      }\n      break;\n";
+    }
+    o << "    default: break;\n  }\n}\n";
+  }
   return o.str();
 }
 
-size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, uint32_t seg_len) {
+size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const LaneLayout& lay) {
+  const size_t el = bits == 64 ? 8 : 4;
+  if (lay.waves > 1) return mw_lds_values(mw_dims(shapes, lay.waves), lay) * el;
   uint32_t nf = 0;
   bool adj = false;
   for (const LaneShape& s : shapes) {
@@ -565,9 +1143,8 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, uint32_t s
       nf = s.f.n > nf ? s.f.n : nf;
     }
   }
-  const size_t el = bits == 64 ? 8 : 4;
   size_t v = size_t(kLaneStage) * kLaneWave;
-  if (adj) v += size_t(seg_len + 1) * nf * kLaneWave;
+  if (adj) v += size_t(lay.seg + 1) * nf * kLaneWave;
   return v * el;
 }
 

@@ -86,7 +86,24 @@ struct LaneShape {
   bool adjoint = false;
   // doubles the larger of the forward and adjoint phases keeps live
   uint32_t live_values() const;
+  // Multi-wave layout (kernel waves > 1): the wave that owns each forward /
+  // adjoint row.  Every wave of a workgroup works on the same instances
+  // (one per lane); stage values go through LDS.
+  std::vector<uint32_t> fwave, awave;
 };
+
+// Kernel-level layout.  waves == 1: one instance per lane, the whole state
+// in registers, independent waves.  waves > 1: a workgroup of `waves` waves
+// runs `lw` instances (lanes >= lw mirror lane % lw), rows split across the
+// waves; `seg` is the adjoint segment length (checkpoint interval) in LDS.
+struct LaneLayout {
+  uint32_t waves = 1, lw = 64, seg = 1;
+};
+
+// Split the rows of `sh` over `waves` waves (largest term cost first onto
+// the least loaded wave) and return the largest per-wave live estimate
+// (coefficients of the larger phase plus two values per owned row).
+uint32_t lane_partition(LaneShape& sh, uint32_t waves);
 
 // samples of the excitation staged per lane, per chunk of steps
 constexpr uint32_t kLaneChunk = 16;
@@ -94,11 +111,10 @@ constexpr uint32_t kLaneStage = kLaneChunk + 3;
 constexpr uint32_t kLaneWave = 64;
 
 // Generated source of one kernel covering `shapes` (class c runs the
-// workgroups of LaneArgs c) in precision `bits`; `seg_len` is the adjoint
-// segment length (checkpoint interval) kept in LDS.
-std::string lane_source(const std::vector<LaneShape>& shapes, int bits, uint32_t seg_len);
-// dynamic LDS bytes of that kernel
-size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, uint32_t seg_len);
+// workgroups of LaneArgs c) in precision `bits` with layout `lay`.
+std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const LaneLayout& lay);
+// LDS bytes of that kernel (dynamic for waves == 1, static otherwise)
+size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const LaneLayout& lay);
 
 struct LaneKernel {
   hipModule_t mod = nullptr;

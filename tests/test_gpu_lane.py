@@ -147,3 +147,60 @@ def test_lane_adjoint_duffing_and_psi():
     p = _lane(w)
     p.setJacobian(*J.as_args())
     assert rel_err(p.getGradient(), oracle_gradient(w, J, 16)) < TOL64
+
+
+def _waves(p, adjoint=False):
+    head = p._laneSource(adjoint).splitlines()[0]
+    return int(head.split(" wave(s)")[0].rsplit(" ", 1)[1])
+
+
+def test_multiwave_forward_interp_and_solver_modes():
+    _case("multi-wave forward chains d8")
+    w = W.chains(n_systems=70, d=8, steps=500, steps_per_sample=10)
+    p = _lane(w)
+    assert _waves(p) > 1
+    q = p.getAmplitudes()
+    assert rel_err(q, oracle_run(w).q) < TOL64
+    p.setSolver("plain")
+    np.testing.assert_array_equal(p.getAmplitudes(), q)
+    p.setSolver("graph", steps_per_launch=23, launches_per_graph=4)
+    np.testing.assert_array_equal(p.getAmplitudes(), q)
+    p.setSolver("persistent")
+    p.setKernel("tile")
+    assert rel_err(p.getAmplitudes(), q) < TOL64
+
+
+def test_multiwave_adjoint_checkpoints_and_solvers():
+    _case("multi-wave adjoint chains d8")
+    w = W.chains(n_systems=20, d=8, steps=300, steps_per_sample=10)
+    J = ADJ.energy_gradient_inputs(w)
+    p = _lane(w)
+    p.setJacobian(*J.as_args())
+    assert _waves(p, True) > 1
+    ref = oracle_gradient(w, J, 64)
+    outs = []
+    for k in (5, 64):
+        p.setTuning(checkpoint_every=k)
+        g = p.getGradient()
+        assert rel_err(g, ref) < TOL64, k
+        outs.append(g)
+    np.testing.assert_array_equal(outs[0], outs[1])
+    p.setSolver("plain")
+    np.testing.assert_array_equal(p.getGradient(), outs[1])
+    p.setSolver("graph", steps_per_launch=17, launches_per_graph=6)
+    np.testing.assert_array_equal(p.getGradient(), outs[1])
+
+
+def test_multiwave_mixed_sizes_and_cfg2_shape():
+    _case("multi-wave heterogeneous d1..d16 + chains d16 adjoint")
+    w = W.heterogeneous(n_systems=120, tmin=30, tmax=400, steps_per_sample=5, n_signals=4,
+                        dof_choices=(1, 2, 4, 8, 16), dof_probs=(0.3, 0.25, 0.2, 0.15, 0.1))
+    p = _lane(w)
+    assert p.kernelInfo()["classes"] == 5
+    assert rel_err(p.getAmplitudes(), oracle_run(w).q) < TOL64
+    w = W.chains(n_systems=40, d=16, steps=200, steps_per_sample=10)
+    J = ADJ.energy_gradient_inputs(w)
+    p = _lane(w)
+    p.setJacobian(*J.as_args())
+    assert rel_err(p.getAmplitudes(), oracle_run(w).q) < TOL64
+    assert rel_err(p.getGradient(), oracle_gradient(w, J, 64)) < TOL64

@@ -40,8 +40,23 @@ def test_generated_source_compiles_for_gfx950(maker, adjoint, classes):
         assert ("typedef double R;" in src) == (bits == 64)
 
 
+def test_multiwave_layout_for_medium_systems():
+    # 16-DOF chains (33 forward rows, 65 adjoint rows) do not fit one lane:
+    # their rows are split over the waves of a workgroup
+    w = W.chains(64, 16, 60, steps_per_sample=10)
+    p = host_solver(w)
+    assert p.kernelInfo()["kernel"] == "lane"
+    head = p._laneSource(False).splitlines()[0]
+    assert "1 wave(s)" not in head and "wave(s) per workgroup" in head
+    p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
+    info = p.kernelInfo(adjoint=True)
+    assert info["kernel"] == "lane" and 1 <= info["checkpoint_every"] <= 256
+    src = p._laneSource(True, compile=True)
+    assert "run_mw<C0, C0W1>" in src
+
+
 def test_large_systems_stay_on_tiles_and_lane_refuses_them():
-    w = W.chains(8, 16, 60, steps_per_sample=10)
+    w = W.chains(4, 64, 60, steps_per_sample=10)
     p = host_solver(w)
     assert p.kernelInfo()["kernel"] == "tile"
     p.setKernel("lane")
