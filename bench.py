@@ -252,6 +252,11 @@ def main():
         "tiles": timing["tiles"],
         "lanes_per_tile": timing["lanes"],
     }
+    # the workload's own parallelism: SIMDs its waves occupy (1024 on the chip)
+    waves = timing["tiles"] * max(1, timing["lanes"] // 64)
+    used = min(1024, waves)
+    roofline["simds_used"] = used
+    roofline["frac_of_used_simds"] = round(achieved_tf / (PEAK_FP64_TFLOPS * used / 1024), 4)
 
     out = {
         "metric": "system-steps/s fwd & fwd+adjoint vs plain at rel err<=1e-10; serving p99 ms",

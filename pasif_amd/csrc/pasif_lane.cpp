@@ -231,6 +231,9 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
 #pragma unroll
     for (int r = 0; r < NA; ++r) lam[r] = ls[r * st];
   bool seg_here = false;  // LDS holds the current segment up to step n
+  // a launch that may stop mid-segment mirrors the segment to HBM so the
+  // next launch can rebuild its LDS copy; one that runs to the end does not
+  const bool mirror = a.budget != 0xFFFFFFFFull;
 
   while (budget > 0u && phase != kDone) {
     if (phase != kBackward) {
@@ -245,7 +248,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           for (int r = 0; r < NF; ++r) {
             q[r] = ck[((u64)sg * NF + r) * st];
             seg[r * WAVE + lane] = q[r];
-            sb[r * st] = q[r];
+            if (mirror) sb[r * st] = q[r];
           }
         } else if (!seg_here) {
           // resumed inside a recompute: the LDS copy of the steps recomputed
@@ -306,7 +309,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
 #pragma unroll
               for (int r = 0; r < NF; ++r) {
                 seg[(m * NF + r) * WAVE + lane] = q[r];
-                sb[((u64)m * NF + r) * st] = q[r];
+                if (mirror) sb[((u64)m * NF + r) * st] = q[r];
               }
             } else {
               if (SE && n == next_save) {
@@ -585,6 +588,9 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
 #pragma unroll
     for (int r = 0; r < SW::RA; ++r) lam[r] = ls[SW::ra(r) * st];
   bool seg_here = false;  // LDS holds the current segment up to step n
+  // a launch that may stop mid-segment mirrors the segment to HBM so the
+  // next launch can rebuild its LDS copy; one that runs to the end does not
+  const bool mirror = a.budget != 0xFFFFFFFFull;
 
   while (budget > 0u && phase != kDone) {
     if (phase != kBackward) {
@@ -598,7 +604,7 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
           for (int r = 0; r < SW::RF; ++r) {
             q[r] = ck[((u64)sg * NF + SW::rf(r)) * st];
             seg[SW::rf(r) * LW] = q[r];
-            sb[SW::rf(r) * st] = q[r];
+            if (mirror) sb[SW::rf(r) * st] = q[r];
           }
         } else if (!seg_here) {
           for (u32 m = 0; m <= n - s0; ++m)
@@ -663,7 +669,7 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
 #pragma unroll
               for (int r = 0; r < SW::RF; ++r) {
                 seg[(m * NFMAX + SW::rf(r)) * LW] = q[r];
-                sb[((u64)m * NF + SW::rf(r)) * st] = q[r];
+                if (mirror) sb[((u64)m * NF + SW::rf(r)) * st] = q[r];
               }
             } else {
               if (SE && n == next_save) {

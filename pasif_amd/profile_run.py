@@ -27,6 +27,7 @@ def main(argv=None):
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--solver", default="persistent")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "lane"])
     ap.add_argument("--repeat", type=int, default=1)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     a = ap.parse_args(argv)
@@ -55,6 +56,7 @@ def main(argv=None):
     p = W.make_solver(w)
     p.setPrecision(a.precision)
     p.setSolver(a.solver)
+    p.setKernel(a.kernel)
     if a.adjoint:
         from .adjoint import energy_gradient_inputs
         p.setJacobian(*energy_gradient_inputs(w).as_args())
@@ -67,7 +69,8 @@ def main(argv=None):
         wall = time.perf_counter() - t0
         t = p._lastTiming()
         print(json.dumps({"workload": w.name, "systems": w.n_systems, "steps": w.numsteps,
-                          "adjoint": a.adjoint, "wall_s": wall, **t}), flush=True)
+                          "adjoint": a.adjoint, "wall_s": wall, **t,
+                          **p.kernelInfo(adjoint=a.adjoint)}), flush=True)
 
 
 if __name__ == "__main__":
