@@ -105,6 +105,10 @@ def adjoint_flops_per_step(p) -> float:
         2.0 * p.globalSystemSize
 
 
+def _progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _timed(p, fn):
     t0 = time.perf_counter()
     out = fn()
@@ -195,6 +199,56 @@ def fwd_adj_parity(steps: int, n: int = 8) -> dict:
             "tolerance_fp64": 1e-10}
 
 
+def batching_section(args) -> dict:
+    """Heterogeneous batch in one persistent launch: 100,000 systems with DOF
+    drawn from {1,..,64} ({0.3,0.25,0.15,0.1,0.1,0.06,0.04}), steps
+    log-uniform on [1e3, 1e5], dt 1e-3, chain structure at each size,
+    interpolated band-limited forcing; tile shape (and checkpoint interval)
+    autotuned on probes capped at 2,000 steps.  FP64 and FP32 forward, FP64
+    forward+adjoint (energy gradient).  Host-side workload generation is not
+    timed."""
+    from pasif_amd import adjoint as ADJ
+    from pasif_amd import autotune as AT
+    from pasif_amd import workloads as W
+    t0 = time.perf_counter()
+    w = W.heterogeneous(n_systems=args.batch_systems, tmin=1e3, tmax=1e5, steps_per_sample=10)
+    p = W.make_solver(w)
+    setup_s = time.perf_counter() - t0
+    _progress(f"batching: workload built in {setup_s:.0f} s")
+    sys_steps = int(np.sum(w.steps_of_system))
+    info = p.kernelInfo()
+    tune = AT.autotune(p, probe_steps=2000, lanes=(128, 256), slots=(3, 1), use_cache=False)
+    _progress(f"batching: forward tuned {tune}")
+    p.getAmplitudes()
+    _, wall, t = _timed(p, p.getAmplitudes)
+    flops = flops_per_step(p) * (sys_steps / w.n_systems)  # mean steps per system
+    sec = {"workload": f"heterogeneous {w.n_systems} systems, DOF 1..64, steps log-uniform "
+                       f"[1e3, 1e5] (total {sys_steps} system-steps), dt 1e-3",
+           "kernel": info["kernel"], "tuning": tune, "setup_s": setup_s,
+           "fwd_fp64": {"value": sys_steps / wall, "unit": "system-steps/s",
+                        "kernel_ms": t["kernel_ms"], "tiles": t["tiles"],
+                        "tflops": flops / (t["kernel_ms"] * 1e-3) / 1e12}}
+    p.setPrecision(32)
+    p.getAmplitudes()
+    _, w32, t32 = _timed(p, p.getAmplitudes)
+    sec["fwd_fp32"] = {"value": sys_steps / w32, "kernel_ms": t32["kernel_ms"]}
+    _progress(f"batching: forward fp64 {t['kernel_ms']:.0f} ms, fp32 {t32['kernel_ms']:.0f} ms")
+    p.setPrecision(64)
+    t1 = time.perf_counter()
+    J = ADJ.energy_gradient_inputs(w)
+    p.setJacobian(*J.as_args())
+    sec["setup_s"] += time.perf_counter() - t1
+    _progress("batching: adjoint inputs built")
+    tune_a = AT.autotune(p, adjoint=True, probe_steps=1000, lanes=(128, 256), slots=(3, 1),
+                         checkpoints=(128, 256), use_cache=False)
+    _progress(f"batching: adjoint tuned {tune_a}")
+    p.getGradient()
+    _, wa, ta = _timed(p, p.getGradient)
+    sec["fwd_adj_fp64"] = {"value": sys_steps / wa, "unit": "system-steps/s (fwd+adj)",
+                           "kernel_ms": ta["kernel_ms"], "tuning": tune_a}
+    return sec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +262,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-systems", type=int, default=8192)
     ap.add_argument("--adj-steps", type=int, default=20_000)
+    ap.add_argument("--batching", action="store_true",
+                    help="run only the heterogeneous-batch measurement (100,000 systems)")
+    ap.add_argument("--batch-systems", type=int, default=100_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: systems are independent and the path needs no "
@@ -216,6 +273,11 @@ def main():
         sys.exit(2)
 
     from pasif_amd import workloads as W
+
+    if args.batching:
+        print(json.dumps({"metric": "heterogeneous batch, one persistent launch",
+                          **batching_section(args)}), flush=True)
+        return
 
     w = W.two_dof(n_systems=args.systems, steps=args.rk_steps)
     p = W.make_solver(w)
