@@ -200,11 +200,13 @@ def fwd_adj_parity(steps: int, n: int = 8) -> dict:
 
 
 def batching_section(args) -> dict:
-    """Heterogeneous batch in one persistent launch: 100,000 systems with DOF
+    """Heterogeneous batch, persistent: 100,000 systems with DOF
     drawn from {1,..,64} ({0.3,0.25,0.15,0.1,0.1,0.06,0.04}), steps
     log-uniform on [1e3, 1e5], dt 1e-3, chain structure at each size,
-    interpolated band-limited forcing; tile shape (and checkpoint interval)
-    autotuned on probes capped at 2,000 steps.  FP64 and FP32 forward, FP64
+    interpolated band-limited forcing.  Systems that fit the lane kernel run
+    on it, the rest on the tile kernel, each family in one persistent launch
+    on its own stream; tile shape (and checkpoint interval) autotuned on
+    probes capped at 2,000 steps.  FP64 and FP32 forward, FP64
     forward+adjoint (energy gradient).  Host-side workload generation is not
     timed."""
     from pasif_amd import adjoint as ADJ
@@ -275,7 +277,8 @@ def main():
     from pasif_amd import workloads as W
 
     if args.batching:
-        print(json.dumps({"metric": "heterogeneous batch, one persistent launch",
+        print(json.dumps({"metric": "heterogeneous batch, persistent (one launch per kernel "
+                                    "family, side by side)",
                           **batching_section(args)}), flush=True)
         return
 

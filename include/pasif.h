@@ -41,6 +41,8 @@ enum {
 
 enum { PASIF_SOLVER_PERSISTENT = 0, PASIF_SOLVER_PLAIN = 1, PASIF_SOLVER_GRAPH = 2 };
 enum { PASIF_KERNEL_AUTO = 0, PASIF_KERNEL_TILE = 1, PASIF_KERNEL_LANE = 2 };
+/* reported by pasif_kernel_info when both families share a solve */
+enum { PASIF_KERNEL_MIXED = 3 };
 
 /* __GpuDriver(excitationSet_, sampleRate_, numsteps_, dCompute_, dSystem_,
  * dSolver_, GPUId_)  (src/__GpuDriver.cu:25-157).  excitations is
@@ -154,9 +156,10 @@ int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launch
  * coefficients in registers, code generated per sparsity structure and
  * compiled at the first solve (small systems: PASIF_ERR_UNSUPPORTED when a
  * system does not fit; at most 8 distinct structures).  PASIF_KERNEL_AUTO
- * (default): the lane kernel when every system fits it, else tiles. */
+ * (default): the lane kernel for the structures that fit it (the 8 with the
+ * most work), the tile kernel for the rest, launched side by side. */
 int pasif_set_kernel(pasif_driver* d, int kernel);
-/* what the packed problem runs on: kernel (TILE or LANE), structure classes
+/* what the packed problem runs on: kernel (TILE, LANE or MIXED), structure classes
  * (lane kernel; 0 for tiles) and the checkpoint interval the adjoint uses
  * (the lane kernel shortens it until a segment fits its LDS) */
 int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* classes,

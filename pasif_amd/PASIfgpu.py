@@ -239,7 +239,7 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
         k, ncls, ck = C.c_int(0), C.c_uint32(0), C.c_uint32(0)
         self._chk(self._pl.pasif_kernel_info(self._h, int(adjoint), C.byref(k), C.byref(ncls),
                                              C.byref(ck)))
-        name = {1: "tile", 2: "lane"}.get(k.value, "?")
+        name = {1: "tile", 2: "lane", 3: "mixed"}.get(k.value, "?")
         return {"kernel": name, "classes": ncls.value, "checkpoint_every": ck.value}
 
     def _laneSource(self, adjoint: bool = False, compile: bool = False) -> str:

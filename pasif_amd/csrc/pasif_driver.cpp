@@ -163,8 +163,9 @@ struct Packed {
   bool valid = false;
   int precision = 0;
   bool adjoint = false;
-  // lane kernel instead of tiles
-  bool lane = false;
+  // lane kernel part (and tile part) of the packed problem
+  bool lane = false, tile = false;
+  uint32_t ltmax = 0;
   std::vector<std::unique_ptr<LaneCls>> lcls;
   std::vector<LaneShape> lshapes;
   LaneKernel lkern;
@@ -189,6 +190,8 @@ struct Packed {
     lshapes.clear();
     largs.release();
     lane = false;
+    tile = false;
+    ltmax = 0;
     valid = false;
   }
 };
@@ -224,6 +227,11 @@ struct pasif_driver {
   int solver = PASIF_SOLVER_PERSISTENT;
   uint32_t tile_lanes = 0, slot_cfg = 0, ck_every = 0, window = 0;
   int kernel_path = PASIF_KERNEL_AUTO;
+  // lane class of every system in the packed problem, -1: tile kernel
+  std::vector<int32_t> lane_cls;
+  // second stream and fork / join events when lane and tile kernels run together
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   uint32_t graph_slice = 256, graph_launches = 16;
 
   Packed pk;
@@ -704,13 +712,23 @@ void validate_program(pasif_driver* d, const HostProgram<R>& hp, const std::vect
   }
 }
 
-std::vector<Instance> make_instances(pasif_driver* d, bool adjoint) {
+// Instances of the problem; `part` keeps those of the lane kernel's systems
+// (kPartLane), of the tile kernel's (kPartTile) or all (kPartAll).
+enum { kPartAll = 0, kPartLane = 1, kPartTile = 2 };
+bool in_part(const pasif_driver* d, uint32_t s, int part) {
+  if (part == kPartAll) return true;
+  const bool lane = s < d->lane_cls.size() && d->lane_cls[s] >= 0;
+  return part == kPartLane ? lane : !lane;
+}
+
+std::vector<Instance> make_instances(pasif_driver* d, bool adjoint, int part = kPartAll) {
   std::vector<Instance> v;
   const uint32_t S = d->fwd.size.size();
   const uint32_t N = d->fwd.n, Na = adjoint ? d->adj.n : 0;
   if (d->mapped) {
     v.reserve(S);
     for (uint32_t s = 0; s < S; ++s) {
+      if (!in_part(d, s, part)) continue;
       Instance in{};
       in.sys = s;
       in.exc = d->map_exc.empty() ? 0 : d->map_exc[s];
@@ -725,6 +743,7 @@ std::vector<Instance> make_instances(pasif_driver* d, bool adjoint) {
     v.reserve(size_t(S) * d->n_exc);
     for (uint32_t p = 0; p < d->n_exc; ++p)
       for (uint32_t s = 0; s < S; ++s) {
+        if (!in_part(d, s, part)) continue;
         Instance in{};
         in.sys = s;
         in.exc = int32_t(p);
@@ -741,7 +760,7 @@ template <typename R>
 void pack_impl(pasif_driver* d, bool adjoint) {
   Packed& pk = d->pk;
   const uint32_t S = d->fwd.size.size();
-  std::vector<Instance> inst = make_instances(d, adjoint);
+  std::vector<Instance> inst = make_instances(d, adjoint, kPartTile);
   // per-system instance counts (Cartesian: every excitation)
   std::vector<uint64_t> mult(S, 0);
   for (const Instance& in : inst) mult[in.sys]++;
@@ -766,6 +785,7 @@ void pack_impl(pasif_driver* d, bool adjoint) {
     uint64_t total = 0;
     bool ok = true;
     for (uint32_t s = 0; s < S && ok; ++s) {
+      if (!in_part(d, s, kPartTile)) continue;
       lf[s] = system_lanes(d->fwd, s, cc);
       if (adjoint) la[s] = system_lanes(d->adj, s, cc);
       if (lf[s] == 0 || (adjoint && la[s] == 0)) ok = false;
@@ -1000,12 +1020,14 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   Packed& pk = d->pk;
   const uint32_t S = uint32_t(d->fwd.size.size());
   const double eps = std::numeric_limits<R>::epsilon();
+  const bool forced_lane = d->kernel_path == PASIF_KERNEL_LANE;
+  d->lane_cls.assign(S, -1);
+  // structure classes of every system
   std::map<std::vector<uint32_t>, uint32_t> cls_of_key;
   std::vector<uint32_t> cls_of(S);
-  std::vector<LaneShape> shapes;
+  std::vector<LaneShape> all;
   std::vector<uint32_t> key;
-  const char* why = nullptr;
-  for (uint32_t s = 0; s < S && !why; ++s) {
+  for (uint32_t s = 0; s < S; ++s) {
     LaneShape sh;
     sh.f = lane_side(d->fwd, s, false, eps);
     sh.adjoint = adjoint;
@@ -1015,15 +1037,49 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     if (adjoint) shape_key(sh.a, key);
     auto it = cls_of_key.find(key);
     if (it == cls_of_key.end()) {
-      if (shapes.size() == kLaneMaxClasses) {
-        why = "the systems have too many distinct structures";
-        break;
-      }
-      it = cls_of_key.emplace(key, uint32_t(shapes.size())).first;
-      shapes.push_back(std::move(sh));
+      it = cls_of_key.emplace(key, uint32_t(all.size())).first;
+      all.push_back(std::move(sh));
     }
     cls_of[s] = it->second;
   }
+  // what each class needs: 1 = one lane, w >= 2 = a workgroup of w waves, 0 = too large
+  std::vector<uint32_t> need(all.size(), 0);
+  std::vector<double> work(all.size(), 0.0);
+  for (size_t c = 0; c < all.size(); ++c) {
+    LaneShape& sh = all[c];
+    if (sh.live_values() <= lane_max_live()) {
+      need[c] = 1;
+    } else {
+      for (uint32_t w = 2; w <= kLaneMaxWaves && !need[c]; ++w)
+        if (lane_partition(sh, w) <= kLaneMaxWaveLive) need[c] = w;
+    }
+  }
+  for (const Instance& in : make_instances(d, adjoint))
+    work[cls_of[in.sys]] += double(in.steps) * (all[cls_of[in.sys]].f.n + all[cls_of[in.sys]].a.n);
+  // the lane kernel takes the classes that fit, heaviest first, up to kLaneMaxClasses
+  std::vector<uint32_t> order;
+  for (uint32_t c = 0; c < all.size(); ++c)
+    if (need[c]) order.push_back(c);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
+  if (order.size() > kLaneMaxClasses) order.resize(kLaneMaxClasses);
+  const char* why = nullptr;
+  if (order.size() < all.size()) {
+    bool too_large = false;
+    for (uint32_t c = 0; c < all.size(); ++c) too_large = too_large || !need[c];
+    why = too_large ? "a system needs more registers than a workgroup of lanes holds"
+                    : "the systems have too many distinct structures";
+    PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED, "lane kernel: %s", why);
+    why = nullptr;
+  }
+  if (order.empty()) return false;
+  std::vector<LaneShape> shapes;
+  std::vector<int32_t> sel_of(all.size(), -1);
+  for (uint32_t c : order) {
+    sel_of[c] = int32_t(shapes.size());
+    shapes.push_back(all[c]);
+  }
+  for (uint32_t s = 0; s < S; ++s) d->lane_cls[s] = sel_of[cls_of[s]];
   // Layout: one wave per instance group when every class fits a lane's
   // registers; otherwise the rows of each instance spread over the fewest
   // waves (up to kLaneMaxWaves) whose share fits.
@@ -1031,23 +1087,15 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   const size_t el = sizeof(R);
   const uint32_t Kreq = d->ck_every ? d->ck_every : 256;
   if (!why) {
-    uint32_t live = 0;
-    for (const LaneShape& sh : shapes) live = std::max(live, sh.live_values());
-    if (live > lane_max_live()) {
-      uint32_t waves = 2;
-      for (LaneShape& sh : shapes) {
-        while (waves <= kLaneMaxWaves && lane_partition(sh, waves) > kLaneMaxWaveLive) ++waves;
-        if (waves > kLaneMaxWaves) {
-          why = "a system needs more registers than a workgroup of lanes holds";
-          break;
-        }
-      }
-      if (!why) {
+    uint32_t waves = 1;
+    for (uint32_t c : order) waves = std::max(waves, need[c]);
+    if (waves > 1) {
+      {
         for (LaneShape& sh : shapes) (void)lane_partition(sh, waves);
         lay.waves = waves;
         // instances per workgroup: enough workgroups for every CU, then the
         // longest checkpoint interval (up to the requested one) that fits LDS
-        const uint64_t total = make_instances(d, adjoint).size();
+        const uint64_t total = make_instances(d, adjoint, kPartLane).size();
         lay.lw = total >= 256ull * 64 ? 64 : (total >= 256ull * 32 ? 32 : 16);
         for (;; lay.lw /= 2) {
           lay.seg = 1;
@@ -1084,16 +1132,15 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     }
   }
   if (why) {
-    PASIF_REQUIRE(d->kernel_path != PASIF_KERNEL_LANE, PASIF_ERR_UNSUPPORTED,
-                  "lane kernel: %s", why);
+    PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED, "lane kernel: %s", why);
+    d->lane_cls.assign(S, -1);
     return false;
   }
   const uint32_t K = adjoint ? lay.seg : 1;
   const uint32_t unit = lay.waves > 1 ? lay.lw : kLaneWave;  // instances per workgroup
-  std::vector<Instance> inst = make_instances(d, adjoint);
+  std::vector<Instance> inst = make_instances(d, adjoint, kPartLane);
   std::vector<std::vector<const Instance*>> members(shapes.size());
-  for (const Instance& in : inst) members[cls_of[in.sys]].push_back(&in);
-  pk.release();
+  for (const Instance& in : inst) members[d->lane_cls[in.sys]].push_back(&in);
   uint32_t block = 0, tmax = 0;
   std::vector<double> cv;
   for (size_t c = 0; c < shapes.size(); ++c) {
@@ -1174,15 +1221,11 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     PASIF_HIP(hipStreamSynchronize(d->stream));
   }
   pk.lane = true;
+  pk.ltmax = tmax;
   pk.adjoint = adjoint;
   pk.precision = int(sizeof(R) * 8);
-  pk.tmax = tmax;
   pk.out_len_f = out_len_f;
   pk.out_len_a = out_len_a;
-  pk.ntiles = block;
-  pk.lanes = kLaneWave;
-  pk.slot_cfg = 0;
-  pk.valid = true;
   if (d->d_compute)
     printf("[pasif] lane kernel: %zu instances in %zu structure class(es), %u workgroups of %u "
            "wave(s), T max %u, checkpoint %u\n",
@@ -1202,16 +1245,37 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
   const uint64_t key = pack_key(d, adjoint);
   if (d->pk.valid && d->pk.key == key) return;
   d->pk.valid = false;
-  bool lane = false;
-  if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool)
-    lane = d->precision == 64 ? lane_plan<double>(d, adjoint) : lane_plan<float>(d, adjoint);
-  if (!lane) {
-    d->pk.release();
+  d->pk.release();
+  d->lane_cls.clear();
+  if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool) {
+    if (d->precision == 64)
+      (void)lane_plan<double>(d, adjoint);
+    else
+      (void)lane_plan<float>(d, adjoint);
+  }
+  // the tile kernel takes every system the lane kernel did not
+  bool rest = !d->pk.lane;
+  for (size_t s = 0; s < d->lane_cls.size() && !rest; ++s) rest = d->lane_cls[s] < 0;
+  if (rest) {
     if (d->precision == 64)
       pack_impl<double>(d, adjoint);
     else
       pack_impl<float>(d, adjoint);
+    d->pk.tile = true;
+  } else {
+    d->pk.ntiles = 0;
+    d->pk.tmax = 0;
   }
+  Packed& pk = d->pk;
+  pk.tmax = std::max(pk.tmax, pk.ltmax);
+  if (pk.lane && !pk.tile) {
+    pk.ntiles = pk.lblocks;  // reported as the launch's workgroups
+    pk.lanes = kLaneWave;
+    pk.slot_cfg = 0;
+  }
+  pk.adjoint = adjoint;
+  pk.precision = d->precision;
+  pk.valid = true;
   d->pk.key = key;
 }
 
@@ -1431,10 +1495,12 @@ void drive(pasif_driver* d, uint64_t launches, F&& launch) {
   d->last_launches = uint32_t(launches);
 }
 
-// Run the lane kernel of the packed problem to completion.
+// Output and run buffers of the lane part and its per-class arguments
+// (budget filled in by the caller).  *work: steps of work of the longest
+// class (3T with the adjoint).
 template <typename R>
-void solve_lane(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
-                std::vector<double>* grad_out) {
+std::vector<LaneArgs> lane_prepare(pasif_driver* d, bool adjoint, uint32_t save_steps,
+                                   uint64_t* work, uint32_t* nslots_out) {
   ensure_excitations<R>(d);
   Packed& pk = d->pk;
   dev_reserve(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
@@ -1449,9 +1515,8 @@ void solve_lane(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     PASIF_HIP(hipMemsetAsync(d->traj.p, 0, size_t(nslots) * pk.out_len_f * sizeof(R), d->stream));
   }
   if (adjoint) dev_reserve(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
-  const uint64_t work = adjoint ? 3ull * pk.tmax : pk.tmax;
-  uint64_t launches = 1;
-  const uint32_t budget = launch_budget(d, work, &launches);
+  *work = adjoint ? 3ull * pk.ltmax : pk.ltmax;
+  *nslots_out = nslots;
   const uint32_t K = pk.lseg;
   const R h = R(1.0 / double(d->sample_rate));
   std::vector<LaneArgs> args(pk.lcls.size());
@@ -1496,7 +1561,6 @@ void solve_lane(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     a.block_begin = L.block_begin;
     a.exc_len = d->exc_len;
     a.interp_w = d->interp_w;
-    a.budget = budget;
     a.window = d->window;
     a.out_len = pk.out_len_f;
     a.save_every = save_steps;
@@ -1510,39 +1574,64 @@ void solve_lane(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     a.h2 = double(R(double(h) / 2.0));
     a.h6 = double(R(double(h) / 6.0));
   }
-  upload(d, pk.largs, args);
-  const void* dargs = pk.largs.p;
-  uint32_t ncls = uint32_t(args.size());
+  return args;
+}
+
+void ensure_aux(pasif_driver* d) {
+  if (d->aux) return;
+  PASIF_HIP(hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking));
+  PASIF_HIP(hipEventCreateWithFlags(&d->fork, hipEventDisableTiming));
+  PASIF_HIP(hipEventCreateWithFlags(&d->join, hipEventDisableTiming));
+}
+
+// Run the packed problem to completion: the lane part, the tile part, or
+// both side by side on two streams.  save_steps > 0 records a trajectory.
+template <typename R>
+void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
+                std::vector<double>* grad_out) {
+  ensure_packed(d, adjoint);
+  Packed& pk = d->pk;
+  const bool lane = pk.lane, tile = pk.tile;
+  uint64_t twork = 0, lwork = 0;
+  uint32_t nslots = 0;
+  TileArgs<R> ta{};
+  if (tile) ta = prepare_run<R>(d, adjoint, save_steps, &twork, &nslots);
+  std::vector<LaneArgs> largs;
+  if (lane) largs = lane_prepare<R>(d, adjoint, save_steps, &lwork, &nslots);
+  uint64_t launches = 1;
+  const uint32_t budget = launch_budget(d, std::max(twork, lwork), &launches);
+  ta.budget = budget;
+  for (LaneArgs& a : largs) a.budget = budget;
+  const void* dargs = nullptr;
+  if (lane) {
+    upload(d, pk.largs, largs);
+    dargs = pk.largs.p;
+  }
+  if (lane && tile) ensure_aux(d);
+  uint32_t ncls = uint32_t(largs.size());
   hipFunction_t fn = pk.lkern.fn;
   const uint32_t blocks = pk.lblocks;
   // multi-wave kernels declare their LDS statically
   const uint32_t lds = pk.llay.waves > 1 ? 0u : uint32_t(pk.llds);
   const uint32_t bdim = kLaneWave * pk.llay.waves;
-  hipStream_t stream = d->stream;
-  drive(d, launches, [&]() -> hipError_t {
-    void* params[] = {const_cast<void**>(&dargs), &ncls};
-    return hipModuleLaunchKernel(fn, blocks, 1, 1, bdim, 1, 1, lds, stream, params, nullptr);
-  });
-  finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
-}
-
-// Run the packed problem to completion.  save_steps > 0 records a trajectory.
-template <typename R>
-void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
-                std::vector<double>* grad_out) {
-  ensure_packed(d, adjoint);
-  if (d->pk.lane) {
-    solve_lane<R>(d, adjoint, save_steps, traj_out, grad_out);
-    return;
-  }
-  uint64_t work = 0;
-  uint32_t nslots = 0;
-  TileArgs<R> a = prepare_run<R>(d, adjoint, save_steps, &work, &nslots);
-  Packed& pk = d->pk;
   const uint32_t nt = pk.ntiles;
-  uint64_t launches = 1;
-  a.budget = launch_budget(d, work, &launches);
-  drive(d, launches, [&]() { return launch_tiles(a, nt, int(pk.slot_cfg), d->stream); });
+  auto lane_launch = [&](hipStream_t st) -> hipError_t {
+    void* params[] = {const_cast<void**>(&dargs), &ncls};
+    return hipModuleLaunchKernel(fn, blocks, 1, 1, bdim, 1, 1, lds, st, params, nullptr);
+  };
+  drive(d, launches, [&]() -> hipError_t {
+    if (lane && tile) {
+      hipError_t e = hipEventRecord(d->fork, d->stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(d->aux, d->fork, 0);
+      if (e == hipSuccess) e = lane_launch(d->aux);
+      if (e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+      if (e == hipSuccess) e = hipEventRecord(d->join, d->aux);
+      if (e == hipSuccess) e = hipStreamWaitEvent(d->stream, d->join, 0);
+      return e;
+    }
+    if (lane) return lane_launch(d->stream);
+    return launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+  });
   finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
 }
 
@@ -1646,6 +1735,12 @@ int pasif_destroy(pasif_driver* d) {
   if (d->dev >= 0) (void)hipSetDevice(d->dev);
   if (d->stream) (void)hipStreamSynchronize(d->stream);
   d->pk.release();
+  if (d->aux) {
+    (void)hipStreamSynchronize(d->aux);
+    (void)hipStreamDestroy(d->aux);
+  }
+  if (d->fork) (void)hipEventDestroy(d->fork);
+  if (d->join) (void)hipEventDestroy(d->join);
   if (!d->pool) {
     if (d->ev0) (void)hipEventDestroy(d->ev0);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
@@ -1940,7 +2035,9 @@ int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* class
                       uint32_t* checkpoint_every) {
   return guarded(d, [&] {
     ensure_packed(d, adjoint != 0);
-    if (kernel) *kernel = d->pk.lane ? PASIF_KERNEL_LANE : PASIF_KERNEL_TILE;
+    if (kernel)
+      *kernel = d->pk.lane ? (d->pk.tile ? PASIF_KERNEL_MIXED : PASIF_KERNEL_LANE)
+                           : PASIF_KERNEL_TILE;
     if (classes) *classes = d->pk.lane ? uint32_t(d->pk.lshapes.size()) : 0u;
     if (checkpoint_every)
       *checkpoint_every = !adjoint ? 0u
@@ -1956,7 +2053,7 @@ int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_
     d->kernel_path = PASIF_KERNEL_LANE;
     const int dev = d->dev;
     d->dev = -1;  // plan on the host only; the compilation below needs no GPU
-    d->pk.valid = false;
+    d->pk.release();
     const int rc = guarded(d, [&] {
       if (d->precision == 64)
         (void)lane_plan<double>(d, adjoint != 0);

@@ -428,6 +428,11 @@ __device__ __forceinline__ void run_tile(const TileArgs<R>& a, const size_t tile
           }
         }
       }
+      if (n == e && !recompute && a.save_every && fown) {
+        // trajectory slots after the tile's last step hold the final state
+        for (uint32_t slot = (T + a.save_every - 1u) / a.save_every; slot < a.nslots; ++slot)
+          a.traj[static_cast<size_t>(slot) * a.out_len + fout] = q;
+      }
       if (n == e) {
         if (recompute) {
           phase = kPhaseBackward;

@@ -378,6 +378,12 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       if (!gen && valid && fsteps > nfirst && fsteps <= nend)
 #pragma unroll
         for (int r = 0; r < NF; ++r) outq[fout + r] = q[r];
+      if (n == e && !recompute && SE && valid) {
+        // trajectory slots after the wave's last step hold the final state
+        for (u32 slot = (T + SE - 1u) / SE; slot < (u32)a.nslots; ++slot)
+#pragma unroll
+          for (int r = 0; r < NF; ++r) traj[(u64)slot * a.out_len + fout + r] = q[r];
+      }
       if (n == e) {
         if (recompute) {
           phase = kBackward;
@@ -728,6 +734,13 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
       if (!gen && valid && fsteps > nfirst && fsteps <= nend)
 #pragma unroll
         for (int r = 0; r < SW::RF; ++r) outq[fout + SW::rf(r)] = q[r];
+      if (n == e && !recompute && SE && valid) {
+        // trajectory slots after the workgroup's last step hold the final state
+        for (u32 slot = (T + SE - 1u) / SE; slot < (u32)a.nslots; ++slot)
+#pragma unroll
+          for (int r = 0; r < SW::RF; ++r)
+            traj[(u64)slot * a.out_len + fout + SW::rf(r)] = q[r];
+      }
       if (n == e) {
         if (recompute) {
           phase = kBackward;

@@ -204,3 +204,23 @@ def test_multiwave_mixed_sizes_and_cfg2_shape():
     p.setJacobian(*J.as_args())
     assert rel_err(p.getAmplitudes(), oracle_run(w).q) < TOL64
     assert rel_err(p.getGradient(), oracle_gradient(w, J, 64)) < TOL64
+
+
+def test_mixed_lane_and_tile_in_one_solve():
+    _case("mixed lane + tile, heterogeneous d1..d64, forward + adjoint + solvers")
+    w = W.heterogeneous(n_systems=90, tmin=30, tmax=300, steps_per_sample=5, n_signals=4)
+    J = ADJ.energy_gradient_inputs(w)
+    p = W.make_solver(w)
+    p.setJacobian(*J.as_args())
+    assert p.kernelInfo()["kernel"] == "mixed"
+    q = p.getAmplitudes()
+    assert rel_err(q, oracle_run(w).q) < TOL64
+    g = p.getGradient()
+    assert rel_err(g, oracle_gradient(w, J, 256)) < TOL64
+    p.setSolver("plain")
+    np.testing.assert_array_equal(p.getAmplitudes(), q)
+    p.setSolver("graph", steps_per_launch=41, launches_per_graph=3)
+    np.testing.assert_array_equal(p.getGradient(), g)
+    tr = p.getTrajectory(saveSteps=11)
+    ref = oracle_run(w, save_steps=11)
+    assert rel_err(tr[1:].T, ref.traj.reshape(ref.traj.shape[0], -1)) < TOL64

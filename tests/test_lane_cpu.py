@@ -78,3 +78,19 @@ def test_checkpoint_interval_shortened_to_fit_lds():
     assert (19 * 64 + (k + 1) * nf * 64) * 8 <= 64 * 1024
     p.setTuning(checkpoint_every=4)
     assert p.kernelInfo(adjoint=True)["checkpoint_every"] == 4
+
+
+def test_mixed_sizes_split_between_lane_and_tile_kernels():
+    # 64-DOF chains do not fit the lane kernel (their energy row alone
+    # exceeds a wave's registers): they run on tiles beside the lane classes
+    # of the smaller sizes (2- and 8-DOF in one lane / 32-DOF across waves)
+    w = W.heterogeneous(60, 30, 300, steps_per_sample=5, n_signals=3,
+                        dof_choices=(2, 8, 32, 64), dof_probs=(1, 1, 1, 1))
+    p = host_solver(w)
+    info = p.kernelInfo()
+    assert info["kernel"] == "mixed" and info["classes"] == 3
+    p.setKernel("tile")
+    assert p.kernelInfo()["kernel"] == "tile"
+    p.setKernel("lane")
+    with pytest.raises(_lib.PasifError):
+        p.kernelInfo()
