@@ -155,24 +155,36 @@ struct SideIn {
 struct LaneCls {
   uint32_t count = 0, stride = 0, waves = 0, block_begin = 0, tmax = 0;
   uint32_t nf = 0, na = 0;
+  uint32_t klass = 0;  // the class's case in its kernel's dispatch
   DevBuf coef_f, coef_a, init_f, init_a, steps, exc, out_f, out_a;  // program
   DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;            // run
+};
+
+// One lane kernel's share of a packed problem: the instances of each of its
+// structure classes and the kernel (with its layout) that runs them.  A
+// driver's plan has one part; a pool job has one per pool lane module it uses.
+struct LanePart {
+  std::vector<std::unique_ptr<LaneCls>> cls;
+  LaneKernel kern;
+  LaneLayout lay;
+  uint32_t blocks = 0, tmax = 0;
+  size_t lds = 0;
+  DevBuf args;
+  int module = -1;  // pool lane module (pool jobs)
 };
 
 struct Packed {
   bool valid = false;
   int precision = 0;
   bool adjoint = false;
-  // lane kernel part (and tile part) of the packed problem
+  // lane kernel parts (and tile part) of the packed problem
   bool lane = false, tile = false;
   uint32_t ltmax = 0;
-  std::vector<std::unique_ptr<LaneCls>> lcls;
+  std::vector<std::unique_ptr<LanePart>> lparts;
+  // the driver's own lane plan (diagnostics): its classes, layout, checkpoint
   std::vector<LaneShape> lshapes;
-  LaneKernel lkern;
-  uint32_t lseg = 1, lblocks = 0;
+  uint32_t lseg = 1;
   LaneLayout llay;
-  size_t llds = 0;
-  DevBuf largs;
   uint32_t lanes = 0, ntiles = 0, slot_cfg = 0, tile_target = 0;
   uint64_t key = 0;
   uint32_t out_len_f = 0, out_len_a = 0, tmax = 0;
@@ -186,9 +198,8 @@ struct Packed {
                      &f_force, &f_val,   &f_idx,  &f_init,  &a_info,  &a_out, &a_src,
                      &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init};
     for (DevBuf* b : all) b->release();
-    lcls.clear();
+    lparts.clear();
     lshapes.clear();
-    largs.release();
     lane = false;
     tile = false;
     ltmax = 0;
@@ -720,6 +731,10 @@ bool in_part(const pasif_driver* d, uint32_t s, int part) {
   const bool lane = s < d->lane_cls.size() && d->lane_cls[s] >= 0;
   return part == kPartLane ? lane : !lane;
 }
+// lane_cls[s] = (lane part << 16) | class index within the part, -1: tiles
+inline int32_t lane_code(int part, uint32_t cls) { return int32_t((uint32_t(part) << 16) | cls); }
+inline int lane_part_of(int32_t code) { return int(uint32_t(code) >> 16); }
+inline uint32_t lane_class_of(int32_t code) { return uint32_t(code) & 0xFFFFu; }
 
 std::vector<Instance> make_instances(pasif_driver* d, bool adjoint, int part = kPartAll) {
   std::vector<Instance> v;
@@ -1016,6 +1031,11 @@ void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, std::vector<
 // Plan the lane kernel for the loaded problem: true if every system fits a
 // lane and the systems form at most kLaneMaxClasses structure classes.
 template <typename R>
+LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
+                     const std::vector<LaneShape>& shapes, const std::vector<uint32_t>& klass,
+                     const LaneLayout& lay);
+
+template <typename R>
 bool lane_plan(pasif_driver* d, bool adjoint) {
   Packed& pk = d->pk;
   const uint32_t S = uint32_t(d->fwd.size.size());
@@ -1079,7 +1099,8 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     sel_of[c] = int32_t(shapes.size());
     shapes.push_back(all[c]);
   }
-  for (uint32_t s = 0; s < S; ++s) d->lane_cls[s] = sel_of[cls_of[s]];
+  for (uint32_t s = 0; s < S; ++s)
+    d->lane_cls[s] = sel_of[cls_of[s]] >= 0 ? lane_code(0, uint32_t(sel_of[cls_of[s]])) : -1;
   // Layout: one wave per instance group when every class fits a lane's
   // registers; otherwise the rows of each instance spread over the fewest
   // waves (up to kLaneMaxWaves) whose share fits.
@@ -1137,14 +1158,59 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     return false;
   }
   const uint32_t K = adjoint ? lay.seg : 1;
+  LaneLayout lk = lay;
+  lk.seg = K;
+  std::vector<uint32_t> klass(shapes.size());
+  for (size_t c = 0; c < shapes.size(); ++c) klass[c] = uint32_t(c);
+  LanePart* part = lane_build<R>(d, adjoint, 0, shapes, klass, lk);
+  const size_t lds = lane_lds_bytes(shapes, int(sizeof(R) * 8), lk);
+  PASIF_REQUIRE(lds <= (lk.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
+                "lane kernel LDS %zu bytes", lds);
+  part->lds = lds;
+  if (d->dev >= 0) {
+    const std::string src = lane_source(shapes, int(sizeof(R) * 8), lk);
+    std::string err;
+    if (!lane_compile(src, d->dev, &part->kern, &err)) {
+      set_err(d, "%s", err.c_str());
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+  }
+  pk.lshapes = shapes;
+  pk.lseg = K;
+  pk.llay = lk;
+  if (d->d_compute)
+    printf("[pasif] lane kernel: %zu structure class(es), %u workgroups of %u wave(s), T max %u, "
+           "checkpoint %u\n",
+           shapes.size(), part->blocks, lk.waves, part->tmax, K);
+  return true;
+}
+
+// Build lane part `part` of the packed problem: the instances of the systems
+// whose lane_cls names this part, one LaneCls per class with instances
+// (sorted by step count, descending), coefficients / initial states /
+// indices uploaded.  shapes[c] is the structure of class c, klass[c] its case
+// in the kernel that will run the part (laid out as `lay`).
+template <typename R>
+LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
+                     const std::vector<LaneShape>& shapes, const std::vector<uint32_t>& klass,
+                     const LaneLayout& lay) {
+  Packed& pk = d->pk;
   const uint32_t unit = lay.waves > 1 ? lay.lw : kLaneWave;  // instances per workgroup
   std::vector<Instance> inst = make_instances(d, adjoint, kPartLane);
   std::vector<std::vector<const Instance*>> members(shapes.size());
-  for (const Instance& in : inst) members[d->lane_cls[in.sys]].push_back(&in);
+  for (const Instance& in : inst) {
+    const int32_t code = d->lane_cls[in.sys];
+    if (lane_part_of(code) == part) members[lane_class_of(code)].push_back(&in);
+  }
+  std::unique_ptr<LanePart> P(new LanePart());
+  P->lay = lay;
   uint32_t block = 0, tmax = 0;
   std::vector<double> cv;
+  const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
+  const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
   for (size_t c = 0; c < shapes.size(); ++c) {
     auto& mem = members[c];
+    if (mem.empty()) continue;
     std::stable_sort(mem.begin(), mem.end(),
                      [](const Instance* x, const Instance* y) { return x->steps > y->steps; });
     std::unique_ptr<LaneCls> L(new LaneCls());
@@ -1155,6 +1221,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     L->block_begin = block;
     L->nf = sh.f.n;
     L->na = adjoint ? sh.a.n : 0;
+    L->klass = klass[c];
     block += L->waves;
     const uint32_t st = L->stride, TF = sh.f.coefs(), TA = adjoint ? sh.a.coefs() : 0;
     std::vector<R> cf(size_t(std::max<uint32_t>(TF, 1)) * st, R(0));
@@ -1165,11 +1232,20 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     std::vector<int32_t> ex(st, -1);
     for (uint32_t i = 0; i < L->count; ++i) {
       const Instance& in = *mem[i];
+      // every output index the kernel writes stays inside the output arrays
+      PASIF_REQUIRE(uint64_t(in.out_f) + sh.f.n <= out_len_f, PASIF_ERR_UNSUPPORTED,
+                    "lane plan: output index out of range");
+      PASIF_REQUIRE(!adjoint || uint64_t(in.out_a) + sh.a.n <= out_len_a,
+                    PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
+      PASIF_REQUIRE(in.exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
+                    "lane plan: excitation out of range");
       lane_coefs(d->fwd, in.sys, sh.f, cv);
+      PASIF_REQUIRE(cv.size() == TF, PASIF_ERR_UNSUPPORTED, "lane plan: structure mismatch");
       for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + i] = R(cv[t]);
       for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
       if (adjoint) {
         lane_coefs(d->adj, in.sys, sh.a, cv);
+        PASIF_REQUIRE(cv.size() == TA, PASIF_ERR_UNSUPPORTED, "lane plan: structure mismatch");
         for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
         for (uint32_t r = 0; r < L->na; ++r)
           qa[size_t(r) * st + i] = R(d->adj.init[d->adj.off[in.sys] + r]);
@@ -1189,48 +1265,18 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     upload(d, L->exc, ex);
     upload(d, L->out_f, of);
     upload(d, L->out_a, oa);
-    pk.lcls.push_back(std::move(L));
+    P->cls.push_back(std::move(L));
   }
-  // every output index the kernel writes stays inside the output arrays
-  const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
-  const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
-  for (size_t c = 0; c < shapes.size(); ++c)
-    for (const Instance* in : members[c]) {
-      PASIF_REQUIRE(uint64_t(in->out_f) + shapes[c].f.n <= out_len_f, PASIF_ERR_UNSUPPORTED,
-                    "lane plan: output index out of range");
-      PASIF_REQUIRE(!adjoint || uint64_t(in->out_a) + shapes[c].a.n <= out_len_a,
-                    PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
-      PASIF_REQUIRE(in->exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
-                    "lane plan: excitation out of range");
-    }
-  pk.lshapes = shapes;
-  pk.lseg = adjoint ? K : 1;
-  pk.llay = lay;
-  pk.llay.seg = pk.lseg;
-  pk.lblocks = block;
-  pk.llds = lane_lds_bytes(shapes, int(sizeof(R) * 8), pk.llay);
-  PASIF_REQUIRE(pk.llds <= (lay.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
-                "lane kernel LDS %zu bytes", pk.llds);
-  if (d->dev >= 0) {
-    const std::string src = lane_source(shapes, int(sizeof(R) * 8), pk.llay);
-    std::string err;
-    if (!lane_compile(src, d->dev, &pk.lkern, &err)) {
-      set_err(d, "%s", err.c_str());
-      throw Fail{PASIF_ERR_DEVICE};
-    }
-    PASIF_HIP(hipStreamSynchronize(d->stream));
-  }
+  if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
+  P->blocks = block;
+  P->tmax = tmax;
+  P->module = part;
   pk.lane = true;
-  pk.ltmax = tmax;
-  pk.adjoint = adjoint;
-  pk.precision = int(sizeof(R) * 8);
+  pk.ltmax = std::max(pk.ltmax, tmax);
   pk.out_len_f = out_len_f;
   pk.out_len_a = out_len_a;
-  if (d->d_compute)
-    printf("[pasif] lane kernel: %zu instances in %zu structure class(es), %u workgroups of %u "
-           "wave(s), T max %u, checkpoint %u\n",
-           inst.size(), shapes.size(), block, lay.waves, tmax, pk.lseg);
-  return true;
+  pk.lparts.push_back(std::move(P));
+  return pk.lparts.back().get();
 }
 
 void require_device(pasif_driver* d) {
@@ -1269,7 +1315,8 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
   Packed& pk = d->pk;
   pk.tmax = std::max(pk.tmax, pk.ltmax);
   if (pk.lane && !pk.tile) {
-    pk.ntiles = pk.lblocks;  // reported as the launch's workgroups
+    pk.ntiles = 0;  // reported: the lane launch's workgroups
+    for (const auto& lp : pk.lparts) pk.ntiles += lp->blocks;
     pk.lanes = kLaneWave;
     pk.slot_cfg = 0;
   }
@@ -1495,12 +1542,9 @@ void drive(pasif_driver* d, uint64_t launches, F&& launch) {
   d->last_launches = uint32_t(launches);
 }
 
-// Output and run buffers of the lane part and its per-class arguments
-// (budget filled in by the caller).  *work: steps of work of the longest
-// class (3T with the adjoint).
+// Output buffers the lane parts write (shared with the tile part).
 template <typename R>
-std::vector<LaneArgs> lane_prepare(pasif_driver* d, bool adjoint, uint32_t save_steps,
-                                   uint64_t* work, uint32_t* nslots_out) {
+uint32_t lane_outputs(pasif_driver* d, bool adjoint, uint32_t save_steps) {
   ensure_excitations<R>(d);
   Packed& pk = d->pk;
   dev_reserve(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
@@ -1515,13 +1559,21 @@ std::vector<LaneArgs> lane_prepare(pasif_driver* d, bool adjoint, uint32_t save_
     PASIF_HIP(hipMemsetAsync(d->traj.p, 0, size_t(nslots) * pk.out_len_f * sizeof(R), d->stream));
   }
   if (adjoint) dev_reserve(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
-  *work = adjoint ? 3ull * pk.ltmax : pk.ltmax;
-  *nslots_out = nslots;
-  const uint32_t K = pk.lseg;
+  return nslots;
+}
+
+// Run buffers of one lane part and its per-class kernel arguments (budget
+// filled in by the caller).  K is the part's checkpoint interval.
+template <typename R>
+std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
+                                     uint32_t save_steps, uint32_t nslots, const void* excit,
+                                     const void* interp) {
+  Packed& pk = d->pk;
+  const uint32_t K = adjoint ? P.lay.seg : 1u;
   const R h = R(1.0 / double(d->sample_rate));
-  std::vector<LaneArgs> args(pk.lcls.size());
-  for (size_t c = 0; c < pk.lcls.size(); ++c) {
-    LaneCls& L = *pk.lcls[c];
+  std::vector<LaneArgs> args(P.cls.size());
+  for (size_t c = 0; c < P.cls.size(); ++c) {
+    LaneCls& L = *P.cls[c];
     const size_t st = L.stride;
     dev_reserve(d, L.qstate, size_t(L.nf) * st * sizeof(R));
     dev_reserve(d, L.progress, size_t(L.waves) * 4 * sizeof(uint32_t));
@@ -1554,11 +1606,12 @@ std::vector<LaneArgs> lane_prepare(pasif_driver* d, bool adjoint, uint32_t save_
     a.out_max = d->window ? d->out_max.p : nullptr;
     a.traj = save_steps ? d->traj.p : nullptr;
     a.out_grad = adjoint ? d->out_grad.p : nullptr;
-    a.excit = d->d_exc.p;
-    a.interp = d->interp_w ? d->d_interp.p : nullptr;
+    a.excit = excit;
+    a.interp = d->interp_w ? interp : nullptr;
     a.stride = st;
     a.count = L.count;
     a.block_begin = L.block_begin;
+    a.klass = L.klass;
     a.exc_len = d->exc_len;
     a.interp_w = d->interp_w;
     a.window = d->window;
@@ -1577,6 +1630,15 @@ std::vector<LaneArgs> lane_prepare(pasif_driver* d, bool adjoint, uint32_t save_
   return args;
 }
 
+hipError_t lane_launch(const LanePart& P, const void* dargs, uint32_t n, hipStream_t st) {
+  // multi-wave kernels declare their LDS statically
+  const uint32_t lds = P.lay.waves > 1 ? 0u : uint32_t(P.lds);
+  void* dp = const_cast<void*>(dargs);
+  void* params[] = {&dp, &n};
+  return hipModuleLaunchKernel(P.kern.fn, P.blocks, 1, 1, kLaneWave * P.lay.waves, 1, 1, lds,
+                               st, params, nullptr);
+}
+
 void ensure_aux(pasif_driver* d) {
   if (d->aux) return;
   PASIF_HIP(hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking));
@@ -1592,45 +1654,43 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   ensure_packed(d, adjoint);
   Packed& pk = d->pk;
   const bool lane = pk.lane, tile = pk.tile;
-  uint64_t twork = 0, lwork = 0;
+  uint64_t twork = 0;
   uint32_t nslots = 0;
   TileArgs<R> ta{};
   if (tile) ta = prepare_run<R>(d, adjoint, save_steps, &twork, &nslots);
-  std::vector<LaneArgs> largs;
-  if (lane) largs = lane_prepare<R>(d, adjoint, save_steps, &lwork, &nslots);
+  std::vector<std::vector<LaneArgs>> largs;
+  if (lane) {
+    nslots = lane_outputs<R>(d, adjoint, save_steps);
+    for (auto& lp : pk.lparts)
+      largs.push_back(lane_part_args<R>(d, *lp, adjoint, save_steps, nslots, d->d_exc.p,
+                                        d->d_interp.p));
+  }
+  const uint64_t lwork = lane ? (adjoint ? 3ull * pk.ltmax : pk.ltmax) : 0;
   uint64_t launches = 1;
   const uint32_t budget = launch_budget(d, std::max(twork, lwork), &launches);
   ta.budget = budget;
-  for (LaneArgs& a : largs) a.budget = budget;
-  const void* dargs = nullptr;
-  if (lane) {
-    upload(d, pk.largs, largs);
-    dargs = pk.largs.p;
+  for (size_t i = 0; i < largs.size(); ++i) {
+    for (LaneArgs& a : largs[i]) a.budget = budget;
+    upload(d, pk.lparts[i]->args, largs[i]);
   }
   if (lane && tile) ensure_aux(d);
-  uint32_t ncls = uint32_t(largs.size());
-  hipFunction_t fn = pk.lkern.fn;
-  const uint32_t blocks = pk.lblocks;
-  // multi-wave kernels declare their LDS statically
-  const uint32_t lds = pk.llay.waves > 1 ? 0u : uint32_t(pk.llds);
-  const uint32_t bdim = kLaneWave * pk.llay.waves;
   const uint32_t nt = pk.ntiles;
-  auto lane_launch = [&](hipStream_t st) -> hipError_t {
-    void* params[] = {const_cast<void**>(&dargs), &ncls};
-    return hipModuleLaunchKernel(fn, blocks, 1, 1, bdim, 1, 1, lds, st, params, nullptr);
-  };
   drive(d, launches, [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    hipStream_t ls = tile ? d->aux : d->stream;
     if (lane && tile) {
-      hipError_t e = hipEventRecord(d->fork, d->stream);
+      e = hipEventRecord(d->fork, d->stream);
       if (e == hipSuccess) e = hipStreamWaitEvent(d->aux, d->fork, 0);
-      if (e == hipSuccess) e = lane_launch(d->aux);
-      if (e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+    }
+    for (size_t i = 0; i < largs.size() && e == hipSuccess; ++i)
+      if (pk.lparts[i]->blocks)
+        e = lane_launch(*pk.lparts[i], pk.lparts[i]->args.p, uint32_t(largs[i].size()), ls);
+    if (tile && e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+    if (lane && tile) {
       if (e == hipSuccess) e = hipEventRecord(d->join, d->aux);
       if (e == hipSuccess) e = hipStreamWaitEvent(d->stream, d->join, 0);
-      return e;
     }
-    if (lane) return lane_launch(d->stream);
-    return launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+    return e;
   });
   finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
 }

@@ -97,6 +97,7 @@ __device__ __forceinline__ u32 umin(u32 a, u32 b) { return a < b ? a : b; }
 enum : u32 { kForward = 0, kRecompute = 1, kBackward = 2, kDone = 3 };
 template <bool B> struct Flag { static constexpr bool value = B; };
 
+
 // e[b .. b + STAGE) of one excitation (indices clamped into [0, L))
 __device__ __forceinline__ void fetch(R (&v)[STAGE], GP(const R) E, u32 Lx, u32 b) {
 #pragma unroll
@@ -1031,6 +1032,14 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
   PASIF_LANE_FIELDS(PASIF_LANE_TEXT)
 #undef PASIF_LANE_TEXT
   o << "};\n";
+  o << "__device__ __forceinline__ u32 find_entry(const LaneArgs* cls, u32 n, u32 b) {\n"
+       "  u32 lo = 0, hi = n;\n"
+       "  while (hi - lo > 1u) {\n"
+       "    const u32 mid = (lo + hi) >> 1;\n"
+       "    if (b >= (u32)cls[mid].block_begin) lo = mid; else hi = mid;\n"
+       "  }\n"
+       "  return (u32)__builtin_amdgcn_readfirstlane((int)lo);\n"
+       "}\n";
   o << "#define WAVE " << kLaneWave << "u\n#define CHUNK " << kLaneChunk << "u\n#define STAGE "
     << kLaneStage << "u\n";
   MwDims dims;
@@ -1121,13 +1130,11 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
       << ") pasif_lane_kernel(const LaneArgs* __restrict__ cls, u32 ncls) {\n"
          "  extern __shared__ R smem[];\n"
          "  const u32 b = blockIdx.x;\n"
-         "  u32 c = 0;\n"
-         "  while (c + 1u < ncls && b >= (u32)cls[c + 1u].block_begin) ++c;\n"
-         "  c = uni(c);\n"
-         "  switch (c) {\n";
+         "  const u32 e = find_entry(cls, ncls, b);\n"
+         "  const LaneArgs& a = cls[e];\n"
+         "  switch (uni((u32)a.klass)) {\n";
     for (size_t c = 0; c < shapes.size(); ++c)
-      o << "    case " << c << ": run<C" << c << ">(cls[" << c << "], b - (u32)cls[" << c
-        << "].block_begin, smem); break;\n";
+      o << "    case " << c << ": run<C" << c << ">(a, b - (u32)a.block_begin, smem); break;\n";
     o << "    default: break;\n  }\n}\n";
   } else {
     o << "extern \"C\" __global__ void __launch_bounds__(" << kLaneWave * lay.waves
@@ -1135,15 +1142,14 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
       << "  __shared__ R smem[" << mw_lds_values(dims, lay) << "];\n"
       << "  const u32 b = blockIdx.x;\n"
          "  const u32 wid = uni(threadIdx.x >> 6);\n"
-         "  u32 c = 0;\n"
-         "  while (c + 1u < ncls && b >= (u32)cls[c + 1u].block_begin) ++c;\n"
-         "  c = uni(c);\n"
-         "  switch (c) {\n";
+         "  const u32 e = find_entry(cls, ncls, b);\n"
+         "  const LaneArgs& a = cls[e];\n"
+         "  switch (uni((u32)a.klass)) {\n";
     for (size_t c = 0; c < shapes.size(); ++c) {
       o << "    case " << c << ":\n      switch (wid) {\n";
       for (uint32_t w = 0; w < lay.waves; ++w)
-        o << "        case " << w << ": run_mw<C" << c << ", C" << c << "W" << w << ">(cls[" << c
-          << "], b - (u32)cls[" << c << "].block_begin, smem); break;\n";
+        o << "        case " << w << ": run_mw<C" << c << ", C" << c << "W" << w
+          << ">(a, b - (u32)a.block_begin, smem); break;\n";
       o << "        default: break;\n      }\n      break;\n";
     }
     o << "    default: break;\n  }\n}\n";

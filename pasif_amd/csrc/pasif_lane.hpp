@@ -46,6 +46,7 @@ namespace pasif {
   X(uint64_t, stride)      /* instance slots (multiple of 64) */            \
   X(uint64_t, count)       /* instances */                                  \
   X(uint64_t, block_begin) /* first workgroup of the class */               \
+  X(uint64_t, klass)       /* the class's case in the kernel's dispatch */  \
   X(uint64_t, exc_len)                                                       \
   X(uint64_t, interp_w)                                                      \
   X(uint64_t, budget)                                                        \
