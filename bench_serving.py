@@ -32,7 +32,7 @@ if ROOT not in sys.path:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--loads", type=float, nargs="+", default=[1000, 5000, 20000])
+    ap.add_argument("--loads", type=float, nargs="+", default=[50, 100, 1000, 5000, 20000])
     ap.add_argument("--duration", type=float, default=10.0)
     ap.add_argument("--max-wall", type=float, default=120.0)
     ap.add_argument("--tmin", type=float, default=1e3)

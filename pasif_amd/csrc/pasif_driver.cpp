@@ -260,6 +260,9 @@ struct pasif_driver {
   // a job of a pool: uses the pool's device and upload stream
   pasif_pool* pool = nullptr;
   uint32_t pool_ck = 0, pool_nck = 0;
+  // a pool job's lane kernel arguments, one vector per lane part (ticks
+  // copy them into the pool's per-module tables)
+  std::vector<std::vector<LaneArgs>> pool_largs;
   // excitations / interpolation rows owned by the pool (shared signal bank)
   const void* ext_exc = nullptr;
   const void* ext_interp = nullptr;
@@ -311,6 +314,25 @@ struct pasif_pool {
     bool has_adj = false;
   };
   std::vector<Entry> reg;
+  // Lane modules over the registered structures, built when lane work is
+  // first submitted: 0 = classes that fit one lane, 1 = classes spread over
+  // the waves of a workgroup.  reg_lane[id] = (module << 16 | class) or -1.
+  struct Module {
+    std::vector<LaneShape> shapes;
+    LaneLayout lay;
+    LaneKernel kern;
+    size_t lds = 0;
+    std::vector<LaneArgs> table;  // this tick's entries
+    void* table_dev = nullptr;
+    size_t table_bytes = 0;
+    uint32_t blocks = 0;
+  };
+  std::vector<Module> mods;
+  bool mods_built = false;
+  std::map<std::vector<uint32_t>, int32_t> key_lane;
+  std::vector<int32_t> reg_lane;
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1284,6 +1306,9 @@ void require_device(pasif_driver* d) {
                 "this handle was created without a GPU (gpu_id < 0): host-side checks only");
 }
 
+template <typename R>
+void pool_lane_parts(pasif_driver* d, bool adjoint);
+
 void ensure_packed(pasif_driver* d, bool adjoint) {
   PASIF_REQUIRE(d->sys_ready, PASIF_ERR_STATE, "the system must be allocated before solving");
   if (adjoint)
@@ -1292,12 +1317,17 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
   if (d->pk.valid && d->pk.key == key) return;
   d->pk.valid = false;
   d->pk.release();
-  d->lane_cls.clear();
+  if (!d->pool) d->lane_cls.clear();
   if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool) {
     if (d->precision == 64)
       (void)lane_plan<double>(d, adjoint);
     else
       (void)lane_plan<float>(d, adjoint);
+  } else if (d->pool && !d->lane_cls.empty()) {
+    if (d->precision == 64)
+      pool_lane_parts<double>(d, adjoint);
+    else
+      pool_lane_parts<float>(d, adjoint);
   }
   // the tile kernel takes every system the lane kernel did not
   bool rest = !d->pk.lane;
@@ -2200,6 +2230,13 @@ double job_bytes(const pasif_driver* d) {
                          &d->d_interp};
   double b = double(d->pk.program_bytes) + double(d->pk.hdr.bytes);
   for (const DevBuf* x : run) b += double(x->bytes);
+  for (const auto& lp : d->pk.lparts)
+    for (const auto& L : lp->cls) {
+      const DevBuf* lb[] = {&L->coef_f, &L->coef_a, &L->init_f, &L->init_a, &L->steps,
+                            &L->exc, &L->out_f, &L->out_a, &L->qstate, &L->lstate,
+                            &L->mxstate, &L->progress, &L->ckpt, &L->segbuf};
+      for (const DevBuf* x : lb) b += double(x->bytes);
+    }
   return b;
 }
 
@@ -2208,6 +2245,7 @@ void release_run(pasif_driver* d) {
                    &d->out_max, &d->out_grad, &d->ckpt, &d->segbuf};
   for (DevBuf* x : run) x->release();
   d->pk.release();
+  d->pool_largs.clear();
 }
 
 // One system's block of a side, re-indexed from 0: factor 0 relative to the
@@ -2329,20 +2367,38 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
   uint64_t work = 0;
   uint32_t nslots = 0;
   int rc = guarded(d, [&] {
-    if (P->precision == 64) {
-      TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
-      d->pool_ck = a.ck_every;
-      d->pool_nck = a.nck;
-    } else {
-      TileArgs<float> a = prepare_run<float>(d, adjoint != 0, 0, &work, &nslots);
-      d->pool_ck = a.ck_every;
-      d->pool_nck = a.nck;
+    ensure_packed(d, adjoint != 0);
+    if (d->pk.tile) {
+      if (P->precision == 64) {
+        TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
+        d->pool_ck = a.ck_every;
+        d->pool_nck = a.nck;
+      } else {
+        TileArgs<float> a = prepare_run<float>(d, adjoint != 0, 0, &work, &nslots);
+        d->pool_ck = a.ck_every;
+        d->pool_nck = a.nck;
+      }
+      PASIF_REQUIRE(d->pk.lanes == P->lanes, PASIF_ERR_UNSUPPORTED,
+                    "a system of this request needs %u lanes, the pool runs %u", d->pk.lanes,
+                    P->lanes);
+      PASIF_REQUIRE(d->pk.slot_cfg == P->slot_cfg, PASIF_ERR_UNSUPPORTED,
+                    "slot configuration mismatch");
     }
-    PASIF_REQUIRE(d->pk.lanes == P->lanes, PASIF_ERR_UNSUPPORTED,
-                  "a system of this request needs %u lanes, the pool runs %u", d->pk.lanes,
-                  P->lanes);
-    PASIF_REQUIRE(d->pk.slot_cfg == P->slot_cfg, PASIF_ERR_UNSUPPORTED,
-                  "slot configuration mismatch");
+    d->pool_largs.clear();
+    if (d->pk.lane) {
+      if (P->precision == 64) {
+        (void)lane_outputs<double>(d, adjoint != 0, 0);
+        for (auto& lp : d->pk.lparts)
+          d->pool_largs.push_back(lane_part_args<double>(d, *lp, adjoint != 0, 0, 0,
+                                                         P->bank.p, P->bank_interp.p));
+      } else {
+        (void)lane_outputs<float>(d, adjoint != 0, 0);
+        for (auto& lp : d->pk.lparts)
+          d->pool_largs.push_back(lane_part_args<float>(d, *lp, adjoint != 0, 0, 0,
+                                                        P->bank.p, P->bank_interp.p));
+      }
+      work = std::max<uint64_t>(work, adjoint ? 3ull * d->pk.ltmax : d->pk.ltmax);
+    }
   });
   if (rc != PASIF_OK) {
     release_run(d);
@@ -2350,7 +2406,8 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
     throw Fail{rc};
   }
   const double bytes = job_bytes(d);
-  const uint32_t tiles = d->pk.ntiles;
+  uint32_t tiles = d->pk.tile ? d->pk.ntiles : 0;
+  for (const auto& lp : d->pk.lparts) tiles += lp->blocks;
   if (!P->jobs.empty() && P->max_tiles && P->tiles_used + tiles > P->max_tiles) {
     release_run(d);
     P->err = "no free slots: the request needs " + std::to_string(tiles) + " tiles, " +
@@ -2369,16 +2426,169 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
   P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false, tiles});
 }
 
+// Lane modules over the registered structures: module 0 runs the classes
+// that fit one lane, module 1 (multi-wave, 16 instances per workgroup) the
+// classes that need a workgroup; classes fitting neither stay on tiles.
+template <typename R>
+void pool_build_modules(pasif_pool* P) {
+  const double eps = std::numeric_limits<R>::epsilon();
+  std::map<std::vector<uint32_t>, uint32_t> cls_of_key;
+  std::vector<LaneShape> all;
+  std::vector<std::vector<uint32_t>> keys;
+  std::vector<uint32_t> count;
+  std::vector<uint32_t> key;
+  for (const auto& e : P->reg) {
+    LaneShape sh;
+    sh.f = lane_side(e.f, 0, false, eps);
+    sh.adjoint = e.has_adj;
+    if (e.has_adj) sh.a = lane_side(e.a, 0, true, eps);
+    key.clear();
+    shape_key(sh.f, key);
+    key.push_back(e.has_adj ? 1u : 0u);
+    if (e.has_adj) shape_key(sh.a, key);
+    auto it = cls_of_key.find(key);
+    if (it == cls_of_key.end()) {
+      it = cls_of_key.emplace(key, uint32_t(all.size())).first;
+      all.push_back(std::move(sh));
+      keys.push_back(key);
+      count.push_back(0);
+    }
+    count[it->second] += 1;
+  }
+  std::vector<uint32_t> need(all.size(), 0);
+  for (size_t c = 0; c < all.size(); ++c) {
+    if (all[c].live_values() <= lane_max_live()) {
+      need[c] = 1;
+    } else {
+      for (uint32_t w = 2; w <= kLaneMaxWaves && !need[c]; ++w)
+        if (lane_partition(all[c], w) <= kLaneMaxWaveLive) need[c] = w;
+    }
+  }
+  P->mods.clear();
+  P->mods.resize(2);
+  const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
+  const size_t el = sizeof(R);
+  for (int m = 0; m < 2; ++m) {
+    std::vector<uint32_t> order;
+    uint32_t waves = 1;
+    for (uint32_t c = 0; c < all.size(); ++c)
+      if (m == 0 ? need[c] == 1 : need[c] >= 2) order.push_back(c);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return count[x] > count[y]; });
+    if (order.size() > kLaneMaxClasses) order.resize(kLaneMaxClasses);
+    for (uint32_t c : order) waves = std::max(waves, need[c]);
+    if (order.empty()) continue;
+    auto& M = P->mods[size_t(m)];
+    for (uint32_t c : order) M.shapes.push_back(all[c]);
+    bool adj = false;
+    uint32_t nf_max = 0;
+    for (const LaneShape& sh : M.shapes) {
+      adj = adj || sh.adjoint;
+      nf_max = std::max(nf_max, sh.f.n);
+    }
+    LaneLayout lay;
+    if (m == 0) {
+      if (adj) {
+        const size_t stage = size_t(kLaneStage) * kLaneWave * el;
+        const size_t fit = (kLaneMaxLds - stage) / (size_t(nf_max) * kLaneWave * el);
+        lay.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+      }
+    } else {
+      for (LaneShape& sh : M.shapes) (void)lane_partition(sh, waves);
+      lay.waves = waves;
+      lay.lw = 16;
+      lay.seg = 1;
+      if (adj) {
+        uint32_t lo = 1, hi = Kreq;
+        lay.seg = 0;
+        while (lo <= hi) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          LaneLayout t = lay;
+          t.seg = mid;
+          if (lane_lds_bytes(M.shapes, int(el * 8), t) <= kMaxLdsBytes) {
+            lay.seg = mid;
+            lo = mid + 1;
+          } else {
+            hi = mid - 1;
+          }
+        }
+      }
+      if (lay.seg == 0 || lane_lds_bytes(M.shapes, int(el * 8), lay) > kMaxLdsBytes) {
+        M.shapes.clear();  // does not fit: these classes stay on tiles
+        continue;
+      }
+    }
+    M.lay = lay;
+    M.lds = lane_lds_bytes(M.shapes, int(el * 8), lay);
+    POOL_REQUIRE(M.lds <= (lay.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
+                 "pool lane module LDS");
+    const std::string src = lane_source(M.shapes, int(el * 8), lay);
+    std::string err;
+    POOL_HIP(hipSetDevice(P->dev));
+    if (!lane_compile(src, P->dev, &M.kern, &err)) {
+      P->err = err;
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+    for (size_t k = 0; k < order.size(); ++k)
+      P->key_lane[keys[order[k]]] = lane_code(m, uint32_t(k));
+  }
+  P->reg_lane.assign(P->reg.size(), -1);
+  size_t id = 0;
+  for (const auto& e : P->reg) {
+    LaneShape sh;
+    sh.f = lane_side(e.f, 0, false, eps);
+    if (e.has_adj) sh.a = lane_side(e.a, 0, true, eps);
+    key.clear();
+    shape_key(sh.f, key);
+    key.push_back(e.has_adj ? 1u : 0u);
+    if (e.has_adj) shape_key(sh.a, key);
+    auto it = P->key_lane.find(key);
+    P->reg_lane[id++] = it == P->key_lane.end() ? -1 : it->second;
+  }
+  P->mods_built = true;
+}
+
+// A pool job's lane parts: its systems of each module, laid out as the module.
+template <typename R>
+void pool_lane_parts(pasif_driver* d, bool adjoint) {
+  pasif_pool* P = d->pool;
+  for (size_t m = 0; m < P->mods.size(); ++m) {
+    bool any = false;
+    for (int32_t c : d->lane_cls) any = any || (c >= 0 && lane_part_of(c) == int(m));
+    if (!any) continue;
+    const auto& M = P->mods[m];
+    std::vector<uint32_t> klass(M.shapes.size());
+    for (size_t c = 0; c < klass.size(); ++c) klass[c] = uint32_t(c);
+    LanePart* lp = lane_build<R>(d, adjoint, int(m), M.shapes, klass, M.lay);
+    lp->kern = M.kern;
+    lp->lds = M.lds;
+  }
+}
+
 template <typename R>
 void pool_tick_impl(pasif_pool* P) {
   std::vector<TileArgs<R>> table;
   P->map_host.clear();
   size_t lds = 0;
+  for (auto& M : P->mods) {
+    M.table.clear();
+    M.blocks = 0;
+  }
   for (auto& j : P->jobs) {
     if (j.finished) continue;
     pasif_driver* d = j.d;
-    uint64_t work = 0;
-    uint32_t nslots = 0;
+    for (size_t i = 0; i < d->pk.lparts.size(); ++i) {
+      const LanePart& lp = *d->pk.lparts[i];
+      auto& M = P->mods[size_t(lp.module)];
+      for (const LaneArgs& a0 : d->pool_largs[i]) {
+        LaneArgs a = a0;
+        a.block_begin = M.blocks + a0.block_begin;
+        a.budget = P->budget;
+        M.table.push_back(a);
+      }
+      M.blocks += lp.blocks;
+    }
+    if (!d->pk.tile) continue;
     // the job's buffers already exist: re-derive its arguments without
     // clearing the progress (prepare_run only clears at admission)
     TileArgs<R> a{};
@@ -2416,8 +2626,6 @@ void pool_tick_impl(pasif_pool* P) {
         a.adjoint = 1;
       }
       a.budget = P->budget;
-      (void)work;
-      (void)nslots;
     }
     const uint32_t id = uint32_t(table.size());
     table.push_back(a);
@@ -2428,36 +2636,74 @@ void pool_tick_impl(pasif_pool* P) {
     lds = std::max(lds, tile_lds_bytes(sizeof(R), P->lanes, true, a.interp_w, a.max_forced));
   }
   const uint32_t nblocks = uint32_t(P->map_host.size() / 2);
-  P->last_blocks = nblocks;
-  if (nblocks == 0) return;
-  const size_t tb = table.size() * sizeof(TileArgs<R>);
-  P->table_host.assign(reinterpret_cast<unsigned char*>(table.data()),
-                       reinterpret_cast<unsigned char*>(table.data()) + tb);
-  if (P->table_bytes < tb) {
-    if (P->table_dev) (void)hipFree(P->table_dev);
-    P->table_dev = nullptr;
-    POOL_HIP(hipMalloc(&P->table_dev, tb));
-    P->table_bytes = tb;
-  }
-  const size_t mb = P->map_host.size() * sizeof(uint32_t);
-  if (P->map_bytes < mb) {
-    if (P->map_dev) (void)hipFree(P->map_dev);
-    P->map_dev = nullptr;
-    POOL_HIP(hipMalloc(&P->map_dev, mb));
-    P->map_bytes = mb;
-  }
+  uint32_t lblocks = 0;
+  for (const auto& M : P->mods) lblocks += M.blocks;
+  P->last_blocks = nblocks + lblocks;
+  if (nblocks == 0 && lblocks == 0) return;
   POOL_HIP(hipStreamWaitEvent(P->stream, P->uploaded, 0));
-  POOL_HIP(hipMemcpyAsync(P->table_dev, P->table_host.data(), tb, hipMemcpyHostToDevice,
-                          P->stream));
-  POOL_HIP(hipMemcpyAsync(P->map_dev, P->map_host.data(), mb, hipMemcpyHostToDevice, P->stream));
-  POOL_HIP(hipEventRecord(P->ev0, P->stream));
-  hipError_t e = launch_pool(static_cast<const TileArgs<R>*>(P->table_dev),
-                             static_cast<const uint32_t*>(P->map_dev), nblocks, P->lanes,
-                             int(P->slot_cfg), lds, P->stream);
-  if (e != hipSuccess) {
-    P->err = std::string("pool kernel launch failed: ") + hipGetErrorName(e);
-    throw Fail{PASIF_ERR_DEVICE};
+  if (nblocks) {
+    const size_t tb = table.size() * sizeof(TileArgs<R>);
+    P->table_host.assign(reinterpret_cast<unsigned char*>(table.data()),
+                         reinterpret_cast<unsigned char*>(table.data()) + tb);
+    if (P->table_bytes < tb) {
+      if (P->table_dev) (void)hipFree(P->table_dev);
+      P->table_dev = nullptr;
+      POOL_HIP(hipMalloc(&P->table_dev, tb));
+      P->table_bytes = tb;
+    }
+    const size_t mb = P->map_host.size() * sizeof(uint32_t);
+    if (P->map_bytes < mb) {
+      if (P->map_dev) (void)hipFree(P->map_dev);
+      P->map_dev = nullptr;
+      POOL_HIP(hipMalloc(&P->map_dev, mb));
+      P->map_bytes = mb;
+    }
+    POOL_HIP(hipMemcpyAsync(P->table_dev, P->table_host.data(), tb, hipMemcpyHostToDevice,
+                            P->stream));
+    POOL_HIP(hipMemcpyAsync(P->map_dev, P->map_host.data(), mb, hipMemcpyHostToDevice,
+                            P->stream));
   }
+  for (auto& M : P->mods) {
+    if (!M.blocks) continue;
+    const size_t b = M.table.size() * sizeof(LaneArgs);
+    if (M.table_bytes < b) {
+      if (M.table_dev) (void)hipFree(M.table_dev);
+      M.table_dev = nullptr;
+      POOL_HIP(hipMalloc(&M.table_dev, b));
+      M.table_bytes = b;
+    }
+    POOL_HIP(hipMemcpyAsync(M.table_dev, M.table.data(), b, hipMemcpyHostToDevice, P->stream));
+  }
+  POOL_HIP(hipEventRecord(P->ev0, P->stream));
+  // lane modules on their own streams beside the tile launch
+  if (lblocks) POOL_HIP(hipEventRecord(P->fork, P->stream));
+  for (size_t m = 0; m < P->mods.size(); ++m) {
+    auto& M = P->mods[m];
+    if (!M.blocks) continue;
+    POOL_HIP(hipStreamWaitEvent(P->aux[m], P->fork, 0));
+    const uint32_t lds_m = M.lay.waves > 1 ? 0u : uint32_t(M.lds);
+    void* dp = M.table_dev;
+    uint32_t n = uint32_t(M.table.size());
+    void* params[] = {&dp, &n};
+    hipError_t e = hipModuleLaunchKernel(M.kern.fn, M.blocks, 1, 1, kLaneWave * M.lay.waves, 1,
+                                         1, lds_m, P->aux[m], params, nullptr);
+    if (e != hipSuccess) {
+      P->err = std::string("pool lane kernel launch failed: ") + hipGetErrorName(e);
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+    POOL_HIP(hipEventRecord(P->join[m], P->aux[m]));
+  }
+  if (nblocks) {
+    hipError_t e = launch_pool(static_cast<const TileArgs<R>*>(P->table_dev),
+                               static_cast<const uint32_t*>(P->map_dev), nblocks, P->lanes,
+                               int(P->slot_cfg), lds, P->stream);
+    if (e != hipSuccess) {
+      P->err = std::string("pool kernel launch failed: ") + hipGetErrorName(e);
+      throw Fail{PASIF_ERR_DEVICE};
+    }
+  }
+  for (size_t m = 0; m < P->mods.size(); ++m)
+    if (P->mods[m].blocks) POOL_HIP(hipStreamWaitEvent(P->stream, P->join[m], 0));
   POOL_HIP(hipEventRecord(P->ev1, P->stream));
   for (auto& j : P->jobs)
     if (!j.finished) j.done += P->budget;
@@ -2494,6 +2740,11 @@ int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile
     POOL_HIP(hipEventCreate(&P->ev0));
     POOL_HIP(hipEventCreate(&P->ev1));
     POOL_HIP(hipEventCreateWithFlags(&P->uploaded, hipEventDisableTiming));
+    for (int m = 0; m < 2; ++m) {
+      POOL_HIP(hipStreamCreateWithFlags(&P->aux[m], hipStreamNonBlocking));
+      POOL_HIP(hipEventCreateWithFlags(&P->join[m], hipEventDisableTiming));
+    }
+    POOL_HIP(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
     P->precision = precision;
     P->lanes = tile_lanes;
     P->slot_cfg = slot_config;
@@ -2610,6 +2861,21 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
       }
       P->reg.push_back(std::move(e));
     }
+    if (P->mods_built) {
+      // structures the lane modules already run join them; new ones use tiles
+      const double eps = P->precision == 64 ? std::numeric_limits<double>::epsilon()
+                                            : double(std::numeric_limits<float>::epsilon());
+      std::vector<uint32_t> key;
+      for (size_t id = first; id < P->reg.size(); ++id) {
+        const auto& e = P->reg[id];
+        key.clear();
+        shape_key(lane_side(e.f, 0, false, eps), key);
+        key.push_back(e.has_adj ? 1u : 0u);
+        if (e.has_adj) shape_key(lane_side(e.a, 0, true, eps), key);
+        auto it = P->key_lane.find(key);
+        P->reg_lane.push_back(it == P->key_lane.end() ? -1 : it->second);
+      }
+    }
     *first_id = uint32_t(first);
     *count = uint32_t(S);
   });
@@ -2634,7 +2900,15 @@ int pasif_pool_submit_systems(pasif_pool* P, uint32_t n, const uint32_t* ids,
                    "excitation " + std::to_string(exc_of_system[i]) + " not in the bank");
       POOL_REQUIRE(steps_of_system[i] > 0, PASIF_ERR_ARGUMENT, "step counts must be positive");
     }
+    if (!P->mods_built) {
+      if (P->precision == 64)
+        pool_build_modules<double>(P);
+      else
+        pool_build_modules<float>(P);
+    }
     std::unique_ptr<pasif_driver> d(new_bank_job(P, 0));
+    d->lane_cls.assign(n, -1);
+    for (uint32_t i = 0; i < n; ++i) d->lane_cls[i] = P->reg_lane[ids[i]];
     std::vector<const RowTerms*> parts(n);
     for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].f;
     concat_blocks(parts, false, {}, d->fwd);
@@ -2726,13 +3000,23 @@ int pasif_pool_collect(pasif_pool* P, pasif_driver* d, double* q, size_t capq, s
     POOL_REQUIRE(j.done >= j.work, PASIF_ERR_STATE, "job not finished");
     POOL_HIP(hipSetDevice(P->dev));
     // every tile must have reached its last phase
-    std::vector<uint32_t> prog(size_t(d->pk.ntiles) * 4);
-    POOL_HIP(hipMemcpy(prog.data(), d->progress.p, prog.size() * 4, hipMemcpyDeviceToHost));
-    for (uint32_t t = 0; t < d->pk.ntiles; ++t) {
-      const bool done = j.adjoint ? prog[4 * t + 1] == kPhaseDone
-                                  : prog[4 * t] >= d->pk.tmax || prog[4 * t + 1] == kPhaseDone;
-      POOL_REQUIRE(done, PASIF_ERR_STATE, "job tiles did not finish");
+    if (d->pk.tile) {
+      std::vector<uint32_t> prog(size_t(d->pk.ntiles) * 4);
+      POOL_HIP(hipMemcpy(prog.data(), d->progress.p, prog.size() * 4, hipMemcpyDeviceToHost));
+      for (uint32_t t = 0; t < d->pk.ntiles; ++t) {
+        const bool done = j.adjoint ? prog[4 * t + 1] == kPhaseDone
+                                    : prog[4 * t] >= d->pk.tmax || prog[4 * t + 1] == kPhaseDone;
+        POOL_REQUIRE(done, PASIF_ERR_STATE, "job tiles did not finish");
+      }
     }
+    for (const auto& lp : d->pk.lparts)
+      for (const auto& L : lp->cls) {
+        std::vector<uint32_t> prog(size_t(L->waves) * 4);
+        POOL_HIP(hipMemcpy(prog.data(), L->progress.p, prog.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t w = 0; w < L->waves; ++w)
+          POOL_REQUIRE(prog[4 * w + 1] == kPhaseDone, PASIF_ERR_STATE,
+                       "job lane workgroups did not finish");
+      }
     std::vector<double> grad_v;
     const int rc = guarded(d, [&] {
       if (P->precision == 64)
@@ -2780,6 +3064,16 @@ int pasif_pool_destroy(pasif_pool* P) {
   }
   P->jobs.clear();
   P->arena.reset();
+  for (int m = 0; m < 2; ++m) {
+    if (P->aux[m]) {
+      (void)hipStreamSynchronize(P->aux[m]);
+      (void)hipStreamDestroy(P->aux[m]);
+    }
+    if (P->join[m]) (void)hipEventDestroy(P->join[m]);
+  }
+  if (P->fork) (void)hipEventDestroy(P->fork);
+  for (auto& M : P->mods)
+    if (M.table_dev) (void)hipFree(M.table_dev);
   if (P->table_dev) (void)hipFree(P->table_dev);
   if (P->map_dev) (void)hipFree(P->map_dev);
   if (P->ev0) (void)hipEventDestroy(P->ev0);
