@@ -164,6 +164,16 @@ int pasif_set_kernel(pasif_driver* d, int kernel);
  * (the lane kernel shortens it until a segment fits its LDS) */
 int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* classes,
                       uint32_t* checkpoint_every);
+/* Lanes per instance in the lane kernel (extension).  0 (default): chosen per
+ * structure class from a cost model (one lane per instance while the GPU's
+ * SIMDs are busy; a group of 2..64 lanes, rows in bands across the group and
+ * values exchanged by lane rotations, when the instances are too few to keep
+ * them busy or a system does not fit one lane's registers).  A power of two
+ * from 1 to 64 requests that group size for every class it fits. */
+int pasif_set_lane_group(pasif_driver* d, uint32_t lanes);
+/* lanes per instance of each lane-kernel class of the packed problem
+ * (out == NULL: count query) */
+int pasif_lane_groups(pasif_driver* d, int adjoint, uint32_t* out, size_t cap, size_t* len);
 /* Diagnostics: the generated HIP source of the lane kernel for the loaded
  * problem (NUL-terminated; out == NULL: length query), and with compile != 0
  * its hipRTC compilation for gfx950, which needs no GPU. */

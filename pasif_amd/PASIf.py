@@ -402,10 +402,14 @@ class PASIf(_GpuDriver):
     def setTuning(self, tile_lanes: int = 0, slot_config: int = 0, checkpoint_every: int = 0):
         self._setTuning(tile_lanes, slot_config, checkpoint_every)
 
-    def setKernel(self, kernel: str = "auto"):
-        """"tile" (any system size), "lane" (one lane per system, state in
-        registers; small systems) or "auto" (lane when every system fits)."""
+    def setKernel(self, kernel: str = "auto", lanes_per_system: int = 0):
+        """"tile" (any system size), "lane" (one lane -- or a group of lanes
+        exchanging values by lane rotations -- per system, state in
+        registers) or "auto" (lane kernel for the structures that fit it,
+        tiles for the rest).  ``lanes_per_system`` 0 lets a cost model pick
+        the group size per structure; a power of two up to 64 requests it."""
         self._setKernel(kernel)
+        self._setLaneGroup(lanes_per_system)
 
     def kernelInfo(self, adjoint: bool = False) -> dict:
         """Kernel the loaded problem runs on, its structure classes and the

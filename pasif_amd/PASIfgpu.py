@@ -235,12 +235,26 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
             raise ValueError(f"kernel must be one of {sorted(self._KERNELS)}")
         self._chk(self._pl.pasif_set_kernel(self._h, self._KERNELS[kernel]))
 
+    def _setLaneGroup(self, lanes: int = 0):
+        """Lanes per instance in the lane kernel: 0 = chosen per structure
+        class, else a power of two up to 64 for every class it fits."""
+        self._chk(self._pl.pasif_set_lane_group(self._h, int(lanes)))
+
+    def _laneGroups(self, adjoint: bool = False) -> list:
+        n = C.c_size_t(0)
+        self._chk(self._pl.pasif_lane_groups(self._h, int(adjoint), None, 0, C.byref(n)))
+        buf = (C.c_uint32 * max(1, n.value))()
+        self._chk(self._pl.pasif_lane_groups(self._h, int(adjoint), buf, n.value, C.byref(n)))
+        return [int(buf[i]) for i in range(n.value)]
+
     def _kernelInfo(self, adjoint: bool = False) -> dict:
         k, ncls, ck = C.c_int(0), C.c_uint32(0), C.c_uint32(0)
         self._chk(self._pl.pasif_kernel_info(self._h, int(adjoint), C.byref(k), C.byref(ncls),
                                              C.byref(ck)))
         name = {1: "tile", 2: "lane", 3: "mixed"}.get(k.value, "?")
-        return {"kernel": name, "classes": ncls.value, "checkpoint_every": ck.value}
+        groups = self._laneGroups(adjoint) if k.value in (2, 3) else []
+        return {"kernel": name, "classes": ncls.value, "checkpoint_every": ck.value,
+                "lanes_per_instance": groups}
 
     def _laneSource(self, adjoint: bool = False, compile: bool = False) -> str:
         """Generated HIP source of the lane kernel (optionally compiled with

@@ -66,6 +66,8 @@ _SIGS = {
     "pasif_get_max_amplitudes": (C.c_int, [_h, _f64p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pasif_last_timing": (C.c_int, [_h, _f64p, _u32p, _u32p, _u32p]),
     "pasif_set_kernel": (C.c_int, [_h, C.c_int]),
+    "pasif_set_lane_group": (C.c_int, [_h, C.c_uint32]),
+    "pasif_lane_groups": (C.c_int, [_h, C.c_int, _u32p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pasif_kernel_info": (C.c_int, [_h, C.c_int, C.POINTER(C.c_int), _u32p, _u32p]),
     "pasif_lane_source": (C.c_int, [_h, C.c_int, C.c_char_p, C.c_size_t,
                                     C.POINTER(C.c_size_t), C.c_int]),

@@ -238,6 +238,8 @@ struct pasif_driver {
   int solver = PASIF_SOLVER_PERSISTENT;
   uint32_t tile_lanes = 0, slot_cfg = 0, ck_every = 0, window = 0;
   int kernel_path = PASIF_KERNEL_AUTO;
+  // lanes per instance requested for lane-kernel classes (0: chosen per class)
+  uint32_t lane_group = 0;
   // lane class of every system in the packed problem, -1: tile kernel
   std::vector<int32_t> lane_cls;
   // second stream and fork / join events when lane and tile kernels run together
@@ -980,7 +982,7 @@ void pack_impl(pasif_driver* d, bool adjoint) {
 uint64_t pack_key(pasif_driver* d, bool adjoint) {
   return (uint64_t(d->precision) << 1) ^ (uint64_t(adjoint) << 8) ^ (uint64_t(d->tile_lanes) << 10) ^
          (uint64_t(d->slot_cfg) << 24) ^ (uint64_t(d->kernel_path) << 28) ^
-         (uint64_t(d->ck_every) << 32);
+         (uint64_t(d->ck_every) << 32) ^ (uint64_t(d->lane_group) << 56);
 }
 
 // ------------------------------------------------------------ lane plan
@@ -1050,6 +1052,120 @@ void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, std::vector<
   for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
 }
 
+// Lanes per instance (powers of two up to a wave) at which a class fits the
+// register budget, with the step cost of each.
+std::vector<std::pair<uint32_t, double>> group_options(const LaneShape& sh, int bits) {
+  std::vector<std::pair<uint32_t, double>> out;
+  for (uint32_t G = 1; G <= kLaneWave; G *= 2) {
+    LaneShape t = sh;
+    if (!lane_group(t, G)) continue;
+    if (lane_live(t) > lane_max_live()) continue;
+    out.push_back({G, lane_step_cost(t, bits)});
+  }
+  return out;
+}
+
+struct ClassLoad {
+  double inst = 0, steps_sum = 0, steps_max = 0;
+};
+
+// Lanes per instance of every class of a single-wave lane kernel.  A class's
+// work is its step cost x instance-steps x G / 64 (wave instructions); the
+// kernel takes about max(total work / SIMDs, longest chain), a chain being a
+// class's step cost x its longest step count (one wave's sequence of steps).
+// Every class starts at its cheapest layout that fits the registers; the
+// class with the longest chain is widened while that shortens the estimate.
+// `want` > 0 requests that many lanes for every class where it fits.
+void choose_groups(std::vector<LaneShape>& shapes, const std::vector<ClassLoad>& load, int bits,
+                   double simds, uint32_t want) {
+  const size_t C = shapes.size();
+  std::vector<std::vector<std::pair<uint32_t, double>>> opt(C);
+  std::vector<size_t> sel(C, 0);
+  for (size_t c = 0; c < C; ++c) {
+    opt[c] = group_options(shapes[c], bits);
+    if (std::getenv("PASIF_LANE_DEBUG")) {
+      for (uint32_t G = 1; G <= kLaneWave; G *= 2) {
+        LaneShape t = shapes[c];
+        const bool ok = lane_group(t, G);
+        fprintf(stderr, "[pasif] class %zu G %u: %s live %u cost %.0f nl %u/%u coefs %u/%u nx %zu\n", c, G,
+                ok ? "ok" : "no", ok ? lane_live(t) : 0u, ok ? lane_step_cost(t, bits) : 0.0,
+                t.gf.nl(), t.ga.nl(), t.gf.coefs(), t.ga.coefs(), t.gx.size());
+      }
+    }
+    if (opt[c].empty()) continue;
+    size_t b = 0;
+    for (size_t k = 0; k < opt[c].size(); ++k) {
+      if (want && opt[c][k].first == want) {
+        b = k;
+        break;
+      }
+      if (opt[c][k].second * opt[c][k].first < opt[c][b].second * opt[c][b].first) b = k;
+    }
+    sel[c] = b;
+  }
+  auto work = [&](size_t c, size_t k) {
+    return opt[c][k].second * load[c].steps_sum * opt[c][k].first / double(kLaneWave);
+  };
+  auto chain = [&](size_t c, size_t k) { return opt[c][k].second * load[c].steps_max; };
+  auto estimate = [&](const std::vector<size_t>& s, size_t* crit) {
+    double w = 0, ch = 0;
+    for (size_t c = 0; c < C; ++c) {
+      if (opt[c].empty()) continue;
+      w += work(c, s[c]);
+      if (chain(c, s[c]) > ch) {
+        ch = chain(c, s[c]);
+        if (crit) *crit = c;
+      }
+    }
+    return std::max(w / simds, ch);
+  };
+  for (int it = 0; it < 64 && !want; ++it) {
+    size_t m = 0;
+    const double est = estimate(sel, &m);
+    if (opt[m].empty() || sel[m] + 1 >= opt[m].size()) break;
+    std::vector<size_t> t = sel;
+    t[m] += 1;
+    if (estimate(t, nullptr) >= 0.99 * est) break;
+    sel = t;
+  }
+  for (size_t c = 0; c < C; ++c)
+    if (!opt[c].empty()) (void)lane_group(shapes[c], opt[c][sel[c]].first);
+}
+
+// SIMDs of the device (4 per CU), 1024 when planning without one
+double device_simds(int dev) {
+  int cus = 0;
+  if (dev >= 0 && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0)
+    return 4.0 * cus;
+  return 1024.0;
+}
+
+// Checkpoint interval of a single-wave adjoint kernel: the requested one,
+// shortened until the segment [K+1][slots][64] and the sample stage fit the
+// LDS one wave may take while every wave of the launch stays resident
+// (160 KB per CU shared by ceil(waves / CUs) waves, at most 64 KB).
+uint32_t single_wave_seg(const std::vector<LaneShape>& shapes, const std::vector<ClassLoad>& load,
+                         size_t el, uint32_t Kreq, int dev) {
+  uint32_t nf_max = 0;
+  double waves = 0;
+  for (size_t c = 0; c < shapes.size(); ++c) {
+    const LaneShape& sh = shapes[c];
+    nf_max = std::max(nf_max, sh.group > 1 ? sh.gf.nl() : sh.f.n);
+    waves += std::ceil(load[c].inst * sh.group / double(kLaneWave));
+  }
+  const double cus = device_simds(dev) / 4.0;
+  const double per_cu = std::max(1.0, std::ceil(waves / cus));
+  const size_t stage = size_t(kLaneStage) * kLaneWave * el;
+  const size_t step = size_t(nf_max) * kLaneWave * el;
+  size_t budget = std::min<size_t>(kLaneMaxLds, size_t(double(kMaxLdsBytes) / per_cu));
+  // at least a few steps per segment even when the waves then queue
+  budget = std::max(budget, stage + 5 * step);
+  budget = std::min<size_t>(budget, kLaneMaxLds);
+  const size_t fit = budget > stage ? (budget - stage) / step : 0;
+  return uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+}
+
 // Plan the lane kernel for the loaded problem: true if every system fits a
 // lane and the systems form at most kLaneMaxClasses structure classes.
 template <typename R>
@@ -1087,17 +1203,28 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   // what each class needs: 1 = one lane, w >= 2 = a workgroup of w waves, 0 = too large
   std::vector<uint32_t> need(all.size(), 0);
   std::vector<double> work(all.size(), 0.0);
+  const int bits = int(sizeof(R) * 8);
   for (size_t c = 0; c < all.size(); ++c) {
     LaneShape& sh = all[c];
-    if (sh.live_values() <= lane_max_live()) {
-      need[c] = 1;
+    // lane_group == 1: one lane per instance only (workgroups of waves for
+    // classes that do not fit one lane)
+    const bool fits = d->lane_group == 1 ? sh.live_values() <= lane_max_live()
+                                         : !group_options(sh, bits).empty();
+    if (fits) {
+      need[c] = 1;  // one lane or a group of lanes per instance
     } else {
       for (uint32_t w = 2; w <= kLaneMaxWaves && !need[c]; ++w)
         if (lane_partition(sh, w) <= kLaneMaxWaveLive) need[c] = w;
     }
   }
-  for (const Instance& in : make_instances(d, adjoint))
-    work[cls_of[in.sys]] += double(in.steps) * (all[cls_of[in.sys]].f.n + all[cls_of[in.sys]].a.n);
+  std::vector<ClassLoad> loads(all.size());
+  for (const Instance& in : make_instances(d, adjoint)) {
+    const uint32_t c = cls_of[in.sys];
+    work[c] += double(in.steps) * (all[c].f.n + all[c].a.n);
+    loads[c].inst += 1;
+    loads[c].steps_sum += in.steps;
+    loads[c].steps_max = std::max(loads[c].steps_max, double(in.steps));
+  }
   // the lane kernel takes the classes that fit, heaviest first, up to kLaneMaxClasses
   std::vector<uint32_t> order;
   for (uint32_t c = 0; c < all.size(); ++c)
@@ -1116,10 +1243,12 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   }
   if (order.empty()) return false;
   std::vector<LaneShape> shapes;
+  std::vector<ClassLoad> sel_load;
   std::vector<int32_t> sel_of(all.size(), -1);
   for (uint32_t c : order) {
     sel_of[c] = int32_t(shapes.size());
     shapes.push_back(all[c]);
+    sel_load.push_back(loads[c]);
   }
   for (uint32_t s = 0; s < S; ++s)
     d->lane_cls[s] = sel_of[cls_of[s]] >= 0 ? lane_code(0, uint32_t(sel_of[cls_of[s]])) : -1;
@@ -1165,13 +1294,9 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
           }
         }
       }
-    } else if (adjoint) {
-      // checkpoint interval: the requested one, shortened until the segment fits LDS
-      uint32_t nf_max = 0;
-      for (const LaneShape& sh : shapes) nf_max = std::max(nf_max, sh.f.n);
-      const size_t stage = size_t(kLaneStage) * kLaneWave * el;
-      const size_t fit = (kLaneMaxLds - stage) / (size_t(nf_max) * kLaneWave * el);
-      lay.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+    } else {
+      choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group);
+      if (adjoint) lay.seg = single_wave_seg(shapes, sel_load, el, Kreq, d->dev);
     }
   }
   if (why) {
@@ -1237,21 +1362,39 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
                      [](const Instance* x, const Instance* y) { return x->steps > y->steps; });
     std::unique_ptr<LaneCls> L(new LaneCls());
     const LaneShape& sh = shapes[c];
+    // lane groups: G lanes per instance, per-lane arrays over lane columns
+    // (instance i, position p at column i * G + p), per-instance arrays over
+    // instances
+    const bool grp = lay.waves == 1 && sh.group > 1;
+    const uint32_t G = grp ? sh.group : 1u;
     L->count = uint32_t(mem.size());
-    L->stride = std::max<uint32_t>(kLaneWave, (L->count + kLaneWave - 1) / kLaneWave * kLaneWave);
-    L->waves = L->stride / unit;
+    if (grp) {
+      const uint32_t ipw = kLaneWave / G;
+      L->waves = std::max<uint32_t>(1, (L->count + ipw - 1) / ipw);
+      L->stride = L->waves * kLaneWave;
+    } else {
+      L->stride = std::max<uint32_t>(kLaneWave, (L->count + kLaneWave - 1) / kLaneWave * kLaneWave);
+      L->waves = L->stride / unit;
+    }
     L->block_begin = block;
-    L->nf = sh.f.n;
-    L->na = adjoint ? sh.a.n : 0;
+    L->nf = grp ? sh.gf.nl() : sh.f.n;
+    L->na = adjoint ? (grp ? sh.ga.nl() : sh.a.n) : 0;
     L->klass = klass[c];
     block += L->waves;
-    const uint32_t st = L->stride, TF = sh.f.coefs(), TA = adjoint ? sh.a.coefs() : 0;
+    const uint32_t st = L->stride;
+    const uint32_t TF = grp ? sh.gf.coefs() : sh.f.coefs();
+    const uint32_t TA = adjoint ? (grp ? sh.ga.coefs() : sh.a.coefs()) : 0;
     std::vector<R> cf(size_t(std::max<uint32_t>(TF, 1)) * st, R(0));
     std::vector<R> ca(size_t(std::max<uint32_t>(TA, 1)) * st, R(0));
     std::vector<R> qf(size_t(L->nf) * st, R(0));
     std::vector<R> qa(size_t(std::max<uint32_t>(L->na, 1)) * st, R(0));
     std::vector<uint32_t> stp(st, 0), of(st, 0), oa(st, 0);
     std::vector<int32_t> ex(st, -1);
+    std::vector<double> lv;
+    // lane slot r of position p of a group side -> the system's row
+    auto slot_row = [](const GSide& gs, uint32_t r, uint32_t p) {
+      return r < gs.RL ? gs.band[r] + p : gs.shared[r - gs.RL];
+    };
     for (uint32_t i = 0; i < L->count; ++i) {
       const Instance& in = *mem[i];
       // every output index the kernel writes stays inside the output arrays
@@ -1262,15 +1405,37 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
       PASIF_REQUIRE(in.exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
                     "lane plan: excitation out of range");
       lane_coefs(d->fwd, in.sys, sh.f, cv);
-      PASIF_REQUIRE(cv.size() == TF, PASIF_ERR_UNSUPPORTED, "lane plan: structure mismatch");
-      for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + i] = R(cv[t]);
-      for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
+      PASIF_REQUIRE(cv.size() == sh.f.coefs(), PASIF_ERR_UNSUPPORTED,
+                    "lane plan: structure mismatch");
+      if (grp) {
+        for (uint32_t p = 0; p < G; ++p) {
+          const size_t col = size_t(i) * G + p;
+          lane_group_coefs(sh.gf, cv, p, lv);
+          for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + col] = R(lv[t]);
+          for (uint32_t r = 0; r < L->nf; ++r)
+            qf[size_t(r) * st + col] = R(d->fwd.init[d->fwd.off[in.sys] + slot_row(sh.gf, r, p)]);
+        }
+      } else {
+        for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + i] = R(cv[t]);
+        for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
+      }
       if (adjoint) {
         lane_coefs(d->adj, in.sys, sh.a, cv);
-        PASIF_REQUIRE(cv.size() == TA, PASIF_ERR_UNSUPPORTED, "lane plan: structure mismatch");
-        for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
-        for (uint32_t r = 0; r < L->na; ++r)
-          qa[size_t(r) * st + i] = R(d->adj.init[d->adj.off[in.sys] + r]);
+        PASIF_REQUIRE(cv.size() == sh.a.coefs(), PASIF_ERR_UNSUPPORTED,
+                      "lane plan: structure mismatch");
+        if (grp) {
+          for (uint32_t p = 0; p < G; ++p) {
+            const size_t col = size_t(i) * G + p;
+            lane_group_coefs(sh.ga, cv, p, lv);
+            for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + col] = R(lv[t]);
+            for (uint32_t r = 0; r < L->na; ++r)
+              qa[size_t(r) * st + col] = R(d->adj.init[d->adj.off[in.sys] + slot_row(sh.ga, r, p)]);
+          }
+        } else {
+          for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
+          for (uint32_t r = 0; r < L->na; ++r)
+            qa[size_t(r) * st + i] = R(d->adj.init[d->adj.off[in.sys] + r]);
+        }
       }
       stp[i] = in.steps;
       ex[i] = in.exc;
@@ -2133,6 +2298,29 @@ int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* class
       *checkpoint_every = !adjoint ? 0u
                           : d->pk.lane ? d->pk.lseg
                                        : (d->ck_every ? d->ck_every : 256u);
+  });
+}
+
+int pasif_set_lane_group(pasif_driver* d, uint32_t lanes) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(lanes <= kLaneWave && (lanes & (lanes - 1u)) == 0u, PASIF_ERR_ARGUMENT,
+                  "lanes per instance must be 0 (automatic) or a power of two up to %u, got %u",
+                  kLaneWave, lanes);
+    d->lane_group = lanes;
+  });
+}
+
+int pasif_lane_groups(pasif_driver* d, int adjoint, uint32_t* out, size_t cap, size_t* len) {
+  return guarded(d, [&] {
+    ensure_packed(d, adjoint != 0);
+    std::vector<uint32_t> g;
+    if (d->pk.lane)
+      for (const LaneShape& sh : d->pk.lshapes) g.push_back(d->pk.llay.waves > 1 ? 1u : sh.group);
+    if (len) *len = g.size();
+    if (out) {
+      PASIF_REQUIRE(cap >= g.size(), PASIF_ERR_ARGUMENT, "buffer holds %zu, need %zu", cap, g.size());
+      std::copy(g.begin(), g.end(), out);
+    }
   });
 }
 

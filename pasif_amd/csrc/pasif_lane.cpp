@@ -12,6 +12,8 @@
 
 #include <hip/hiprtc.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -86,6 +88,336 @@ uint32_t lane_partition(LaneShape& sh, uint32_t waves) {
   return worst;
 }
 
+uint32_t GSide::coefs() const {
+  uint32_t t = 0;
+  for (const GRow& r : rows) t += uint32_t(r.terms.size()) + (r.force.empty() ? 0u : 1u);
+  return t;
+}
+
+namespace {
+
+uint32_t log2u(uint32_t G) {
+  uint32_t l = 0;
+  while ((1u << l) < G) ++l;
+  return l;
+}
+
+// Bands of G consecutive rows and shared rows of a side, by dynamic
+// programming over per-stage cost estimates: a band costs the union of its
+// rows' terms (terms compared by kind and factor offsets from their row), a
+// shared row its terms spread over G lanes plus the butterfly.
+void plan_bands(const LaneSide& sd, uint32_t G, uint32_t lo, std::vector<uint32_t>* band,
+                std::vector<uint32_t>* shared) {
+  const uint32_t n = sd.n, lg = log2u(G);
+  auto key_of = [](const LaneTerm& t, uint32_t g) {
+    std::vector<int64_t> k{int64_t(t.kind)};
+    for (uint32_t a = 0; a <= t.kind; ++a) k.push_back(int64_t(t.f[a]) - int64_t(g));
+    return k;
+  };
+  auto band_cost = [&](uint32_t g) {
+    std::map<std::pair<std::vector<int64_t>, int>, int> uni;
+    double c = 2.0;
+    for (uint32_t p = 0; p < G; ++p) {
+      std::map<std::vector<int64_t>, int> occ;
+      for (const LaneTerm& t : sd.rows[g + p]) {
+        std::vector<int64_t> k = key_of(t, g + p);
+        const int o = occ[k]++;
+        if (uni.emplace(std::make_pair(k, o), 1).second) c += t.kind + 1.0;
+      }
+    }
+    return c;  // every lane of the band runs the union once
+  };
+  auto shared_cost = [&](uint32_t g) {
+    double c = 0;
+    for (const LaneTerm& t : sd.rows[g]) c += t.kind + 1.0;
+    return (G > 1 ? std::ceil(c / G) + 3.0 * lg : c) + 2.0;
+  };
+  std::vector<double> best(n + 1, 0.0);
+  std::vector<uint8_t> take(n + 1, 0);
+  for (uint32_t g = n; g-- > lo;) {
+    best[g] = shared_cost(g) + best[g + 1];
+    if (g + G <= n) {
+      const double b = band_cost(g) + best[g + G];
+      if (b <= best[g]) {
+        best[g] = b;
+        take[g] = 1;
+      }
+    }
+  }
+  for (uint32_t g = lo; g < n;) {
+    if (take[g]) {
+      band->push_back(g);
+      g += G;
+    } else {
+      shared->push_back(g);
+      g += 1;
+    }
+  }
+}
+
+// Bands at lo, lo + G, ... while they fit; the rest shared.
+void aligned_bands(uint32_t n, uint32_t G, uint32_t lo, std::vector<uint32_t>* band,
+                   std::vector<uint32_t>* shared) {
+  uint32_t g = lo;
+  for (; g + G <= n; g += G) band->push_back(g);
+  for (; g < n; ++g) shared->push_back(g);
+}
+
+// Lane slot and group position of every row of a side.
+struct GMap {
+  std::vector<uint32_t> slot, pos;
+  std::vector<bool> shared;
+};
+
+GMap group_map(uint32_t n, uint32_t G, const GSide& gs) {
+  GMap m;
+  m.slot.assign(n, 0);
+  m.pos.assign(n, 0);
+  m.shared.assign(n, false);
+  for (uint32_t r = 0; r < gs.RL; ++r)
+    for (uint32_t p = 0; p < G; ++p) {
+      m.slot[gs.band[r] + p] = r;
+      m.pos[gs.band[r] + p] = p;
+    }
+  for (uint32_t s = 0; s < gs.NS; ++s) {
+    m.slot[gs.shared[s]] = gs.RL + s;
+    m.shared[gs.shared[s]] = true;
+  }
+  return m;
+}
+
+GSrc src_of(const GMap& m, uint32_t row, uint32_t p, uint32_t G) {
+  GSrc s;
+  s.slot = m.slot[row];
+  s.shared = m.shared[row];
+  s.d = s.shared ? 0u : (m.pos[row] + G - p) % G;
+  return s;
+}
+
+// Union programs of a side laid out as gs.band / gs.shared; `own` maps the
+// side's rows, `setp` the rows its set-point factors index.
+void build_gside(const LaneSide& sd, uint32_t G, const GMap& own, const GMap& setp, GSide* gs) {
+  std::vector<uint32_t> base(sd.n);
+  std::vector<int32_t> force(sd.n, -1);
+  uint32_t t = 0;
+  for (uint32_t r = 0; r < sd.n; ++r) {
+    base[r] = t;
+    t += uint32_t(sd.rows[r].size());
+  }
+  for (size_t i = 0; i < sd.forced.size(); ++i) force[sd.forced[i]] = int32_t(t + i);
+  gs->rows.assign(gs->nl(), GRow());
+  using Key = std::vector<uint32_t>;
+  auto add_term = [&](GRow& row, std::map<Key, size_t>& where, std::map<Key, uint32_t>& occ,
+                      const LaneTerm& tm, uint32_t p, int32_t coef) {
+    GTerm gt;
+    gt.kind = tm.kind;
+    Key k{tm.kind};
+    for (uint32_t a = 0; a <= tm.kind; ++a) {
+      gt.f[a] = src_of(a == 0 ? own : setp, tm.f[a], p, G);
+      k.push_back(gt.f[a].shared ? 1u : 0u);
+      k.push_back(gt.f[a].slot);
+      k.push_back(gt.f[a].d);
+    }
+    const uint32_t o = occ[k]++;
+    k.push_back(o);
+    auto it = where.find(k);
+    if (it == where.end()) {
+      gt.coef.assign(G, -1);
+      row.terms.push_back(gt);
+      it = where.emplace(k, row.terms.size() - 1).first;
+    }
+    row.terms[it->second].coef[p] = coef;
+  };
+  for (uint32_t r = 0; r < gs->RL; ++r) {
+    GRow& row = gs->rows[r];
+    std::map<Key, size_t> where;
+    for (uint32_t p = 0; p < G; ++p) {
+      const uint32_t g = gs->band[r] + p;
+      std::map<Key, uint32_t> occ;
+      for (size_t e = 0; e < sd.rows[g].size(); ++e)
+        add_term(row, where, occ, sd.rows[g][e], p, int32_t(base[g] + e));
+      if (force[g] >= 0) {
+        if (row.force.empty()) row.force.assign(G, -1);
+        row.force[p] = force[g];
+      }
+    }
+  }
+  for (uint32_t s = 0; s < gs->NS; ++s) {
+    const uint32_t g = gs->shared[s];
+    GRow& row = gs->rows[gs->RL + s];
+    std::map<Key, size_t> where;
+    std::vector<std::map<Key, uint32_t>> occ(G);
+    for (size_t e = 0; e < sd.rows[g].size(); ++e) {
+      const LaneTerm& tm = sd.rows[g][e];
+      // the term runs on the position owning its first banded factor
+      uint32_t pt = 0;
+      for (uint32_t a = 0; a <= tm.kind; ++a) {
+        const GMap& m = a == 0 ? own : setp;
+        if (!m.shared[tm.f[a]]) {
+          pt = m.pos[tm.f[a]];
+          break;
+        }
+      }
+      add_term(row, where, occ[pt], tm, pt, int32_t(base[g] + e));
+    }
+    if (force[g] >= 0) {
+      row.force.assign(G, -1);
+      row.force[0] = force[g];
+    }
+  }
+}
+
+uint32_t remote_count(const GSide& gs, bool setpoints_own) {
+  std::vector<GSrc> v;
+  for (const GRow& row : gs.rows)
+    for (const GTerm& t : row.terms)
+      for (uint32_t a = 0; a <= (setpoints_own ? t.kind : 0u); ++a)
+        if (!t.f[a].shared && t.f[a].d != 0) v.push_back(t.f[a]);
+  std::sort(v.begin(), v.end());
+  return uint32_t(std::unique(v.begin(), v.end()) - v.begin());
+}
+
+}  // namespace
+
+namespace {
+
+// Per-stage vector instructions of one lane of a group side: its union
+// terms, kind sums, forces, butterflies of shared rows and rotations.
+double gside_stage_cost(const GSide& gs, uint32_t G, bool setpoints_own, int bits) {
+  const uint32_t lg = log2u(G);
+  const double words = bits == 64 ? 2.0 : 1.0;
+  const double rot = G == 8 ? 3.0 * words : (G >= 32 ? 2.0 * words : words);
+  double stage = 2.0 * gs.nl();
+  for (uint32_t r = 0; r < gs.nl(); ++r) {
+    const GRow& row = gs.rows[r];
+    bool kinds[4] = {false, false, false, false};
+    for (const GTerm& t : row.terms) {
+      stage += t.kind + 1.0;
+      kinds[t.kind] = true;
+    }
+    const int nk = kinds[0] + kinds[1] + kinds[2] + kinds[3];
+    stage += nk > 1 ? nk - 1 : 0;
+    stage += row.force.empty() ? 0 : 1;
+    if (r >= gs.RL) stage += lg * (rot + 1.0);
+  }
+  return stage + remote_count(gs, setpoints_own) * rot;
+}
+
+// The cheapest of the candidate band plans of a side (each built exactly).
+// `mirror` (adjoint side): the forward plan, whose rows the first rows of
+// the adjoint state mirror.
+bool best_gside(const LaneSide& sd, uint32_t G, const GSide* mirror, uint32_t nmirror,
+                const GMap* setp, int bits, GSide* out) {
+  std::vector<std::pair<std::vector<uint32_t>, std::vector<uint32_t>>> plans;
+  {
+    std::vector<uint32_t> b, sh;
+    plan_bands(sd, G, 0, &b, &sh);
+    plans.push_back({b, sh});
+  }
+  {
+    std::vector<uint32_t> b, sh;
+    aligned_bands(sd.n, G, 0, &b, &sh);
+    plans.push_back({b, sh});
+  }
+  if (mirror && nmirror <= sd.n) {
+    for (int k = 0; k < 2; ++k) {
+      std::vector<uint32_t> b = mirror->band, sh = mirror->shared;
+      if (k == 0) plan_bands(sd, G, nmirror, &b, &sh);
+      else aligned_bands(sd.n, G, nmirror, &b, &sh);
+      plans.push_back({b, sh});
+    }
+  }
+  double best = -1;
+  for (auto& pl : plans) {
+    GSide gs;
+    gs.band = pl.first;
+    gs.shared = pl.second;
+    std::sort(gs.shared.begin(), gs.shared.end());
+    gs.RL = uint32_t(gs.band.size());
+    gs.NS = uint32_t(gs.shared.size());
+    if (gs.RL == 0) continue;
+    const GMap own = group_map(sd.n, G, gs);
+    build_gside(sd, G, own, setp ? *setp : own, &gs);
+    const double c = gside_stage_cost(gs, G, setp == nullptr, bits);
+    if (best < 0 || c < best) {
+      best = c;
+      *out = std::move(gs);
+    }
+  }
+  return best >= 0;
+}
+
+}  // namespace
+
+bool lane_group(LaneShape& sh, uint32_t G) {
+  sh.gf = GSide();
+  sh.ga = GSide();
+  sh.gx.clear();
+  sh.group = 1;
+  if (G == 1) return true;
+  if (G > kLaneWave || (kLaneWave % G) != 0 || (G & (G - 1)) != 0) return false;
+  if (!best_gside(sh.f, G, nullptr, 0, nullptr, 64, &sh.gf)) return false;
+  if (sh.adjoint) {
+    const GMap mf = group_map(sh.f.n, G, sh.gf);
+    if (!best_gside(sh.a, G, &sh.gf, sh.f.n, &mf, 64, &sh.ga)) return false;
+    for (const GRow& row : sh.ga.rows)
+      for (const GTerm& t : row.terms)
+        for (uint32_t a = 1; a <= t.kind; ++a) sh.gx.push_back(t.f[a]);
+    std::sort(sh.gx.begin(), sh.gx.end());
+    sh.gx.erase(std::unique(sh.gx.begin(), sh.gx.end()), sh.gx.end());
+  }
+  sh.group = G;
+  return true;
+}
+
+uint32_t lane_live(const LaneShape& sh) {
+  if (sh.group <= 1) return sh.live_values();
+  const uint32_t fwd = 4 * sh.gf.nl() + sh.gf.coefs() + remote_count(sh.gf, true);
+  const uint32_t adj = sh.adjoint ? 4 * sh.ga.nl() + sh.ga.coefs() + remote_count(sh.ga, false) +
+                                        3 * uint32_t(sh.gx.size()) + sh.gf.nl()
+                                  : 0;
+  return std::max(fwd, adj);
+}
+
+double lane_step_cost(const LaneShape& sh, int bits) {
+  const uint32_t G = sh.group;
+  auto side = [&](const LaneSide& sd, const GSide& gs, bool setpoints_own) {
+    double stage = 0, rows = 0;
+    if (G <= 1) {
+      rows = sd.n;
+      for (const auto& row : sd.rows) {
+        bool kinds[4] = {false, false, false, false};
+        for (const LaneTerm& t : row) {
+          stage += t.kind + 1.0;
+          kinds[t.kind] = true;
+        }
+        const int nk = kinds[0] + kinds[1] + kinds[2] + kinds[3];
+        stage += nk > 1 ? nk - 1 : 0;
+      }
+      stage += double(sd.forced.size()) + 2.0 * rows;
+    } else {
+      rows = gs.nl();
+      stage = gside_stage_cost(gs, G, setpoints_own, bits);
+    }
+    return 4.0 * stage + 2.0 * rows;
+  };
+  const double forcing = 14.0, loop = 8.0;
+  const double f = side(sh.f, sh.gf, true) + forcing + loop;
+  if (!sh.adjoint) return f;
+  const double nx = G <= 1 ? sh.f.n : double(sh.gx.size());
+  const double a = side(sh.a, sh.ga, false) + 3.0 * nx + loop;
+  return 2.0 * f + a;
+}
+
+void lane_group_coefs(const GSide& gs, const std::vector<double>& sys, uint32_t p,
+                      std::vector<double>& out) {
+  out.clear();
+  for (const GRow& row : gs.rows) {
+    for (const GTerm& t : row.terms) out.push_back(t.coef[p] >= 0 ? sys[size_t(t.coef[p])] : 0.0);
+    if (!row.force.empty()) out.push_back(row.force[p] >= 0 ? sys[size_t(row.force[p])] : 0.0);
+  }
+}
+
 namespace {
 
 // Device-side runtime shared by every class (R, LaneArgs and the class
@@ -96,6 +428,79 @@ __device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfir
 __device__ __forceinline__ u32 umin(u32 a, u32 b) { return a < b ? a : b; }
 enum : u32 { kForward = 0, kRecompute = 1, kBackward = 2, kDone = 3 };
 template <bool B> struct Flag { static constexpr bool value = B; };
+
+// ------------------------------------------------------------ lane groups
+// A value moved between lanes by a DPP control (32-bit halves for doubles);
+// every lane is a valid source for the controls used here.
+template <int CTRL, class T>
+__device__ __forceinline__ T dpp(T x) {
+  if constexpr (sizeof(T) == 8) {
+    const u64 b = __builtin_bit_cast(u64, x);
+    const u32 lo = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)b, CTRL, 0xF, 0xF, false);
+    const u32 hi = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(T, ((u64)hi << 32) | (u64)lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF,
+                                                             0xF, false));
+  }
+}
+// through the LDS crossbar (no LDS storage): lane l reads lane src
+template <class T>
+__device__ __forceinline__ T bperm(T x, u32 src) {
+  const int addr = (int)(src << 2);
+  if constexpr (sizeof(T) == 8) {
+    const u64 b = __builtin_bit_cast(u64, x);
+    const u32 lo = (u32)__builtin_amdgcn_ds_bpermute(addr, (int)(u32)b);
+    const u32 hi = (u32)__builtin_amdgcn_ds_bpermute(addr, (int)(u32)(b >> 32));
+    return __builtin_bit_cast(T, ((u64)hi << 32) | (u64)lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
+  }
+}
+// quad_perm control: lane l of each quad reads lane (l + D) mod G of its
+// aligned group of G (G = 2 or 4)
+constexpr int quad_ctrl(u32 G, u32 D) {
+  int c = 0;
+  for (u32 l = 0; l < 4u; ++l) c |= (int)(((l & ~(G - 1u)) | ((l + D) & (G - 1u))) & 3u) << (2u * l);
+  return c;
+}
+// Value of position (p + D) mod G of the lane's aligned group of G lanes.
+template <u32 G, u32 D>
+__device__ __forceinline__ R rot(R x) {
+  if constexpr (D % G == 0u) {
+    return x;
+  } else if constexpr (G == 2u || G == 4u) {
+    return dpp<quad_ctrl(G, D), R>(x);
+  } else if constexpr (G == 8u) {
+    // row_shl:D where p + D stays in the group, row_shr:(8 - D) where it wraps
+    const R a = dpp<0x100 + (int)D, R>(x);
+    const R b = dpp<0x110 + 8 - (int)D, R>(x);
+    return ((threadIdx.x & 7u) + D < 8u) ? a : b;
+  } else if constexpr (G == 16u) {
+    return dpp<0x120 + 16 - (int)D, R>(x);  // row_ror:(16 - D)
+  } else {
+    const u32 l = threadIdx.x & 63u;
+    return bperm<R>(x, (l & ~(G - 1u)) | ((l + D) & (G - 1u)));
+  }
+}
+// Sum over the group by an xor butterfly: lanes add their partner's value
+// in the same order of operands up to commutation, so every lane of the
+// group ends with bitwise the same sum.
+template <u32 G>
+__device__ __forceinline__ R gsum(R x) {
+  if constexpr (G >= 2u) x = x + rot<2u, 1u>(x);
+  if constexpr (G >= 4u) x = x + rot<4u, 2u>(x);
+  if constexpr (G >= 8u) x = x + rot<8u, 4u>(x);
+  if constexpr (G >= 16u) x = x + rot<16u, 8u>(x);
+  if constexpr (G >= 32u) x = x + rot<32u, 16u>(x);
+  if constexpr (G >= 64u) x = x + rot<64u, 32u>(x);
+  return x;
+}
+// lane of position (p + D) mod G of the lane's group
+template <u32 G, u32 D>
+__device__ __forceinline__ u32 glane(u32 lane) {
+  return (lane & ~(G - 1u)) | ((lane + D) & (G - 1u));
+}
 
 
 // e[b .. b + STAGE) of one excitation (indices clamped into [0, L))
@@ -158,8 +563,11 @@ __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
 }
 
 // One wave of one class through its phases; same schedule and resumable
-// state as the tile kernel.  LDS: eb [STAGE][WAVE] staged samples of the
-// current chunk of steps (the next chunk is prefetched into registers),
+// state as the tile kernel.  S::G lanes integrate one instance (1: one
+// instance per lane); per-lane arrays (coefficients, states, checkpoints)
+// are indexed by the lane's column, per-instance ones (steps, excitation,
+// output bases) by the instance.  LDS: eb [STAGE][WAVE] staged samples of
+// the current chunk of steps (the next chunk is prefetched into registers),
 // then seg [K+1][NF][WAVE] (adjoint) the recomputed segment.
 template <class S>
 __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
@@ -167,9 +575,14 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr int NA = S::NA > 0 ? S::NA : 1;
   constexpr int TF = S::TF > 0 ? S::TF : 1;
   constexpr int TA = S::TA > 0 ? S::TA : 1;
+  constexpr int NX = S::NX > 0 ? S::NX : 1;
+  constexpr u32 G = S::G;
+  constexpr u32 IPW = WAVE / G;  // instances per wave
   const u32 lane = threadIdx.x;
+  const u32 p = lane & (G - 1u);
   const u64 st = a.stride;
-  const u32 i = wave * WAVE + lane;
+  const u32 col = wave * WAVE + lane;
+  const u32 i = wave * IPW + lane / G;
   const bool valid = i < (u32)a.count;
   R* eb = smem + lane;
   R* seg = smem + STAGE * WAVE;
@@ -180,24 +593,24 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   const u32 SE = (u32)a.save_every;
   const R h = R(a.h), h2 = R(a.h2), h6 = R(a.h6);
   GP(const u32) steps = (GP(const u32))a.steps;
-  // instances are sorted by step count, descending: the wave's first lane
-  // has the most steps, its last valid lane the fewest
-  const u32 T = uni(steps[wave * WAVE]);
-  const u32 Tmin = uni(steps[umin(wave * WAVE + WAVE, (u32)a.count) - 1u]);
+  // instances are sorted by step count, descending: the wave's first
+  // instance has the most steps, its last valid one the fewest
+  const u32 T = uni(steps[wave * IPW]);
+  const u32 Tmin = uni(steps[umin(wave * IPW + IPW, (u32)a.count) - 1u]);
   const u32 fsteps = steps[i];
   const i32 exc = ((GP(const i32))a.exc)[i];
   const bool forced = exc >= 0;
   const u32 fout = ((GP(const u32))a.out_f)[i];
   const u32 aout = ((GP(const u32))a.out_a)[i];
-  GP(const R) cf = (GP(const R))a.coef_f + i;
-  GP(const R) ca = (GP(const R))a.coef_a + i;
-  GP(const R) qi = (GP(const R))a.init_f + i;
-  GP(const R) li = (GP(const R))a.init_a + i;
-  GP(R) qs = (GP(R))a.qstate + i;
-  GP(R) ms = (GP(R))a.mxstate + i;
-  GP(R) ls = (GP(R))a.lstate + i;
-  GP(R) ck = (GP(R))a.ckpt + i;
-  GP(R) sb = (GP(R))a.segbuf + i;
+  GP(const R) cf = (GP(const R))a.coef_f + col;
+  GP(const R) ca = (GP(const R))a.coef_a + col;
+  GP(const R) qi = (GP(const R))a.init_f + col;
+  GP(const R) li = (GP(const R))a.init_a + col;
+  GP(R) qs = (GP(R))a.qstate + col;
+  GP(R) ms = (GP(R))a.mxstate + col;
+  GP(R) ls = (GP(R))a.lstate + col;
+  GP(R) ck = (GP(R))a.ckpt + col;
+  GP(R) sb = (GP(R))a.segbuf + col;
   GP(R) outq = (GP(R))a.out_q;
   GP(R) outm = (GP(R))a.out_max;
   GP(R) outg = (GP(R))a.out_grad;
@@ -209,6 +622,14 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   u32 phase = uni(pst[1]);
   u32 sg = uni(pst[2]);
   u32 budget = (u32)a.budget;
+  // output row of lane slot r: band slots hold row rowf(r) + p, a shared
+  // slot's row is written by position 0 only (-1 elsewhere)
+  auto orow_f = [&](int r) -> int {
+    return r < S::RLF ? S::rowf(r) + (int)p : (p == 0u ? S::rowf(r) : -1);
+  };
+  auto orow_a = [&](int r) -> int {
+    return r < S::RLA ? S::rowa(r) + (int)p : (p == 0u ? S::rowa(r) : -1);
+  };
 
   R q[NF], mx[NF], lam[NA];
 #pragma unroll
@@ -220,7 +641,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
     for (int r = 0; r < NF; ++r) q[r] = qi[r * st];
     if (fsteps == 0u && valid)
 #pragma unroll
-      for (int r = 0; r < NF; ++r) outq[fout + r] = q[r];
+      for (int r = 0; r < NF; ++r)
+        if (orow_f(r) >= 0) outq[fout + orow_f(r)] = q[r];
   } else {
 #pragma unroll
     for (int r = 0; r < NF; ++r) {
@@ -266,7 +688,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       budget -= nend - n;
       const u32 nfirst = n;
       // per-step bookkeeping (checkpoints, segment, trajectory, window,
-      // per-lane output) only when a run needs it
+      // per-instance output) only when a run needs it
       const bool gen = recompute || adjoint || SE != 0u || a.window != 0ull;
       u32 next_ck = ((n + K - 1u) / K) * K;
       u32 next_save = SE ? ((n + SE - 1u) / SE) * SE : 0u;
@@ -317,7 +739,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
                 const u32 slot = n / SE;
                 if (slot < (u32)a.nslots && valid)
 #pragma unroll
-                  for (int r = 0; r < NF; ++r) traj[(u64)slot * a.out_len + fout + r] = q[r];
+                  for (int r = 0; r < NF; ++r)
+                    if (orow_f(r) >= 0) traj[(u64)slot * a.out_len + fout + orow_f(r)] = q[r];
                 next_save += SE;
               }
               if (a.window && (ALL || n < fsteps) && n + 1u + (u32)a.window > fsteps)
@@ -326,8 +749,9 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
               if (n + 1u == fsteps && valid) {
 #pragma unroll
                 for (int r = 0; r < NF; ++r) {
-                  outq[fout + r] = q[r];
-                  if (a.window) outm[fout + r] = mx[r];
+                  if (orow_f(r) < 0) continue;
+                  outq[fout + orow_f(r)] = q[r];
+                  if (a.window) outm[fout + orow_f(r)] = mx[r];
                 }
               }
             }
@@ -374,16 +798,18 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           }
         }
       }
-      // without per-step bookkeeping: a lane whose last step fell in this
-      // range kept its final state since (inactive steps leave it unchanged)
+      // without per-step bookkeeping: an instance whose last step fell in
+      // this range kept its final state since (inactive steps leave it)
       if (!gen && valid && fsteps > nfirst && fsteps <= nend)
 #pragma unroll
-        for (int r = 0; r < NF; ++r) outq[fout + r] = q[r];
+        for (int r = 0; r < NF; ++r)
+          if (orow_f(r) >= 0) outq[fout + orow_f(r)] = q[r];
       if (n == e && !recompute && SE && valid) {
         // trajectory slots after the wave's last step hold the final state
         for (u32 slot = (T + SE - 1u) / SE; slot < (u32)a.nslots; ++slot)
 #pragma unroll
-          for (int r = 0; r < NF; ++r) traj[(u64)slot * a.out_len + fout + r] = q[r];
+          for (int r = 0; r < NF; ++r)
+            if (orow_f(r) >= 0) traj[(u64)slot * a.out_len + fout + orow_f(r)] = q[r];
       }
       if (n == e) {
         if (recompute) {
@@ -398,7 +824,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           phase = kDone;
           if (adjoint && valid)
 #pragma unroll
-            for (int r = 0; r < S::NA; ++r) outg[aout + r] = li[r * st];
+            for (int r = 0; r < S::NA; ++r)
+              if (orow_a(r) >= 0) outg[aout + orow_a(r)] = li[r * st];
         }
       }
     } else if (S::ADJ) {
@@ -418,13 +845,10 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       auto back_to = [&](auto all_t, u32 stop) {
         constexpr bool ALL = decltype(all_t)::value;
         for (; n > stop; --n) {
-          R xc[NF], xb[NF], xp[NF];
+          R xc[NX], xb[NX], xp[NX];
+          S::xload(xc, xp, seg + (n - s0) * (NF * WAVE), seg + (n - 1u - s0) * (NF * WAVE), lane);
 #pragma unroll
-          for (int r = 0; r < NF; ++r) {
-            xc[r] = seg[((n - s0) * NF + r) * WAVE + lane];
-            xp[r] = seg[((n - 1u - s0) * NF + r) * WAVE + lane];
-            xb[r] = R(0.5) * (xc[r] + xp[r]);
-          }
+          for (int j = 0; j < NX; ++j) xb[j] = R(0.5) * (xc[j] + xp[j]);
           rk4<NA, ALL>(lam, [&](R* k, const R* y, int s) {
             S::rhs_a(k, y, s == 0 ? xc : (s == 3 ? xp : xb), c);
           }, h, h2, h6, n <= fsteps);
@@ -438,7 +862,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           phase = kDone;
           if (valid)
 #pragma unroll
-            for (int r = 0; r < S::NA; ++r) outg[aout + r] = lam[r];
+            for (int r = 0; r < S::NA; ++r)
+              if (orow_a(r) >= 0) outg[aout + orow_a(r)] = lam[r];
         } else {
           sg -= 1u;
           n = sg * K;
@@ -934,6 +1359,67 @@ void emit_wave_side(std::ostringstream& o, const LaneSide& sd, const std::vector
   }
 }
 
+// Right-hand side of one side of a lane-group class: the own-state values
+// the program reads from other positions of the group (rotated once per
+// stage), then every lane slot: band slots directly, shared slots as
+// partial sums added across the group.  Coefficients c[t] in per-lane slot
+// order (row by row, terms then force); set points x[j] index `gx`.
+void emit_group_side(std::ostringstream& o, const GSide& gs, uint32_t G, bool forward,
+                     const std::vector<GSrc>& gx) {
+  std::vector<GSrc> rem;
+  for (const GRow& row : gs.rows)
+    for (const GTerm& t : row.terms)
+      for (uint32_t a = 0; a <= (forward ? t.kind : 0u); ++a)
+        if (!t.f[a].shared && t.f[a].d != 0) rem.push_back(t.f[a]);
+  std::sort(rem.begin(), rem.end());
+  rem.erase(std::unique(rem.begin(), rem.end()), rem.end());
+  for (const GSrc& r : rem)
+    o << "    const R y" << r.slot << "_" << r.d << " = rot<" << G << "u, " << r.d << "u>(y["
+      << r.slot << "]);\n";
+  auto S = [&](const GSrc& r) {
+    return (r.shared || r.d == 0) ? "y[" + std::to_string(r.slot) + "]"
+                                  : "y" + std::to_string(r.slot) + "_" + std::to_string(r.d);
+  };
+  auto X = [&](const GSrc& r) {
+    if (forward) return S(r);
+    const size_t j = size_t(std::lower_bound(gx.begin(), gx.end(), r) - gx.begin());
+    return "x[" + std::to_string(j) + "]";
+  };
+  uint32_t t = 0;
+  for (uint32_t r = 0; r < gs.nl(); ++r) {
+    const GRow& row = gs.rows[r];
+    std::string k;
+    for (int kind = 0; kind < 4; ++kind) {
+      std::string acc;
+      for (size_t e = 0; e < row.terms.size(); ++e) {
+        const GTerm& tm = row.terms[e];
+        if (int(tm.kind) != kind) continue;
+        const std::string coef = "c[" + std::to_string(t + e) + "]";
+        std::string prod;
+        if (kind == 0) {
+          prod = acc.empty() ? coef + " * " + S(tm.f[0])
+                             : "fmaR(" + coef + ", " + S(tm.f[0]) + ", " + acc + ")";
+        } else {
+          std::string l = coef + " * " + S(tm.f[0]);
+          for (int f = 1; f < kind; ++f) l = "(" + l + ") * " + X(tm.f[f]);
+          const std::string last = X(tm.f[kind]);
+          prod = acc.empty() ? "(" + l + ") * " + last : "fmaR(" + l + ", " + last + ", " + acc + ")";
+        }
+        const std::string name = "s" + std::to_string(kind) + "_" + std::to_string(r);
+        o << "    " << (acc.empty() ? "R " : "") << name << " = " << prod << ";\n";
+        acc = name;
+      }
+      if (!acc.empty()) k = k.empty() ? acc : "(" + k + " + " + acc + ")";
+    }
+    if (k.empty()) k = "R(0)";
+    const uint32_t tf = t + uint32_t(row.terms.size());
+    if (forward && !row.force.empty()) k = "fmaR(c[" + std::to_string(tf) + "], u, " + k + ")";
+    if (r >= gs.RL && G > 1) k = "gsum<" + std::to_string(G) + "u>(" + k + ")";
+    o << "    k[" << r << "] = " << k << ";\n";
+    t = tf + (row.force.empty() ? 0u : 1u);
+  }
+}
+
 std::string ternary_imap(const std::vector<int>& v) {
   std::string e = "-1";
   for (size_t i = v.size(); i-- > 0;)
@@ -1059,17 +1545,65 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
   o << kLaneRuntime;
   for (size_t c = 0; c < shapes.size(); ++c) {
     const LaneShape& sh = shapes[c];
+    const bool grp = !mw && sh.group > 1;
+    const uint32_t G = grp ? sh.group : 1u;
+    const uint32_t nf = grp ? sh.gf.nl() : sh.f.n;
+    const uint32_t na = sh.adjoint ? (grp ? sh.ga.nl() : sh.a.n) : 0u;
     o << "struct C" << c << " {\n";
-    o << "  enum { NF = " << sh.f.n << ", NA = " << (sh.adjoint ? sh.a.n : 0)
-      << ", TF = " << sh.f.coefs() << ", TA = " << (sh.adjoint ? sh.a.coefs() : 0)
-      << ", ADJ = " << (sh.adjoint ? 1 : 0) << " };\n";
+    o << "  enum { NF = " << nf << ", NA = " << na
+      << ", TF = " << (grp ? sh.gf.coefs() : sh.f.coefs())
+      << ", TA = " << (sh.adjoint ? (grp ? sh.ga.coefs() : sh.a.coefs()) : 0u)
+      << ", ADJ = " << (sh.adjoint ? 1 : 0) << ", G = " << G
+      << ", RLF = " << (grp ? sh.gf.RL : nf) << ", RLA = " << (grp ? sh.ga.RL : na)
+      << ", NX = " << (grp ? uint32_t(sh.gx.size()) : nf) << " };\n";
     if (!mw) {
+      std::vector<uint32_t> rf, ra;
+      if (grp) {
+        rf = sh.gf.band;
+        rf.insert(rf.end(), sh.gf.shared.begin(), sh.gf.shared.end());
+        ra = sh.ga.band;
+        ra.insert(ra.end(), sh.ga.shared.begin(), sh.ga.shared.end());
+        o << "  static __device__ __forceinline__ constexpr int rowf(int r) { return "
+          << ternary_map(rf) << "; }\n";
+        o << "  static __device__ __forceinline__ constexpr int rowa(int r) { return "
+          << ternary_map(ra) << "; }\n";
+      } else {
+        o << "  static __device__ __forceinline__ constexpr int rowf(int r) { return r; }\n";
+        o << "  static __device__ __forceinline__ constexpr int rowa(int r) { return r; }\n";
+      }
       o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* y, const R* c, R u) {\n";
-      emit_rhs(o, sh.f, true, "y", "y");
+      if (grp) emit_group_side(o, sh.gf, G, true, sh.gx);
+      else emit_rhs(o, sh.f, true, "y", "y");
+      o << "  }\n";
+      // set points of the reverse step: forward values read from the
+      // segment in LDS (another position's column for rotated sources)
+      o << "  static __device__ __forceinline__ void xload(R* xc, R* xp, const R* sc, const R* sp, "
+           "u32 lane) {\n";
+      if (grp) {
+        std::vector<uint32_t> ds;
+        for (const GSrc& x : sh.gx)
+          if (!x.shared && x.d != 0) ds.push_back(x.d);
+        std::sort(ds.begin(), ds.end());
+        ds.erase(std::unique(ds.begin(), ds.end()), ds.end());
+        for (uint32_t dd : ds)
+          o << "    const u32 l" << dd << " = glane<" << G << "u, " << dd << "u>(lane);\n";
+        for (size_t j = 0; j < sh.gx.size(); ++j) {
+          const GSrc& x = sh.gx[j];
+          const std::string col = (x.shared || x.d == 0) ? "lane" : "l" + std::to_string(x.d);
+          o << "    xc[" << j << "] = sc[" << x.slot << "u * WAVE + " << col << "];\n";
+          o << "    xp[" << j << "] = sp[" << x.slot << "u * WAVE + " << col << "];\n";
+        }
+      } else if (sh.adjoint) {
+        o << "#pragma unroll\n    for (int j = 0; j < NF; ++j) {\n"
+             "      xc[j] = sc[j * WAVE + lane];\n      xp[j] = sp[j * WAVE + lane];\n    }\n";
+      }
       o << "  }\n";
       o << "  static __device__ __forceinline__ void rhs_a(R* k, const R* y, const R* x, "
            "const R* c) {\n";
-      if (sh.adjoint) emit_rhs(o, sh.a, false, "y", "x");
+      if (sh.adjoint) {
+        if (grp) emit_group_side(o, sh.ga, G, false, sh.gx);
+        else emit_rhs(o, sh.a, false, "y", "x");
+      }
       o << "  }\n";
     }
     o << "};\n";
@@ -1165,7 +1699,8 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const Lane
   for (const LaneShape& s : shapes) {
     if (s.adjoint) {
       adj = true;
-      nf = s.f.n > nf ? s.f.n : nf;
+      const uint32_t k = s.group > 1 ? s.gf.nl() : s.f.n;
+      nf = k > nf ? k : nf;
     }
   }
   size_t v = size_t(kLaneStage) * kLaneWave;

@@ -82,6 +82,42 @@ struct LaneSide {
   uint32_t coefs() const;                   // coefficient slots incl. forcing
 };
 
+// Lane groups: G lanes (a power of two, aligned within the wave) integrate
+// one instance.  A side's rows are laid out in bands of G consecutive rows
+// (band slot r of position p holds row band[r] + p) and shared rows (every
+// lane of the group holds the same value).  The right-hand side of a band
+// slot is one program for all positions (the union of the positions' terms,
+// each lane with its own coefficients, zero where its row lacks the term);
+// a factor owned by another position comes through a lane rotation within
+// the group.  A shared row's terms are spread over the positions and summed
+// across the group with an xor butterfly (every lane gets the same sum).
+struct GSrc {
+  bool shared = false;
+  uint32_t slot = 0;  // lane slot (band slots first, then shared rows)
+  uint32_t d = 0;     // rotation: the value of position (p + d) mod G
+  bool operator<(const GSrc& o) const {
+    return shared != o.shared ? shared < o.shared : (slot != o.slot ? slot < o.slot : d < o.d);
+  }
+  bool operator==(const GSrc& o) const { return shared == o.shared && slot == o.slot && d == o.d; }
+};
+struct GTerm {
+  uint32_t kind = 0;
+  GSrc f[4];
+  std::vector<int32_t> coef;  // [G] system coefficient slot per position, -1 = none
+};
+struct GRow {
+  std::vector<GTerm> terms;
+  std::vector<int32_t> force;  // [G] force slot per position, empty if no position is forced
+};
+struct GSide {
+  uint32_t RL = 0, NS = 0;
+  std::vector<uint32_t> band;    // first row of each band slot
+  std::vector<uint32_t> shared;  // row of each shared slot
+  std::vector<GRow> rows;        // [RL + NS]
+  uint32_t nl() const { return RL + NS; }
+  uint32_t coefs() const;        // per-lane coefficient slots: row by row, terms then force
+};
+
 struct LaneShape {
   LaneSide f, a;
   bool adjoint = false;
@@ -91,10 +127,29 @@ struct LaneShape {
   // adjoint row.  Every wave of a workgroup works on the same instances
   // (one per lane); stage values go through LDS.
   std::vector<uint32_t> fwave, awave;
+  // Lane-group layout (group > 1, single-wave kernels): the programs of both
+  // sides and the forward values (set points) the adjoint side reads.
+  uint32_t group = 1;
+  GSide gf, ga;
+  std::vector<GSrc> gx;
 };
 
-// Kernel-level layout.  waves == 1: one instance per lane, the whole state
-// in registers, independent waves.  waves > 1: a workgroup of `waves` waves
+// Lay `sh` out for groups of G lanes (G = 1 restores one lane per instance).
+// False if G does not divide the wave or no band fits.
+bool lane_group(LaneShape& sh, uint32_t G);
+// values (of the working type) one lane keeps live in the larger phase
+uint32_t lane_live(const LaneShape& sh);
+// estimated vector instructions per step of one wave: forward step, and
+// forward + recompute + reverse step of an adjoint run
+double lane_step_cost(const LaneShape& sh, int bits);
+// system coefficient slots (lane_coefs order) -> the lane coefficients of
+// position p of a group side (0 where the position has no such term)
+void lane_group_coefs(const GSide& gs, const std::vector<double>& sys, uint32_t p,
+                      std::vector<double>& out);
+
+// Kernel-level layout.  waves == 1: one instance per lane (or per group of
+// lanes, LaneShape::group), the whole state in registers, independent
+// waves.  waves > 1: a workgroup of `waves` waves
 // runs `lw` instances (lanes >= lw mirror lane % lw), rows split across the
 // waves; `seg` is the adjoint segment length (checkpoint interval) in LDS.
 struct LaneLayout {

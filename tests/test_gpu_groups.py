@@ -1,0 +1,129 @@
+"""GPU parity of lane groups: G lanes (2..64) integrate one instance, rows in
+bands across the group, factors of other positions by lane rotations (DPP
+within rows of 16 lanes, the LDS crossbar for 32 and 64), shared rows summed
+by an xor butterfly.  Every group size is checked against the FP64 oracle
+(1e-10 relative), forward and adjoint, on regular bands (chains) and on
+irregular ones (Duffing, random systems with forces on several rows); launch
+boundaries and checkpoint intervals leave results bitwise unchanged.
+"""
+import numpy as np
+import pytest
+
+from pasif_amd import adjoint as ADJ
+from pasif_amd import workloads as W
+from tests.test_gpu_adjoint import oracle_gradient
+from tests.test_gpu_forward import TOL64, oracle_run, rel_err
+from tests.test_gpu_lane import _random_cartesian
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print(f"[gpu case] groups {name}", flush=True)
+
+
+def _grouped(w, G, adjoint=False, J=None):
+    p = W.make_solver(w)
+    p.setKernel("lane", G)
+    if adjoint:
+        p.setJacobian(*J.as_args())
+    assert p.kernelInfo(adjoint)["lanes_per_instance"] == [G]
+    return p
+
+
+FORWARD = [
+    (2, lambda: W.two_dof(40, 600)),
+    (2, lambda: W.duffing(16, 800)),
+    (4, lambda: W.chains(20, 2, 500, steps_per_sample=10)),
+    (8, lambda: W.chains(20, 4, 500, steps_per_sample=10)),
+    (16, lambda: W.chains(20, 8, 400, steps_per_sample=10)),
+    (16, lambda: W.chains(20, 16, 400, steps_per_sample=10)),
+    (32, lambda: W.chains(20, 16, 400, steps_per_sample=10)),
+    (64, lambda: W.chains(6, 32, 300, steps_per_sample=10)),
+]
+
+
+@pytest.mark.parametrize("G,maker", FORWARD)
+def test_group_forward_matches_oracle(G, maker):
+    w = maker()
+    _case(f"forward G={G} {w.name} n_systems={w.n_systems}")
+    p = _grouped(w, G)
+    q = p.getAmplitudes()
+    assert rel_err(q, oracle_run(w).q) < TOL64
+
+
+ADJOINT = [
+    (2, lambda: W.chains(16, 2, 300, steps_per_sample=10)),
+    (4, lambda: W.chains(16, 2, 300, steps_per_sample=10)),
+    (8, lambda: W.chains(16, 4, 300, steps_per_sample=10)),
+    (16, lambda: W.chains(16, 16, 250, steps_per_sample=10)),
+    (32, lambda: W.chains(16, 16, 250, steps_per_sample=10)),
+    (64, lambda: W.chains(4, 64, 150, steps_per_sample=10)),
+]
+
+
+@pytest.mark.parametrize("G,maker", ADJOINT)
+def test_group_adjoint_matches_oracle_and_is_checkpoint_invariant(G, maker):
+    w = maker()
+    _case(f"adjoint G={G} {w.name} n_systems={w.n_systems}")
+    J = ADJ.energy_gradient_inputs(w)
+    p = _grouped(w, G, True, J)
+    ref = oracle_gradient(w, J, 64)
+    outs = []
+    for k in (3, 64):
+        p.setTuning(checkpoint_every=k)
+        g = p.getGradient()
+        assert rel_err(g, ref) < TOL64, k
+        outs.append(g)
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_group_irregular_systems_trajectory_and_window():
+    _case("random cartesian G=2 and G=4, trajectory, window")
+    w = _random_cartesian()
+    ref = oracle_run(w, save_steps=7)
+    for G in (2, 4):
+        p = W.make_solver(w)
+        p.setKernel("lane", G)
+        groups = p.kernelInfo()["lanes_per_instance"]
+        assert G in groups, groups
+        q = p.getAmplitudes()
+        assert rel_err(q, ref.q) < TOL64
+        tr = p.getTrajectory(saveSteps=7)
+        assert rel_err(tr[1:].T, ref.traj.reshape(ref.traj.shape[0], -1)) < TOL64
+    w = W.duffing(n_systems=24, steps=900)
+    p = _grouped(w, 2)
+    q, mx = p.getMaxAmplitudes(window=250)
+    ref = oracle_run(w, window=250)
+    assert rel_err(q, ref.q) < TOL64
+    assert rel_err(mx, ref.maxabs) < TOL64
+
+
+def test_group_uneven_steps_and_solver_modes_bitwise():
+    _case("heterogeneous d1..d64 automatic groups, plain + graph, fwd + adj")
+    w = W.heterogeneous(n_systems=90, tmin=30, tmax=300, steps_per_sample=5, n_signals=4)
+    J = ADJ.energy_gradient_inputs(w)
+    p = W.make_solver(w)
+    p.setJacobian(*J.as_args())
+    info = p.kernelInfo()
+    assert info["kernel"] == "lane" and max(info["lanes_per_instance"]) > 1
+    q = p.getAmplitudes()
+    assert rel_err(q, oracle_run(w).q) < TOL64
+    g = p.getGradient()
+    assert rel_err(g, oracle_gradient(w, J, 256)) < TOL64
+    p.setSolver("plain")
+    np.testing.assert_array_equal(p.getAmplitudes(), q)
+    p.setSolver("graph", steps_per_launch=41, launches_per_graph=3)
+    np.testing.assert_array_equal(p.getGradient(), g)
+    np.testing.assert_array_equal(p.getAmplitudes(), q)
+
+
+def test_group_fp32():
+    _case("fp32 G=2 two_dof, G=16 chains d8")
+    for G, w in ((2, W.two_dof(n_systems=32, steps=1000)),
+                 (16, W.chains(12, 8, 400, steps_per_sample=10))):
+        p = _grouped(w, G)
+        p.setPrecision(32)
+        q = p.getAmplitudes()
+        assert rel_err(q, oracle_run(w, dtype=np.float32).q) < 1e-4
+        assert rel_err(q, oracle_run(w).q) < 5e-3

@@ -22,9 +22,9 @@ def _case(name):
     print(f"[gpu case] lane {name}", flush=True)
 
 
-def _lane(w):
+def _lane(w, lanes_per_system=0):
     p = W.make_solver(w)
-    p.setKernel("lane")
+    p.setKernel("lane", lanes_per_system)
     return p
 
 
@@ -157,7 +157,7 @@ def _waves(p, adjoint=False):
 def test_multiwave_forward_interp_and_solver_modes():
     _case("multi-wave forward chains d8")
     w = W.chains(n_systems=70, d=8, steps=500, steps_per_sample=10)
-    p = _lane(w)
+    p = _lane(w, 1)
     assert _waves(p) > 1
     q = p.getAmplitudes()
     assert rel_err(q, oracle_run(w).q) < TOL64
@@ -174,7 +174,7 @@ def test_multiwave_adjoint_checkpoints_and_solvers():
     _case("multi-wave adjoint chains d8")
     w = W.chains(n_systems=20, d=8, steps=300, steps_per_sample=10)
     J = ADJ.energy_gradient_inputs(w)
-    p = _lane(w)
+    p = _lane(w, 1)
     p.setJacobian(*J.as_args())
     assert _waves(p, True) > 1
     ref = oracle_gradient(w, J, 64)
@@ -195,12 +195,12 @@ def test_multiwave_mixed_sizes_and_cfg2_shape():
     _case("multi-wave heterogeneous d1..d16 + chains d16 adjoint")
     w = W.heterogeneous(n_systems=120, tmin=30, tmax=400, steps_per_sample=5, n_signals=4,
                         dof_choices=(1, 2, 4, 8, 16), dof_probs=(0.3, 0.25, 0.2, 0.15, 0.1))
-    p = _lane(w)
+    p = _lane(w, 1)
     assert p.kernelInfo()["classes"] == 5
     assert rel_err(p.getAmplitudes(), oracle_run(w).q) < TOL64
     w = W.chains(n_systems=40, d=16, steps=200, steps_per_sample=10)
     J = ADJ.energy_gradient_inputs(w)
-    p = _lane(w)
+    p = _lane(w, 1)
     p.setJacobian(*J.as_args())
     assert rel_err(p.getAmplitudes(), oracle_run(w).q) < TOL64
     assert rel_err(p.getGradient(), oracle_gradient(w, J, 64)) < TOL64
@@ -211,6 +211,7 @@ def test_mixed_lane_and_tile_in_one_solve():
     w = W.heterogeneous(n_systems=90, tmin=30, tmax=300, steps_per_sample=5, n_signals=4)
     J = ADJ.energy_gradient_inputs(w)
     p = W.make_solver(w)
+    p.setKernel("auto", 1)  # one lane per instance: the 64-DOF systems go to tiles
     p.setJacobian(*J.as_args())
     assert p.kernelInfo()["kernel"] == "mixed"
     q = p.getAmplitudes()

@@ -42,9 +42,11 @@ def test_generated_source_compiles_for_gfx950(maker, adjoint, classes):
 
 def test_multiwave_layout_for_medium_systems():
     # 16-DOF chains (33 forward rows, 65 adjoint rows) do not fit one lane:
-    # their rows are split over the waves of a workgroup
+    # with one lane per instance requested, their rows are split over the
+    # waves of a workgroup
     w = W.chains(64, 16, 60, steps_per_sample=10)
     p = host_solver(w)
+    p.setKernel("auto", 1)
     assert p.kernelInfo()["kernel"] == "lane"
     head = p._laneSource(False).splitlines()[0]
     assert "1 wave(s)" not in head and "wave(s) per workgroup" in head
@@ -56,10 +58,14 @@ def test_multiwave_layout_for_medium_systems():
 
 
 def test_large_systems_stay_on_tiles_and_lane_refuses_them():
+    # one lane per instance: 64-DOF chains fit neither a lane nor a workgroup
+    # of waves; a group of 64 lanes holds them
     w = W.chains(4, 64, 60, steps_per_sample=10)
     p = host_solver(w)
+    assert p.kernelInfo()["lanes_per_instance"] == [64]
+    p.setKernel("auto", 1)
     assert p.kernelInfo()["kernel"] == "tile"
-    p.setKernel("lane")
+    p.setKernel("lane", 1)
     with pytest.raises(_lib.PasifError) as e:
         p.kernelInfo()
     assert e.value.code == 3 and "registers" in str(e.value)
@@ -70,6 +76,7 @@ def test_large_systems_stay_on_tiles_and_lane_refuses_them():
 def test_checkpoint_interval_shortened_to_fit_lds():
     w = W.chains(8, 2, 60, steps_per_sample=10)
     p = host_solver(w)
+    p.setKernel("lane", 1)
     p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
     p.setTuning(checkpoint_every=256)
     k = p.kernelInfo(adjoint=True)["checkpoint_every"]
@@ -81,16 +88,78 @@ def test_checkpoint_interval_shortened_to_fit_lds():
 
 
 def test_mixed_sizes_split_between_lane_and_tile_kernels():
-    # 64-DOF chains do not fit the lane kernel (their energy row alone
-    # exceeds a wave's registers): they run on tiles beside the lane classes
-    # of the smaller sizes (2- and 8-DOF in one lane / 32-DOF across waves)
+    # With one lane per instance, 64-DOF chains do not fit the lane kernel
+    # (their energy row alone exceeds a wave's registers): they run on tiles
+    # beside the lane classes of the smaller sizes (2- and 8-DOF in one lane,
+    # 32-DOF across waves).  Lane groups hold every size.
     w = W.heterogeneous(60, 30, 300, steps_per_sample=5, n_signals=3,
                         dof_choices=(2, 8, 32, 64), dof_probs=(1, 1, 1, 1))
     p = host_solver(w)
     info = p.kernelInfo()
+    assert info["kernel"] == "lane" and info["classes"] == 4
+    p.setKernel("auto", 1)
+    info = p.kernelInfo()
     assert info["kernel"] == "mixed" and info["classes"] == 3
     p.setKernel("tile")
     assert p.kernelInfo()["kernel"] == "tile"
-    p.setKernel("lane")
+    p.setKernel("lane", 1)
     with pytest.raises(_lib.PasifError):
         p.kernelInfo()
+
+
+# (workload, adjoint, lanes per instance) covering every group size, regular
+# bands (chains) and irregular ones (Duffing: a band of 2 rows + a shared row)
+GROUP_CASES = [
+    (lambda: W.two_dof(40, 50), False, 2),
+    (lambda: W.duffing(16, 50), True, 2),
+    (lambda: W.chains(20, 2, 60, steps_per_sample=10), True, 4),
+    (lambda: W.chains(20, 4, 60, steps_per_sample=10), True, 8),
+    (lambda: W.chains(20, 8, 60, steps_per_sample=10), False, 16),
+    (lambda: W.chains(20, 16, 60, steps_per_sample=10), True, 16),
+    (lambda: W.chains(20, 16, 60, steps_per_sample=10), True, 32),
+    (lambda: W.chains(6, 32, 60, steps_per_sample=10), False, 64),
+    (lambda: W.chains(4, 64, 60, steps_per_sample=10), True, 64),
+]
+
+
+@pytest.mark.parametrize("maker,adjoint,G", GROUP_CASES)
+def test_group_layouts_compile_for_gfx950(maker, adjoint, G):
+    w = maker()
+    p = host_solver(w)
+    if adjoint:
+        p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
+    p.setKernel("lane", G)
+    assert p.kernelInfo(adjoint)["lanes_per_instance"] == [G]
+    src = p._laneSource(adjoint, compile=True)
+    assert f"G = {G}," in src
+    p.setPrecision(32)
+    p._laneSource(adjoint, compile=True)
+
+
+def test_chain_group_program_is_the_stencil():
+    # 16-DOF chain on 16 lanes: lane p holds x_p, v_p and the shared energy
+    # row; neighbours come by rotations +-1, the energy sum by the butterfly
+    w = W.chains(64, 16, 60, steps_per_sample=10)
+    p = host_solver(w)
+    p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
+    p.setKernel("lane", 16)
+    src = p._laneSource(True)
+    fwd = src[src.index("struct C0 {"):src.index("static __device__ __forceinline__ void xload")]
+    assert "NF = 3," in fwd and "RLF = 2," in fwd
+    for r in ("rot<16u, 1u>(y[0])", "rot<16u, 15u>(y[0])", "rot<16u, 1u>(y[1])",
+              "rot<16u, 15u>(y[1])", "gsum<16u>("):
+        assert r in fwd
+    assert fwd.count("rot<16u,") == 4
+    assert "TF = 15," in fwd
+
+
+def test_automatic_groups_widen_few_instances_only():
+    # few instances: the chain of steps of one wave bounds the run, so the
+    # planner spreads each instance over lanes; many instances fill every
+    # SIMD already and stay one per lane
+    few = host_solver(W.two_dof(64, 2000))
+    assert few.kernelInfo()["lanes_per_instance"] == [2]
+    many = host_solver(W.two_dof(200000, 20))
+    assert many.kernelInfo()["lanes_per_instance"] == [1]
+    with pytest.raises(_lib.PasifError):
+        few.setKernel("lane", 3)
