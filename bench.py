@@ -139,6 +139,7 @@ def fwd_adj_section(args) -> dict:
                        f"interpolated band-limited forcing, gradient of E(T) w.r.t. 16 K_ii + "
                        f"16 Lambda per system, FP64",
            "kernel": info["kernel"], "checkpoint_every": info["checkpoint_every"],
+           "lanes_per_system": info.get("lanes_per_instance"),
            "value": sys_steps / wall, "unit": "system-steps/s (fwd+adj)",
            "kernel_ms": t["kernel_ms"], "workgroups": t["tiles"],
            "roofline": {"achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -339,7 +340,8 @@ def main():
         "config": {"workload": f"two_dof {w.n_systems} systems x {w.numsteps} RK4 steps, "
                                f"dt 5e-4, sinusoidal forcing, forward, FP64",
                    "systems": w.n_systems, "rk_steps": w.numsteps, "solver": "persistent",
-                   "kernel": kinfo["kernel"], "structure_classes": kinfo["classes"]},
+                   "kernel": kinfo["kernel"], "structure_classes": kinfo["classes"],
+                   "lanes_per_system": kinfo.get("lanes_per_instance")},
         "roofline": roofline,
     }
 

@@ -1141,6 +1141,23 @@ double device_simds(int dev) {
   return 1024.0;
 }
 
+// Waves per SIMD a single-wave kernel should keep resident: enough for all
+// its waves to run at once, as far as the largest class's live values
+// (plus temporaries) leave registers for (512 VGPRs per SIMD lane).
+uint32_t single_wave_occupancy(const std::vector<LaneShape>& shapes,
+                               const std::vector<ClassLoad>& load, int bits, int dev) {
+  double waves = 0;
+  uint32_t live = 0;
+  for (size_t c = 0; c < shapes.size(); ++c) {
+    waves += std::ceil(load[c].inst * shapes[c].group / double(kLaneWave));
+    live = std::max(live, lane_live(shapes[c]));
+  }
+  const uint32_t regs = live * uint32_t(bits / 32) + 48;
+  const uint32_t by_regs = std::max<uint32_t>(1, 512 / regs);
+  const uint32_t by_waves = uint32_t(std::min(8.0, std::max(1.0, std::ceil(waves / device_simds(dev)))));
+  return std::min(by_regs, by_waves);
+}
+
 // Checkpoint interval of a single-wave adjoint kernel: the requested one,
 // shortened until the segment [K+1][slots][64] and the sample stage fit the
 // LDS one wave may take while every wave of the launch stays resident
@@ -1149,14 +1166,15 @@ uint32_t single_wave_seg(const std::vector<LaneShape>& shapes, const std::vector
                          size_t el, uint32_t Kreq, int dev) {
   uint32_t nf_max = 0;
   double waves = 0;
+  size_t stage = 0;
   for (size_t c = 0; c < shapes.size(); ++c) {
     const LaneShape& sh = shapes[c];
     nf_max = std::max(nf_max, sh.group > 1 ? sh.gf.nl() : sh.f.n);
     waves += std::ceil(load[c].inst * sh.group / double(kLaneWave));
+    stage = std::max(stage, size_t(kLaneStage) * (kLaneWave / sh.group) * el);
   }
   const double cus = device_simds(dev) / 4.0;
   const double per_cu = std::max(1.0, std::ceil(waves / cus));
-  const size_t stage = size_t(kLaneStage) * kLaneWave * el;
   const size_t step = size_t(nf_max) * kLaneWave * el;
   size_t budget = std::min<size_t>(kLaneMaxLds, size_t(double(kMaxLdsBytes) / per_cu));
   // at least a few steps per segment even when the waves then queue
@@ -1297,6 +1315,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     } else {
       choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group);
       if (adjoint) lay.seg = single_wave_seg(shapes, sel_load, el, Kreq, d->dev);
+      lay.occ = single_wave_occupancy(shapes, sel_load, bits, d->dev);
     }
   }
   if (why) {

@@ -519,6 +519,33 @@ __device__ __forceinline__ void put(R* e, const R (&v)[STAGE], u32 Lx, u32 b, bo
   for (u32 u = 0; u < STAGE; ++u) e[u * SS] = (forced && b + u < Lx) ? v[u] : R(0);
 }
 
+// Samples of a group's chunk: position p of a group of G lanes loads
+// e[b + u] for u = p, p + G, ... < STAGE and stores them into the group's
+// column of the stage [STAGE][WAVE / G] (`e` points at it); the group then
+// reads its samples there.
+template <u32 G, u32 NPF>
+__device__ __forceinline__ void fetch_g(R (&v)[NPF], GP(const R) E, u32 Lx, u32 b, u32 p) {
+#pragma unroll
+  for (u32 k = 0; k < NPF; ++k) {
+    const u32 i = b + p + k * G;
+    v[k] = E[i < Lx ? i : Lx - 1u];
+  }
+}
+template <u32 G, u32 NPF>
+__device__ __forceinline__ void put_g(R* e, const R (&v)[NPF], u32 Lx, u32 b, bool forced, u32 p) {
+#pragma unroll
+  for (u32 k = 0; k < NPF; ++k) {
+    const u32 u = p + k * G;
+    if (u < STAGE) e[u * (WAVE / G)] = (forced && b + u < Lx) ? v[k] : R(0);
+  }
+  if constexpr (G > 1u) {
+    // the other lanes of the group read these samples next
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  }
+}
+
 // Mi[1][j] e[i] + Mi[0][j] e[i+1] on the half step i0*W + j0 + t, zero
 // unless 0 < i < L (src/kernels.cu:114-124)
 template <u32 SS>
@@ -566,9 +593,10 @@ __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
 // state as the tile kernel.  S::G lanes integrate one instance (1: one
 // instance per lane); per-lane arrays (coefficients, states, checkpoints)
 // are indexed by the lane's column, per-instance ones (steps, excitation,
-// output bases) by the instance.  LDS: eb [STAGE][WAVE] staged samples of
-// the current chunk of steps (the next chunk is prefetched into registers),
-// then seg [K+1][NF][WAVE] (adjoint) the recomputed segment.
+// output bases) by the instance.  LDS: eb [STAGE][WAVE / G] staged samples
+// of the current chunk of steps, one column per instance (the next chunk is
+// prefetched into registers), then seg [K+1][NF][WAVE] (adjoint) the
+// recomputed segment.
 template <class S>
 __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr int NF = S::NF;
@@ -578,14 +606,15 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr int NX = S::NX > 0 ? S::NX : 1;
   constexpr u32 G = S::G;
   constexpr u32 IPW = WAVE / G;  // instances per wave
+  constexpr u32 NPF = (STAGE + G - 1u) / G;  // samples each lane of a group stages
   const u32 lane = threadIdx.x;
   const u32 p = lane & (G - 1u);
   const u64 st = a.stride;
   const u32 col = wave * WAVE + lane;
   const u32 i = wave * IPW + lane / G;
   const bool valid = i < (u32)a.count;
-  R* eb = smem + lane;
-  R* seg = smem + STAGE * WAVE;
+  R* eb = smem + lane / G;  // the instance's staged samples
+  R* seg = smem + STAGE * IPW;
   const bool adjoint = S::ADJ && a.adjoint != 0ull;
   const u32 K = adjoint ? (u32)a.ck_every : 1u;
   const u32 W = (u32)a.interp_w;
@@ -696,7 +725,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       u32 hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
       u32 ebase = 0, left = 0, pbase = 0;
       bool have = false;
-      R pre[STAGE];
+      R pre[NPF];
       auto steps_to = [&](auto all_t, auto gen_t, auto zoh_t, u32 stop) {
         constexpr bool ALL = decltype(all_t)::value;
         constexpr bool GEN = decltype(gen_t)::value;
@@ -704,14 +733,14 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         for (; n < stop; ++n) {
           R u0, u1, u3;
           if (ZOH) {
-            const R e0 = eb[(n - ebase) * WAVE], e1 = eb[(n + 1u - ebase) * WAVE];
+            const R e0 = eb[(n - ebase) * IPW], e1 = eb[(n + 1u - ebase) * IPW];
             u0 = (n > 0u && n < Lx) ? e0 : R(0);
             u1 = u0;
             u3 = e1;
           } else {
-            u0 = interp_at<WAVE>(eb, ebase, wts, W, Lx, hi, hj, 0u);
-            u1 = interp_at<WAVE>(eb, ebase, wts, W, Lx, hi, hj, 1u);
-            u3 = interp_at<WAVE>(eb, ebase, wts, W, Lx, hi, hj, 2u);
+            u0 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 0u);
+            u1 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 1u);
+            u3 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 2u);
             hj += 2u;
             if (hj >= W) {
               hj -= W;
@@ -764,16 +793,16 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           // now), and the next chunk's are requested
           const u32 base = W ? hi : n;
           if (have && pbase == base) {
-            put<WAVE>(eb, pre, Lx, base, forced);
+            put_g<G, NPF>(eb, pre, Lx, base, forced, p);
           } else {
-            R v[STAGE];
-            fetch(v, E, Lx, base);
-            put<WAVE>(eb, v, Lx, base, forced);
+            R v[NPF];
+            fetch_g<G, NPF>(v, E, Lx, base, p);
+            put_g<G, NPF>(eb, v, Lx, base, forced, p);
           }
           ebase = base;
           left = CHUNK;
           pbase = W ? (2u * (n + CHUNK)) / W : n + CHUNK;
-          fetch(pre, E, Lx, pbase);
+          fetch_g<G, NPF>(pre, E, Lx, pbase, p);
           have = true;
         }
         u32 stop = umin(nend, n + left);
@@ -1510,7 +1539,7 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
   std::ostringstream o;
   o << "// generated lane kernel: " << shapes.size() << " structure class(es), FP" << bits
     << ", " << lay.waves << " wave(s) per workgroup, " << (mw ? lay.lw : kLaneWave)
-    << " instances per workgroup, segment " << lay.seg << "\n";
+    << " instances per workgroup, segment " << lay.seg << ", occupancy " << lay.occ << "\n";
   o << "typedef unsigned int u32;\ntypedef int i32;\ntypedef unsigned long long u64;\n";
   o << "typedef " << (bits == 64 ? "double" : "float") << " R;\n";
   o << "struct LaneArgs {\n";
@@ -1660,8 +1689,9 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
     }
   }
   if (!mw) {
-    o << "extern \"C\" __global__ void __launch_bounds__(" << kLaneWave
-      << ") pasif_lane_kernel(const LaneArgs* __restrict__ cls, u32 ncls) {\n"
+    o << "extern \"C\" __global__ void __launch_bounds__(" << kLaneWave << ") ";
+    if (lay.occ > 1) o << "__attribute__((amdgpu_waves_per_eu(" << lay.occ << "))) ";
+    o << "pasif_lane_kernel(const LaneArgs* __restrict__ cls, u32 ncls) {\n"
          "  extern __shared__ R smem[];\n"
          "  const u32 b = blockIdx.x;\n"
          "  const u32 e = find_entry(cls, ncls, b);\n"
@@ -1694,17 +1724,14 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
 size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const LaneLayout& lay) {
   const size_t el = bits == 64 ? 8 : 4;
   if (lay.waves > 1) return mw_lds_values(mw_dims(shapes, lay.waves), lay) * el;
-  uint32_t nf = 0;
-  bool adj = false;
+  // per class: the sample stage (a column per instance), then the segment
+  size_t v = 0;
   for (const LaneShape& s : shapes) {
-    if (s.adjoint) {
-      adj = true;
-      const uint32_t k = s.group > 1 ? s.gf.nl() : s.f.n;
-      nf = k > nf ? k : nf;
-    }
+    const uint32_t G = s.group > 1 ? s.group : 1u;
+    size_t c = size_t(kLaneStage) * (kLaneWave / G);
+    if (s.adjoint) c += size_t(lay.seg + 1) * (G > 1 ? s.gf.nl() : s.f.n) * kLaneWave;
+    v = std::max(v, c);
   }
-  size_t v = size_t(kLaneStage) * kLaneWave;
-  if (adj) v += size_t(lay.seg + 1) * nf * kLaneWave;
   return v * el;
 }
 

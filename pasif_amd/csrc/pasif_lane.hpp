@@ -154,6 +154,9 @@ void lane_group_coefs(const GSide& gs, const std::vector<double>& sys, uint32_t 
 // waves; `seg` is the adjoint segment length (checkpoint interval) in LDS.
 struct LaneLayout {
   uint32_t waves = 1, lw = 64, seg = 1;
+  // single-wave kernels: waves each SIMD should hold at once (the register
+  // budget the compiler gets is 512 / occ VGPRs); 0 or 1: no target
+  uint32_t occ = 0;
 };
 
 // Split the rows of `sh` over `waves` waves (largest term cost first onto

@@ -28,6 +28,8 @@ def main(argv=None):
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--solver", default="persistent")
     ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "lane"])
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="lanes per system in the lane kernel (0: chosen per structure)")
     ap.add_argument("--repeat", type=int, default=1)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     a = ap.parse_args(argv)
@@ -56,7 +58,7 @@ def main(argv=None):
     p = W.make_solver(w)
     p.setPrecision(a.precision)
     p.setSolver(a.solver)
-    p.setKernel(a.kernel)
+    p.setKernel(a.kernel, a.lanes)
     if a.adjoint:
         from .adjoint import energy_gradient_inputs
         p.setJacobian(*energy_gradient_inputs(w).as_args())
