@@ -156,8 +156,12 @@ struct LaneCls {
   uint32_t count = 0, stride = 0, waves = 0, block_begin = 0, tmax = 0;
   uint32_t nf = 0, na = 0;
   uint32_t klass = 0;  // the class's case in its kernel's dispatch
-  DevBuf coef_f, coef_a, init_f, init_a, steps, exc, out_f, out_a;  // program
-  DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;            // run
+  uint32_t seg = 1;    // checkpoint interval (adjoint)
+  // single-wave adjoint: checkpoint elements, each wave's block sized by its
+  // longest instance (offsets in ckoff)
+  uint64_t ck_elems = 0;
+  DevBuf coef_f, coef_a, init_f, init_a, steps, exc, out_f, out_a, ckoff;  // program
+  DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;                  // run
 };
 
 // One lane kernel's share of a packed problem: the instances of each of its
@@ -1158,30 +1162,49 @@ uint32_t single_wave_occupancy(const std::vector<LaneShape>& shapes,
   return std::min(by_regs, by_waves);
 }
 
-// Checkpoint interval of a single-wave adjoint kernel: the requested one,
-// shortened until the segment [K+1][slots][64] and the sample stage fit the
-// LDS one wave may take while every wave of the launch stays resident
-// (160 KB per CU shared by ceil(waves / CUs) waves, at most 64 KB).
-uint32_t single_wave_seg(const std::vector<LaneShape>& shapes, const std::vector<ClassLoad>& load,
-                         size_t el, uint32_t Kreq, int dev) {
-  uint32_t nf_max = 0;
+// Checkpoint intervals of the classes of a single-wave adjoint kernel: the
+// requested one, shortened until a class's segment [K+1][slots][64] and its
+// sample stage fit the LDS one wave may take while every wave of the launch
+// stays resident (160 KB per CU shared by ceil(waves / CUs) waves, at most
+// 64 KB, at least 5 steps of segment).  When the checkpoints of all waves
+// (a state every K steps of each wave's longest instance) would take more
+// than half the free device memory, the LDS share grows (fewer resident
+// waves, longer intervals) until they fit.  Returns the shortest interval.
+uint32_t single_wave_segs(std::vector<LaneShape>& shapes, const std::vector<ClassLoad>& load,
+                          size_t el, uint32_t Kreq, int dev) {
   double waves = 0;
-  size_t stage = 0;
-  for (size_t c = 0; c < shapes.size(); ++c) {
-    const LaneShape& sh = shapes[c];
-    nf_max = std::max(nf_max, sh.group > 1 ? sh.gf.nl() : sh.f.n);
-    waves += std::ceil(load[c].inst * sh.group / double(kLaneWave));
-    stage = std::max(stage, size_t(kLaneStage) * (kLaneWave / sh.group) * el);
-  }
+  for (size_t c = 0; c < shapes.size(); ++c)
+    waves += std::ceil(load[c].inst * shapes[c].group / double(kLaneWave));
   const double cus = device_simds(dev) / 4.0;
   const double per_cu = std::max(1.0, std::ceil(waves / cus));
-  const size_t step = size_t(nf_max) * kLaneWave * el;
+  double cap = 64e9;
+  size_t fr = 0, tot = 0;
+  if (dev >= 0 && hipMemGetInfo(&fr, &tot) == hipSuccess) cap = 0.5 * double(fr);
   size_t budget = std::min<size_t>(kLaneMaxLds, size_t(double(kMaxLdsBytes) / per_cu));
-  // at least a few steps per segment even when the waves then queue
-  budget = std::max(budget, stage + 5 * step);
-  budget = std::min<size_t>(budget, kLaneMaxLds);
-  const size_t fit = budget > stage ? (budget - stage) / step : 0;
-  return uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+  uint32_t kmin = Kreq;
+  for (;;) {
+    double bytes = 0;
+    kmin = Kreq;
+    for (size_t c = 0; c < shapes.size(); ++c) {
+      LaneShape& sh = shapes[c];
+      const uint32_t G = sh.group > 1 ? sh.group : 1u;
+      const size_t stage = size_t(kLaneStage) * (kLaneWave / G) * el;
+      const size_t step = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
+      const size_t b = std::min<size_t>(kLaneMaxLds, std::max(budget, stage + 6 * step));
+      const size_t fit = b > stage ? (b - stage) / step : 0;
+      sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+      kmin = std::min(kmin, sh.seg);
+      // a checkpoint of every wave's lanes every seg steps of its longest instance
+      bytes += (load[c].steps_sum * G / double(kLaneWave) / sh.seg + load[c].inst * G / kLaneWave) *
+               double(step);
+    }
+    if (std::getenv("PASIF_LANE_DEBUG"))
+      fprintf(stderr, "[pasif] LDS share %zu B: checkpoints %.3g GB (cap %.3g GB), shortest interval %u\n",
+              budget, bytes / 1e9, cap / 1e9, kmin);
+    if (bytes <= cap || budget >= kLaneMaxLds) break;
+    budget = std::min<size_t>(kLaneMaxLds, budget + budget / 4 + 1024);
+  }
+  return kmin;
 }
 
 // Plan the lane kernel for the loaded problem: true if every system fits a
@@ -1314,7 +1337,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
       }
     } else {
       choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group);
-      if (adjoint) lay.seg = single_wave_seg(shapes, sel_load, el, Kreq, d->dev);
+      if (adjoint) lay.seg = single_wave_segs(shapes, sel_load, el, Kreq, d->dev);
       lay.occ = single_wave_occupancy(shapes, sel_load, bits, d->dev);
     }
   }
@@ -1463,6 +1486,20 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
       L->tmax = std::max(L->tmax, in.steps);
     }
     tmax = std::max(tmax, L->tmax);
+    L->seg = adjoint ? ((lay.waves == 1 && sh.seg) ? sh.seg : lay.seg) : 1u;
+    if (adjoint && lay.waves == 1) {
+      // wave w runs instances [w*ipw, (w+1)*ipw), the first with the most steps
+      const uint32_t ipw = kLaneWave / G;
+      std::vector<uint64_t> off(L->waves, 0);
+      uint64_t tot = 0;
+      for (uint32_t w = 0; w < L->waves; ++w) {
+        const uint32_t Tw = w * ipw < L->count ? stp[size_t(w) * ipw] : 0u;
+        off[w] = tot;
+        tot += uint64_t(std::max<uint32_t>(1, (Tw + L->seg - 1) / L->seg)) * L->nf * kLaneWave;
+      }
+      L->ck_elems = tot;
+      upload(d, L->ckoff, off);
+    }
     upload(d, L->coef_f, cf);
     upload(d, L->coef_a, ca);
     upload(d, L->init_f, qf);
@@ -1783,11 +1820,11 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
                                      uint32_t save_steps, uint32_t nslots, const void* excit,
                                      const void* interp) {
   Packed& pk = d->pk;
-  const uint32_t K = adjoint ? P.lay.seg : 1u;
   const R h = R(1.0 / double(d->sample_rate));
   std::vector<LaneArgs> args(P.cls.size());
   for (size_t c = 0; c < P.cls.size(); ++c) {
     LaneCls& L = *P.cls[c];
+    const uint32_t K = adjoint ? L.seg : 1u;
     const size_t st = L.stride;
     dev_reserve(d, L.qstate, size_t(L.nf) * st * sizeof(R));
     dev_reserve(d, L.progress, size_t(L.waves) * 4 * sizeof(uint32_t));
@@ -1797,7 +1834,10 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
     if (adjoint) {
       nck = std::max<uint32_t>(1, (L.tmax + K - 1) / K);
       dev_reserve(d, L.lstate, size_t(std::max<uint32_t>(L.na, 1)) * st * sizeof(R));
-      dev_reserve(d, L.ckpt, size_t(nck) * L.nf * st * sizeof(R));
+      if (P.lay.waves == 1)
+        dev_reserve(d, L.ckpt, size_t(L.ck_elems) * sizeof(R));
+      else
+        dev_reserve(d, L.ckpt, size_t(nck) * L.nf * st * sizeof(R));
       dev_reserve(d, L.segbuf, size_t(K + 1) * L.nf * st * sizeof(R));
     }
     LaneArgs& a = args[c];
@@ -1815,6 +1855,7 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
     a.mxstate = L.mxstate.p;
     a.progress = L.progress.p;
     a.ckpt = L.ckpt.p;
+    a.ck_off = L.ckoff.p;
     a.segbuf = L.segbuf.p;
     a.out_q = d->out_q.p;
     a.out_max = d->window ? d->out_max.p : nullptr;
@@ -2440,8 +2481,8 @@ double job_bytes(const pasif_driver* d) {
   for (const auto& lp : d->pk.lparts)
     for (const auto& L : lp->cls) {
       const DevBuf* lb[] = {&L->coef_f, &L->coef_a, &L->init_f, &L->init_a, &L->steps,
-                            &L->exc, &L->out_f, &L->out_a, &L->qstate, &L->lstate,
-                            &L->mxstate, &L->progress, &L->ckpt, &L->segbuf};
+                            &L->exc, &L->out_f, &L->out_a, &L->ckoff, &L->qstate,
+                            &L->lstate, &L->mxstate, &L->progress, &L->ckpt, &L->segbuf};
       for (const DevBuf* x : lb) b += double(x->bytes);
     }
   return b;

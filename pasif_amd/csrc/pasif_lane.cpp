@@ -638,7 +638,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   GP(R) qs = (GP(R))a.qstate + col;
   GP(R) ms = (GP(R))a.mxstate + col;
   GP(R) ls = (GP(R))a.lstate + col;
-  GP(R) ck = (GP(R))a.ckpt + col;
+  // checkpoints: the wave's block [nck_w][NF][WAVE] (sized by its longest instance)
+  GP(R) ck = (GP(R))a.ckpt + (adjoint ? ((GP(const u64))a.ck_off)[wave] : 0ull) + lane;
   GP(R) sb = (GP(R))a.segbuf + col;
   GP(R) outq = (GP(R))a.out_q;
   GP(R) outm = (GP(R))a.out_max;
@@ -698,7 +699,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         if (n == s0) {
 #pragma unroll
           for (int r = 0; r < NF; ++r) {
-            q[r] = ck[((u64)sg * NF + r) * st];
+            q[r] = ck[((u64)sg * NF + r) * WAVE];
             seg[r * WAVE + lane] = q[r];
             if (mirror) sb[r * st] = q[r];
           }
@@ -749,7 +750,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           }
           if (GEN && adjoint && !recompute && n == next_ck) {
 #pragma unroll
-            for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * st] = q[r];
+            for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * WAVE] = q[r];
             next_ck += K;
           }
           rk4<NF, ALL>(q, [&](R* k, const R* y, int s) {
@@ -1728,8 +1729,9 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const Lane
   size_t v = 0;
   for (const LaneShape& s : shapes) {
     const uint32_t G = s.group > 1 ? s.group : 1u;
+    const uint32_t K = s.seg ? s.seg : lay.seg;
     size_t c = size_t(kLaneStage) * (kLaneWave / G);
-    if (s.adjoint) c += size_t(lay.seg + 1) * (G > 1 ? s.gf.nl() : s.f.n) * kLaneWave;
+    if (s.adjoint) c += size_t(K + 1) * (G > 1 ? s.gf.nl() : s.f.n) * kLaneWave;
     v = std::max(v, c);
   }
   return v * el;

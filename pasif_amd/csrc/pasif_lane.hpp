@@ -35,7 +35,9 @@ namespace pasif {
   X(void*, lstate)         /* [NA][stride] */                               \
   X(void*, mxstate)        /* [NF][stride] running max |q| */               \
   X(void*, progress)       /* u32 [waves][4]: step, phase, segment */       \
-  X(void*, ckpt)           /* [nck][NF][stride] checkpoints */              \
+  X(void*, ckpt)           /* checkpoints (single-wave: per-wave blocks) */ \
+  X(const void*, ck_off)   /* u64 [waves] first element of each wave's      \
+                              checkpoint block [nck_w][NF][64] (single-wave) */ \
   X(void*, segbuf)         /* [K+1][NF][stride] recomputed segment */       \
   X(void*, out_q)                                                            \
   X(void*, out_max)                                                          \
@@ -132,6 +134,8 @@ struct LaneShape {
   uint32_t group = 1;
   GSide gf, ga;
   std::vector<GSrc> gx;
+  // single-wave kernels: this class's checkpoint interval (0: the kernel's)
+  uint32_t seg = 0;
 };
 
 // Lay `sh` out for groups of G lanes (G = 1 restores one lane per instance).
