@@ -2675,8 +2675,11 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
 }
 
 // Lane modules over the registered structures: module 0 runs the classes
-// that fit one lane, module 1 (multi-wave, 16 instances per workgroup) the
-// classes that need a workgroup; classes fitting neither stay on tiles.
+// that fit one lane or a group of lanes (each at its cheapest group size:
+// requests arrive continuously, so throughput, not one request's chain of
+// steps, sets the layout), module 1 (multi-wave, 16 instances per
+// workgroup) the classes that need a workgroup; classes fitting neither
+// stay on tiles.
 template <typename R>
 void pool_build_modules(pasif_pool* P) {
   const double eps = std::numeric_limits<R>::epsilon();
@@ -2704,8 +2707,14 @@ void pool_build_modules(pasif_pool* P) {
     count[it->second] += 1;
   }
   std::vector<uint32_t> need(all.size(), 0);
+  const int bits = int(sizeof(R) * 8);
   for (size_t c = 0; c < all.size(); ++c) {
-    if (all[c].live_values() <= lane_max_live()) {
+    const auto opt = group_options(all[c], bits);
+    if (!opt.empty()) {
+      size_t b = 0;
+      for (size_t k = 1; k < opt.size(); ++k)
+        if (opt[k].second * opt[k].first < opt[b].second * opt[b].first) b = k;
+      (void)lane_group(all[c], opt[b].first);
       need[c] = 1;
     } else {
       for (uint32_t w = 2; w <= kLaneMaxWaves && !need[c]; ++w)
@@ -2729,18 +2738,26 @@ void pool_build_modules(pasif_pool* P) {
     auto& M = P->mods[size_t(m)];
     for (uint32_t c : order) M.shapes.push_back(all[c]);
     bool adj = false;
-    uint32_t nf_max = 0;
-    for (const LaneShape& sh : M.shapes) {
-      adj = adj || sh.adjoint;
-      nf_max = std::max(nf_max, sh.f.n);
-    }
+    for (const LaneShape& sh : M.shapes) adj = adj || sh.adjoint;
     LaneLayout lay;
     if (m == 0) {
-      if (adj) {
-        const size_t stage = size_t(kLaneStage) * kLaneWave * el;
-        const size_t fit = (kLaneMaxLds - stage) / (size_t(nf_max) * kLaneWave * el);
-        lay.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+      // a fixed LDS share per wave (eight waves per CU resident) sets each
+      // class's checkpoint interval; registers for at most four waves per SIMD
+      uint32_t live = 0;
+      lay.seg = Kreq;
+      for (LaneShape& sh : M.shapes) {
+        live = std::max(live, lane_live(sh));
+        if (!sh.adjoint) continue;
+        const uint32_t G = sh.group > 1 ? sh.group : 1u;
+        const size_t stage = size_t(kLaneStage) * (kLaneWave / G) * el;
+        const size_t step = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
+        const size_t b = std::min<size_t>(kLaneMaxLds, std::max<size_t>(kMaxLdsBytes / 8, stage + 6 * step));
+        const size_t fit = b > stage ? (b - stage) / step : 0;
+        sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+        lay.seg = std::min(lay.seg, sh.seg);
       }
+      if (!adj) lay.seg = 1;
+      lay.occ = std::min<uint32_t>(4, std::max<uint32_t>(1, 512 / (live * uint32_t(bits / 32) + 48)));
     } else {
       for (LaneShape& sh : M.shapes) (void)lane_partition(sh, waves);
       lay.waves = waves;
@@ -3109,7 +3126,16 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
       }
       P->reg.push_back(std::move(e));
     }
-    if (P->mods_built) {
+    if (P->jobs.empty()) {
+      // no job runs on the lane modules yet: (re)build them over every
+      // registered structure now, so that no request waits for a compilation
+      P->mods_built = false;
+      P->key_lane.clear();
+      if (P->precision == 64)
+        pool_build_modules<double>(P);
+      else
+        pool_build_modules<float>(P);
+    } else if (P->mods_built) {
       // structures the lane modules already run join them; new ones use tiles
       const double eps = P->precision == 64 ? std::numeric_limits<double>::epsilon()
                                             : double(std::numeric_limits<float>::epsilon());
