@@ -211,7 +211,9 @@ int pasif_pool_tick(pasif_pool* pool, int wait, uint32_t* active_tiles);
 int pasif_pool_wait(pasif_pool* pool, double* kernel_ms);
 int pasif_pool_job_finished(pasif_pool* pool, const pasif_driver* job, int* finished);
 /* final states [dof] and, for adjoint jobs, lambda_0 [adjoint dof]; frees the
- * job's device memory and removes it from the pool (then pasif_destroy it) */
+ * job's device memory and removes it from the pool (then pasif_destroy it).
+ * A job finished by an earlier tick is left out of later ticks and may be
+ * collected while one runs; otherwise collect after pasif_pool_wait. */
 int pasif_pool_collect(pasif_pool* pool, pasif_driver* job, double* q, size_t capq, size_t* lenq,
                        double* grad, size_t capg, size_t* leng);
 int pasif_pool_stats(const pasif_pool* pool, uint32_t* jobs, double* used_bytes,

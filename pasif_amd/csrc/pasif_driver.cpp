@@ -2839,6 +2839,11 @@ void pool_tick_impl(pasif_pool* P) {
     M.table.clear();
     M.blocks = 0;
   }
+  // jobs whose work the completed ticks covered leave the launches: their
+  // buffers are then untouched by the GPU and may be collected while a
+  // later tick runs
+  for (auto& j : P->jobs)
+    if (j.done >= j.work) j.finished = true;
   for (auto& j : P->jobs) {
     if (j.finished) continue;
     pasif_driver* d = j.d;
@@ -3255,7 +3260,7 @@ int pasif_pool_job_finished(pasif_pool* P, const pasif_driver* d, int* finished)
     *finished = 0;
     for (auto& j : P->jobs)
       if (j.d == d) {
-        *finished = (!P->in_flight && j.done >= j.work) ? 1 : 0;
+        *finished = (j.finished || (!P->in_flight && j.done >= j.work)) ? 1 : 0;
         return;
       }
     POOL_REQUIRE(false, PASIF_ERR_ARGUMENT, "job not in this pool");
@@ -3265,12 +3270,13 @@ int pasif_pool_job_finished(pasif_pool* P, const pasif_driver* d, int* finished)
 int pasif_pool_collect(pasif_pool* P, pasif_driver* d, double* q, size_t capq, size_t* lenq,
                        double* grad, size_t capg, size_t* leng) {
   return pool_guarded(P, [&] {
-    POOL_REQUIRE(!P->in_flight, PASIF_ERR_STATE, "wait for the tick first");
     size_t k = 0;
     for (; k < P->jobs.size(); ++k)
       if (P->jobs[k].d == d) break;
     POOL_REQUIRE(k < P->jobs.size(), PASIF_ERR_ARGUMENT, "job not in this pool");
     auto& j = P->jobs[k];
+    // a job in the running tick waits for it; one the tick left out does not
+    POOL_REQUIRE(!P->in_flight || j.finished, PASIF_ERR_STATE, "wait for the tick first");
     POOL_REQUIRE(j.done >= j.work, PASIF_ERR_STATE, "job not finished");
     POOL_HIP(hipSetDevice(P->dev));
     // every tile must have reached its last phase

@@ -436,11 +436,22 @@ class SolveServer:
         admit_s = 0.0
         by_id = self.pool.ids is not None
 
+        pending = []                # finished after the last tick, not collected yet
+
         def arrivals_until(now):
             nonlocal i
             while i < n_req and arr[i] <= now:
                 queue.append(i)
                 i += 1
+
+        def collect_pending():
+            nonlocal blocked
+            for job, r in pending:
+                r.result, r.grad = self.pool.collect(job, r)
+                r.done = time.perf_counter() - t0
+                done.append(r)
+                blocked = False
+            pending.clear()
 
         def admit_one() -> bool:
             """Admit the oldest queued request; False if none or refused."""
@@ -470,12 +481,16 @@ class SolveServer:
                 break
             arrivals_until(now)
             if not self.inflight:
+                collect_pending()
                 while admit_one():
                     pass
             if self.inflight:
                 self.pool.tick(wait=False)
                 ticks += 1
-                # admit while the launch runs; new jobs join the next tick
+                # while the launch runs: collect what the previous tick
+                # finished (ticks skip finished jobs), then admit; new jobs
+                # join the next tick
+                collect_pending()
                 while self.pool.running():
                     arrivals_until(time.perf_counter() - t0)
                     if not admit_one():
@@ -483,24 +498,16 @@ class SolveServer:
                 busy_ms += self.pool.wait()
                 keep = []
                 for job, r in self.inflight:
-                    if self.pool.finished(job):
-                        r.result, r.grad = self.pool.collect(job, r)
-                        r.done = time.perf_counter() - t0
-                        done.append(r)
-                        blocked = False
-                    else:
-                        keep.append((job, r))
+                    (pending if self.pool.finished(job) else keep).append((job, r))
                 self.inflight = keep
-                # fill freed slots before the next launch
-                arrivals_until(time.perf_counter() - t0)
-                while admit_one():
-                    if not queue:
-                        break
+            elif pending:
+                collect_pending()
             elif i >= n_req and not queue:
                 break
             else:
                 nxt = arr[i] if i < n_req else now
                 time.sleep(max(0.0, min(0.01, nxt - now)))
+        collect_pending()
         wall = time.perf_counter() - t0
         lat = np.array([(r.done - r.arrival) * 1e3 for r in done]) if done else np.zeros(1)
         waiting = [(wall - r.arrival) * 1e3 for _, r in self.inflight] + \

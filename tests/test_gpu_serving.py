@@ -233,3 +233,37 @@ def test_lane_served_request_alone_equals_in_mix_and_oracle():
     oq, og = _oracle(target, bank, 1000, interp)
     assert rel_err(alone[0], oq) < TOL64
     assert rel_err(alone[1], og) < TOL64
+
+
+def test_server_results_equal_alone_runs():
+    """The server collects a finished job while the next tick runs (ticks
+    leave finished jobs out): every served result is bitwise the result of
+    its request run alone on a fresh pool."""
+    print("[gpu case] serving collect during tick", flush=True)
+    rng = np.random.default_rng(46)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16, 64), variants=2)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    from pasif_amd.workloads import linear_interp_matrix
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=150, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        return pool
+
+    reqs = [S.draw_request(lib, rng, i, 6, tmin=100, tmax=1500, max_systems=10)
+            for i in range(10)]
+    for i, r in enumerate(reqs):
+        r.arrival = 0.002 * i
+    rep = S.SolveServer(new_pool()).run(reqs, max_wall_s=120)
+    assert rep.completed == len(reqs)
+    for r in reqs:
+        pool = new_pool()
+        j = pool.submit_ids(r)
+        while not pool.finished(j):
+            pool.tick(wait=True)
+        q, g = pool.collect(j, r)
+        np.testing.assert_array_equal(r.result, q)
+        if r.adjoint:
+            np.testing.assert_array_equal(r.grad, g)
