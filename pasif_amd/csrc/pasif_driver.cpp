@@ -1006,27 +1006,57 @@ constexpr uint32_t kLaneMaxWaves = 16;
 constexpr uint32_t kLaneMaxWaveLive = 100;
 constexpr size_t kLaneMaxLds = 64 * 1024;
 
-// Structure of system s: its terms in coefficient order with local factor
-// rows, and (forward) the rows with a force above the working type's eps.
+// Terms of row r of system s with identical monomials merged: for the
+// forward side every factor is the state, so a term is identified by its
+// kind and its sorted factor rows (Gamma[o,j,k] and Gamma[o,k,j] are one
+// monomial); for the adjoint side the own factor (lambda) stays apart from
+// the set-point factors, which are sorted.  Each merged term lists the
+// entries (kind, entry index) whose coefficients it sums, in entry order.
+struct MergedTerm {
+  LaneTerm t;
+  std::vector<std::pair<int, uint32_t>> src;
+};
+void merged_row(const RowTerms& rt, uint32_t s, uint32_t r, bool adjoint,
+                std::vector<MergedTerm>& out) {
+  out.clear();
+  const uint32_t off = rt.off[s];
+  const uint32_t xo = adjoint ? rt.xoff[s] : off;
+  const uint32_t g = off + r;
+  std::map<std::vector<uint32_t>, size_t> where;
+  for (int k = 0; k < 4; ++k) {
+    const int nf = k + 1;
+    for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e) {
+      LaneTerm t{};
+      t.kind = uint32_t(k);
+      t.f[0] = rt.idx[k][size_t(e) * nf] - off;
+      for (int a = 1; a < nf; ++a) t.f[a] = rt.idx[k][size_t(e) * nf + a] - xo;
+      if (!adjoint) std::sort(t.f, t.f + nf);
+      else std::sort(t.f + 1, t.f + nf);
+      std::vector<uint32_t> key{t.kind};
+      for (int a = 0; a < nf; ++a) key.push_back(t.f[a]);
+      auto it = where.find(key);
+      if (it == where.end()) {
+        it = where.emplace(key, out.size()).first;
+        out.push_back({t, {}});
+      }
+      out[it->second].src.push_back({k, e});
+    }
+  }
+}
+
+// Structure of system s: its merged terms in coefficient order with local
+// factor rows, and (forward) the rows with a force above the working type's
+// eps.
 LaneSide lane_side(const RowTerms& rt, uint32_t s, bool adjoint, double eps) {
   LaneSide sd;
   const uint32_t off = rt.off[s], n = rt.size[s];
-  const uint32_t xo = adjoint ? rt.xoff[s] : off;
   sd.n = n;
   sd.rows.resize(n);
+  std::vector<MergedTerm> mt;
   for (uint32_t r = 0; r < n; ++r) {
-    const uint32_t g = off + r;
-    for (int k = 0; k < 4; ++k) {
-      const int nf = k + 1;
-      for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e) {
-        LaneTerm t{};
-        t.kind = uint32_t(k);
-        t.f[0] = rt.idx[k][size_t(e) * nf] - off;
-        for (int a = 1; a < nf; ++a) t.f[a] = rt.idx[k][size_t(e) * nf + a] - xo;
-        sd.rows[r].push_back(t);
-      }
-    }
-    if (!adjoint && std::fabs(rt.force[g]) > eps) sd.forced.push_back(r);
+    merged_row(rt, s, r, adjoint, mt);
+    for (const MergedTerm& m : mt) sd.rows[r].push_back(m.t);
+    if (!adjoint && std::fabs(rt.force[off + r]) > eps) sd.forced.push_back(r);
   }
   return sd;
 }
@@ -1044,15 +1074,23 @@ void shape_key(const LaneSide& sd, std::vector<uint32_t>& key) {
   for (uint32_t r : sd.forced) key.push_back(r);
 }
 
-// Coefficients of system s in the class's slot order (terms row by row,
-// kinds ascending, then the force of each forced row).
-void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, std::vector<double>& out) {
+// Coefficients of system s in the class's slot order (merged terms row by
+// row, kinds ascending, each the sum of its entries in entry order; then the
+// force of each forced row).
+void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint,
+                std::vector<double>& out) {
   out.clear();
   const uint32_t off = rt.off[s];
-  for (uint32_t r = 0; r < sd.n; ++r)
-    for (int k = 0; k < 4; ++k)
-      for (uint32_t e = rt.ptr[k][off + r]; e < rt.ptr[k][off + r + 1]; ++e)
-        out.push_back(rt.val[k][e]);
+  std::vector<MergedTerm> mt;
+  for (uint32_t r = 0; r < sd.n; ++r) {
+    merged_row(rt, s, r, adjoint, mt);
+    for (const MergedTerm& m : mt) {
+      double v = 0.0;
+      for (size_t q = 0; q < m.src.size(); ++q)
+        v = q == 0 ? rt.val[m.src[q].first][m.src[q].second] : v + rt.val[m.src[q].first][m.src[q].second];
+      out.push_back(v);
+    }
+  }
   for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
 }
 
@@ -1446,7 +1484,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
                     PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
       PASIF_REQUIRE(in.exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
                     "lane plan: excitation out of range");
-      lane_coefs(d->fwd, in.sys, sh.f, cv);
+      lane_coefs(d->fwd, in.sys, sh.f, false, cv);
       PASIF_REQUIRE(cv.size() == sh.f.coefs(), PASIF_ERR_UNSUPPORTED,
                     "lane plan: structure mismatch");
       if (grp) {
@@ -1462,7 +1500,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
         for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
       }
       if (adjoint) {
-        lane_coefs(d->adj, in.sys, sh.a, cv);
+        lane_coefs(d->adj, in.sys, sh.a, true, cv);
         PASIF_REQUIRE(cv.size() == sh.a.coefs(), PASIF_ERR_UNSUPPORTED,
                       "lane plan: structure mismatch");
         if (grp) {

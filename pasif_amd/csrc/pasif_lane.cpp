@@ -879,8 +879,14 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           S::xload(xc, xp, seg + (n - s0) * (NF * WAVE), seg + (n - 1u - s0) * (NF * WAVE), lane);
 #pragma unroll
           for (int j = 0; j < NX; ++j) xb[j] = R(0.5) * (xc[j] + xp[j]);
+          // effective coefficients per set point: x_n, the midpoint (stages
+          // 2 and 3), x_{n-1}
+          R e[S::NE > 0 ? S::NE : 1];
           rk4<NA, ALL>(lam, [&](R* k, const R* y, int s) {
-            S::rhs_a(k, y, s == 0 ? xc : (s == 3 ? xp : xb), c);
+            if (s == 0) S::eff_a(e, xc, c);
+            else if (s == 1) S::eff_a(e, xb, c);
+            else if (s == 3) S::eff_a(e, xp, c);
+            S::lin_a(k, y, e);
           }, h, h2, h6, n <= fsteps);
         }
       };
@@ -1389,6 +1395,105 @@ void emit_wave_side(std::ostringstream& o, const LaneSide& sd, const std::vector
   }
 }
 
+// The adjoint right-hand side is linear in the adjoint state: every row is
+// sum_o e_o(x) y_o over its own factors o, with e_o(x) the sum of its
+// terms' coefficients times their set-point factors.  eff_a evaluates the
+// e_o once per set point (x_n, the midpoint -- shared by the two middle
+// stages -- and x_{n-1}); lin_a combines them with the stage's values.
+struct ETerm {
+  uint32_t kind;
+  std::string own;
+  std::vector<std::string> xs;
+  uint32_t coef;
+};
+struct ERow {
+  std::vector<ETerm> terms;
+  bool shared = false;
+};
+void emit_adjoint_linear(std::ostringstream& oe, std::ostringstream& ol, const std::vector<ERow>& rows,
+                         uint32_t G, uint32_t* ne) {
+  uint32_t j = 0;
+  for (size_t r = 0; r < rows.size(); ++r) {
+    std::vector<std::string> owns;
+    std::vector<std::string> effs;
+    for (const ETerm& t : rows[r].terms) {
+      size_t o = 0;
+      while (o < owns.size() && owns[o] != t.own) ++o;
+      if (o == owns.size()) {
+        owns.push_back(t.own);
+        effs.push_back("");
+      }
+      const std::string c = "c[" + std::to_string(t.coef) + "]";
+      std::string& e = effs[o];
+      if (t.kind == 0) {
+        e = e.empty() ? c : "(" + e + " + " + c + ")";
+      } else {
+        std::string l = c;
+        for (uint32_t f = 0; f + 1 < t.kind; ++f) l = "(" + l + ") * " + t.xs[f];
+        const std::string last = t.xs[t.kind - 1];
+        e = e.empty() ? "(" + l + ") * " + last : "fmaR(" + l + ", " + last + ", " + e + ")";
+      }
+    }
+    std::string k;
+    for (size_t o = 0; o < owns.size(); ++o) {
+      oe << "    e[" << j << "] = " << effs[o] << ";\n";
+      const std::string ej = "e[" + std::to_string(j) + "]";
+      k = k.empty() ? ej + " * " + owns[o] : "fmaR(" + ej + ", " + owns[o] + ", " + k + ")";
+      ++j;
+    }
+    if (k.empty()) {
+      k = "R(0)";
+    } else if (rows[r].shared && G > 1) {
+      k = "gsum<" + std::to_string(G) + "u>(" + k + ")";
+    }
+    ol << "    k[" << r << "] = " << k << ";\n";
+  }
+  *ne = j;
+}
+
+// Adjoint rows of a lane-group side for emit_adjoint_linear.
+std::vector<ERow> group_adjoint_rows(const GSide& gs, const std::vector<GSrc>& gx) {
+  std::vector<ERow> rows(gs.nl());
+  uint32_t t = 0;
+  for (uint32_t r = 0; r < gs.nl(); ++r) {
+    const GRow& row = gs.rows[r];
+    rows[r].shared = r >= gs.RL;
+    for (size_t e = 0; e < row.terms.size(); ++e) {
+      const GTerm& tm = row.terms[e];
+      ETerm et;
+      et.kind = tm.kind;
+      et.coef = t + uint32_t(e);
+      const GSrc& o = tm.f[0];
+      et.own = (o.shared || o.d == 0) ? "y[" + std::to_string(o.slot) + "]"
+                                      : "y" + std::to_string(o.slot) + "_" + std::to_string(o.d);
+      for (uint32_t a = 1; a <= tm.kind; ++a) {
+        const size_t jx = size_t(std::lower_bound(gx.begin(), gx.end(), tm.f[a]) - gx.begin());
+        et.xs.push_back("x[" + std::to_string(jx) + "]");
+      }
+      rows[r].terms.push_back(et);
+    }
+    t += uint32_t(row.terms.size()) + (row.force.empty() ? 0u : 1u);
+  }
+  return rows;
+}
+
+// Adjoint rows of a one-lane side (coefficients in the side's slot order).
+std::vector<ERow> lane_adjoint_rows(const LaneSide& sd) {
+  std::vector<ERow> rows(sd.n);
+  uint32_t t = 0;
+  for (uint32_t r = 0; r < sd.n; ++r) {
+    for (const LaneTerm& tm : sd.rows[r]) {
+      ETerm et;
+      et.kind = tm.kind;
+      et.coef = t++;
+      et.own = "y[" + std::to_string(tm.f[0]) + "]";
+      for (uint32_t a = 1; a <= tm.kind; ++a) et.xs.push_back("x[" + std::to_string(tm.f[a]) + "]");
+      rows[r].terms.push_back(et);
+    }
+  }
+  return rows;
+}
+
 // Right-hand side of one side of a lane-group class: the own-state values
 // the program reads from other positions of the group (rotated once per
 // stage), then every lane slot: band slots directly, shared slots as
@@ -1441,10 +1546,11 @@ void emit_group_side(std::ostringstream& o, const GSide& gs, uint32_t G, bool fo
       }
       if (!acc.empty()) k = k.empty() ? acc : "(" + k + " + " + acc + ")";
     }
+    const bool empty = k.empty() && (!forward || row.force.empty());
     if (k.empty()) k = "R(0)";
     const uint32_t tf = t + uint32_t(row.terms.size());
     if (forward && !row.force.empty()) k = "fmaR(c[" + std::to_string(tf) + "], u, " + k + ")";
-    if (r >= gs.RL && G > 1) k = "gsum<" + std::to_string(G) + "u>(" + k + ")";
+    if (r >= gs.RL && G > 1 && !empty) k = "gsum<" + std::to_string(G) + "u>(" + k + ")";
     o << "    k[" << r << "] = " << k << ";\n";
     t = tf + (row.force.empty() ? 0u : 1u);
   }
@@ -1628,13 +1734,31 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
              "      xc[j] = sc[j * WAVE + lane];\n      xp[j] = sp[j * WAVE + lane];\n    }\n";
       }
       o << "  }\n";
-      o << "  static __device__ __forceinline__ void rhs_a(R* k, const R* y, const R* x, "
-           "const R* c) {\n";
+      std::ostringstream oe, ol;
+      uint32_t ne = 0;
       if (sh.adjoint) {
-        if (grp) emit_group_side(o, sh.ga, G, false, sh.gx);
-        else emit_rhs(o, sh.a, false, "y", "x");
+        const std::vector<ERow> rows = grp ? group_adjoint_rows(sh.ga, sh.gx) : lane_adjoint_rows(sh.a);
+        if (grp) {
+          // own-state values of other positions, rotated once per stage
+          std::vector<GSrc> rem;
+          for (const GRow& row : sh.ga.rows)
+            for (const GTerm& t : row.terms)
+              if (!t.f[0].shared && t.f[0].d != 0) rem.push_back(t.f[0]);
+          std::sort(rem.begin(), rem.end());
+          rem.erase(std::unique(rem.begin(), rem.end()), rem.end());
+          for (const GSrc& r : rem)
+            ol << "    const R y" << r.slot << "_" << r.d << " = rot<" << G << "u, " << r.d
+               << "u>(y[" << r.slot << "]);\n";
+        }
+        std::ostringstream ol2;
+        emit_adjoint_linear(oe, ol2, rows, G, &ne);
+        ol << ol2.str();
       }
-      o << "  }\n";
+      o << "  enum { NE = " << ne << " };\n";
+      o << "  static __device__ __forceinline__ void eff_a(R* e, const R* x, const R* c) {\n"
+        << oe.str() << "  }\n";
+      o << "  static __device__ __forceinline__ void lin_a(R* k, const R* y, const R* e) {\n"
+        << ol.str() << "  }\n";
     }
     o << "};\n";
     if (!mw) continue;

@@ -150,7 +150,14 @@ def test_chain_group_program_is_the_stencil():
               "rot<16u, 15u>(y[1])", "gsum<16u>("):
         assert r in fwd
     assert fwd.count("rot<16u,") == 4
-    assert "TF = 15," in fwd
+    # per lane: x row 1 term, v row 6 + 3 + 1 terms and the force, energy row
+    # 2 terms (v_p^2 and v_p v_{p+1}: the two orders of an off-diagonal
+    # energy entry are one monomial, merged)
+    assert "TF = 14," in fwd
+    # the adjoint is linear in lambda: effective coefficients per set point,
+    # one multiply-add per lambda factor and stage
+    adj = src[src.index("void eff_a"):]
+    assert "void lin_a" in adj and "TA = 18," in src
 
 
 def test_automatic_groups_widen_few_instances_only():
