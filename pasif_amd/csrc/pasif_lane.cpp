@@ -694,6 +694,10 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       R c[TF];
 #pragma unroll
       for (int t = 0; t < S::TF; ++t) c[t] = cf[t * st];
+      // the phase's loads land here: no wait for them inside the step loops,
+      // where the in-order load counter would also hold back the prefetched
+      // samples of the next chunk
+      __builtin_amdgcn_s_waitcnt(0);
       const u32 s0 = sg * K;
       if (recompute) {
         if (n == s0) {
@@ -870,6 +874,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           for (int r = 0; r < NF; ++r) seg[(m * NF + r) * WAVE + lane] = sb[((u64)m * NF + r) * st];
         seg_here = true;
       }
+      __builtin_amdgcn_s_waitcnt(0);
       const u32 nend = (n - s0 < budget) ? s0 : n - budget;
       budget -= n - nend;
       auto back_to = [&](auto all_t, u32 stop) {
