@@ -1226,7 +1226,7 @@ uint32_t single_wave_segs(std::vector<LaneShape>& shapes, const std::vector<Clas
     for (size_t c = 0; c < shapes.size(); ++c) {
       LaneShape& sh = shapes[c];
       const uint32_t G = sh.group > 1 ? sh.group : 1u;
-      const size_t stage = size_t(kLaneStage) * (kLaneWave / G) * el;
+      const size_t stage = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G) * el;
       const size_t step = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
       const size_t b = std::min<size_t>(kLaneMaxLds, std::max(budget, stage + 6 * step));
       const size_t fit = b > stage ? (b - stage) / step : 0;
@@ -2787,7 +2787,7 @@ void pool_build_modules(pasif_pool* P) {
         live = std::max(live, lane_live(sh));
         if (!sh.adjoint) continue;
         const uint32_t G = sh.group > 1 ? sh.group : 1u;
-        const size_t stage = size_t(kLaneStage) * (kLaneWave / G) * el;
+        const size_t stage = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G) * el;
         const size_t step = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
         const size_t b = std::min<size_t>(kLaneMaxLds, std::max<size_t>(kMaxLdsBytes / 8, stage + 6 * step));
         const size_t fit = b > stage ? (b - stage) / step : 0;

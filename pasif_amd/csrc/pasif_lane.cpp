@@ -607,14 +607,19 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr u32 G = S::G;
   constexpr u32 IPW = WAVE / G;  // instances per wave
   constexpr u32 NPF = (STAGE + G - 1u) / G;  // samples each lane of a group stages
+  // groups with interpolated forcing: the chunk's forcing values (3 per
+  // step) computed once by the group's lanes into a table
+  constexpr u32 FT = G > 1u ? 3u * CHUNK : 0u;
+  constexpr u32 NFT = (FT + G - 1u) / G;
   const u32 lane = threadIdx.x;
   const u32 p = lane & (G - 1u);
   const u64 st = a.stride;
   const u32 col = wave * WAVE + lane;
   const u32 i = wave * IPW + lane / G;
   const bool valid = i < (u32)a.count;
-  R* eb = smem + lane / G;  // the instance's staged samples
-  R* seg = smem + STAGE * IPW;
+  R* eb = smem + lane / G;                // the instance's staged samples
+  R* ft = smem + STAGE * IPW + lane / G;  // its forcing table (groups)
+  R* seg = smem + (STAGE + FT) * IPW;
   const bool adjoint = S::ADJ && a.adjoint != 0ull;
   const u32 K = adjoint ? (u32)a.ck_every : 1u;
   const u32 W = (u32)a.interp_w;
@@ -728,7 +733,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       u32 next_save = SE ? ((n + SE - 1u) / SE) * SE : 0u;
       // forcing cursor: half-step position 2n = hi*W + hj, staged chunk
       u32 hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
-      u32 ebase = 0, left = 0, pbase = 0;
+      u32 ebase = 0, left = 0, pbase = 0, nchunk = 0;
       bool have = false;
       R pre[NPF];
       auto steps_to = [&](auto all_t, auto gen_t, auto zoh_t, u32 stop) {
@@ -743,9 +748,16 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             u1 = u0;
             u3 = e1;
           } else {
-            u0 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 0u);
-            u1 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 1u);
-            u3 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 2u);
+            if constexpr (FT > 0u) {
+              const u32 m = 3u * (n - nchunk);
+              u0 = ft[m * IPW];
+              u1 = ft[(m + 1u) * IPW];
+              u3 = ft[(m + 2u) * IPW];
+            } else {
+              u0 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 0u);
+              u1 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 1u);
+              u3 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 2u);
+            }
             hj += 2u;
             if (hj >= W) {
               hj -= W;
@@ -805,6 +817,26 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             put_g<G, NPF>(eb, v, Lx, base, forced, p);
           }
           ebase = base;
+          nchunk = n;
+          if constexpr (FT > 0u) {
+            if (W != 0u) {
+              // value k of the table: step nchunk + k / 3, half step
+              // 2 (nchunk + k / 3) + (0, 1, 2)[k % 3] = hi W + hj + ...
+#pragma unroll
+              for (u32 v = 0; v < NFT; ++v) {
+                const u32 k = p + v * G;
+                if (k < FT) {
+                  const u32 m = k / 3u;
+                  const u32 hs = hj + 2u * m + (k - 3u * m);
+                  const u32 di = hs / W;
+                  ft[k * IPW] = interp_at<IPW>(eb, ebase, wts, W, Lx, hi + di, hs - di * W, 0u);
+                }
+              }
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            }
+          }
           left = CHUNK;
           pbase = W ? (2u * (n + CHUNK)) / W : n + CHUNK;
           fetch_g<G, NPF>(pre, E, Lx, pbase, p);
@@ -1859,7 +1891,7 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const Lane
   for (const LaneShape& s : shapes) {
     const uint32_t G = s.group > 1 ? s.group : 1u;
     const uint32_t K = s.seg ? s.seg : lay.seg;
-    size_t c = size_t(kLaneStage) * (kLaneWave / G);
+    size_t c = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G);
     if (s.adjoint) c += size_t(K + 1) * (G > 1 ? s.gf.nl() : s.f.n) * kLaneWave;
     v = std::max(v, c);
   }
