@@ -366,6 +366,16 @@ bool lane_group(LaneShape& sh, uint32_t G) {
     std::sort(sh.gx.begin(), sh.gx.end());
     sh.gx.erase(std::unique(sh.gx.begin(), sh.gx.end()), sh.gx.end());
   }
+  // forward shared slots no forward term and no adjoint set point reads
+  sh.gf.sink.assign(sh.gf.nl(), 0);
+  for (uint32_t r = sh.gf.RL; r < sh.gf.nl(); ++r) sh.gf.sink[r] = 1;
+  for (const GRow& row : sh.gf.rows)
+    for (const GTerm& t : row.terms)
+      for (uint32_t a = 0; a <= t.kind; ++a)
+        if (t.f[a].shared) sh.gf.sink[t.f[a].slot] = 0;
+  for (const GSrc& x : sh.gx)
+    if (x.shared) sh.gf.sink[x.slot] = 0;
+  sh.ga.sink.assign(sh.ga.nl(), 0);
   sh.group = G;
   return true;
 }
@@ -561,7 +571,14 @@ __device__ __forceinline__ R interp_at(const R* e, u32 b, GP(const R) wts, u32 W
 }
 
 // one RK4 step of N rows; rhs(k, y, stage).  ALL: every lane is active.
-template <int N, bool ALL, class F>
+// K::sink(r): row r is a shared row that no term reads; rhs leaves its
+// group-partial slope in k[r], which is accumulated per lane over the four
+// stages and summed across the group (K::G lanes) once per step.
+struct NoSink {
+  static constexpr unsigned G = 1;
+  static __device__ __forceinline__ constexpr bool sink(int) { return false; }
+};
+template <int N, bool ALL, class K, class F>
 __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
   R y[N], k[N], acc[N];
 #pragma unroll
@@ -573,15 +590,16 @@ __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
     for (int r = 0; r < N; ++r) {
       if (s == 0) {
         acc[r] = k[r];
-        y[r] = fmaR(h2, k[r], q[r]);
+        if (!K::sink(r)) y[r] = fmaR(h2, k[r], q[r]);
       } else if (s == 1) {
         acc[r] = fmaR(R(2), k[r], acc[r]);
-        y[r] = fmaR(h2, k[r], q[r]);
+        if (!K::sink(r)) y[r] = fmaR(h2, k[r], q[r]);
       } else if (s == 2) {
         acc[r] = fmaR(R(2), k[r], acc[r]);
-        y[r] = fmaR(h, k[r], q[r]);
+        if (!K::sink(r)) y[r] = fmaR(h, k[r], q[r]);
       } else {
         acc[r] = acc[r] + k[r];
+        if (K::sink(r)) acc[r] = gsum<K::G>(acc[r]);
         const R qn = fmaR(h6, acc[r], q[r]);
         q[r] = (ALL || active) ? qn : q[r];
       }
@@ -769,7 +787,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * WAVE] = q[r];
             next_ck += K;
           }
-          rk4<NF, ALL>(q, [&](R* k, const R* y, int s) {
+          rk4<NF, ALL, S>(q, [&](R* k, const R* y, int s) {
             S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
           }, h, h2, h6, n < fsteps);
           if (GEN) {
@@ -919,7 +937,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           // effective coefficients per set point: x_n, the midpoint (stages
           // 2 and 3), x_{n-1}
           R e[S::NE > 0 ? S::NE : 1];
-          rk4<NA, ALL>(lam, [&](R* k, const R* y, int s) {
+          rk4<NA, ALL, NoSink>(lam, [&](R* k, const R* y, int s) {
             if (s == 0) S::eff_a(e, xc, c);
             else if (s == 1) S::eff_a(e, xb, c);
             else if (s == 3) S::eff_a(e, xp, c);
@@ -1584,10 +1602,12 @@ void emit_group_side(std::ostringstream& o, const GSide& gs, uint32_t G, bool fo
       if (!acc.empty()) k = k.empty() ? acc : "(" + k + " + " + acc + ")";
     }
     const bool empty = k.empty() && (!forward || row.force.empty());
+    const bool sink = r < gs.sink.size() && gs.sink[r];
     if (k.empty()) k = "R(0)";
     const uint32_t tf = t + uint32_t(row.terms.size());
     if (forward && !row.force.empty()) k = "fmaR(c[" + std::to_string(tf) + "], u, " + k + ")";
-    if (r >= gs.RL && G > 1 && !empty) k = "gsum<" + std::to_string(G) + "u>(" + k + ")";
+    // a sink's partial slope is summed across the group once per step (rk4)
+    if (r >= gs.RL && G > 1 && !empty && !sink) k = "gsum<" + std::to_string(G) + "u>(" + k + ")";
     o << "    k[" << r << "] = " << k << ";\n";
     t = tf + (row.force.empty() ? 0u : 1u);
   }
@@ -1740,9 +1760,15 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
           << ternary_map(rf) << "; }\n";
         o << "  static __device__ __forceinline__ constexpr int rowa(int r) { return "
           << ternary_map(ra) << "; }\n";
+        std::string sk = "false";
+        for (uint32_t r = 0; r < sh.gf.sink.size(); ++r)
+          if (sh.gf.sink[r]) sk = "r == " + std::to_string(r) + " || " + sk;
+        o << "  static __device__ __forceinline__ constexpr bool sink(int r) { return " << sk
+          << "; }\n";
       } else {
         o << "  static __device__ __forceinline__ constexpr int rowf(int r) { return r; }\n";
         o << "  static __device__ __forceinline__ constexpr int rowa(int r) { return r; }\n";
+        o << "  static __device__ __forceinline__ constexpr bool sink(int) { return false; }\n";
       }
       o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* y, const R* c, R u) {\n";
       if (grp) emit_group_side(o, sh.gf, G, true, sh.gx);

@@ -116,6 +116,9 @@ struct GSide {
   std::vector<uint32_t> band;    // first row of each band slot
   std::vector<uint32_t> shared;  // row of each shared slot
   std::vector<GRow> rows;        // [RL + NS]
+  // shared slots no term reads (forward: nor the adjoint's set points): the
+  // group sums their slopes once per step instead of once per stage
+  std::vector<uint8_t> sink;     // [RL + NS]
   uint32_t nl() const { return RL + NS; }
   uint32_t coefs() const;        // per-lane coefficient slots: row by row, terms then force
 };
