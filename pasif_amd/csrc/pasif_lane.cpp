@@ -440,18 +440,19 @@ enum : u32 { kForward = 0, kRecompute = 1, kBackward = 2, kDone = 3 };
 template <bool B> struct Flag { static constexpr bool value = B; };
 
 // ------------------------------------------------------------ lane groups
-// A value moved between lanes by a DPP control (32-bit halves for doubles);
-// every lane is a valid source for the controls used here.
+// A value moved between lanes by a DPP control (32-bit halves for doubles).
+// A lane whose source falls outside its row reads 0 (bound_ctrl); no old
+// value is kept, so no register is initialised for it.
 template <int CTRL, class T>
 __device__ __forceinline__ T dpp(T x) {
   if constexpr (sizeof(T) == 8) {
     const u64 b = __builtin_bit_cast(u64, x);
-    const u32 lo = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)b, CTRL, 0xF, 0xF, false);
-    const u32 hi = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)(b >> 32), CTRL, 0xF, 0xF, false);
+    const u32 lo = (u32)__builtin_amdgcn_mov_dpp((int)(u32)b, CTRL, 0xF, 0xF, true);
+    const u32 hi = (u32)__builtin_amdgcn_mov_dpp((int)(u32)(b >> 32), CTRL, 0xF, 0xF, true);
     return __builtin_bit_cast(T, ((u64)hi << 32) | (u64)lo);
   } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF,
-                                                             0xF, false));
+    return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF,
+                                                          true));
   }
 }
 // through the LDS crossbar (no LDS storage): lane l reads lane src
