@@ -1473,7 +1473,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     std::vector<double> lv;
     // lane slot r of position p of a group side -> the system's row
     auto slot_row = [](const GSide& gs, uint32_t r, uint32_t p) {
-      return r < gs.RL ? gs.band[r] + p : gs.shared[r - gs.RL];
+      return r < gs.RL ? gs.band[r] + p * gs.stride[r] : gs.shared[r - gs.RL];
     };
     for (uint32_t i = 0; i < L->count; ++i) {
       const Instance& in = *mem[i];

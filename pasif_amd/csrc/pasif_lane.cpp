@@ -155,11 +155,17 @@ void plan_bands(const LaneSide& sd, uint32_t G, uint32_t lo, std::vector<uint32_
   }
 }
 
-// Bands at lo, lo + G, ... while they fit; the rest shared.
-void aligned_bands(uint32_t n, uint32_t G, uint32_t lo, std::vector<uint32_t>* band,
-                   std::vector<uint32_t>* shared) {
+// Blocks of G m rows from lo while they fit, position p holding rows
+// m p .. m p + m - 1 of each (m = 1: bands of G consecutive rows); the rest
+// shared.
+void blocked_bands(uint32_t n, uint32_t G, uint32_t lo, uint32_t m, std::vector<uint32_t>* band,
+                   std::vector<uint32_t>* stride, std::vector<uint32_t>* shared) {
   uint32_t g = lo;
-  for (; g + G <= n; g += G) band->push_back(g);
+  for (; g + G * m <= n; g += G * m)
+    for (uint32_t j = 0; j < m; ++j) {
+      band->push_back(g + j);
+      stride->push_back(m);
+    }
   for (; g < n; ++g) shared->push_back(g);
 }
 
@@ -176,8 +182,8 @@ GMap group_map(uint32_t n, uint32_t G, const GSide& gs) {
   m.shared.assign(n, false);
   for (uint32_t r = 0; r < gs.RL; ++r)
     for (uint32_t p = 0; p < G; ++p) {
-      m.slot[gs.band[r] + p] = r;
-      m.pos[gs.band[r] + p] = p;
+      m.slot[gs.band[r] + p * gs.stride[r]] = r;
+      m.pos[gs.band[r] + p * gs.stride[r]] = p;
     }
   for (uint32_t s = 0; s < gs.NS; ++s) {
     m.slot[gs.shared[s]] = gs.RL + s;
@@ -232,7 +238,7 @@ void build_gside(const LaneSide& sd, uint32_t G, const GMap& own, const GMap& se
     GRow& row = gs->rows[r];
     std::map<Key, size_t> where;
     for (uint32_t p = 0; p < G; ++p) {
-      const uint32_t g = gs->band[r] + p;
+      const uint32_t g = gs->band[r] + p * gs->stride[r];
       std::map<Key, uint32_t> occ;
       for (size_t e = 0; e < sd.rows[g].size(); ++e)
         add_term(row, where, occ, sd.rows[g][e], p, int32_t(base[g] + e));
@@ -308,30 +314,31 @@ double gside_stage_cost(const GSide& gs, uint32_t G, bool setpoints_own, int bit
 // the adjoint state mirror.
 bool best_gside(const LaneSide& sd, uint32_t G, const GSide* mirror, uint32_t nmirror,
                 const GMap* setp, int bits, GSide* out) {
-  std::vector<std::pair<std::vector<uint32_t>, std::vector<uint32_t>>> plans;
-  {
-    std::vector<uint32_t> b, sh;
-    plan_bands(sd, G, 0, &b, &sh);
-    plans.push_back({b, sh});
-  }
-  {
-    std::vector<uint32_t> b, sh;
-    aligned_bands(sd.n, G, 0, &b, &sh);
-    plans.push_back({b, sh});
-  }
-  if (mirror && nmirror <= sd.n) {
-    for (int k = 0; k < 2; ++k) {
-      std::vector<uint32_t> b = mirror->band, sh = mirror->shared;
-      if (k == 0) plan_bands(sd, G, nmirror, &b, &sh);
-      else aligned_bands(sd.n, G, nmirror, &b, &sh);
-      plans.push_back({b, sh});
+  struct Plan {
+    std::vector<uint32_t> band, stride, shared;
+  };
+  std::vector<Plan> plans;
+  // rows from lo: dynamic programming over consecutive bands, or aligned
+  // blocks of G m rows for m = 1, 2, 4, ...
+  auto from = [&](const Plan& head, uint32_t lo) {
+    Plan pl = head;
+    plan_bands(sd, G, lo, &pl.band, &pl.shared);
+    pl.stride.resize(pl.band.size(), 1u);
+    plans.push_back(pl);
+    for (uint32_t m = 1; lo + G * m <= sd.n; m *= 2) {
+      Plan q = head;
+      blocked_bands(sd.n, G, lo, m, &q.band, &q.stride, &q.shared);
+      plans.push_back(q);
     }
-  }
+  };
+  from(Plan(), 0);
+  if (mirror && nmirror <= sd.n) from(Plan{mirror->band, mirror->stride, mirror->shared}, nmirror);
   double best = -1;
   for (auto& pl : plans) {
     GSide gs;
-    gs.band = pl.first;
-    gs.shared = pl.second;
+    gs.band = pl.band;
+    gs.stride = pl.stride;
+    gs.shared = pl.shared;
     std::sort(gs.shared.begin(), gs.shared.end());
     gs.RL = uint32_t(gs.band.size());
     gs.NS = uint32_t(gs.shared.size());
@@ -679,10 +686,10 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   // output row of lane slot r: band slots hold row rowf(r) + p, a shared
   // slot's row is written by position 0 only (-1 elsewhere)
   auto orow_f = [&](int r) -> int {
-    return r < S::RLF ? S::rowf(r) + (int)p : (p == 0u ? S::rowf(r) : -1);
+    return r < S::RLF ? S::rowf(r) + (int)p * S::stridef(r) : (p == 0u ? S::rowf(r) : -1);
   };
   auto orow_a = [&](int r) -> int {
-    return r < S::RLA ? S::rowa(r) + (int)p : (p == 0u ? S::rowa(r) : -1);
+    return r < S::RLA ? S::rowa(r) + (int)p * S::stridea(r) : (p == 0u ? S::rowa(r) : -1);
   };
 
   R q[NF], mx[NF], lam[NA];
@@ -1761,6 +1768,10 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
           << ternary_map(rf) << "; }\n";
         o << "  static __device__ __forceinline__ constexpr int rowa(int r) { return "
           << ternary_map(ra) << "; }\n";
+        o << "  static __device__ __forceinline__ constexpr int stridef(int r) { return "
+          << ternary_map(sh.gf.stride) << "; }\n";
+        o << "  static __device__ __forceinline__ constexpr int stridea(int r) { return "
+          << ternary_map(sh.ga.stride) << "; }\n";
         std::string sk = "false";
         for (uint32_t r = 0; r < sh.gf.sink.size(); ++r)
           if (sh.gf.sink[r]) sk = "r == " + std::to_string(r) + " || " + sk;
@@ -1769,6 +1780,8 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
       } else {
         o << "  static __device__ __forceinline__ constexpr int rowf(int r) { return r; }\n";
         o << "  static __device__ __forceinline__ constexpr int rowa(int r) { return r; }\n";
+        o << "  static __device__ __forceinline__ constexpr int stridef(int) { return 1; }\n";
+        o << "  static __device__ __forceinline__ constexpr int stridea(int) { return 1; }\n";
         o << "  static __device__ __forceinline__ constexpr bool sink(int) { return false; }\n";
       }
       o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* y, const R* c, R u) {\n";

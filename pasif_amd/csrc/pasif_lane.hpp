@@ -85,8 +85,8 @@ struct LaneSide {
 };
 
 // Lane groups: G lanes (a power of two, aligned within the wave) integrate
-// one instance.  A side's rows are laid out in bands of G consecutive rows
-// (band slot r of position p holds row band[r] + p) and shared rows (every
+// one instance.  A side's rows are laid out in bands of G rows (band slot r
+// of position p holds row band[r] + p * stride[r]) and shared rows (every
 // lane of the group holds the same value).  The right-hand side of a band
 // slot is one program for all positions (the union of the positions' terms,
 // each lane with its own coefficients, zero where its row lacks the term);
@@ -113,7 +113,11 @@ struct GRow {
 };
 struct GSide {
   uint32_t RL = 0, NS = 0;
+  // band slot r of position p holds row band[r] + p * stride[r]: stride 1
+  // for G consecutive rows, m for the j-th slot of a block of G m rows in
+  // which position p holds rows m p .. m p + m - 1
   std::vector<uint32_t> band;    // first row of each band slot
+  std::vector<uint32_t> stride;  // row step between positions of each band slot
   std::vector<uint32_t> shared;  // row of each shared slot
   std::vector<GRow> rows;        // [RL + NS]
   // shared slots no term reads (forward: nor the adjoint's set points): the
