@@ -683,8 +683,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   u32 phase = uni(pst[1]);
   u32 sg = uni(pst[2]);
   u32 budget = (u32)a.budget;
-  // output row of lane slot r: band slots hold row rowf(r) + p, a shared
-  // slot's row is written by position 0 only (-1 elsewhere)
+  // output row of lane slot r: band slots hold row rowf(r) + p stridef(r), a
+  // shared slot's row is written by position 0 only (-1 elsewhere)
   auto orow_f = [&](int r) -> int {
     return r < S::RLF ? S::rowf(r) + (int)p * S::stridef(r) : (p == 0u ? S::rowf(r) : -1);
   };
