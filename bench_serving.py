@@ -95,10 +95,11 @@ def main():
         pool.collect(job, r)
         pool.close()
         rates.append(r.system_steps / wall)
-    print(json.dumps({"metric": "plain kernel per request", "requests": a.plain_sample,
-                      "system_steps_per_s_mean": float(np.mean(rates)),
-                      "note": "one launch per step (steps_per_tick=1), request alone"}),
-          flush=True)
+    if rates:
+        print(json.dumps({"metric": "plain kernel per request", "requests": a.plain_sample,
+                          "system_steps_per_s_mean": float(np.mean(rates)),
+                          "note": "one launch per step (steps_per_tick=1), request alone"}),
+              flush=True)
 
 
 if __name__ == "__main__":

@@ -160,8 +160,11 @@ struct LaneCls {
   // single-wave adjoint: checkpoint elements, each wave's block sized by its
   // longest instance (offsets in ckoff)
   uint64_t ck_elems = 0;
-  DevBuf coef_f, coef_a, init_f, init_a, steps, exc, out_f, out_a, ckoff;  // program
-  DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;                  // run
+  // program arrays: views into the part's one program buffer
+  const void *coef_f = nullptr, *coef_a = nullptr, *init_f = nullptr, *init_a = nullptr,
+             *steps = nullptr, *exc = nullptr, *out_f = nullptr, *out_a = nullptr,
+             *ckoff = nullptr;
+  DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;  // run
 };
 
 // One lane kernel's share of a packed problem: the instances of each of its
@@ -169,6 +172,7 @@ struct LaneCls {
 // driver's plan has one part; a pool job has one per pool lane module it uses.
 struct LanePart {
   std::vector<std::unique_ptr<LaneCls>> cls;
+  DevBuf prog;  // every class's program arrays, uploaded in one copy
   LaneKernel kern;
   LaneLayout lay;
   uint32_t blocks = 0, tmax = 0;
@@ -1010,19 +1014,16 @@ constexpr size_t kLaneMaxLds = 64 * 1024;
 // forward side every factor is the state, so a term is identified by its
 // kind and its sorted factor rows (Gamma[o,j,k] and Gamma[o,k,j] are one
 // monomial); for the adjoint side the own factor (lambda) stays apart from
-// the set-point factors, which are sorted.  Each merged term lists the
-// entries (kind, entry index) whose coefficients it sums, in entry order.
-struct MergedTerm {
-  LaneTerm t;
-  std::vector<std::pair<int, uint32_t>> src;
-};
+// the set-point factors, which are sorted.  `which[e]` is the merged term of
+// the row's e-th entry (entries in kind order, then entry order).  Rows have
+// few terms: a linear scan, no allocation beyond the reused outputs.
 void merged_row(const RowTerms& rt, uint32_t s, uint32_t r, bool adjoint,
-                std::vector<MergedTerm>& out) {
+                std::vector<LaneTerm>& out, std::vector<uint32_t>& which) {
   out.clear();
+  which.clear();
   const uint32_t off = rt.off[s];
   const uint32_t xo = adjoint ? rt.xoff[s] : off;
   const uint32_t g = off + r;
-  std::map<std::vector<uint32_t>, size_t> where;
   for (int k = 0; k < 4; ++k) {
     const int nf = k + 1;
     for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e) {
@@ -1032,14 +1033,16 @@ void merged_row(const RowTerms& rt, uint32_t s, uint32_t r, bool adjoint,
       for (int a = 1; a < nf; ++a) t.f[a] = rt.idx[k][size_t(e) * nf + a] - xo;
       if (!adjoint) std::sort(t.f, t.f + nf);
       else std::sort(t.f + 1, t.f + nf);
-      std::vector<uint32_t> key{t.kind};
-      for (int a = 0; a < nf; ++a) key.push_back(t.f[a]);
-      auto it = where.find(key);
-      if (it == where.end()) {
-        it = where.emplace(key, out.size()).first;
-        out.push_back({t, {}});
+      uint32_t m = 0;
+      for (; m < out.size(); ++m) {
+        const LaneTerm& o = out[m];
+        if (o.kind != t.kind) continue;
+        bool same = true;
+        for (int a = 0; a < nf && same; ++a) same = o.f[a] == t.f[a];
+        if (same) break;
       }
-      out[it->second].src.push_back({k, e});
+      if (m == out.size()) out.push_back(t);
+      which.push_back(m);
     }
   }
 }
@@ -1052,10 +1055,11 @@ LaneSide lane_side(const RowTerms& rt, uint32_t s, bool adjoint, double eps) {
   const uint32_t off = rt.off[s], n = rt.size[s];
   sd.n = n;
   sd.rows.resize(n);
-  std::vector<MergedTerm> mt;
+  std::vector<LaneTerm> mt;
+  std::vector<uint32_t> which;
   for (uint32_t r = 0; r < n; ++r) {
-    merged_row(rt, s, r, adjoint, mt);
-    for (const MergedTerm& m : mt) sd.rows[r].push_back(m.t);
+    merged_row(rt, s, r, adjoint, mt, which);
+    sd.rows[r] = mt;
     if (!adjoint && std::fabs(rt.force[off + r]) > eps) sd.forced.push_back(r);
   }
   return sd;
@@ -1081,15 +1085,26 @@ void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint
                 std::vector<double>& out) {
   out.clear();
   const uint32_t off = rt.off[s];
-  std::vector<MergedTerm> mt;
+  std::vector<LaneTerm> mt;
+  std::vector<uint32_t> which;
+  std::vector<double> v;
   for (uint32_t r = 0; r < sd.n; ++r) {
-    merged_row(rt, s, r, adjoint, mt);
-    for (const MergedTerm& m : mt) {
-      double v = 0.0;
-      for (size_t q = 0; q < m.src.size(); ++q)
-        v = q == 0 ? rt.val[m.src[q].first][m.src[q].second] : v + rt.val[m.src[q].first][m.src[q].second];
-      out.push_back(v);
-    }
+    merged_row(rt, s, r, adjoint, mt, which);
+    v.assign(mt.size(), 0.0);
+    // merged terms appear in the order of their first entries
+    uint32_t q = 0, fresh = 0;
+    const uint32_t g = off + r;
+    for (int k = 0; k < 4; ++k)
+      for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e, ++q) {
+        const uint32_t m = which[q];
+        if (m == fresh) {
+          v[m] = rt.val[k][e];
+          ++fresh;
+        } else {
+          v[m] += rt.val[k][e];
+        }
+      }
+    out.insert(out.end(), v.begin(), v.end());
   }
   for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
 }
@@ -1433,6 +1448,16 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
   P->lay = lay;
   uint32_t block = 0, tmax = 0;
   std::vector<double> cv;
+  // the part's program arrays, 256-byte aligned in one host buffer: one
+  // upload for the whole part; (class, field, offset) patched in after it
+  std::vector<unsigned char> host;
+  std::vector<std::pair<const void**, size_t>> views;
+  auto stage = [&](const void** where, const void* data, size_t bytes) {
+    const size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + std::max<size_t>(bytes, 1));
+    if (bytes) std::memcpy(host.data() + off, data, bytes);
+    views.push_back({where, off});
+  };
   const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
   const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
   for (size_t c = 0; c < shapes.size(); ++c) {
@@ -1536,18 +1561,21 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
         tot += uint64_t(std::max<uint32_t>(1, (Tw + L->seg - 1) / L->seg)) * L->nf * kLaneWave;
       }
       L->ck_elems = tot;
-      upload(d, L->ckoff, off);
+      stage(&L->ckoff, off.data(), off.size() * sizeof(uint64_t));
     }
-    upload(d, L->coef_f, cf);
-    upload(d, L->coef_a, ca);
-    upload(d, L->init_f, qf);
-    upload(d, L->init_a, qa);
-    upload(d, L->steps, stp);
-    upload(d, L->exc, ex);
-    upload(d, L->out_f, of);
-    upload(d, L->out_a, oa);
+    stage(&L->coef_f, cf.data(), cf.size() * sizeof(R));
+    stage(&L->coef_a, ca.data(), ca.size() * sizeof(R));
+    stage(&L->init_f, qf.data(), qf.size() * sizeof(R));
+    stage(&L->init_a, qa.data(), qa.size() * sizeof(R));
+    stage(&L->steps, stp.data(), stp.size() * sizeof(uint32_t));
+    stage(&L->exc, ex.data(), ex.size() * sizeof(int32_t));
+    stage(&L->out_f, of.data(), of.size() * sizeof(uint32_t));
+    stage(&L->out_a, oa.data(), oa.size() * sizeof(uint32_t));
     P->cls.push_back(std::move(L));
   }
+  upload(d, P->prog, host);
+  for (auto& v : views)
+    *v.first = P->prog.p ? static_cast<const unsigned char*>(P->prog.p) + v.second : nullptr;
   if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
   P->blocks = block;
   P->tmax = tmax;
@@ -1880,20 +1908,20 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
     }
     LaneArgs& a = args[c];
     std::memset(&a, 0, sizeof a);
-    a.coef_f = L.coef_f.p;
-    a.coef_a = L.coef_a.p;
-    a.init_f = L.init_f.p;
-    a.init_a = L.init_a.p;
-    a.steps = L.steps.p;
-    a.exc = L.exc.p;
-    a.out_f = L.out_f.p;
-    a.out_a = L.out_a.p;
+    a.coef_f = L.coef_f;
+    a.coef_a = L.coef_a;
+    a.init_f = L.init_f;
+    a.init_a = L.init_a;
+    a.steps = L.steps;
+    a.exc = L.exc;
+    a.out_f = L.out_f;
+    a.out_a = L.out_a;
     a.qstate = L.qstate.p;
     a.lstate = L.lstate.p;
     a.mxstate = L.mxstate.p;
     a.progress = L.progress.p;
     a.ckpt = L.ckpt.p;
-    a.ck_off = L.ckoff.p;
+    a.ck_off = L.ckoff;
     a.segbuf = L.segbuf.p;
     a.out_q = d->out_q.p;
     a.out_max = d->window ? d->out_max.p : nullptr;
@@ -2516,13 +2544,14 @@ double job_bytes(const pasif_driver* d) {
                          &d->d_interp};
   double b = double(d->pk.program_bytes) + double(d->pk.hdr.bytes);
   for (const DevBuf* x : run) b += double(x->bytes);
-  for (const auto& lp : d->pk.lparts)
+  for (const auto& lp : d->pk.lparts) {
+    b += double(lp->prog.bytes);
     for (const auto& L : lp->cls) {
-      const DevBuf* lb[] = {&L->coef_f, &L->coef_a, &L->init_f, &L->init_a, &L->steps,
-                            &L->exc, &L->out_f, &L->out_a, &L->ckoff, &L->qstate,
-                            &L->lstate, &L->mxstate, &L->progress, &L->ckpt, &L->segbuf};
+      const DevBuf* lb[] = {&L->qstate, &L->lstate, &L->mxstate, &L->progress, &L->ckpt,
+                            &L->segbuf};
       for (const DevBuf* x : lb) b += double(x->bytes);
     }
+  }
   return b;
 }
 

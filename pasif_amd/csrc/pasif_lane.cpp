@@ -356,7 +356,62 @@ bool best_gside(const LaneSide& sd, uint32_t G, const GSide* mirror, uint32_t nm
 
 }  // namespace
 
+namespace {
+// Group layouts already planned in this process, by structure and group
+// size (planning builds several candidate programs per size).
+struct GroupPlan {
+  bool ok = false;
+  GSide gf, ga;
+  std::vector<GSrc> gx;
+};
+std::mutex g_plan_mu;
+std::map<std::string, GroupPlan> g_plans;
+
+void key_side(const LaneSide& sd, std::string& k) {
+  auto put = [&](uint32_t v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
+  put(sd.n);
+  for (const auto& row : sd.rows) {
+    put(uint32_t(row.size()));
+    for (const LaneTerm& t : row) {
+      put(t.kind);
+      for (uint32_t a = 0; a <= t.kind; ++a) put(t.f[a]);
+    }
+  }
+  put(uint32_t(sd.forced.size()));
+  for (uint32_t r : sd.forced) put(r);
+}
+bool plan_group(LaneShape& sh, uint32_t G);
+}  // namespace
+
 bool lane_group(LaneShape& sh, uint32_t G) {
+  if (G <= 1) return plan_group(sh, G);
+  std::string key(reinterpret_cast<const char*>(&G), sizeof G);
+  key.push_back(sh.adjoint ? 'a' : 'f');
+  key_side(sh.f, key);
+  if (sh.adjoint) key_side(sh.a, key);
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+      sh.gf = it->second.gf;
+      sh.ga = it->second.ga;
+      sh.gx = it->second.gx;
+      sh.group = it->second.ok ? G : 1u;
+      return it->second.ok;
+    }
+  }
+  GroupPlan gp;
+  gp.ok = plan_group(sh, G);
+  gp.gf = sh.gf;
+  gp.ga = sh.ga;
+  gp.gx = sh.gx;
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plans.emplace(std::move(key), std::move(gp));
+  return sh.group == G;
+}
+
+namespace {
+bool plan_group(LaneShape& sh, uint32_t G) {
   sh.gf = GSide();
   sh.ga = GSide();
   sh.gx.clear();
@@ -386,6 +441,7 @@ bool lane_group(LaneShape& sh, uint32_t G) {
   sh.group = G;
   return true;
 }
+}  // namespace
 
 uint32_t lane_live(const LaneShape& sh) {
   if (sh.group <= 1) return sh.live_values();
