@@ -38,7 +38,7 @@ def main():
     ap.add_argument("--tmin", type=float, default=1e3)
     ap.add_argument("--tmax", type=float, default=1e5)
     ap.add_argument("--max-systems", type=int, default=256)
-    ap.add_argument("--steps-per-tick", type=int, default=2000)
+    ap.add_argument("--steps-per-tick", type=int, default=5000)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--lanes", type=int, default=512)
     ap.add_argument("--slots", type=int, default=8192,
