@@ -202,7 +202,8 @@ GSrc src_of(const GMap& m, uint32_t row, uint32_t p, uint32_t G) {
 
 // Union programs of a side laid out as gs.band / gs.shared; `own` maps the
 // side's rows, `setp` the rows its set-point factors index.
-void build_gside(const LaneSide& sd, uint32_t G, const GMap& own, const GMap& setp, GSide* gs) {
+void build_gside(const LaneSide& sd, uint32_t G, const GMap& own, const GMap& setp, bool forward,
+                 GSide* gs) {
   std::vector<uint32_t> base(sd.n);
   std::vector<int32_t> force(sd.n, -1);
   uint32_t t = 0;
@@ -218,8 +219,12 @@ void build_gside(const LaneSide& sd, uint32_t G, const GMap& own, const GMap& se
     GTerm gt;
     gt.kind = tm.kind;
     Key k{tm.kind};
+    for (uint32_t a = 0; a <= tm.kind; ++a) gt.f[a] = src_of(a == 0 ? own : setp, tm.f[a], p, G);
+    // one monomial, one slot: the factors in a canonical order relative to
+    // the lane (all of them on the forward side, where every factor is the
+    // state; the set-point factors on the adjoint side)
+    std::sort(gt.f + (forward ? 0 : 1), gt.f + tm.kind + 1);
     for (uint32_t a = 0; a <= tm.kind; ++a) {
-      gt.f[a] = src_of(a == 0 ? own : setp, tm.f[a], p, G);
       k.push_back(gt.f[a].shared ? 1u : 0u);
       k.push_back(gt.f[a].slot);
       k.push_back(gt.f[a].d);
@@ -344,7 +349,7 @@ bool best_gside(const LaneSide& sd, uint32_t G, const GSide* mirror, uint32_t nm
     gs.NS = uint32_t(gs.shared.size());
     if (gs.RL == 0) continue;
     const GMap own = group_map(sd.n, G, gs);
-    build_gside(sd, G, own, setp ? *setp : own, &gs);
+    build_gside(sd, G, own, setp ? *setp : own, setp == nullptr, &gs);
     const double c = gside_stage_cost(gs, G, setp == nullptr, bits);
     if (best < 0 || c < best) {
       best = c;
