@@ -1480,6 +1480,25 @@ void emit_row(std::ostringstream& o, const LaneSide& sd, const Slots& sl, uint32
   o << "    k[" << kidx << "] = " << k << ";\n";
 }
 
+// Products of state factors emitted once per right-hand side evaluation
+// (`m<k>` temporaries, keyed by the factor names in order), so terms sharing a
+// monomial -- x0^2 in a Gamma and a Lambda term -- share its multiplies.
+struct Monos {
+  std::map<std::string, std::string> names;
+  std::string get(std::ostringstream& o, const std::vector<std::string>& fs, size_t n) {
+    if (n == 1) return fs[0];
+    std::string key;
+    for (size_t i = 0; i < n; ++i) key += fs[i] + ",";
+    auto it = names.find(key);
+    if (it != names.end()) return it->second;
+    const std::string left = get(o, fs, n - 1);
+    const std::string name = "m" + std::to_string(names.size());
+    o << "    const R " << name << " = " << left << " * " << fs[n - 1] << ";\n";
+    names.emplace(key, name);
+    return name;
+  }
+};
+
 // Single-wave right-hand side: every row, coefficients c[global slot].
 void emit_rhs(std::ostringstream& o, const LaneSide& sd, bool forced_input, const char* S,
               const char* X) {
@@ -1490,6 +1509,37 @@ void emit_rhs(std::ostringstream& o, const LaneSide& sd, bool forced_input, cons
              [](uint32_t gs) { return "c[" + std::to_string(gs) + "]"; },
              [&](uint32_t j) { return sn + "[" + std::to_string(j) + "]"; },
              [&](uint32_t j) { return xn + "[" + std::to_string(j) + "]"; });
+}
+
+// Single-wave forward right-hand side with shared monomials: each kind
+// summed in term order (coefficient times the cached product of its state
+// factors), kinds added in order, then the force.
+void emit_rhs_forward(std::ostringstream& o, const LaneSide& sd) {
+  const Slots sl = slots_of(sd);
+  Monos monos;
+  auto y = [](uint32_t j) { return "y[" + std::to_string(j) + "]"; };
+  for (uint32_t r = 0; r < sd.n; ++r) {
+    std::string k;
+    for (int kind = 0; kind < 4; ++kind) {
+      std::string acc;
+      uint32_t slot = sl.base[r];
+      for (const LaneTerm& tm : sd.rows[r]) {
+        const std::string coef = "c[" + std::to_string(slot++) + "]";
+        if (int(tm.kind) != kind) continue;
+        std::vector<std::string> fs;
+        for (uint32_t a = 0; a <= tm.kind; ++a) fs.push_back(y(tm.f[a]));
+        const std::string m = monos.get(o, fs, fs.size());
+        const std::string prod = acc.empty() ? coef + " * " + m : "fmaR(" + coef + ", " + m + ", " + acc + ")";
+        const std::string name = "s" + std::to_string(kind) + "_" + std::to_string(r);
+        o << "    " << (acc.empty() ? "R " : "") << name << " = " << prod << ";\n";
+        acc = name;
+      }
+      if (!acc.empty()) k = k.empty() ? acc : "(" + k + " + " + acc + ")";
+    }
+    if (k.empty()) k = "R(0)";
+    if (sl.force[r] >= 0) k = "fmaR(c[" + std::to_string(sl.force[r]) + "], u, " + k + ")";
+    o << "    k[" << r << "] = " << k << ";\n";
+  }
 }
 
 // One wave's rows of a side in a multi-wave class: its own coefficient
@@ -1644,6 +1694,7 @@ void emit_group_side(std::ostringstream& o, const GSide& gs, uint32_t G, bool fo
     const size_t j = size_t(std::lower_bound(gx.begin(), gx.end(), r) - gx.begin());
     return "x[" + std::to_string(j) + "]";
   };
+  Monos monos;
   uint32_t t = 0;
   for (uint32_t r = 0; r < gs.nl(); ++r) {
     const GRow& row = gs.rows[r];
@@ -1658,6 +1709,11 @@ void emit_group_side(std::ostringstream& o, const GSide& gs, uint32_t G, bool fo
         if (kind == 0) {
           prod = acc.empty() ? coef + " * " + S(tm.f[0])
                              : "fmaR(" + coef + ", " + S(tm.f[0]) + ", " + acc + ")";
+        } else if (forward) {
+          std::vector<std::string> fs{S(tm.f[0])};
+          for (int f = 1; f <= kind; ++f) fs.push_back(X(tm.f[f]));
+          const std::string m = monos.get(o, fs, fs.size());
+          prod = acc.empty() ? coef + " * " + m : "fmaR(" + coef + ", " + m + ", " + acc + ")";
         } else {
           std::string l = coef + " * " + S(tm.f[0]);
           for (int f = 1; f < kind; ++f) l = "(" + l + ") * " + X(tm.f[f]);
@@ -1847,7 +1903,7 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
       }
       o << "  static __device__ __forceinline__ void rhs_f(R* k, const R* y, const R* c, R u) {\n";
       if (grp) emit_group_side(o, sh.gf, G, true, sh.gx);
-      else emit_rhs(o, sh.f, true, "y", "y");
+      else emit_rhs_forward(o, sh.f);
       o << "  }\n";
       // set points of the reverse step: forward values read from the
       // segment in LDS (another position's column for rotated sources)
