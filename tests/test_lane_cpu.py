@@ -2,6 +2,7 @@
 checkpoint shortening, and hipRTC compilation of the generated source for
 gfx950 (hipRTC needs no device)."""
 import os
+import re
 
 import numpy as np
 import pytest
@@ -173,3 +174,22 @@ def test_automatic_groups_widen_few_instances_only():
     assert many.kernelInfo()["lanes_per_instance"] == [1]
     with pytest.raises(_lib.PasifError):
         few.setKernel("lane", 3)
+
+
+def test_blocked_layout_for_long_chains():
+    # 64-DOF chain on 16 lanes: blocks of 64 rows, lane p holding DOFs
+    # 4p .. 4p+3 (band stride 4), so neighbours across lanes are one rotation
+    # away in either direction and no lane pads for another's seam
+    w = W.chains(8, 64, 60, steps_per_sample=10)
+    p = host_solver(w)
+    p.setKernel("lane", 16)
+    assert p.kernelInfo()["lanes_per_instance"] == [16]
+    src = p._laneSource(False)
+    strides = src[src.index("int stridef(int r) { return"):].split(";")[0]
+    assert "? 4" in strides and "? 1" not in strides
+    fwd = src[src.index("void rhs_f"):src.index("void xload")]
+    rots = set(re.findall(r"rot<16u, (\d+)u>", fwd))
+    assert rots == {"1", "15"}, rots
+    # 4 x rows of 1 term, 4 v rows of the chain stencil, the energy row
+    assert int(re.search(r"TF = (\d+),", src).group(1)) <= 60
+    p._laneSource(False, compile=True)
