@@ -1,6 +1,7 @@
 """Autotuner of the tile kernel: tile lanes (systems per workgroup), slot
 configuration (segment width per term kind) and, for the adjoint, the
-checkpoint interval.
+checkpoint interval (also the cap on the lane kernel's interval; a problem
+entirely on the lane kernel probes only that).
 
 Each candidate runs the loaded workload with every system's step count
 capped at ``probe_steps`` and is timed with the library's HIP events; the
@@ -74,6 +75,11 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
         p.setSystemExcitations(exc, cap)
         cks = list(checkpoints) if checkpoints is not None else (list(CHECKPOINTS) if adjoint
                                                                  else [0])
+        # every system on the lane kernel: tile shapes change nothing, only the
+        # checkpoint interval (a cap on the lane kernel's own) is probed
+        lane_only = p.kernelInfo(adjoint)["kernel"] == "lane"
+        if lane_only:
+            lanes, slots = list(lanes)[:1], list(slots)[:1]
         trials = []
         try:
             for ln in lanes:
@@ -96,7 +102,7 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
             raise RuntimeError("no tile configuration fits this problem")
         ms, ln, sl, ck = min(trials)
         best = {"tile_lanes": ln, "slot_config": sl, "checkpoint_every": ck, "probe_ms": ms,
-                "probe_steps": probe_steps, "trials": len(trials)}
+                "probe_steps": probe_steps, "trials": len(trials), "lane_only": lane_only}
         if use_cache:
             cache[key] = best
             _save_cache(cache)
