@@ -171,6 +171,11 @@ int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* class
  * them busy or a system does not fit one lane's registers).  A power of two
  * from 1 to 64 requests that group size for every class it fits. */
 int pasif_set_lane_group(pasif_driver* d, uint32_t lanes);
+/* Moves every class's group size `bias` steps along the sizes that fit it
+ * (clamped), from the cost model's choice: +1 doubles the lanes per instance
+ * where possible, -1 halves them.  Ignored when pasif_set_lane_group fixes a
+ * size.  The autotuner probes -1, 0 and +1. (extension) */
+int pasif_set_lane_bias(pasif_driver* d, int bias);
 /* lanes per instance of each lane-kernel class of the packed problem
  * (out == NULL: count query) */
 int pasif_lane_groups(pasif_driver* d, int adjoint, uint32_t* out, size_t cap, size_t* len);

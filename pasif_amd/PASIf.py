@@ -402,14 +402,17 @@ class PASIf(_GpuDriver):
     def setTuning(self, tile_lanes: int = 0, slot_config: int = 0, checkpoint_every: int = 0):
         self._setTuning(tile_lanes, slot_config, checkpoint_every)
 
-    def setKernel(self, kernel: str = "auto", lanes_per_system: int = 0):
+    def setKernel(self, kernel: str = "auto", lanes_per_system: int = 0, group_bias: int = 0):
         """"tile" (any system size), "lane" (one lane -- or a group of lanes
         exchanging values by lane rotations -- per system, state in
         registers) or "auto" (lane kernel for the structures that fit it,
         tiles for the rest).  ``lanes_per_system`` 0 lets a cost model pick
-        the group size per structure; a power of two up to 64 requests it."""
+        the group size per structure (``group_bias`` moves every choice that
+        many steps along the sizes that fit; the autotuner probes it); a
+        power of two up to 64 requests that size."""
         self._setKernel(kernel)
         self._setLaneGroup(lanes_per_system)
+        self._setLaneBias(group_bias)
 
     def kernelInfo(self, adjoint: bool = False) -> dict:
         """Kernel the loaded problem runs on, its structure classes and the

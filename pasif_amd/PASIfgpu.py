@@ -240,6 +240,11 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
         class, else a power of two up to 64 for every class it fits."""
         self._chk(self._pl.pasif_set_lane_group(self._h, int(lanes)))
 
+    def _setLaneBias(self, bias: int = 0):
+        """Steps along each class's feasible group sizes from the cost
+        model's choice (+1: twice the lanes per instance where possible)."""
+        self._chk(self._pl.pasif_set_lane_bias(self._h, int(bias)))
+
     def _laneGroups(self, adjoint: bool = False) -> list:
         n = C.c_size_t(0)
         self._chk(self._pl.pasif_lane_groups(self._h, int(adjoint), None, 0, C.byref(n)))

@@ -67,6 +67,7 @@ _SIGS = {
     "pasif_last_timing": (C.c_int, [_h, _f64p, _u32p, _u32p, _u32p]),
     "pasif_set_kernel": (C.c_int, [_h, C.c_int]),
     "pasif_set_lane_group": (C.c_int, [_h, C.c_uint32]),
+    "pasif_set_lane_bias": (C.c_int, [_h, C.c_int]),
     "pasif_lane_groups": (C.c_int, [_h, C.c_int, _u32p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pasif_kernel_info": (C.c_int, [_h, C.c_int, C.POINTER(C.c_int), _u32p, _u32p]),
     "pasif_lane_source": (C.c_int, [_h, C.c_int, C.c_char_p, C.c_size_t,

@@ -1,7 +1,8 @@
-"""Autotuner of the tile kernel: tile lanes (systems per workgroup), slot
-configuration (segment width per term kind) and, for the adjoint, the
+"""Autotuner: tile lanes (systems per workgroup) and slot configuration
+(segment width per term kind) of the tile kernel, for the adjoint the
 checkpoint interval (also the cap on the lane kernel's interval; a problem
-entirely on the lane kernel probes only that).
+entirely on the lane kernel probes only that), and the lane kernel's group
+sizes (the cost model's choice, or every class one size narrower or wider).
 
 Each candidate runs the loaded workload with every system's step count
 capped at ``probe_steps`` and is timed with the library's HIP events; the
@@ -60,8 +61,9 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
              lanes: Iterable[int] = LANES, slots: Iterable[int] = SLOTS,
              checkpoints: Optional[Iterable[int]] = None, use_cache: bool = True,
              apply: bool = True, verbose: bool = False) -> dict:
-    """Pick the fastest (tile lanes, slot configuration[, checkpoint]) for the
-    problem loaded in PASIf instance ``p``; apply it unless ``apply=False``."""
+    """Pick the fastest (tile lanes, slot configuration[, checkpoint], lane
+    group bias) for the problem loaded in PASIf instance ``p``; apply it
+    unless ``apply=False``."""
     key = signature(p, adjoint)
     cache = _load_cache() if use_cache else {}
     if key in cache:
@@ -77,7 +79,8 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
                                                                  else [0])
         # every system on the lane kernel: tile shapes change nothing, only the
         # checkpoint interval (a cap on the lane kernel's own) is probed
-        lane_only = p.kernelInfo(adjoint)["kernel"] == "lane"
+        kernel = p.kernelInfo(adjoint)["kernel"]
+        lane_only = kernel == "lane"
         if lane_only:
             lanes, slots = list(lanes)[:1], list(slots)[:1]
         trials = []
@@ -96,16 +99,39 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
                         trials.append((ms, ln, sl, ck))
                         if verbose:
                             print(f"[autotune] lanes {ln} slots {sl} ck {ck}: {ms:.3f} ms")
+            if not trials:
+                raise RuntimeError("no tile configuration fits this problem")
+            ms, ln, sl, ck = min(trials)
+            # the lane kernel's group sizes: the cost model's choice, and every
+            # class one step narrower / wider where that fits
+            bias = 0
+            if kernel in ("lane", "mixed"):
+                p.setTuning(ln, sl, ck)
+                for b in (1, -1):
+                    try:
+                        p._setLaneBias(b)
+                        (p.getGradient if adjoint else p.getAmplitudes)()
+                        mb = p._lastTiming()["kernel_ms"]
+                    except Exception as e:
+                        if verbose:
+                            print(f"[autotune] lane bias {b}: {e}")
+                        continue
+                    finally:
+                        p._setLaneBias(0)
+                    if verbose:
+                        print(f"[autotune] lane bias {b}: {mb:.3f} ms")
+                    trials.append((mb, ln, sl, ck))
+                    if mb < 0.98 * ms:       # a clear gain over the cost model's choice
+                        ms, bias = mb, b
         finally:
             p.setSystemExcitations(exc, steps_full)
-        if not trials:
-            raise RuntimeError("no tile configuration fits this problem")
-        ms, ln, sl, ck = min(trials)
         best = {"tile_lanes": ln, "slot_config": sl, "checkpoint_every": ck, "probe_ms": ms,
-                "probe_steps": probe_steps, "trials": len(trials), "lane_only": lane_only}
+                "probe_steps": probe_steps, "trials": len(trials), "lane_only": lane_only,
+                "lane_bias": bias}
         if use_cache:
             cache[key] = best
             _save_cache(cache)
     if apply:
         p.setTuning(best["tile_lanes"], best["slot_config"], best["checkpoint_every"])
+        p._setLaneBias(best.get("lane_bias", 0))
     return best

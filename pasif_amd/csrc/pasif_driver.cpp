@@ -248,6 +248,9 @@ struct pasif_driver {
   int kernel_path = PASIF_KERNEL_AUTO;
   // lanes per instance requested for lane-kernel classes (0: chosen per class)
   uint32_t lane_group = 0;
+  // steps along each class's feasible group sizes from the cost model's
+  // choice (the autotuner probes -1, 0, +1)
+  int lane_bias = 0;
   // lane class of every system in the packed problem, -1: tile kernel
   std::vector<int32_t> lane_cls;
   // second stream and fork / join events when lane and tile kernels run together
@@ -987,10 +990,21 @@ void pack_impl(pasif_driver* d, bool adjoint) {
            pk.program_bytes / 1048576.0);
 }
 
+// 0 for an automatic group size, log2(lanes) + 1 otherwise (3 bits)
+uint32_t lane_group_code(uint32_t lanes) {
+  uint32_t c = 0;
+  while (lanes) {
+    ++c;
+    lanes >>= 1;
+  }
+  return c;
+}
+
 uint64_t pack_key(pasif_driver* d, bool adjoint) {
   return (uint64_t(d->precision) << 1) ^ (uint64_t(adjoint) << 8) ^ (uint64_t(d->tile_lanes) << 10) ^
          (uint64_t(d->slot_cfg) << 24) ^ (uint64_t(d->kernel_path) << 28) ^
-         (uint64_t(d->ck_every) << 32) ^ (uint64_t(d->lane_group) << 56);
+         (uint64_t(d->ck_every) << 32) ^ (uint64_t(lane_group_code(d->lane_group)) << 56) ^
+         (uint64_t(uint32_t(d->lane_bias + 8) & 15u) << 59);
 }
 
 // ------------------------------------------------------------ lane plan
@@ -1134,7 +1148,7 @@ struct ClassLoad {
 // class with the longest chain is widened while that shortens the estimate.
 // `want` > 0 requests that many lanes for every class where it fits.
 void choose_groups(std::vector<LaneShape>& shapes, const std::vector<ClassLoad>& load, int bits,
-                   double simds, uint32_t want) {
+                   double simds, uint32_t want, int bias) {
   const size_t C = shapes.size();
   std::vector<std::vector<std::pair<uint32_t, double>>> opt(C);
   std::vector<size_t> sel(C, 0);
@@ -1185,8 +1199,12 @@ void choose_groups(std::vector<LaneShape>& shapes, const std::vector<ClassLoad>&
     if (estimate(t, nullptr) >= 0.99 * est) break;
     sel = t;
   }
-  for (size_t c = 0; c < C; ++c)
-    if (!opt[c].empty()) (void)lane_group(shapes[c], opt[c][sel[c]].first);
+  for (size_t c = 0; c < C; ++c) {
+    if (opt[c].empty()) continue;
+    // the bias moves each class along its feasible sizes (clamped)
+    const long k = std::min<long>(long(opt[c].size()) - 1, std::max<long>(0, long(sel[c]) + (want ? 0 : bias)));
+    (void)lane_group(shapes[c], opt[c][size_t(k)].first);
+  }
 }
 
 // SIMDs of the device (4 per CU), 1024 when planning without one
@@ -1389,7 +1407,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
         }
       }
     } else {
-      choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group);
+      choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group, d->lane_bias);
       if (adjoint) lay.seg = single_wave_segs(shapes, sel_load, el, Kreq, d->dev);
       lay.occ = single_wave_occupancy(shapes, sel_load, bits, d->dev);
     }
@@ -2433,6 +2451,13 @@ int pasif_set_lane_group(pasif_driver* d, uint32_t lanes) {
                   "lanes per instance must be 0 (automatic) or a power of two up to %u, got %u",
                   kLaneWave, lanes);
     d->lane_group = lanes;
+  });
+}
+
+int pasif_set_lane_bias(pasif_driver* d, int bias) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(bias >= -6 && bias <= 6, PASIF_ERR_ARGUMENT, "lane bias %d outside [-6, 6]", bias);
+    d->lane_bias = bias;
   });
 }
 

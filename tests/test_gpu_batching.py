@@ -45,8 +45,10 @@ def test_autotuned_configuration_keeps_parity(tmp_path, monkeypatch):
     w = W.chains(n_systems=64, d=16, steps=500, steps_per_sample=10)
     p = W.make_solver(w)
     # on the lane kernel (the default here) tile shapes change nothing
+    # (only the group sizes are probed: the model's, one size narrower, wider)
     lane = AT.autotune(p, probe_steps=100, use_cache=False)
-    assert lane["lane_only"] and lane["trials"] == 1
+    assert lane["lane_only"] and 1 <= lane["trials"] <= 3 and lane["lane_bias"] in (-1, 0, 1)
+    assert rel_err(p.getAmplitudes(), oracle_run(w).q) < TOL64
     p.setKernel("tile")
     best = AT.autotune(p, probe_steps=100)
     assert best["trials"] >= 3 and not best["lane_only"]
