@@ -506,6 +506,11 @@ __device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfir
 __device__ __forceinline__ u32 umin(u32 a, u32 b) { return a < b ? a : b; }
 enum : u32 { kForward = 0, kRecompute = 1, kBackward = 2, kDone = 3 };
 template <bool B> struct Flag { static constexpr bool value = B; };
+// per-step bookkeeping of a forward step loop: none, checkpoints (forward
+// pass of an adjoint run), the recomputed segment, or trajectory / window /
+// per-step outputs (with checkpoints when the run also has an adjoint)
+enum : u32 { mPlain = 0, mCk = 1, mRec = 2, mFull = 3 };
+template <u32 M> struct Mode { static constexpr u32 value = M; };
 
 // ------------------------------------------------------------ lane groups
 // A value moved between lanes by a DPP control (32-bit halves for doubles).
@@ -813,9 +818,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       const u32 nend = (e - n < budget) ? e : n + budget;
       budget -= nend - n;
       const u32 nfirst = n;
-      // per-step bookkeeping (checkpoints, segment, trajectory, window,
-      // per-instance output) only when a run needs it
-      const bool gen = recompute || adjoint || SE != 0u || a.window != 0ull;
+      // per-step bookkeeping only as far as the phase needs it
+      const u32 mode = recompute ? mRec : ((SE != 0u || a.window != 0ull) ? mFull : (adjoint ? mCk : mPlain));
       u32 next_ck = ((n + K - 1u) / K) * K;
       u32 next_save = SE ? ((n + SE - 1u) / SE) * SE : 0u;
       // forcing cursor: half-step position 2n = hi*W + hj, staged chunk
@@ -823,9 +827,9 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       u32 ebase = 0, left = 0, pbase = 0, nchunk = 0;
       bool have = false;
       R pre[NPF];
-      auto steps_to = [&](auto all_t, auto gen_t, auto zoh_t, u32 stop) {
+      auto steps_to = [&](auto all_t, auto mode_t, auto zoh_t, u32 stop) {
         constexpr bool ALL = decltype(all_t)::value;
-        constexpr bool GEN = decltype(gen_t)::value;
+        constexpr u32 MODE = decltype(mode_t)::value;
         constexpr bool ZOH = decltype(zoh_t)::value;
         for (; n < stop; ++n) {
           R u0, u1, u3;
@@ -851,7 +855,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
               hi += 1u;
             }
           }
-          if (GEN && adjoint && !recompute && n == next_ck) {
+          if ((MODE == mCk || MODE == mFull) && adjoint && !recompute && n == next_ck) {
 #pragma unroll
             for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * WAVE] = q[r];
             next_ck += K;
@@ -859,15 +863,16 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           rk4<NF, ALL, S>(q, [&](R* k, const R* y, int s) {
             S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
           }, h, h2, h6, n < fsteps);
-          if (GEN) {
-            if (recompute) {
-              const u32 m = n + 1u - s0;
+          if constexpr (MODE == mRec) {
+            const u32 m = n + 1u - s0;
 #pragma unroll
-              for (int r = 0; r < NF; ++r) {
-                seg[(m * NF + r) * WAVE + lane] = q[r];
-                if (mirror) sb[((u64)m * NF + r) * st] = q[r];
-              }
-            } else {
+            for (int r = 0; r < NF; ++r) {
+              seg[(m * NF + r) * WAVE + lane] = q[r];
+              if (mirror) sb[((u64)m * NF + r) * st] = q[r];
+            }
+          }
+          if constexpr (MODE == mFull) {
+            {
               if (SE && n == next_save) {
                 const u32 slot = n / SE;
                 if (slot < (u32)a.nslots && valid)
@@ -933,27 +938,23 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         if (n < Tmin && Tmin < stop) stop = Tmin;
         left -= stop - n;
         const bool all = n < Tmin;
-        if (gen) {
+        auto in_mode = [&](auto mode_t) {
           if (W == 0u) {
-            if (all) steps_to(Flag<true>(), Flag<true>(), Flag<true>(), stop);
-            else steps_to(Flag<false>(), Flag<true>(), Flag<true>(), stop);
+            if (all) steps_to(Flag<true>(), mode_t, Flag<true>(), stop);
+            else steps_to(Flag<false>(), mode_t, Flag<true>(), stop);
           } else {
-            if (all) steps_to(Flag<true>(), Flag<true>(), Flag<false>(), stop);
-            else steps_to(Flag<false>(), Flag<true>(), Flag<false>(), stop);
+            if (all) steps_to(Flag<true>(), mode_t, Flag<false>(), stop);
+            else steps_to(Flag<false>(), mode_t, Flag<false>(), stop);
           }
-        } else {
-          if (W == 0u) {
-            if (all) steps_to(Flag<true>(), Flag<false>(), Flag<true>(), stop);
-            else steps_to(Flag<false>(), Flag<false>(), Flag<true>(), stop);
-          } else {
-            if (all) steps_to(Flag<true>(), Flag<false>(), Flag<false>(), stop);
-            else steps_to(Flag<false>(), Flag<false>(), Flag<false>(), stop);
-          }
-        }
+        };
+        if (mode == mPlain) in_mode(Mode<mPlain>());
+        else if (mode == mCk) in_mode(Mode<mCk>());
+        else if (mode == mRec) in_mode(Mode<mRec>());
+        else in_mode(Mode<mFull>());
       }
-      // without per-step bookkeeping: an instance whose last step fell in
-      // this range kept its final state since (inactive steps leave it)
-      if (!gen && valid && fsteps > nfirst && fsteps <= nend)
+      // without per-step outputs: an instance whose last step fell in this
+      // range kept its final state since (inactive steps leave it)
+      if ((mode == mPlain || mode == mCk) && valid && fsteps > nfirst && fsteps <= nend)
 #pragma unroll
         for (int r = 0; r < NF; ++r)
           if (orow_f(r) >= 0) outq[fout + orow_f(r)] = q[r];
