@@ -606,7 +606,8 @@ __device__ __forceinline__ void put(R* e, const R (&v)[STAGE], u32 Lx, u32 b, bo
 // Samples of a group's chunk: position p of a group of G lanes loads
 // e[b + u] for u = p, p + G, ... < STAGE and stores them into the group's
 // column of the stage [STAGE][WAVE / G] (`e` points at it); the group then
-// reads its samples there.
+// reads its samples there.  Under zero-order hold (zoh) sample 0 is staged
+// as 0: it only ever drives step 0's first stages, which take no forcing.
 template <u32 G, u32 NPF>
 __device__ __forceinline__ void fetch_g(R (&v)[NPF], GP(const R) E, u32 Lx, u32 b, u32 p) {
 #pragma unroll
@@ -616,11 +617,13 @@ __device__ __forceinline__ void fetch_g(R (&v)[NPF], GP(const R) E, u32 Lx, u32 
   }
 }
 template <u32 G, u32 NPF>
-__device__ __forceinline__ void put_g(R* e, const R (&v)[NPF], u32 Lx, u32 b, bool forced, u32 p) {
+__device__ __forceinline__ void put_g(R* e, const R (&v)[NPF], u32 Lx, u32 b, bool forced, u32 p,
+                                      bool zoh) {
 #pragma unroll
   for (u32 k = 0; k < NPF; ++k) {
     const u32 u = p + k * G;
-    if (u < STAGE) e[u * (WAVE / G)] = (forced && b + u < Lx) ? v[k] : R(0);
+    if (u < STAGE)
+      e[u * (WAVE / G)] = (forced && b + u < Lx && !(zoh && b + u == 0u)) ? v[k] : R(0);
   }
   if constexpr (G > 1u) {
     // the other lanes of the group read these samples next
@@ -641,7 +644,11 @@ __device__ __forceinline__ R interp_at(const R* e, u32 b, GP(const R) wts, u32 W
   const u32 i = i0 + (next ? 1u : 0u);
   const R ei = e[(i - b) * SS];
   const R ei1 = e[(i + 1u - b) * SS];
-  return (i > 0u && i < Lx) ? fmaR(wts[W + j], ei, wts[j] * ei1) : R(0);
+  // samples past the end are staged as 0; i = 0 is masked by a product with
+  // a uniform 0 or 1 (a select here turns into a branch around the loads,
+  // at which a lone wave waits for them at once)
+  const R keep = i > 0u ? R(1) : R(0);
+  return fmaR(wts[W + j], ei, wts[j] * ei1) * keep;
 }
 
 // one RK4 step of N rows; rhs(k, y, stage).  ALL: every lane is active.
@@ -834,10 +841,10 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         for (; n < stop; ++n) {
           R u0, u1, u3;
           if (ZOH) {
-            const R e0 = eb[(n - ebase) * IPW], e1 = eb[(n + 1u - ebase) * IPW];
-            u0 = (n > 0u && n < Lx) ? e0 : R(0);
+            // 0 < n < L or 0: sample 0 and those past the end are staged as 0
+            u0 = eb[(n - ebase) * IPW];
             u1 = u0;
-            u3 = e1;
+            u3 = eb[(n + 1u - ebase) * IPW];
           } else {
             if constexpr (FT > 0u) {
               const u32 m = 3u * (n - nchunk);
@@ -902,11 +909,11 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           // now), and the next chunk's are requested
           const u32 base = W ? hi : n;
           if (have && pbase == base) {
-            put_g<G, NPF>(eb, pre, Lx, base, forced, p);
+            put_g<G, NPF>(eb, pre, Lx, base, forced, p, W == 0u);
           } else {
             R v[NPF];
             fetch_g<G, NPF>(v, E, Lx, base, p);
-            put_g<G, NPF>(eb, v, Lx, base, forced, p);
+            put_g<G, NPF>(eb, v, Lx, base, forced, p, W == 0u);
           }
           ebase = base;
           nchunk = n;
