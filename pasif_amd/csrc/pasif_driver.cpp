@@ -1260,14 +1260,17 @@ uint32_t single_wave_segs(std::vector<LaneShape>& shapes, const std::vector<Clas
       LaneShape& sh = shapes[c];
       const uint32_t G = sh.group > 1 ? sh.group : 1u;
       const size_t stage = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G) * el;
-      const size_t step = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
+      // a step of the segment in LDS: the rows the reverse step reads; a
+      // checkpoint: the whole forward state
+      const size_t step = size_t(lane_seg_rows(sh)) * kLaneWave * el;
+      const size_t ckstep = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
       const size_t b = std::min<size_t>(kLaneMaxLds, std::max(budget, stage + 6 * step));
       const size_t fit = b > stage ? (b - stage) / step : 0;
       sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
       kmin = std::min(kmin, sh.seg);
       // a checkpoint of every wave's lanes every seg steps of its longest instance
       bytes += (load[c].steps_sum * G / double(kLaneWave) / sh.seg + load[c].inst * G / kLaneWave) *
-               double(step);
+               double(ckstep);
     }
     if (std::getenv("PASIF_LANE_DEBUG"))
       fprintf(stderr, "[pasif] LDS share %zu B: checkpoints %.3g GB (cap %.3g GB), shortest interval %u\n",
@@ -2842,7 +2845,7 @@ void pool_build_modules(pasif_pool* P) {
         if (!sh.adjoint) continue;
         const uint32_t G = sh.group > 1 ? sh.group : 1u;
         const size_t stage = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G) * el;
-        const size_t step = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
+        const size_t step = size_t(lane_seg_rows(sh)) * kLaneWave * el;
         const size_t b = std::min<size_t>(kLaneMaxLds, std::max<size_t>(kMaxLdsBytes / 8, stage + 6 * step));
         const size_t fit = b > stage ? (b - stage) / step : 0;
         sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));

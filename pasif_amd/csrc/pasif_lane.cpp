@@ -448,6 +448,26 @@ bool plan_group(LaneShape& sh, uint32_t G) {
 }
 }  // namespace
 
+uint32_t lane_seg_rows(const LaneShape& sh, std::vector<int>* map) {
+  const bool grp = sh.group > 1;
+  const uint32_t nf = grp ? sh.gf.nl() : sh.f.n;
+  std::vector<int> m(nf, -1);
+  if (sh.adjoint) {
+    if (grp) {
+      for (const GSrc& x : sh.gx) m[x.slot] = 0;
+    } else {
+      for (const auto& row : sh.a.rows)
+        for (const LaneTerm& t : row)
+          for (uint32_t a = 1; a <= t.kind; ++a) m[t.f[a]] = 0;
+    }
+  }
+  uint32_t rows = 0;
+  for (int& v : m)
+    if (v == 0) v = int(rows++);
+  if (map) *map = m;
+  return std::max(1u, rows);
+}
+
 uint32_t lane_live(const LaneShape& sh) {
   if (sh.group <= 1) return sh.live_values();
   const uint32_t fwd = 4 * sh.gf.nl() + sh.gf.coefs() + remote_count(sh.gf, true);
@@ -703,6 +723,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr int TF = S::TF > 0 ? S::TF : 1;
   constexpr int TA = S::TA > 0 ? S::TA : 1;
   constexpr int NX = S::NX > 0 ? S::NX : 1;
+  constexpr u32 NSEG = S::NSEG;  // rows per step of the recomputed segment
   constexpr u32 G = S::G;
   constexpr u32 IPW = WAVE / G;  // instances per wave
   constexpr u32 NPF = (STAGE + G - 1u) / G;  // samples each lane of a group stages
@@ -808,7 +829,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
 #pragma unroll
           for (int r = 0; r < NF; ++r) {
             q[r] = ck[((u64)sg * NF + r) * WAVE];
-            seg[r * WAVE + lane] = q[r];
+            if (S::segr(r) >= 0) seg[S::segr(r) * WAVE + lane] = q[r];
             if (mirror) sb[r * st] = q[r];
           }
         } else if (!seg_here) {
@@ -817,7 +838,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           for (u32 m = 0; m <= n - s0; ++m)
 #pragma unroll
             for (int r = 0; r < NF; ++r)
-              seg[(m * NF + r) * WAVE + lane] = sb[((u64)m * NF + r) * st];
+              if (S::segr(r) >= 0) seg[(m * NSEG + S::segr(r)) * WAVE + lane] = sb[((u64)m * NF + r) * st];
         }
         seg_here = true;
       }
@@ -874,7 +895,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             const u32 m = n + 1u - s0;
 #pragma unroll
             for (int r = 0; r < NF; ++r) {
-              seg[(m * NF + r) * WAVE + lane] = q[r];
+              if (S::segr(r) >= 0) seg[(m * NSEG + S::segr(r)) * WAVE + lane] = q[r];
               if (mirror) sb[((u64)m * NF + r) * st] = q[r];
             }
           }
@@ -998,7 +1019,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         // resumed in the reverse phase: reload the segment this launch needs
         for (u32 m = 0; m <= n - s0; ++m)
 #pragma unroll
-          for (int r = 0; r < NF; ++r) seg[(m * NF + r) * WAVE + lane] = sb[((u64)m * NF + r) * st];
+          for (int r = 0; r < NF; ++r)
+            if (S::segr(r) >= 0) seg[(m * NSEG + S::segr(r)) * WAVE + lane] = sb[((u64)m * NF + r) * st];
         seg_here = true;
       }
       __builtin_amdgcn_s_waitcnt(0);
@@ -1008,7 +1030,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         constexpr bool ALL = decltype(all_t)::value;
         for (; n > stop; --n) {
           R xc[NX], xb[NX], xp[NX];
-          S::xload(xc, xp, seg + (n - s0) * (NF * WAVE), seg + (n - 1u - s0) * (NF * WAVE), lane);
+          S::xload(xc, xp, seg + (n - s0) * (NSEG * WAVE), seg + (n - 1u - s0) * (NSEG * WAVE), lane);
 #pragma unroll
           for (int j = 0; j < NX; ++j) xb[j] = R(0.5) * (xc[j] + xp[j]);
           // effective coefficients per set point: x_n, the midpoint (stages
@@ -1760,6 +1782,13 @@ std::string ternary_map(const std::vector<uint32_t>& v) {
   return e;
 }
 
+std::string ternary_map_i(const std::vector<int>& v) {
+  std::string e = "-1";
+  for (size_t i = v.size(); i-- > 0;)
+    e = "r == " + std::to_string(i) + " ? " + std::to_string(v[i]) + " : (" + e + ")";
+  return e;
+}
+
 uint64_t fnv1a(const std::string& s) {
   uint64_t h = 1469598103934665603ull;
   for (unsigned char ch : s) {
@@ -1881,8 +1910,13 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
       << ", TA = " << (sh.adjoint ? (grp ? sh.ga.coefs() : sh.a.coefs()) : 0u)
       << ", ADJ = " << (sh.adjoint ? 1 : 0) << ", G = " << G
       << ", RLF = " << (grp ? sh.gf.RL : nf) << ", RLA = " << (grp ? sh.ga.RL : na)
-      << ", NX = " << (grp ? uint32_t(sh.gx.size()) : nf) << " };\n";
+      << ", NX = " << (grp ? uint32_t(sh.gx.size()) : nf);
+    std::vector<int> segmap;
+    if (!mw) o << ", NSEG = " << lane_seg_rows(sh, &segmap);
+    o << " };\n";
     if (!mw) {
+      o << "  static __device__ __forceinline__ constexpr int segr(int r) { return "
+        << ternary_map_i(segmap) << "; }\n";
       std::vector<uint32_t> rf, ra;
       if (grp) {
         rf = sh.gf.band;
@@ -1928,12 +1962,20 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
         for (size_t j = 0; j < sh.gx.size(); ++j) {
           const GSrc& x = sh.gx[j];
           const std::string col = (x.shared || x.d == 0) ? "lane" : "l" + std::to_string(x.d);
-          o << "    xc[" << j << "] = sc[" << x.slot << "u * WAVE + " << col << "];\n";
-          o << "    xp[" << j << "] = sp[" << x.slot << "u * WAVE + " << col << "];\n";
+          const int r = segmap[x.slot];
+          o << "    xc[" << j << "] = sc[" << r << "u * WAVE + " << col << "];\n";
+          o << "    xp[" << j << "] = sp[" << r << "u * WAVE + " << col << "];\n";
         }
       } else if (sh.adjoint) {
-        o << "#pragma unroll\n    for (int j = 0; j < NF; ++j) {\n"
-             "      xc[j] = sc[j * WAVE + lane];\n      xp[j] = sp[j * WAVE + lane];\n    }\n";
+        // rows the reverse step does not read are not in the segment
+        for (uint32_t j = 0; j < nf; ++j) {
+          if (segmap[j] < 0) {
+            o << "    xc[" << j << "] = R(0);\n    xp[" << j << "] = R(0);\n";
+          } else {
+            o << "    xc[" << j << "] = sc[" << segmap[j] << "u * WAVE + lane];\n";
+            o << "    xp[" << j << "] = sp[" << segmap[j] << "u * WAVE + lane];\n";
+          }
+        }
       }
       o << "  }\n";
       std::ostringstream oe, ol;
@@ -2057,7 +2099,7 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const Lane
     const uint32_t G = s.group > 1 ? s.group : 1u;
     const uint32_t K = s.seg ? s.seg : lay.seg;
     size_t c = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G);
-    if (s.adjoint) c += size_t(K + 1) * (G > 1 ? s.gf.nl() : s.f.n) * kLaneWave;
+    if (s.adjoint) c += size_t(K + 1) * lane_seg_rows(s) * kLaneWave;
     v = std::max(v, c);
   }
   return v * el;

@@ -150,6 +150,10 @@ struct LaneShape {
 bool lane_group(LaneShape& sh, uint32_t G);
 // values (of the working type) one lane keeps live in the larger phase
 uint32_t lane_live(const LaneShape& sh);
+// adjoint runs: the forward lane slots the reverse step reads as set points,
+// which are all the recomputed segment in LDS holds.  map[r] = the slot's row
+// in the segment or -1; returns the segment's rows per step (at least 1).
+uint32_t lane_seg_rows(const LaneShape& sh, std::vector<int>* map = nullptr);
 // estimated vector instructions per step of one wave: forward step, and
 // forward + recompute + reverse step of an adjoint run
 double lane_step_cost(const LaneShape& sh, int bits);

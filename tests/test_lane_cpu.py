@@ -81,9 +81,12 @@ def test_checkpoint_interval_shortened_to_fit_lds():
     p.setJacobian(*ADJ.energy_gradient_inputs(w).as_args())
     p.setTuning(checkpoint_every=256)
     k = p.kernelInfo(adjoint=True)["checkpoint_every"]
-    nf = 5
+    # the segment holds the rows the reverse step reads: x0, x1, v0, v1 (not
+    # the energy row E, which no adjoint term reads)
+    nseg = 4
     assert 1 <= k < 256
-    assert (19 * 64 + (k + 1) * nf * 64) * 8 <= 64 * 1024
+    assert (19 * 64 + (k + 1) * nseg * 64) * 8 <= 64 * 1024
+    assert (19 * 64 + (k + 2) * nseg * 64) * 8 > 64 * 1024
     p.setTuning(checkpoint_every=4)
     assert p.kernelInfo(adjoint=True)["checkpoint_every"] == 4
 
