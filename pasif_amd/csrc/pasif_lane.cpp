@@ -819,16 +819,27 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       R c[TF];
 #pragma unroll
       for (int t = 0; t < S::TF; ++t) c[t] = cf[t * st];
+      const u32 s0 = sg * K;
+      // the checkpoint a recompute starts from and the samples of the
+      // phase's first chunk are requested with the coefficients
+      if (recompute && n == s0)
+#pragma unroll
+        for (int r = 0; r < NF; ++r) q[r] = ck[((u64)sg * NF + r) * WAVE];
+      // forcing cursor: half-step position 2n = hi*W + hj, staged chunk
+      u32 hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
+      u32 ebase = 0, left = 0, nchunk = 0;
+      u32 pbase = W ? hi : n;
+      R pre[NPF];
+      fetch_g<G, NPF>(pre, E, Lx, pbase, p);
+      bool have = true;
       // the phase's loads land here: no wait for them inside the step loops,
       // where the in-order load counter would also hold back the prefetched
       // samples of the next chunk
       __builtin_amdgcn_s_waitcnt(0);
-      const u32 s0 = sg * K;
       if (recompute) {
         if (n == s0) {
 #pragma unroll
           for (int r = 0; r < NF; ++r) {
-            q[r] = ck[((u64)sg * NF + r) * WAVE];
             if (S::segr(r) >= 0) seg[S::segr(r) * WAVE + lane] = q[r];
             if (mirror) sb[r * st] = q[r];
           }
@@ -850,11 +861,6 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       const u32 mode = recompute ? mRec : ((SE != 0u || a.window != 0ull) ? mFull : (adjoint ? mCk : mPlain));
       u32 next_ck = ((n + K - 1u) / K) * K;
       u32 next_save = SE ? ((n + SE - 1u) / SE) * SE : 0u;
-      // forcing cursor: half-step position 2n = hi*W + hj, staged chunk
-      u32 hi = W ? (2u * n) / W : 0u, hj = W ? 2u * n - hi * W : 0u;
-      u32 ebase = 0, left = 0, pbase = 0, nchunk = 0;
-      bool have = false;
-      R pre[NPF];
       auto steps_to = [&](auto all_t, auto mode_t, auto zoh_t, u32 stop) {
         constexpr bool ALL = decltype(all_t)::value;
         constexpr u32 MODE = decltype(mode_t)::value;
