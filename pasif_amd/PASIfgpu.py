@@ -208,6 +208,11 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
 
     def _setPrecision(self, bits: int):
         self._chk(self._pl.pasif_set_precision(self._h, int(bits)))
+        self._bits = int(bits)
+
+    def _precision(self) -> int:
+        """Working precision in bits (64 unless set)."""
+        return getattr(self, "_bits", 64)
 
     def _setSolver(self, solver: str):
         code = {"persistent": _lib.PASIF_SOLVER_PERSISTENT, "plain": _lib.PASIF_SOLVER_PLAIN,

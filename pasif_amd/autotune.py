@@ -5,11 +5,14 @@ entirely on the lane kernel probes only that), and the lane kernel's group
 sizes (the cost model's choice, or every class one size narrower or wider).
 
 Each candidate runs the loaded workload with every system's step count
-capped at ``probe_steps`` and is timed with the library's HIP events; the
-fastest wins.  Results depend only on the kernel choice (numerics differ at
+scaled by one factor so the longest has ``probe_steps`` (the ratios between
+systems stay, and with them the lane planner's group choice, which weighs
+the longest chain of steps against the total work) and is timed with the
+library's HIP events; the fastest wins.  Results depend only on the kernel choice (numerics differ at
 round-off level between slot configurations: a row's terms are summed in
 different segment groupings), so the tuner records the configuration, not
-results.  Choices are cached per workload signature in a JSON file.
+results.  Choices are cached per workload signature (with the precision) in
+a JSON file.
 """
 from __future__ import annotations
 
@@ -34,9 +37,20 @@ def signature(p, adjoint: bool) -> str:
     A, G, L, f, q0 = p.getMatrices()
     parts = [str(len(q0)), str(A.tocsr().nnz), str(len(np.asarray(G.val).reshape(-1))),
              str(len(np.asarray(L.val).reshape(-1))), str(p.numberOfSystems),
-             str(int(p._max_steps())), str(adjoint),
+             str(int(p._max_steps())), str(adjoint), str(p._precision()),
              ",".join(str(int(x)) for x in np.bincount(np.asarray(p.system_sizes)))]
     return hashlib.sha1("|".join(parts).encode()).hexdigest()[:16]
+
+
+def probe_caps(steps, probe_steps: int) -> np.ndarray:
+    """Step counts of a probe: every system's steps times one factor that
+    brings the longest to ``probe_steps`` (never more than its own; systems
+    with steps keep at least one, systems without stay at none)."""
+    full = np.asarray(steps, dtype=np.int64)
+    if full.size == 0:
+        return full
+    scale = min(1.0, probe_steps / max(1, int(full.max())))
+    return np.where(full > 0, np.maximum(1, np.ceil(full * scale)), 0).astype(np.int64)
 
 
 def _load_cache() -> dict:
@@ -72,7 +86,7 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
         steps_full = p._sys_steps
         exc = p._sys_exc
         n = p.numberOfSystems
-        cap = np.minimum(steps_full if steps_full is not None else np.full(n, p.numsteps),
+        cap = probe_caps(steps_full if steps_full is not None else np.full(n, p.numsteps),
                          probe_steps)
         p.setSystemExcitations(exc, cap)
         cks = list(checkpoints) if checkpoints is not None else (list(CHECKPOINTS) if adjoint

@@ -30,6 +30,8 @@ def main(argv=None):
     ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "lane"])
     ap.add_argument("--lanes", type=int, default=0,
                     help="lanes per system in the lane kernel (0: chosen per structure)")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="adjoint checkpoint interval (0: the planner's choice)")
     ap.add_argument("--repeat", type=int, default=1)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     a = ap.parse_args(argv)
@@ -62,6 +64,8 @@ def main(argv=None):
     if a.adjoint:
         from .adjoint import energy_gradient_inputs
         p.setJacobian(*energy_gradient_inputs(w).as_args())
+    if a.checkpoint_every:
+        p.setTuning(checkpoint_every=a.checkpoint_every)
     for _ in range(a.repeat):
         t0 = time.perf_counter()
         if a.adjoint:

@@ -196,3 +196,24 @@ def test_blocked_layout_for_long_chains():
     # 4 x rows of 1 term, 4 v rows of the chain stencil, the energy row
     assert int(re.search(r"TF = (\d+),", src).group(1)) <= 60
     p._laneSource(False, compile=True)
+
+
+def test_autotune_probe_keeps_step_ratios_and_precision_in_key():
+    from pasif_amd import autotune as AT
+    steps = np.array([100000, 50000, 1000, 1, 0])
+    cap = AT.probe_caps(steps, 2000)
+    # one factor (0.02): the longest runs 2,000, ratios stay, none grows,
+    # a system with steps keeps one, one without keeps none
+    assert cap.tolist() == [2000, 1000, 20, 1, 0]
+    assert (AT.probe_caps(np.array([500, 300]), 2000) == [500, 300]).all()
+    # the group plan weighs the longest chain against the total work: on the
+    # scaled probe it is the plan of the full problem
+    w = W.heterogeneous(300, 30, 3000, steps_per_sample=5, n_signals=3,
+                        dof_choices=(1, 2, 4, 8), dof_probs=(1, 1, 1, 1))
+    p = host_solver(w)
+    full = p.kernelInfo()["lanes_per_instance"]
+    p.setSystemExcitations(p._sys_exc, AT.probe_caps(p._sys_steps, 300))
+    assert p.kernelInfo()["lanes_per_instance"] == full
+    key64 = AT.signature(p, False)
+    p.setPrecision(32)
+    assert AT.signature(p, False) != key64
