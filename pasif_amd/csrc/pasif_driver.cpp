@@ -2806,9 +2806,19 @@ void pool_build_modules(pasif_pool* P) {
   for (size_t c = 0; c < all.size(); ++c) {
     const auto opt = group_options(all[c], bits);
     if (!opt.empty()) {
-      size_t b = 0;
-      for (size_t k = 1; k < opt.size(); ++k)
-        if (opt[k].second * opt[k].first < opt[b].second * opt[b].first) b = k;
+      // the shortest step among the sizes within 2x of the least work: a
+      // tick lasts `budget` steps of the slowest class, and a request waits
+      // for its slowest system
+      double wmin = 0;
+      for (size_t k = 0; k < opt.size(); ++k) {
+        const double wk = opt[k].second * opt[k].first;
+        if (k == 0 || wk < wmin) wmin = wk;
+      }
+      size_t b = opt.size();
+      for (size_t k = 0; k < opt.size(); ++k)
+        if (opt[k].second * opt[k].first <= 2.0 * wmin &&
+            (b == opt.size() || opt[k].second < opt[b].second))
+          b = k;
       (void)lane_group(all[c], opt[b].first);
       need[c] = 1;
     } else {
