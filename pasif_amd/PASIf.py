@@ -37,7 +37,7 @@ import os
 from typing import List, Optional, Sequence
 
 import numpy as np
-from scipy.sparse import coo_matrix, dia_matrix
+from scipy.sparse import coo_matrix, dia_matrix, issparse
 
 from .PASIfgpu import __GpuDriver as _GpuDriver
 
@@ -136,13 +136,45 @@ def _cat_tensors(tensors: Sequence[coo_tensor]) -> coo_tensor:
     return out
 
 
+def _coo_parts(m):
+    """(row, col, value) arrays of one matrix, in the entry order
+    ``coo_matrix(m)`` gives, read straight from COO / CSR / CSC storage
+    (building a scipy object per system dominates the assembly of large
+    batches)."""
+    if issparse(m):
+        fmt = m.format
+        if fmt == "coo":
+            return m.row, m.col, m.data
+        if fmt == "csr":
+            return np.repeat(np.arange(m.shape[0]), np.diff(m.indptr)), m.indices, m.data
+        c = m.tocoo()
+        return c.row, c.col, c.data
+    a = np.asarray(m, dtype=np.float64)
+    r, c = np.nonzero(a)
+    return r, c, a[r, c]
+
+
+def _diagonal(m) -> np.ndarray:
+    """Main diagonal of one matrix (a DIA matrix's offset-0 row read
+    directly)."""
+    if issparse(m) and m.format == "dia":
+        k = np.nonzero(np.asarray(m.offsets) == 0)[0]
+        if k.size == 1:
+            n = min(m.shape)
+            d = np.zeros(n, dtype=np.float64)
+            row = np.asarray(m.data[k[0]], dtype=np.float64)[:n]
+            d[:row.size] = row
+            return d
+    return np.asarray(m.diagonal(), dtype=np.float64)
+
+
 def _cat_matrices(mats, offs, n):
     rows, cols, vals = [], [], []
     for m, o in zip(mats, offs):
-        c = coo_matrix(m)
-        rows.append(c.row.astype(np.int64) + o)
-        cols.append(c.col.astype(np.int64) + o)
-        vals.append(np.asarray(c.data, dtype=np.float64))
+        r, c, v = _coo_parts(m)
+        rows.append(np.asarray(r, dtype=np.int64) + o)
+        cols.append(np.asarray(c, dtype=np.int64) + o)
+        vals.append(np.asarray(v, dtype=np.float64))
     return coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(n, n))
 
@@ -167,7 +199,7 @@ def assemble_arrays(vecM, vecB, vecK, vecGamma, vecLambda, vecForce, vecQ0, vecP
     sizes = np.array([m.shape[0] for m in vecM], dtype=np.int64)
     offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
     n = int(sizes.sum())
-    diag = np.concatenate([np.asarray(m.diagonal(), dtype=np.float64) for m in vecM])
+    diag = np.concatenate([_diagonal(m) for m in vecM])
     Dm = _neg_inverse_diag(dia_matrix((diag, [0]), shape=(n, n))).tocsc()
     Bs = coo_matrix(Dm.dot(_cat_matrices(vecB, offs, n)))
     Ks = coo_matrix(Dm.dot(_cat_matrices(vecK, offs, n)))
@@ -483,7 +515,7 @@ class PASIf(_GpuDriver):
         n = int(sizes.sum())
         self.globalSystemSize = n
         self.system_sizes = sizes
-        diag = np.concatenate([np.asarray(m.diagonal(), dtype=np.float64) for m in inputVecM])
+        diag = np.concatenate([_diagonal(m) for m in inputVecM])
         self.system_M = dia_matrix((diag, [0]), shape=(n, n))
         self.system_B = _cat_matrices(inputVecB, offs, n)
         self.system_K = _cat_matrices(inputVecK, offs, n)
@@ -502,7 +534,7 @@ class PASIf(_GpuDriver):
         n = int(sizes.sum())
         self.globalAdjointSize = n
         self.jacobian_sizes = sizes
-        diag = np.concatenate([np.asarray(m.diagonal(), dtype=np.float64) for m in inputVecM])
+        diag = np.concatenate([_diagonal(m) for m in inputVecM])
         self.jacobian_M = dia_matrix((diag, [0]), shape=(n, n))
         self.jacobian_B = _cat_matrices(inputVecB, offs, n)
         self.jacobian_K = _cat_matrices(inputVecK, offs, n)

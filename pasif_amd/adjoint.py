@@ -32,7 +32,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 from scipy.sparse import coo_matrix, dia_matrix
 
-from .PASIf import coo_tensor
+from .PASIf import _coo_parts, _diagonal, coo_tensor
 
 # a parameter: ("K", r, c) | ("B", r, c) | ("Gamma", entry) | ("Lambda", entry)
 Param = Tuple
@@ -62,19 +62,47 @@ def _entries(t, order):
     return ix, v
 
 
-def _tensor(dims, idx: List[Sequence[int]], val: List[float]):
+def _tensor(dims, idx, val):
     t = coo_tensor(list(dims))
-    t.val = np.asarray(val, dtype=np.float64)
+    t.val = np.asarray(val, dtype=np.float64).reshape(-1)
     t.indices = (np.asarray(idx, dtype=np.int64).reshape(-1) if len(idx)
                  else np.zeros(0, dtype=np.int64))
     return t
+
+
+# The adjoint mass (-I) and the empty B' depend only on the size: one shared,
+# read-only matrix per size (building a scipy object per system dominates the
+# assembly of large batches).
+_SHARED = {}
+
+
+def _minus_identity(na: int) -> dia_matrix:
+    key = ("M", na)
+    if key not in _SHARED:
+        _SHARED[key] = dia_matrix((-np.ones(na), [0]), shape=(na, na))
+    return _SHARED[key]
+
+
+def _empty(na: int) -> coo_matrix:
+    key = ("B", na)
+    if key not in _SHARED:
+        _SHARED[key] = coo_matrix((na, na))
+    return _SHARED[key]
+
+
+def _interleave(ix: np.ndarray, perms) -> np.ndarray:
+    """Rows ``ix[e][perm]`` for every entry e and, within it, every perm in
+    order: entry-major, as the term-by-term construction lists them."""
+    if ix.shape[0] == 0:
+        return np.zeros((0, ix.shape[1]), dtype=np.int64)
+    return np.stack([ix[:, list(p)] for p in perms], axis=1).reshape(-1, ix.shape[1])
 
 
 def system_jacobian(M, B, K, G, L, params: Sequence[Param] = (),
                     terminal: Optional[np.ndarray] = None):
     """Adjoint inputs of one system: ``(M', B', K', G', L', f', lambda_T, Psi')``.
     ``terminal`` is dL/dx(T) (length n); accumulators start at 0."""
-    m = np.asarray(M.diagonal(), dtype=np.float64)
+    m = _diagonal(M)
     n = m.size
     D = -1.0 / m
     npar = len(params)
@@ -82,45 +110,50 @@ def system_jacobian(M, B, K, G, L, params: Sequence[Param] = (),
     # A' = A^T from the scaled B and K entries (duplicates are summed later)
     rows, cols, vals = [], [], []
     for mat in (B, K):
-        c = coo_matrix(mat)
-        rows.append(c.col.astype(np.int64))
-        cols.append(c.row.astype(np.int64))
-        vals.append(D[c.row] * np.asarray(c.data, dtype=np.float64))
+        r, c, v = _coo_parts(mat)
+        r = np.asarray(r, dtype=np.int64)
+        rows.append(np.asarray(c, dtype=np.int64))
+        cols.append(r)
+        vals.append(D[r] * np.asarray(v, dtype=np.float64))
     Kp = coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                     shape=(na, na))
-    Bp = coo_matrix((na, na))
+    Bp = _empty(na)
     gi, gv = _entries(G, 3)
     li, lv = _entries(L, 4)
-    G_idx, G_val, L_idx, L_val, P_idx, P_val = [], [], [], [], [], []
-    for (o, j, k), v in zip(gi, gv):
-        s = D[o] * v
-        G_idx += [(j, o, k), (k, o, j)]
-        G_val += [s, s]
-    for (o, j, k, l), v in zip(li, lv):
-        s = D[o] * v
-        L_idx += [(j, o, k, l), (k, o, j, l), (l, o, j, k)]
-        L_val += [s, s, s]
-    for p, par in enumerate(params):
-        g = n + p
-        kind = par[0]
-        if kind in ("K", "B"):
-            r, c = int(par[1]), int(par[2])
-            G_idx.append((g, r, c))
-            G_val.append(D[r])
-        elif kind == "Gamma":
-            o, j, k = (int(x) for x in gi[int(par[1])])
-            L_idx.append((g, o, j, k))
-            L_val.append(D[o])
-        elif kind == "Lambda":
-            o, j, k, l = (int(x) for x in li[int(par[1])])
-            P_idx.append((g, o, j, k, l))
-            P_val.append(D[o])
-        else:
-            raise ValueError(f"unknown parameter kind {kind!r}")
+    # G[o, j, k] -> G'[j, o, k], G'[k, o, j]; L[o, j, k, l] -> L'[j, o, k, l],
+    # L'[k, o, j, l], L'[l, o, j, k] (each entry's images consecutive)
+    G_idx = [_interleave(gi, ((1, 0, 2), (2, 0, 1)))]
+    G_val = [np.repeat(D[gi[:, 0]] * gv, 2)]
+    L_idx = [_interleave(li, ((1, 0, 2, 3), (2, 0, 1, 3), (3, 0, 1, 2)))]
+    L_val = [np.repeat(D[li[:, 0]] * lv, 3)]
+    # parameters, each kind into its tensor in parameter order
+    kinds = [par[0] for par in params]
+    for k in kinds:
+        if k not in ("K", "B", "Gamma", "Lambda"):
+            raise ValueError(f"unknown parameter kind {k!r}")
+    sel = [p for p, k in enumerate(kinds) if k in ("K", "B")]
+    if sel:
+        rc = np.array([(int(params[p][1]), int(params[p][2])) for p in sel], dtype=np.int64)
+        G_idx.append(np.column_stack([n + np.asarray(sel, dtype=np.int64), rc]))
+        G_val.append(D[rc[:, 0]])
+    sel = [p for p, k in enumerate(kinds) if k == "Gamma"]
+    if sel:
+        e = gi[np.array([int(params[p][1]) for p in sel], dtype=np.int64)]
+        L_idx.append(np.column_stack([n + np.asarray(sel, dtype=np.int64), e]))
+        L_val.append(D[e[:, 0]])
+    P_idx, P_val = [np.zeros((0, 5), dtype=np.int64)], [np.zeros(0)]
+    sel = [p for p, k in enumerate(kinds) if k == "Lambda"]
+    if sel:
+        e = li[np.array([int(params[p][1]) for p in sel], dtype=np.int64)]
+        P_idx.append(np.column_stack([n + np.asarray(sel, dtype=np.int64), e]))
+        P_val.append(D[e[:, 0]])
+    G_idx, G_val = np.concatenate(G_idx), np.concatenate(G_val)
+    L_idx, L_val = np.concatenate(L_idx), np.concatenate(L_val)
+    P_idx, P_val = np.concatenate(P_idx), np.concatenate(P_val)
     lamT = np.zeros(na)
     if terminal is not None:
         lamT[:n] = np.asarray(terminal, dtype=np.float64)
-    Mp = dia_matrix((-np.ones(na), [0]), shape=(na, na))
+    Mp = _minus_identity(na)
     return (Mp, Bp, Kp, _tensor([na, na, n], G_idx, G_val),
             _tensor([na, na, n, n], L_idx, L_val), np.zeros(na), lamT,
             _tensor([na, na, n, n, n], P_idx, P_val))
