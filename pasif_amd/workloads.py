@@ -141,38 +141,60 @@ def band_limited(rng, n_samples: int, cutoff: float = 0.1) -> np.ndarray:
     return y / (np.sqrt(np.mean(y ** 2)) + 1e-300)
 
 
-def chain_system(d: int, s: float, gvals: np.ndarray, lvals: np.ndarray, with_energy=True):
-    """User inputs of one chain of ``d`` DOF: ``K = tridiag(-1, 2, -1) s``,
-    ``B = 0.02 K + 0.01 I``, nearest-neighbour Gamma (values ``gvals``),
-    on-site Lambda (``lvals``), energy probe ``E' = v^T B v``."""
+_CHAIN = {}
+
+
+def _chain_template(d: int, with_energy: bool) -> dict:
+    """Index patterns of a chain of ``d`` DOF (the same for every scale):
+    built once per size."""
+    key = (d, with_energy)
+    t = _CHAIN.get(key)
+    if t is not None:
+        return t
     n = 2 * d + (1 if with_energy else 0)
-    Kd = (np.diag(np.full(d, 2.0)) - np.diag(np.ones(d - 1), 1) - np.diag(np.ones(d - 1), -1)) * s \
-        if d > 1 else np.array([[2.0 * s]])
-    Bd = 0.02 * Kd + 0.01 * np.eye(d)
-    rows, cols, vals = list(range(d)), list(range(d, 2 * d)), [-1.0] * d
-    br, bc = np.nonzero(Bd)
-    rows += list(d + br)
-    cols += list(d + bc)
-    vals += list(Bd[br, bc])
-    B = coo_matrix((vals, (rows, cols)), shape=(n, n))
-    kr, kc = np.nonzero(Kd)
-    K = coo_matrix((Kd[kr, kc], (d + kr, kc)), shape=(n, n))
+    T = (np.diag(np.full(d, 2.0)) - np.diag(np.ones(d - 1), 1) - np.diag(np.ones(d - 1), -1)) \
+        if d > 1 else np.array([[2.0]])
+    br, bc = np.nonzero(0.02 * T + 0.01 * np.eye(d))
+    kr, kc = np.nonzero(T)
     gi = chain_gamma_index(d)
     G_idx = list(gi.reshape(-1))
-    G_val = list(gvals[: len(gi)])
     if with_energy:
         for j, k in zip(br, bc):
             G_idx += [2 * d, d + j, d + k]
-            G_val.append(-Bd[j, k])
     L_idx = []
     for i in range(d):
         L_idx += [d + i, i, i, i]
-    G = _tensor([n] * 3, G_val, G_idx)
-    L = _tensor([n] * 4, lvals[:d], L_idx)
-    M = dia_matrix((np.ones(n), [0]), shape=(n, n))
     f = np.zeros(n)
     f[d] = 1.0
-    return M, B, K, G, L, f, np.zeros(n), Kd, Bd
+    t = {"n": n, "T": T, "br": br, "bc": bc, "kr": kr, "kc": kc, "ngi": len(gi),
+         "B_rows": np.concatenate([np.arange(d), d + br]),
+         "B_cols": np.concatenate([np.arange(d, 2 * d), d + bc]),
+         "K_rows": d + kr, "K_cols": kc,
+         "G_idx": np.asarray(G_idx, dtype=np.int64), "L_idx": np.asarray(L_idx, dtype=np.int64),
+         "M": dia_matrix((np.ones(n), [0]), shape=(n, n)), "f": f}
+    _CHAIN[key] = t
+    return t
+
+
+def chain_system(d: int, s: float, gvals: np.ndarray, lvals: np.ndarray, with_energy=True):
+    """User inputs of one chain of ``d`` DOF: ``K = tridiag(-1, 2, -1) s``,
+    ``B = 0.02 K + 0.01 I``, nearest-neighbour Gamma (values ``gvals``),
+    on-site Lambda (``lvals``), energy probe ``E' = v^T B v``.  The mass
+    matrix and force pattern are shared by the chains of one size."""
+    t = _chain_template(d, with_energy)
+    n = t["n"]
+    Kd = t["T"] * s
+    Bd = 0.02 * Kd + 0.01 * np.eye(d)
+    br, bc = t["br"], t["bc"]
+    B = coo_matrix((np.concatenate([np.full(d, -1.0), Bd[br, bc]]), (t["B_rows"], t["B_cols"])),
+                   shape=(n, n))
+    K = coo_matrix((Kd[t["kr"], t["kc"]], (t["K_rows"], t["K_cols"])), shape=(n, n))
+    G_val = np.asarray(gvals[: t["ngi"]], dtype=np.float64)
+    if with_energy:
+        G_val = np.concatenate([G_val, -Bd[br, bc]])
+    G = _tensor([n] * 3, G_val, t["G_idx"])
+    L = _tensor([n] * 4, lvals[:d], t["L_idx"])
+    return t["M"], B, K, G, L, t["f"], np.zeros(n), Kd, Bd
 
 
 def chain_gamma_index(d: int) -> np.ndarray:
