@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--steps-per-tick", type=int, default=5000)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--lanes", type=int, default=512)
-    ap.add_argument("--slots", type=int, default=8192,
+    ap.add_argument("--slots", type=int, default=32768,
                     help="resident tile cap (0: HBM budget only)")
     ap.add_argument("--plain-sample", type=int, default=3)
     ap.add_argument("--seed", type=int, default=4)

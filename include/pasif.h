@@ -198,6 +198,11 @@ typedef struct pasif_pool pasif_pool;
 int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile_lanes,
                       uint32_t slot_config, uint32_t checkpoint_every, uint32_t steps_per_tick,
                       double hbm_budget_bytes);
+/* Destroys the pool.  Job handles it created stay with their owners: jobs
+ * still resident lose their device memory and every such handle is detached
+ * from the pool (pasif_destroy it as usual).  pasif_destroy on a job handle
+ * removes the job from its pool first (a job a tick in flight still runs is
+ * deleted by the pool once that tick is over). */
 int pasif_pool_destroy(pasif_pool* pool);
 const char* pasif_pool_last_error(const pasif_pool* pool);
 /* Signal bank shared by jobs created with excitations == NULL: [n_exc][len]

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -19,6 +20,7 @@
 #include <limits>
 #include <memory>
 #include <map>
+#include <set>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -272,6 +274,9 @@ struct pasif_driver {
   uint32_t last_launches = 0;
   // a job of a pool: uses the pool's device and upload stream
   pasif_pool* pool = nullptr;
+  // destroyed by its owner while a tick in flight still ran it: the pool
+  // deletes it once the tick is over
+  bool orphan = false;
   uint32_t pool_ck = 0, pool_nck = 0;
   // a pool job's lane kernel arguments, one vector per lane part (ticks
   // copy them into the pool's per-module tables)
@@ -303,6 +308,9 @@ struct pasif_pool {
     uint32_t tiles;
   };
   std::vector<Job> jobs;
+  // every job handle the pool created and that is not destroyed yet (when
+  // the pool goes first, their device memory is freed and they are detached)
+  std::set<pasif_driver*> handles;
   // shared signal bank, uploaded once in the pool's precision
   DevBuf bank, bank_interp;
   uint32_t bank_n = 0, bank_len = 0, bank_w = 0, bank_rate = 0;
@@ -317,6 +325,11 @@ struct pasif_pool {
   uint32_t last_blocks = 0;
   // slot cap: resident tiles (0 = only the HBM budget limits admission)
   uint32_t max_tiles = 0, tiles_used = 0;
+  // admission time by phase (PASIF_POOL_TIMING set: printed when the pool is
+  // destroyed): concatenation, packing, run buffers and arguments, the rest
+  bool timing = std::getenv("PASIF_POOL_TIMING") != nullptr;
+  double tm[4] = {0, 0, 0, 0};
+  uint64_t tm_n = 0;
   // device memory of the jobs
   std::unique_ptr<Arena> arena;
   // registered systems: one forward block each (local indices) and, when
@@ -2090,6 +2103,9 @@ void set_coo(pasif_driver* d, CooIn& t, int order, const uint32_t* n, const doub
   t.set = true;
 }
 
+// pool job handles (defined with the pool)
+void pool_remove_job(pasif_pool* P, size_t k);
+void pool_detach(pasif_driver* d);
 }  // namespace
 
 // =================================================================== C ABI
@@ -2130,9 +2146,21 @@ int pasif_create(pasif_driver** out, const double* excitations, uint32_t n_exc, 
 int pasif_destroy(pasif_driver* d) {
   if (!d) return PASIF_OK;
   if (d->pool) {
-    // still a member of a pool: leave it there (the pool destroys it)
-    for (const auto& j : d->pool->jobs)
-      if (j.d == d) return PASIF_ERR_STATE;
+    // a pool job: leaves the pool (a resident one that a tick in flight
+    // still runs is deleted by the pool once the tick is over)
+    pasif_pool* P = d->pool;
+    for (size_t k = 0; k < P->jobs.size(); ++k) {
+      if (P->jobs[k].d != d) continue;
+      if (P->in_flight && !P->jobs[k].finished) {
+        d->orphan = true;
+        return PASIF_OK;
+      }
+      (void)hipSetDevice(P->dev);
+      pool_remove_job(P, k);
+      break;
+    }
+    P->handles.erase(d);
+    pool_detach(d);
   }
   if (d->dev >= 0) (void)hipSetDevice(d->dev);
   if (d->stream) (void)hipStreamSynchronize(d->stream);
@@ -2690,6 +2718,47 @@ pasif_driver* new_bank_job(pasif_pool* P, uint32_t numsteps) {
   return d;
 }
 
+// Remove job k from the pool and free its device memory (its handle stays
+// with the caller).
+void pool_remove_job(pasif_pool* P, size_t k) {
+  auto& j = P->jobs[k];
+  pasif_driver* d = j.d;
+  release_run(d);
+  d->d_exc.release();
+  d->d_interp.release();
+  d->d_exc_dirty = true;
+  P->used -= j.bytes;
+  P->tiles_used -= j.tiles;
+  P->jobs.erase(P->jobs.begin() + long(k));
+}
+
+// A pool handle's device memory (all of it in the pool's arena) is freed and
+// the handle no longer refers to the pool: pasif_destroy deletes it as a
+// plain driver.
+void pool_detach(pasif_driver* d) {
+  release_run(d);
+  d->d_exc.release();
+  d->d_interp.release();
+  d->traj.release();
+  d->pool = nullptr;
+  d->stream = nullptr;
+  d->ext_exc = nullptr;
+  d->ext_interp = nullptr;
+}
+
+// Delete the jobs their owners destroyed while a tick ran them (those no
+// launch in flight uses).
+void pool_reap(pasif_pool* P) {
+  for (size_t k = P->jobs.size(); k-- > 0;) {
+    pasif_driver* d = P->jobs[k].d;
+    if (!d->orphan || (P->in_flight && !P->jobs[k].finished)) continue;
+    pool_remove_job(P, k);
+    P->handles.erase(d);
+    pool_detach(d);
+    delete d;
+  }
+}
+
 // Pack a loaded job, allocate its run buffers from the arena and admit it
 // if the slot cap and the HBM budget allow (PASIF_ERR_MEMORY otherwise,
 // leaving the job unadmitted with no device memory).
@@ -2709,8 +2778,12 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
   d->window = 0;
   uint64_t work = 0;
   uint32_t nslots = 0;
+  using clk = std::chrono::steady_clock;
+  const auto c0 = clk::now();
+  auto c1 = c0;
   int rc = guarded(d, [&] {
     ensure_packed(d, adjoint != 0);
+    c1 = clk::now();
     if (d->pk.tile) {
       if (P->precision == 64) {
         TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
@@ -2763,9 +2836,14 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
              " bytes, " + std::to_string(P->hbm_budget - P->used) + " are free";
     throw Fail{PASIF_ERR_MEMORY};
   }
+  const auto c2 = clk::now();
   POOL_HIP(hipEventRecord(P->uploaded, P->upload));
   P->used += bytes;
   P->tiles_used += tiles;
+  if (P->timing) {
+    P->tm[1] += std::chrono::duration<double, std::milli>(c1 - c0).count();
+    P->tm[2] += std::chrono::duration<double, std::milli>(c2 - c1).count();
+  }
   P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false, tiles});
 }
 
@@ -3142,6 +3220,7 @@ int pasif_pool_job_create(pasif_pool* P, pasif_driver** job, const double* excit
     POOL_REQUIRE(job != nullptr, PASIF_ERR_ARGUMENT, "null job pointer");
     if (excitations == nullptr) {  // the pool's signal bank: excitation ids index it
       *job = new_bank_job(P, numsteps);
+      P->handles.insert(*job);
       return;
     }
     std::unique_ptr<pasif_driver> d(new pasif_driver());
@@ -3161,6 +3240,7 @@ int pasif_pool_job_create(pasif_pool* P, pasif_driver** job, const double* excit
       POOL_REQUIRE(rc == PASIF_OK, rc, d->err);
     }
     *job = d.release();
+    P->handles.insert(*job);
   });
 }
 
@@ -3290,6 +3370,7 @@ int pasif_pool_submit_systems(pasif_pool* P, uint32_t n, const uint32_t* ids,
       else
         pool_build_modules<float>(P);
     }
+    const auto c0 = std::chrono::steady_clock::now();
     std::unique_ptr<pasif_driver> d(new_bank_job(P, 0));
     d->lane_cls.assign(n, -1);
     for (uint32_t i = 0; i < n; ++i) d->lane_cls[i] = P->reg_lane[ids[i]];
@@ -3308,8 +3389,16 @@ int pasif_pool_submit_systems(pasif_pool* P, uint32_t n, const uint32_t* ids,
     d->layout_systems = n;
     d->sys_ready = true;
     d->adj_ready = adjoint != 0;
+    const auto c1 = std::chrono::steady_clock::now();
     admit_job(P, d.get(), adjoint);
     *job = d.release();
+    P->handles.insert(*job);
+    if (P->timing) {
+      const auto c2 = std::chrono::steady_clock::now();
+      P->tm[0] += std::chrono::duration<double, std::milli>(c1 - c0).count();
+      P->tm[3] += std::chrono::duration<double, std::milli>(c2 - c1).count();
+      ++P->tm_n;
+    }
   });
 }
 
@@ -3331,6 +3420,7 @@ int pasif_pool_tick(pasif_pool* P, int wait, uint32_t* active_tiles) {
   return pool_guarded(P, [&] {
     POOL_REQUIRE(!P->in_flight, PASIF_ERR_STATE, "a tick is already in flight");
     POOL_HIP(hipSetDevice(P->dev));
+    pool_reap(P);
     if (P->precision == 64)
       pool_tick_impl<double>(P);
     else
@@ -3355,6 +3445,7 @@ int pasif_pool_wait(pasif_pool* P, double* kernel_ms) {
       P->last_ms = ms;
       P->in_flight = false;
     }
+    pool_reap(P);
     if (kernel_ms) *kernel_ms = P->last_ms;
   });
 }
@@ -3416,13 +3507,7 @@ int pasif_pool_collect(pasif_pool* P, pasif_driver* d, double* q, size_t capq, s
       P->err = d->err;
       throw Fail{rc};
     }
-    release_run(d);
-    d->d_exc.release();
-    d->d_interp.release();
-    d->d_exc_dirty = true;
-    P->used -= j.bytes;
-    P->tiles_used -= j.tiles;
-    P->jobs.erase(P->jobs.begin() + long(k));
+    pool_remove_job(P, k);
   });
 }
 
@@ -3438,16 +3523,26 @@ int pasif_pool_stats(const pasif_pool* P, uint32_t* jobs, double* used_bytes,
 
 int pasif_pool_destroy(pasif_pool* P) {
   if (!P) return PASIF_OK;
+  if (P->timing && P->tm_n)
+    fprintf(stderr,
+            "[pasif] admission of %llu requests, ms per request: concatenation %.3f, "
+            "packing %.3f, run buffers and arguments %.3f, total in admit %.3f\n",
+            (unsigned long long)P->tm_n, P->tm[0] / double(P->tm_n), P->tm[1] / double(P->tm_n),
+            P->tm[2] / double(P->tm_n), P->tm[3] / double(P->tm_n));
   (void)hipSetDevice(P->dev);
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   if (P->upload) (void)hipStreamSynchronize(P->upload);
-  for (auto& j : P->jobs) {
-    release_run(j.d);
-    j.d->pool = nullptr;
-    j.d->stream = nullptr;
-    delete j.d;
-  }
+  // orphaned jobs are deleted; every other handle the pool created stays
+  // with its owner, detached (pasif_destroy it as usual)
+  for (auto& j : P->jobs)
+    if (j.d->orphan) {
+      P->handles.erase(j.d);
+      pool_detach(j.d);
+      delete j.d;
+    }
   P->jobs.clear();
+  for (pasif_driver* d : P->handles) pool_detach(d);
+  P->handles.clear();
   P->arena.reset();
   for (int m = 0; m < 2; ++m) {
     if (P->aux[m]) {

@@ -267,3 +267,58 @@ def test_server_results_equal_alone_runs():
         np.testing.assert_array_equal(r.result, q)
         if r.adjoint:
             np.testing.assert_array_equal(r.grad, g)
+
+
+def test_job_handles_leave_and_outlive_the_pool():
+    """A job handle destroyed while resident leaves the pool (at once, or once
+    the tick in flight that runs it is over) and frees its memory; the other
+    jobs' results are unaffected (bitwise equal to their runs alone); a pool
+    destroyed with jobs still resident leaves their handles to their owners,
+    who destroy them afterwards."""
+    print("[gpu case] serving job handle lifetime", flush=True)
+    rng = np.random.default_rng(45)
+    lib = S.SystemLibrary(dofs=(1, 2, 4), variants=2)
+    bank = S.signal_bank(rng, 4, 3000, steps_per_sample=10)
+    from pasif_amd.workloads import linear_interp_matrix
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=256, slot_config=1, steps_per_tick=200, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        return pool
+
+    reqs = [S.draw_request(lib, rng, i, 4, tmin=300, tmax=1200, max_systems=12)
+            for i in range(4)]
+    alone = new_pool()
+    j = alone.submit_ids(reqs[0])
+    while not alone.finished(j):
+        alone.tick(wait=True)
+    ref = alone.collect(j, reqs[0])
+    alone.close()
+
+    pool = new_pool()
+    jobs = [pool.submit_ids(r) for r in reqs]
+    assert all(x is not None for x in jobs)
+    used4 = pool.stats()["used_bytes"]
+    # destroyed while resident, no tick in flight: leaves at once
+    assert pool.lib.pasif_destroy(jobs[1]._h) == 0
+    jobs[1]._h = None
+    st = pool.stats()
+    assert st["jobs"] == 3 and st["used_bytes"] < used4
+    # destroyed while a tick runs it: the pool deletes it after the tick
+    pool.tick(wait=False)
+    assert pool.lib.pasif_destroy(jobs[2]._h) == 0
+    jobs[2]._h = None
+    pool.wait()
+    assert pool.stats()["jobs"] == 2
+    while not pool.finished(jobs[0]):
+        pool.tick(wait=True)
+    got = pool.collect(jobs[0], reqs[0])
+    assert np.array_equal(got[0], ref[0])
+    if reqs[0].adjoint:
+        assert np.array_equal(got[1], ref[1])
+    # the pool goes first with job 3 resident; its handle is destroyed after
+    pool.close()
+    assert pool.lib.pasif_destroy(jobs[3]._h) == 0
+    jobs[3]._h = None
