@@ -326,9 +326,10 @@ struct pasif_pool {
   // slot cap: resident tiles (0 = only the HBM budget limits admission)
   uint32_t max_tiles = 0, tiles_used = 0;
   // admission time by phase (PASIF_POOL_TIMING set: printed when the pool is
-  // destroyed): concatenation, packing, run buffers and arguments, the rest
+  // destroyed): concatenation, packing, run buffers and arguments, all of
+  // admission, the lane programs' uploads (part of packing)
   bool timing = std::getenv("PASIF_POOL_TIMING") != nullptr;
-  double tm[4] = {0, 0, 0, 0};
+  double tm[5] = {0, 0, 0, 0, 0};
   uint64_t tm_n = 0;
   // device memory of the jobs
   std::unique_ptr<Arena> arena;
@@ -1607,10 +1608,14 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     stage(&L->out_a, oa.data(), oa.size() * sizeof(uint32_t));
     P->cls.push_back(std::move(L));
   }
+  const auto u0 = std::chrono::steady_clock::now();
   upload(d, P->prog, host);
   for (auto& v : views)
     *v.first = P->prog.p ? static_cast<const unsigned char*>(P->prog.p) + v.second : nullptr;
   if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
+  if (d->pool && d->pool->timing)
+    d->pool->tm[4] +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - u0).count();
   P->blocks = block;
   P->tmax = tmax;
   P->module = part;
@@ -3526,9 +3531,10 @@ int pasif_pool_destroy(pasif_pool* P) {
   if (P->timing && P->tm_n)
     fprintf(stderr,
             "[pasif] admission of %llu requests, ms per request: concatenation %.3f, "
-            "packing %.3f, run buffers and arguments %.3f, total in admit %.3f\n",
+            "packing %.3f (of which lane program uploads %.3f), run buffers and arguments "
+            "%.3f, total in admit %.3f\n",
             (unsigned long long)P->tm_n, P->tm[0] / double(P->tm_n), P->tm[1] / double(P->tm_n),
-            P->tm[2] / double(P->tm_n), P->tm[3] / double(P->tm_n));
+            P->tm[4] / double(P->tm_n), P->tm[2] / double(P->tm_n), P->tm[3] / double(P->tm_n));
   (void)hipSetDevice(P->dev);
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   if (P->upload) (void)hipStreamSynchronize(P->upload);
