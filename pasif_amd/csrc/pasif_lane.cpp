@@ -1826,6 +1826,25 @@ const char* kJitOptions[] = {"--gpu-architecture=gfx950", "-O3", "-std=c++17", "
                              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
 constexpr int kJitOptionCount = sizeof(kJitOptions) / sizeof(kJitOptions[0]);
 
+// the options above, then any from PASIF_JIT_FLAGS (space separated, e.g.
+// "-mllvm -amdgpu-sched-strategy=max-ilp"); they are part of the cache key
+std::vector<std::string> jit_options() {
+  std::vector<std::string> o(kJitOptions, kJitOptions + kJitOptionCount);
+  if (const char* e = std::getenv("PASIF_JIT_FLAGS")) {
+    std::string t;
+    for (const char* c = e;; ++c) {
+      if (*c == ' ' || *c == '\0') {
+        if (!t.empty()) o.push_back(t);
+        t.clear();
+        if (*c == '\0') break;
+      } else {
+        t += *c;
+      }
+    }
+  }
+  return o;
+}
+
 std::mutex g_jit_mu;
 std::map<std::pair<int, uint64_t>, LaneKernel> g_jit;
 
@@ -2120,7 +2139,10 @@ bool rtc_compile(const std::string& src, std::vector<char>* code, std::string* e
     *err = "hipRTC: cannot create the lane kernel program";
     return false;
   }
-  const hiprtcResult rc = hiprtcCompileProgram(prog, kJitOptionCount, kJitOptions);
+  const std::vector<std::string> opt = jit_options();
+  std::vector<const char*> optp;
+  for (const std::string& x : opt) optp.push_back(x.c_str());
+  const hiprtcResult rc = hiprtcCompileProgram(prog, int(optp.size()), optp.data());
   if (rc != HIPRTC_SUCCESS) {
     size_t ls = 0;
     hiprtcGetProgramLogSize(prog, &ls);
@@ -2149,7 +2171,7 @@ bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* 
 
 bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err) {
   std::string opts;
-  for (int i = 0; i < kJitOptionCount; ++i) opts += std::string(kJitOptions[i]) + " ";
+  for (const std::string& x : jit_options()) opts += x + " ";
   const uint64_t key = fnv1a(src + opts);
   std::lock_guard<std::mutex> lk(g_jit_mu);
   auto it = g_jit.find({dev, key});
