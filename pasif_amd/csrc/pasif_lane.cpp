@@ -656,8 +656,8 @@ __device__ __forceinline__ void put_g(R* e, const R (&v)[NPF], u32 Lx, u32 b, bo
 // Mi[1][j] e[i] + Mi[0][j] e[i+1] on the half step i0*W + j0 + t, zero
 // unless 0 < i < L (src/kernels.cu:114-124)
 template <u32 SS>
-__device__ __forceinline__ R interp_at(const R* e, u32 b, GP(const R) wts, u32 W, u32 Lx, u32 i0,
-                                       u32 j0, u32 t) {
+__device__ __forceinline__ R interp_at(const R* e, u32 b, GP(const R) wts, u32 W, u32 i0, u32 j0,
+                                       u32 t) {
   u32 j = j0 + t;
   const bool next = j >= W;
   j = next ? j - W : j;
@@ -879,9 +879,9 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
               u1 = ft[(m + 1u) * IPW];
               u3 = ft[(m + 2u) * IPW];
             } else {
-              u0 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 0u);
-              u1 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 1u);
-              u3 = interp_at<IPW>(eb, ebase, wts, W, Lx, hi, hj, 2u);
+              u0 = interp_at<IPW>(eb, ebase, wts, W, hi, hj, 0u);
+              u1 = interp_at<IPW>(eb, ebase, wts, W, hi, hj, 1u);
+              u3 = interp_at<IPW>(eb, ebase, wts, W, hi, hj, 2u);
             }
             hj += 2u;
             if (hj >= W) {
@@ -955,7 +955,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
                   const u32 m = k / 3u;
                   const u32 hs = hj + 2u * m + (k - 3u * m);
                   const u32 di = hs / W;
-                  ft[k * IPW] = interp_at<IPW>(eb, ebase, wts, W, Lx, hi + di, hs - di * W, 0u);
+                  ft[k * IPW] = interp_at<IPW>(eb, ebase, wts, W, hi + di, hs - di * W, 0u);
                 }
               }
               __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
@@ -1271,9 +1271,9 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
               u1 = u0;
               u3 = e1;
             } else {
-              u0 = interp_at<LW>(eb, ebase, wts, W, Lx, hi, hj, 0u);
-              u1 = interp_at<LW>(eb, ebase, wts, W, Lx, hi, hj, 1u);
-              u3 = interp_at<LW>(eb, ebase, wts, W, Lx, hi, hj, 2u);
+              u0 = interp_at<LW>(eb, ebase, wts, W, hi, hj, 0u);
+              u1 = interp_at<LW>(eb, ebase, wts, W, hi, hj, 1u);
+              u3 = interp_at<LW>(eb, ebase, wts, W, hi, hj, 2u);
             }
           }
           if (!ZOH) {
