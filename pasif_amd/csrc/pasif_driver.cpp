@@ -308,6 +308,8 @@ struct pasif_pool {
     uint32_t tiles;
   };
   std::vector<Job> jobs;
+  // host staging of the lane programs of admitted jobs (reused)
+  std::vector<unsigned char> stage_host;
   // every job handle the pool created and that is not destroyed yet (when
   // the pool goes first, their device memory is freed and they are detached)
   std::set<pasif_driver*> handles;
@@ -449,6 +451,17 @@ void upload(pasif_driver* d, DevBuf& b, const std::vector<T>& h) {
   if (!h.empty())
     PASIF_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice,
                              d->stream));
+}
+
+void upload_bytes(pasif_driver* d, DevBuf& b, const void* h, size_t bytes) {
+  if (d->dev < 0) {  // host-only handle: packing and validation without a GPU
+    b.release();
+    b.bytes = bytes;
+    return;
+  }
+  dev_alloc(d, b, bytes);
+  if (bytes)
+    PASIF_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, d->stream));
 }
 
 void check_rows(pasif_driver* d, const CsrIn& K, const char* what) {
@@ -1137,6 +1150,92 @@ void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint
   for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
 }
 
+// lane_coefs through a merge map cached from one system of a class: the raw
+// entries of every row and kind (factor rows relative to the system) and the
+// merged slot each entry adds into.  A system whose raw entries are the same
+// takes its coefficients by one pass over its values (same slots, same
+// summation order as lane_coefs); any other goes through lane_coefs.
+struct CoefMap {
+  bool valid = false;
+  uint32_t n = 0, nslots = 0;
+  std::vector<uint32_t> cnt;    // [n][4] entries per row and kind
+  std::vector<uint32_t> rel;    // every entry's factor rows, relative, in order
+  std::vector<uint32_t> slot;   // every entry's merged slot
+  std::vector<uint8_t> first;   // 1: the first entry of its slot (assigned, not added)
+};
+
+void lane_coefs_mapped(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint,
+                       CoefMap& cm, std::vector<double>& out) {
+  const uint32_t off = rt.off[s];
+  const uint32_t xo = adjoint ? rt.xoff[s] : off;
+  const uint32_t n = rt.size[s];
+  if (!cm.valid) {
+    std::vector<LaneTerm> mt;
+    std::vector<uint32_t> which;
+    cm.n = n;
+    cm.cnt.assign(size_t(n) * 4, 0);
+    cm.rel.clear();
+    cm.slot.clear();
+    cm.first.clear();
+    uint32_t base = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+      merged_row(rt, s, r, adjoint, mt, which);
+      const uint32_t g = off + r;
+      uint32_t q = 0, fresh = 0;
+      for (int k = 0; k < 4; ++k) {
+        const int nf = k + 1;
+        cm.cnt[size_t(r) * 4 + k] = rt.ptr[k][g + 1] - rt.ptr[k][g];
+        for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e, ++q) {
+          cm.rel.push_back(rt.idx[k][size_t(e) * nf] - off);
+          for (int a = 1; a < nf; ++a) cm.rel.push_back(rt.idx[k][size_t(e) * nf + a] - xo);
+          cm.slot.push_back(base + which[q]);
+          cm.first.push_back(which[q] == fresh ? 1 : 0);
+          if (which[q] == fresh) ++fresh;
+        }
+      }
+      base += uint32_t(mt.size());
+    }
+    cm.nslots = base;
+    cm.valid = true;
+  } else {
+    // the same raw entries as the cached system?
+    bool same = n == cm.n;
+    size_t x = 0;
+    for (uint32_t r = 0; r < n && same; ++r) {
+      const uint32_t g = off + r;
+      for (int k = 0; k < 4 && same; ++k) {
+        const int nf = k + 1;
+        if (rt.ptr[k][g + 1] - rt.ptr[k][g] != cm.cnt[size_t(r) * 4 + k]) {
+          same = false;
+          break;
+        }
+        for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1] && same; ++e) {
+          same = rt.idx[k][size_t(e) * nf] - off == cm.rel[x++];
+          for (int a = 1; a < nf && same; ++a)
+            same = rt.idx[k][size_t(e) * nf + a] - xo == cm.rel[x++];
+        }
+      }
+    }
+    if (!same) {
+      lane_coefs(rt, s, sd, adjoint, out);
+      return;
+    }
+  }
+  out.assign(cm.nslots, 0.0);
+  size_t q = 0;
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t g = off + r;
+    for (int k = 0; k < 4; ++k)
+      for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e, ++q) {
+        if (cm.first[q])
+          out[cm.slot[q]] = rt.val[k][e];
+        else
+          out[cm.slot[q]] += rt.val[k][e];
+      }
+  }
+  for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
+}
+
 // Lanes per instance (powers of two up to a wave) at which a class fits the
 // register budget, with the step cost of each.
 std::vector<std::pair<uint32_t, double>> group_options(const LaneShape& sh, int bits) {
@@ -1483,18 +1582,27 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
   P->lay = lay;
   uint32_t block = 0, tmax = 0;
   std::vector<double> cv;
-  // the part's program arrays, 256-byte aligned in one host buffer: one
-  // upload for the whole part; (class, field, offset) patched in after it
-  std::vector<unsigned char> host;
+  // the part's program arrays, 256-byte aligned in one host buffer (a pool
+  // job reuses the pool's): one upload for the whole part.  Every class's
+  // fields are placed first, the buffer is cleared once, then the fields are
+  // written in place; (field, offset) is patched in after the upload.
+  std::vector<unsigned char> own;
+  std::vector<unsigned char>& host = d->pool ? d->pool->stage_host : own;
   std::vector<std::pair<const void**, size_t>> views;
-  auto stage = [&](const void** where, const void* data, size_t bytes) {
-    const size_t off = (host.size() + 255) / 256 * 256;
-    host.resize(off + std::max<size_t>(bytes, 1));
-    if (bytes) std::memcpy(host.data() + off, data, bytes);
+  size_t top = 0;
+  auto place = [&](const void** where, size_t bytes) {
+    const size_t off = (top + 255) / 256 * 256;
+    top = off + std::max<size_t>(bytes, 1);
     views.push_back({where, off});
+    return off;
   };
   const uint32_t out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
   const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
+  // classes and their fields' places
+  struct Fields {
+    size_t cf, ca, qf, qa, stp, ex, of, oa, ck;
+  };
+  std::vector<Fields> fld(shapes.size());
   for (size_t c = 0; c < shapes.size(); ++c) {
     auto& mem = members[c];
     if (mem.empty()) continue;
@@ -1520,17 +1628,48 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     L->nf = grp ? sh.gf.nl() : sh.f.n;
     L->na = adjoint ? (grp ? sh.ga.nl() : sh.a.n) : 0;
     L->klass = klass[c];
+    L->seg = adjoint ? ((lay.waves == 1 && sh.seg) ? sh.seg : lay.seg) : 1u;
     block += L->waves;
-    const uint32_t st = L->stride;
+    const size_t st = L->stride;
     const uint32_t TF = grp ? sh.gf.coefs() : sh.f.coefs();
     const uint32_t TA = adjoint ? (grp ? sh.ga.coefs() : sh.a.coefs()) : 0;
-    std::vector<R> cf(size_t(std::max<uint32_t>(TF, 1)) * st, R(0));
-    std::vector<R> ca(size_t(std::max<uint32_t>(TA, 1)) * st, R(0));
-    std::vector<R> qf(size_t(L->nf) * st, R(0));
-    std::vector<R> qa(size_t(std::max<uint32_t>(L->na, 1)) * st, R(0));
-    std::vector<uint32_t> stp(st, 0), of(st, 0), oa(st, 0);
-    std::vector<int32_t> ex(st, -1);
+    Fields& f = fld[c];
+    f.ck = (adjoint && lay.waves == 1) ? place(&L->ckoff, L->waves * sizeof(uint64_t)) : 0;
+    f.cf = place(&L->coef_f, size_t(std::max<uint32_t>(TF, 1)) * st * sizeof(R));
+    f.ca = place(&L->coef_a, size_t(std::max<uint32_t>(TA, 1)) * st * sizeof(R));
+    f.qf = place(&L->init_f, size_t(L->nf) * st * sizeof(R));
+    f.qa = place(&L->init_a, size_t(std::max<uint32_t>(L->na, 1)) * st * sizeof(R));
+    f.stp = place(&L->steps, st * sizeof(uint32_t));
+    f.ex = place(&L->exc, st * sizeof(int32_t));
+    f.of = place(&L->out_f, st * sizeof(uint32_t));
+    f.oa = place(&L->out_a, st * sizeof(uint32_t));
+    P->cls.push_back(std::move(L));
+  }
+  if (host.size() < top) host.resize(top);
+  std::memset(host.data(), 0, top);
+  size_t k = 0;
+  for (size_t c = 0; c < shapes.size(); ++c) {
+    auto& mem = members[c];
+    if (mem.empty()) continue;
+    LaneCls* L = P->cls[k++].get();
+    const LaneShape& sh = shapes[c];
+    const Fields& f = fld[c];
+    const bool grp = lay.waves == 1 && sh.group > 1;
+    const uint32_t G = grp ? sh.group : 1u;
+    const size_t st = L->stride;
+    const uint32_t TF = grp ? sh.gf.coefs() : sh.f.coefs();
+    const uint32_t TA = adjoint ? (grp ? sh.ga.coefs() : sh.a.coefs()) : 0;
+    R* cf = reinterpret_cast<R*>(host.data() + f.cf);
+    R* ca = reinterpret_cast<R*>(host.data() + f.ca);
+    R* qf = reinterpret_cast<R*>(host.data() + f.qf);
+    R* qa = reinterpret_cast<R*>(host.data() + f.qa);
+    uint32_t* stp = reinterpret_cast<uint32_t*>(host.data() + f.stp);
+    int32_t* ex = reinterpret_cast<int32_t*>(host.data() + f.ex);
+    uint32_t* of = reinterpret_cast<uint32_t*>(host.data() + f.of);
+    uint32_t* oa = reinterpret_cast<uint32_t*>(host.data() + f.oa);
+    std::fill(ex, ex + st, -1);
     std::vector<double> lv;
+    CoefMap map_f, map_a;
     // lane slot r of position p of a group side -> the system's row
     auto slot_row = [](const GSide& gs, uint32_t r, uint32_t p) {
       return r < gs.RL ? gs.band[r] + p * gs.stride[r] : gs.shared[r - gs.RL];
@@ -1544,7 +1683,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
                     PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
       PASIF_REQUIRE(in.exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
                     "lane plan: excitation out of range");
-      lane_coefs(d->fwd, in.sys, sh.f, false, cv);
+      lane_coefs_mapped(d->fwd, in.sys, sh.f, false, map_f, cv);
       PASIF_REQUIRE(cv.size() == sh.f.coefs(), PASIF_ERR_UNSUPPORTED,
                     "lane plan: structure mismatch");
       if (grp) {
@@ -1560,7 +1699,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
         for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
       }
       if (adjoint) {
-        lane_coefs(d->adj, in.sys, sh.a, true, cv);
+        lane_coefs_mapped(d->adj, in.sys, sh.a, true, map_a, cv);
         PASIF_REQUIRE(cv.size() == sh.a.coefs(), PASIF_ERR_UNSUPPORTED,
                       "lane plan: structure mismatch");
         if (grp) {
@@ -1584,11 +1723,10 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
       L->tmax = std::max(L->tmax, in.steps);
     }
     tmax = std::max(tmax, L->tmax);
-    L->seg = adjoint ? ((lay.waves == 1 && sh.seg) ? sh.seg : lay.seg) : 1u;
     if (adjoint && lay.waves == 1) {
       // wave w runs instances [w*ipw, (w+1)*ipw), the first with the most steps
       const uint32_t ipw = kLaneWave / G;
-      std::vector<uint64_t> off(L->waves, 0);
+      uint64_t* off = reinterpret_cast<uint64_t*>(host.data() + f.ck);
       uint64_t tot = 0;
       for (uint32_t w = 0; w < L->waves; ++w) {
         const uint32_t Tw = w * ipw < L->count ? stp[size_t(w) * ipw] : 0u;
@@ -1596,20 +1734,10 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
         tot += uint64_t(std::max<uint32_t>(1, (Tw + L->seg - 1) / L->seg)) * L->nf * kLaneWave;
       }
       L->ck_elems = tot;
-      stage(&L->ckoff, off.data(), off.size() * sizeof(uint64_t));
     }
-    stage(&L->coef_f, cf.data(), cf.size() * sizeof(R));
-    stage(&L->coef_a, ca.data(), ca.size() * sizeof(R));
-    stage(&L->init_f, qf.data(), qf.size() * sizeof(R));
-    stage(&L->init_a, qa.data(), qa.size() * sizeof(R));
-    stage(&L->steps, stp.data(), stp.size() * sizeof(uint32_t));
-    stage(&L->exc, ex.data(), ex.size() * sizeof(int32_t));
-    stage(&L->out_f, of.data(), of.size() * sizeof(uint32_t));
-    stage(&L->out_a, oa.data(), oa.size() * sizeof(uint32_t));
-    P->cls.push_back(std::move(L));
   }
   const auto u0 = std::chrono::steady_clock::now();
-  upload(d, P->prog, host);
+  upload_bytes(d, P->prog, host.data(), top);
   for (auto& v : views)
     *v.first = P->prog.p ? static_cast<const unsigned char*>(P->prog.p) + v.second : nullptr;
   if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
