@@ -183,17 +183,36 @@ class Request:
         return int(np.sum(self.steps))
 
 
+def _draw_tables(lib: SystemLibrary):
+    """The library's sizes with their probabilities, variant counts and
+    entries, computed once per library."""
+    t = getattr(lib, "_draw", None)
+    if t is None:
+        sizes = [int(d) for d in W.DOF_CHOICES if int(d) in lib.entries]
+        probs = np.array([p for d, p in zip(W.DOF_CHOICES, W.DOF_PROBS) if int(d) in lib.entries])
+        ents = [lib.entries[d] for d in sizes]
+        keys = [np.array([e.key for e in es], dtype=np.int64) for es in ents]
+        t = (np.arange(len(sizes)), probs / probs.sum(),
+             np.array([len(es) for es in ents], dtype=np.int64), ents, keys)
+        lib._draw = t
+    return t
+
+
 def draw_request(lib: SystemLibrary, rng, rid: int, n_signals: int, tmin=1e3, tmax=1e5,
                  max_systems: int = 256, adjoint_frac: float = 0.7) -> Request:
+    """A request of 1..max_systems library systems (sizes by the DOF
+    probabilities, a uniform variant of each), log-uniform step counts on
+    [tmin, tmax] and uniform signal ids; forward+adjoint with probability
+    ``adjoint_frac``."""
+    idx, probs, nvar, ents, keys = _draw_tables(lib)
     n = int(rng.integers(1, max_systems + 1))
-    sizes = np.array([d for d in W.DOF_CHOICES if int(d) in lib.entries])
-    probs = np.array([p for d, p in zip(W.DOF_CHOICES, W.DOF_PROBS) if int(d) in lib.entries])
-    dofs = rng.choice(sizes, size=n, p=probs / probs.sum())
-    systems = [lib.entries[int(d)][int(rng.integers(len(lib.entries[int(d)])))] for d in dofs]
+    di = rng.choice(idx, size=n, p=probs)
+    var = rng.integers(0, nvar[di])
+    systems = [ents[a][b] for a, b in zip(di.tolist(), var.tolist())]
     steps = np.exp(rng.uniform(np.log(tmin), np.log(tmax), n)).astype(np.int64)
+    k = np.array([keys[a][b] for a, b in zip(di.tolist(), var.tolist())], dtype=np.int64)
     return Request(rid, systems, rng.integers(0, n_signals, n), steps,
-                   bool(rng.uniform() < adjoint_frac),
-                   keys=np.array([s.key for s in systems], dtype=np.int64))
+                   bool(rng.uniform() < adjoint_frac), keys=k)
 
 
 # --------------------------------------------------------------- pool
