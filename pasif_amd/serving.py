@@ -422,7 +422,11 @@ class ServeReport:
 class SolveServer:
     def __init__(self, pool: Pool):
         self.pool = pool
-        self.inflight = []          # (job, request)
+        # admitted jobs by the tick after which their work is covered: a job
+        # of at most T steps per system needs ceil(T (3 with the adjoint) /
+        # steps_per_tick) ticks, so only the jobs due are asked whether they
+        # finished
+        self.due = {}               # tick -> [(job, request)]
 
     def run(self, requests: Sequence[Request], horizon_s: Optional[float] = None,
             drain: bool = True, max_wall_s: float = 600.0,
@@ -456,6 +460,7 @@ class SolveServer:
         by_id = self.pool.ids is not None
 
         pending = []                # finished after the last tick, not collected yet
+        nres = [0]                  # resident (admitted, not finished) jobs
 
         def arrivals_until(now):
             nonlocal i
@@ -487,7 +492,10 @@ class SolveServer:
                 blocked = True
                 return False
             r.admitted = time.perf_counter() - t0
-            self.inflight.append((job, r))
+            work = int(np.max(r.steps)) * (3 if r.adjoint else 1)
+            due = ticks + max(1, -(-work // self.pool.steps_per_tick))
+            self.due.setdefault(due, []).append((job, r))
+            nres[0] += 1
             admitted += 1
             queue.popleft()
             return True
@@ -499,11 +507,11 @@ class SolveServer:
             if horizon_s is not None and now > horizon_s and not drain:
                 break
             arrivals_until(now)
-            if not self.inflight:
+            if not nres[0]:
                 collect_pending()
                 while admit_one():
                     pass
-            if self.inflight:
+            if nres[0]:
                 self.pool.tick(wait=False)
                 ticks += 1
                 # while the launch runs: collect what the previous tick
@@ -515,10 +523,13 @@ class SolveServer:
                     if not admit_one():
                         break
                 busy_ms += self.pool.wait()
-                keep = []
-                for job, r in self.inflight:
-                    (pending if self.pool.finished(job) else keep).append((job, r))
-                self.inflight = keep
+                for t in [t for t in self.due if t <= ticks]:
+                    for job, r in self.due.pop(t):
+                        if self.pool.finished(job):
+                            pending.append((job, r))
+                            nres[0] -= 1
+                        else:               # not expected: ask again after the next tick
+                            self.due.setdefault(ticks + 1, []).append((job, r))
             elif pending:
                 collect_pending()
             elif i >= n_req and not queue:
@@ -529,7 +540,7 @@ class SolveServer:
         collect_pending()
         wall = time.perf_counter() - t0
         lat = np.array([(r.done - r.arrival) * 1e3 for r in done]) if done else np.zeros(1)
-        waiting = [(wall - r.arrival) * 1e3 for _, r in self.inflight] + \
+        waiting = [(wall - r.arrival) * 1e3 for lst in self.due.values() for _, r in lst] + \
             [(wall - arr[k]) * 1e3 for k in queue]
         lat_all = np.concatenate([lat, np.array(waiting)]) if waiting else lat
         steps = sum(r.system_steps for r in done)
