@@ -1526,6 +1526,10 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
       choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group, d->lane_bias);
       if (adjoint) lay.seg = single_wave_segs(shapes, sel_load, el, Kreq, d->dev);
       lay.occ = single_wave_occupancy(shapes, sel_load, bits, d->dev);
+      double waves = 0;
+      for (size_t c = 0; c < shapes.size(); ++c)
+        waves += std::ceil(sel_load[c].inst * shapes[c].group / double(kLaneWave));
+      lay.ilp = waves <= device_simds(d->dev);
     }
   }
   if (why) {
@@ -1546,7 +1550,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   if (d->dev >= 0) {
     const std::string src = lane_source(shapes, int(sizeof(R) * 8), lk);
     std::string err;
-    if (!lane_compile(src, d->dev, &part->kern, &err)) {
+    if (!lane_compile(src, d->dev, &part->kern, &err, lk.ilp)) {
       set_err(d, "%s", err.c_str());
       throw Fail{PASIF_ERR_DEVICE};
     }

@@ -1828,8 +1828,12 @@ constexpr int kJitOptionCount = sizeof(kJitOptions) / sizeof(kJitOptions[0]);
 
 // the options above, then any from PASIF_JIT_FLAGS (space separated, e.g.
 // "-mllvm -amdgpu-sched-strategy=max-ilp"); they are part of the cache key
-std::vector<std::string> jit_options() {
+std::vector<std::string> jit_options(bool ilp = false) {
   std::vector<std::string> o(kJitOptions, kJitOptions + kJitOptionCount);
+  if (ilp) {
+    o.push_back("-mllvm");
+    o.push_back("-amdgpu-sched-strategy=max-ilp");
+  }
   if (const char* e = std::getenv("PASIF_JIT_FLAGS")) {
     std::string t;
     for (const char* c = e;; ++c) {
@@ -2132,14 +2136,15 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const Lane
 
 namespace {
 // hipRTC compilation of `src` to a gfx950 code object
-bool rtc_compile(const std::string& src, std::vector<char>* code, std::string* err) {
+bool rtc_compile(const std::string& src, std::vector<char>* code, std::string* err,
+                 bool ilp = false) {
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "pasif_lane.hip", 0, nullptr, nullptr) !=
       HIPRTC_SUCCESS) {
     *err = "hipRTC: cannot create the lane kernel program";
     return false;
   }
-  const std::vector<std::string> opt = jit_options();
+  const std::vector<std::string> opt = jit_options(ilp);
   std::vector<const char*> optp;
   for (const std::string& x : opt) optp.push_back(x.c_str());
   const hiprtcResult rc = hiprtcCompileProgram(prog, int(optp.size()), optp.data());
@@ -2169,9 +2174,9 @@ bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* 
   return true;
 }
 
-bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err) {
+bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err, bool ilp) {
   std::string opts;
-  for (const std::string& x : jit_options()) opts += x + " ";
+  for (const std::string& x : jit_options(ilp)) opts += x + " ";
   const uint64_t key = fnv1a(src + opts);
   std::lock_guard<std::mutex> lk(g_jit_mu);
   auto it = g_jit.find({dev, key});
@@ -2197,7 +2202,7 @@ bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string*
     }
   }
   if (code.empty()) {
-    if (!rtc_compile(src, &code, err)) return false;
+    if (!rtc_compile(src, &code, err, ilp)) return false;
     if (!path.empty()) {
       mkdirs(dir);
       const std::string tmp = path + ".tmp" + std::to_string(long(getpid()));

@@ -172,6 +172,10 @@ struct LaneLayout {
   // single-wave kernels: waves each SIMD should hold at once (the register
   // budget the compiler gets is 512 / occ VGPRs); 0 or 1: no target
   uint32_t occ = 0;
+  // single-wave kernels whose waves each have a SIMD to themselves: a lone
+  // wave's dependency chains bound them, so they are compiled with the
+  // scheduler that maximises instruction-level parallelism
+  bool ilp = false;
 };
 
 // Split the rows of `sh` over `waves` waves (largest term cost first onto
@@ -196,7 +200,8 @@ struct LaneKernel {
 };
 // Compile (or fetch from the caches) the kernel of `src` for device `dev`.
 // Returns false with a message in *err on failure.
-bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err);
+bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err,
+                  bool ilp = false);
 // hipRTC compilation only (no device needed): size of the code object
 bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* err);
 
