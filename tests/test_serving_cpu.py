@@ -1,0 +1,81 @@
+"""The serving loop's host logic without a GPU: SolveServer against a
+stand-in pool that keeps the library pool's accounting (a tick adds
+steps_per_tick steps of work to every job admitted before it started; a job
+of at most T steps per system is finished once that covers T, 3T with the
+adjoint; a slot cap refuses submissions until jobs are collected)."""
+import numpy as np
+
+from pasif_amd import serving as S
+
+
+class _Job:
+    def __init__(self, req, work):
+        self.req, self.work, self.done, self.joined = req, work, 0, False
+
+
+class FakePool:
+    ids = None
+
+    def __init__(self, steps_per_tick, slots):
+        self.steps_per_tick = steps_per_tick
+        self.slots = slots
+        self.jobs = []
+        self.in_flight = False
+        self.finished_calls = 0
+        self.collected = []
+
+    def submit(self, req):
+        if len(self.jobs) >= self.slots:
+            return None
+        j = _Job(req, int(np.max(req.steps)) * (3 if req.adjoint else 1))
+        self.jobs.append(j)
+        return j
+
+    def tick(self, wait):
+        assert not self.in_flight
+        for j in self.jobs:
+            j.joined = True
+            if j.done < j.work:
+                j.done += self.steps_per_tick
+        self.in_flight = True
+        return len(self.jobs)
+
+    def running(self):
+        return False
+
+    def wait(self):
+        self.in_flight = False
+        return 1.0
+
+    def finished(self, job):
+        self.finished_calls += 1
+        return not self.in_flight and job.done >= job.work
+
+    def collect(self, job, req):
+        assert job.done >= job.work and job.req is req
+        self.jobs.remove(job)
+        self.collected.append(req.rid)
+        return np.zeros(1), (np.zeros(1) if req.adjoint else None)
+
+
+def _requests(rng, n):
+    out = []
+    for k in range(n):
+        m = int(rng.integers(1, 5))
+        out.append(S.Request(k, [None] * m, np.zeros(m, dtype=np.int64),
+                             rng.integers(1, 5000, m), bool(rng.uniform() < 0.7)))
+    return out
+
+
+def test_server_asks_only_due_jobs_and_completes_every_request():
+    rng = np.random.default_rng(3)
+    reqs = _requests(rng, 60)
+    pool = FakePool(steps_per_tick=700, slots=8)
+    rep = S.SolveServer(pool).run(reqs, arrivals=np.zeros(len(reqs)))
+    assert rep.completed == rep.admitted == len(reqs) and rep.backlog == 0
+    assert sorted(pool.collected) == list(range(len(reqs)))
+    # each job is asked once, at the tick that covers its work
+    assert pool.finished_calls == len(reqs)
+    # no job waited past that tick: total ticks are what the slot cap forces
+    assert rep.ticks <= sum(-(-int(np.max(r.steps)) * (3 if r.adjoint else 1) // 700)
+                            for r in reqs)
