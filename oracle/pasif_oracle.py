@@ -308,8 +308,8 @@ class InstanceLayout:
     (``parallelizeThroughExcitations``, ``src/__GpuDriver.cu:896-938``;
     replica ``p`` reads excitation ``d_indice/systemStride == p``,
     ``src/kernels.cu:85``): that is ``cartesian_layout``.  ``mapped_layout``
-    is this build's extension: one replica, each system its own excitation id
-    and its own step count.
+    is an extension beyond the reference: one replica, each system its own
+    excitation id and its own step count.
     """
     exc_id: np.ndarray          # [P][N] excitation index per replica/dof
     steps: np.ndarray           # [P][N] step count per replica/dof
@@ -459,7 +459,7 @@ def rk4_forward(sys: Assembled, forcing: Optional[Forcing], sample_rate: int,
     ``t % save_steps == 0`` (``src/__GpuDriver.cu:669-687``): slot ``j`` holds
     ``q_{j*s+1}``; ``floor(T/s)`` slots (``PASIf.py:412``).  ``window > 0``
     returns ``max |q_m|`` over the post-step states ``m`` in
-    ``(T-window, T]`` of each DOF (this build's fused reduction).
+    ``(T-window, T]`` of each DOF (a fused reduction the reference does not have).
     ``keep_states`` returns the states ``q_n`` for the listed step numbers
     (0 = initial state)."""
     dt = np.dtype(dtype)
