@@ -241,8 +241,7 @@ def batching_section(args) -> dict:
     _progress(f"batching: forward fp64 {t['kernel_ms']:.0f} ms, fp32 {t32['kernel_ms']:.0f} ms")
     p.setPrecision(64)
     t1 = time.perf_counter()
-    J = ADJ.energy_gradient_inputs(w)
-    p.setJacobian(*J.as_args())
+    p.setJacobianBlocks(ADJ.energy_gradient_blocks(w))
     sec["setup_s"] += time.perf_counter() - t1
     _progress("batching: adjoint inputs built")
     tune_a = AT.autotune(p, adjoint=True, probe_steps=5000, lanes=(128, 256), slots=(3, 1),

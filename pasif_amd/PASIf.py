@@ -332,6 +332,41 @@ class PASIf(_GpuDriver):
                            vecInitialConditions, vecPsi)
         self.__unfoldJacobians(vecM, vecB, vecK, vecGamma, vecLambda, vecForcePattern,
                                vecInitialConditions, vecPsi)
+        self.__loadJacobian()
+
+    def setJacobianBlocks(self, blocks):
+        """``setJacobian`` for adjoint systems given already concatenated
+        (``pasif_amd.adjoint.jacobian_blocks`` / ``energy_gradient_blocks``:
+        the arrays ``setJacobian`` unfolds from per-system inputs, entry for
+        entry), without a scipy object per system -- an extension for large
+        batches."""
+        self.jacobianSet = False
+        self.globalAdjointSize = 0
+        if not self.systemSet:
+            raise ValueError("The system must be set before the jacobian")
+        sizes = np.asarray(blocks.sizes, dtype=np.int64)
+        assert sizes.size == self.numberOfSystems, \
+            "The number of jacobian must be the same as the number of systems."
+        assert np.array_equal(np.asarray(blocks.fwd_sizes), np.asarray(self.system_sizes)), \
+            "The adjoint blocks were built for systems of other sizes."
+        n = int(sizes.sum())
+        self.globalAdjointSize = n
+        self.jacobian_sizes = sizes
+        self.jacobian_M = dia_matrix((-np.ones(n), [0]), shape=(n, n))
+        self.jacobian_B = coo_matrix((n, n))
+        self.jacobian_K = blocks.K
+        cp = []
+        for t in (blocks.gamma, blocks.lam, blocks.psi):
+            c = coo_tensor(list(t.dimensions))
+            c.val, c.indices = t.val, t.indices
+            cp.append(c)
+        self.jacobian_Gamma, self.jacobian_Lambda, self.jacobian_Psi = cp
+        self.jacobian_forcePattern = np.zeros(n)
+        self.jacobian_initialConditions = np.asarray(blocks.init, dtype=np.float64)
+        self.__loadJacobian()
+
+    def __loadJacobian(self):
+        """Scale the unfolded adjoint system and hand it to the driver."""
         self.__jacobianPreprocessing()
         jac_csr_K = (self.jacobian_B + self.jacobian_K).tocsr()
         jac_csr_K.sort_indices()

@@ -189,3 +189,148 @@ def energy_gradient_inputs(w, params_of_system=None) -> JacobianInputs:
         term[-1] = 1.0
         terms.append(term)
     return jacobian_inputs(w.vecM, w.vecB, w.vecK, w.vecGamma, w.vecLambda, params, terms)
+
+
+@dataclass
+class JacobianBlocks:
+    """The adjoint systems of many systems already concatenated block-diagonal:
+    exactly what ``PASIf.setJacobian`` unfolds from the per-system inputs of
+    ``jacobian_inputs`` (same entries in the same order), built without a
+    scipy object per system.  Pass it to ``PASIf.setJacobianBlocks``."""
+    sizes: np.ndarray        # adjoint size of each system
+    fwd_sizes: np.ndarray    # forward size of each system
+    K: coo_matrix            # A' = A^T entries (the unfolded K'; B' is empty)
+    gamma: coo_tensor        # [na, na, n]
+    lam: coo_tensor          # [na, na, n, n]
+    psi: coo_tensor          # [na, na, n, n, n]
+    init: np.ndarray         # lambda(T) and zero accumulators
+
+
+def jacobian_blocks(vecM, vecB, vecK, vecGamma, vecLambda,
+                    params: Sequence[Sequence[Param]], terminals) -> JacobianBlocks:
+    """``jacobian_inputs`` concatenated: systems with the same entry pattern
+    (sizes, B / K / Gamma / Lambda indices and parameters) are computed
+    together, then every system's entries are put back in system order."""
+    S = len(vecM)
+    raw, groups = [], {}
+    for s in range(S):
+        m = _diagonal(vecM[s])
+        rB, cB, vB = (np.asarray(x) for x in _coo_parts(vecB[s]))
+        rK, cK, vK = (np.asarray(x) for x in _coo_parts(vecK[s]))
+        gi, gv = _entries(vecGamma[s], 3)
+        li, lv = _entries(vecLambda[s], 4)
+        ps = tuple(tuple(p) for p in params[s])
+        key = (m.size, rB.astype(np.int64).tobytes(), cB.astype(np.int64).tobytes(),
+               rK.astype(np.int64).tobytes(), cK.astype(np.int64).tobytes(), gi.tobytes(),
+               li.tobytes(), ps)
+        groups.setdefault(key, []).append(s)
+        raw.append((m, vB, vK, gv, lv))
+    n_f = np.array([raw[s][0].size for s in range(S)], dtype=np.int64)
+    n_a = np.array([raw[s][0].size + len(params[s]) for s in range(S)], dtype=np.int64)
+    off_f = np.concatenate([[0], np.cumsum(n_f)[:-1]])
+    off_a = np.concatenate([[0], np.cumsum(n_a)[:-1]])
+    parts = {k: [] for k in ("K", "G", "L", "P")}
+    init = [None] * S
+    for key, members in groups.items():
+        s0 = members[0]
+        sel = np.asarray(members, dtype=np.int64)
+        n = int(key[0])
+        npar = len(params[s0])
+        rB, cB, vB = (np.asarray(x) for x in _coo_parts(vecB[s0]))
+        rK, cK, vK = (np.asarray(x) for x in _coo_parts(vecK[s0]))
+        gi, _ = _entries(vecGamma[s0], 3)
+        li, _ = _entries(vecLambda[s0], 4)
+        D = -1.0 / np.stack([raw[s][0] for s in members])                # [k, n]
+        # A' = A^T: B entries, then K entries, transposed, scaled by D of the row
+        rows = np.concatenate([cB, cK]).astype(np.int64)
+        cols = np.concatenate([rB, rK]).astype(np.int64)
+        rBi, rKi = rB.astype(np.int64), rK.astype(np.int64)
+        kv = np.concatenate([D[:, rBi] * np.stack([raw[s][1] for s in members]),
+                             D[:, rKi] * np.stack([raw[s][2] for s in members])], axis=1)
+        parts["K"].append((sel, np.stack([rows, cols], axis=1), kv, 2))
+        # tensor images of the forward terms, then the parameters' entries
+        gvals = np.stack([raw[s][3] for s in members]) if gi.shape[0] else np.zeros((len(sel), 0))
+        lvals = np.stack([raw[s][4] for s in members]) if li.shape[0] else np.zeros((len(sel), 0))
+        G_idx = [_interleave(gi, ((1, 0, 2), (2, 0, 1)))]
+        G_val = [np.repeat(D[:, gi[:, 0]] * gvals, 2, axis=1)]
+        L_idx = [_interleave(li, ((1, 0, 2, 3), (2, 0, 1, 3), (3, 0, 1, 2)))]
+        L_val = [np.repeat(D[:, li[:, 0]] * lvals, 3, axis=1)]
+        P_idx, P_val = [np.zeros((0, 5), dtype=np.int64)], [np.zeros((len(sel), 0))]
+        kinds = [p[0] for p in params[s0]]
+        for k in kinds:
+            if k not in ("K", "B", "Gamma", "Lambda"):
+                raise ValueError(f"unknown parameter kind {k!r}")
+        ps = [p for p, k in enumerate(kinds) if k in ("K", "B")]
+        if ps:
+            rc = np.array([(int(params[s0][p][1]), int(params[s0][p][2])) for p in ps],
+                          dtype=np.int64)
+            G_idx.append(np.column_stack([n + np.asarray(ps, dtype=np.int64), rc]))
+            G_val.append(D[:, rc[:, 0]])
+        ps = [p for p, k in enumerate(kinds) if k == "Gamma"]
+        if ps:
+            e = gi[np.array([int(params[s0][p][1]) for p in ps], dtype=np.int64)]
+            L_idx.append(np.column_stack([n + np.asarray(ps, dtype=np.int64), e]))
+            L_val.append(D[:, e[:, 0]])
+        ps = [p for p, k in enumerate(kinds) if k == "Lambda"]
+        if ps:
+            e = li[np.array([int(params[s0][p][1]) for p in ps], dtype=np.int64)]
+            P_idx.append(np.column_stack([n + np.asarray(ps, dtype=np.int64), e]))
+            P_val.append(D[:, e[:, 0]])
+        parts["G"].append((sel, np.concatenate(G_idx), np.concatenate(G_val, axis=1), 2))
+        parts["L"].append((sel, np.concatenate(L_idx), np.concatenate(L_val, axis=1), 2))
+        parts["P"].append((sel, np.concatenate(P_idx), np.concatenate(P_val, axis=1), 2))
+        for s in members:
+            lam_t = np.zeros(n + npar)
+            if terminals[s] is not None:
+                lam_t[:n] = np.asarray(terminals[s], dtype=np.float64)
+            init[s] = lam_t
+
+    def assemble(kind, order):
+        """every group's entries offset into the concatenation (the first two
+        axes by adjoint sizes, the rest by forward sizes), in system order"""
+        sys_ids, idx, val = [], [], []
+        for sel, li_, v, na_axes in parts[kind]:
+            k, e = v.shape
+            if e == 0:
+                continue
+            off = np.concatenate([np.repeat(off_a[sel][:, None], na_axes, axis=1),
+                                  np.repeat(off_f[sel][:, None], order - na_axes, axis=1)], axis=1)
+            idx.append((li_[None, :, :] + off[:, None, :]).reshape(-1, order))
+            val.append(v.reshape(-1))
+            sys_ids.append(np.repeat(sel, e))
+        if not idx:
+            return np.zeros((0, order), dtype=np.int64), np.zeros(0)
+        sid = np.concatenate(sys_ids)
+        o = np.argsort(sid, kind="stable")
+        return np.concatenate(idx)[o], np.concatenate(val)[o]
+
+    Na, Nf = int(n_a.sum()), int(n_f.sum())
+    ki, kv = assemble("K", 2)
+    K = coo_matrix((kv, (ki[:, 0], ki[:, 1])), shape=(Na, Na))
+    gi, gv = assemble("G", 3)
+    li, lv = assemble("L", 4)
+    pi, pv = assemble("P", 5)
+    return JacobianBlocks(sizes=n_a, fwd_sizes=n_f, K=K,
+                          gamma=_tensor([Na, Na, Nf], gi, gv),
+                          lam=_tensor([Na, Na, Nf, Nf], li, lv),
+                          psi=_tensor([Na, Na, Nf, Nf, Nf], pi, pv),
+                          init=np.concatenate(init) if S else np.zeros(0))
+
+
+def energy_gradient_blocks(w, params_of_system=None) -> JacobianBlocks:
+    """``energy_gradient_inputs`` as ``JacobianBlocks``."""
+    params, terms = [], []
+    for s in range(w.n_systems):
+        n = w.vecM[s].shape[0]
+        d = (n - 1) // 2
+        if params_of_system is not None:
+            ps = params_of_system[s]
+        else:
+            ps = [("K", d + i, i) for i in range(d)]
+            nl = len(np.asarray(w.vecLambda[s].val).reshape(-1))
+            ps += [("Lambda", t) for t in range(nl)]
+        params.append(ps)
+        term = np.zeros(n)
+        term[-1] = 1.0
+        terms.append(term)
+    return jacobian_blocks(w.vecM, w.vecB, w.vecK, w.vecGamma, w.vecLambda, params, terms)
