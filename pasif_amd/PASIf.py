@@ -121,18 +121,23 @@ def _cat_tensors(tensors: Sequence[coo_tensor]) -> coo_tensor:
     """Block-diagonal concatenation of a list of tensors in one pass
     (``PASIf.py:504-519`` / ``584-586`` without mutating the inputs)."""
     nd = len(tensors[0].dimensions)
-    dims = np.array([t.dimensions for t in tensors], dtype=np.int64).reshape(len(tensors), nd)
-    offs = np.concatenate([np.zeros((1, nd), dtype=np.int64), np.cumsum(dims, axis=0)[:-1]])
-    idx, val = [], []
-    for t, o in zip(tensors, offs):
-        ix, v = t._arrays()
+    for t in tensors:
         if len(t.dimensions) != nd:
             raise Exception("The number of dimensions of the two tensors are not the same")
-        idx.append(ix + o[None, :])
+    dims = np.array([t.dimensions for t in tensors], dtype=np.int64).reshape(len(tensors), nd)
+    offs = np.concatenate([np.zeros((1, nd), dtype=np.int64), np.cumsum(dims, axis=0)[:-1]])
+    # every tensor's entries gathered as they are, offsets added once
+    idx, val = [], []
+    for t in tensors:
+        v = np.asarray(t.val, dtype=np.float64).reshape(-1)
         val.append(v)
+        idx.append(np.asarray(t.indices).reshape(-1) if v.size else np.zeros(0, dtype=np.int64))
     out = coo_tensor(list(int(x) for x in dims.sum(axis=0)))
     out.val = np.concatenate(val) if val else np.zeros(0)
-    out.indices = np.concatenate(idx, axis=0).reshape(-1) if idx else np.zeros(0, dtype=np.int64)
+    counts = np.array([v.size for v in val], dtype=np.int64)
+    ix = (np.concatenate(idx).astype(np.int64, copy=False).reshape(-1, nd) if idx
+          else np.zeros((0, nd), dtype=np.int64))
+    out.indices = (ix + np.repeat(offs, counts, axis=0)).reshape(-1)
     return out
 
 
@@ -170,12 +175,16 @@ def _diagonal(m) -> np.ndarray:
 
 def _cat_matrices(mats, offs, n):
     rows, cols, vals = [], [], []
-    for m, o in zip(mats, offs):
+    for m in mats:
         r, c, v = _coo_parts(m)
-        rows.append(np.asarray(r, dtype=np.int64) + o)
-        cols.append(np.asarray(c, dtype=np.int64) + o)
-        vals.append(np.asarray(v, dtype=np.float64))
-    return coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+        rows.append(r)
+        cols.append(c)
+        vals.append(v)
+    # offsets added once for all blocks
+    shift = np.repeat(np.asarray(offs, dtype=np.int64), [len(v) for v in vals])
+    return coo_matrix((np.concatenate(vals).astype(np.float64, copy=False),
+                       (np.concatenate(rows).astype(np.int64, copy=False) + shift,
+                        np.concatenate(cols).astype(np.int64, copy=False) + shift)),
                       shape=(n, n))
 
 
