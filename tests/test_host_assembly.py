@@ -111,3 +111,39 @@ def test_adjoint_assembler_entries_in_order():
     assert np.array_equal(lamT[:n], np.arange(n)) and not lamT[n:].any()
     with pytest.raises(ValueError):
         ADJ.system_jacobian(M, B, K, G, L, [("Psi", 0)])
+
+
+@pytest.mark.parametrize("name", ["two_dof", "duffing", "chains", "heterogeneous"])
+def test_block_diagonal_assembly_matches_scipy_and_a_plain_loop(name):
+    """setSystems' concatenation (offsets added once per batch) equals
+    scipy's block_diag for B and K and a system-by-system loop for the
+    tensors, after the -M^-1 scaling."""
+    import os
+    import scipy.sparse as sp
+    from pasif_amd import workloads as W
+    w = {"two_dof": lambda: W.two_dof(50, 20), "duffing": lambda: W.duffing(16, 20),
+         "chains": lambda: W.chains(12, 5, 40),
+         "heterogeneous": lambda: W.heterogeneous(60, 30, 300, steps_per_sample=5,
+                                                  n_signals=3)}[name]()
+    os.environ["LOCAL_RANK"] = "-1"
+    try:
+        p = W.make_solver(w)
+    finally:
+        del os.environ["LOCAL_RANK"]
+    D = -1.0 / np.concatenate([np.asarray(m.diagonal()) for m in w.vecM])
+    for got, mats in ((p.system_B, w.vecB), (p.system_K, w.vecK)):
+        ref = sp.diags(D) @ sp.block_diag([sp.coo_matrix(m) for m in mats])
+        assert np.array_equal(got.toarray(), ref.toarray())
+    for got, ts in ((p.system_Gamma, w.vecGamma), (p.system_Lambda, w.vecLambda)):
+        nd = len(ts[0].dimensions)
+        off = np.zeros(nd, dtype=np.int64)
+        idx, val = [], []
+        for t in ts:
+            v = np.asarray(t.val, dtype=np.float64).reshape(-1)
+            ix = np.asarray(t.indices, dtype=np.int64).reshape(-1, nd)
+            idx.append(ix + off)
+            val.append(v)
+            off = off + np.asarray(t.dimensions, dtype=np.int64)
+        idx, val = np.concatenate(idx), np.concatenate(val)
+        assert np.array_equal(np.asarray(got.indices).reshape(-1, nd), idx)
+        assert np.array_equal(np.asarray(got.val), val * D[idx[:, 0]])
