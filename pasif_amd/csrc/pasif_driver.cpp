@@ -1550,6 +1550,10 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   if (d->dev >= 0) {
     const std::string src = lane_source(shapes, int(sizeof(R) * 8), lk);
     std::string err;
+    if (!lane_jit_flags_ok(&err)) {
+      set_err(d, "%s", err.c_str());
+      throw Fail{PASIF_ERR_ARGUMENT};
+    }
     if (!lane_compile(src, d->dev, &part->kern, &err, lk.ilp)) {
       set_err(d, "%s", err.c_str());
       throw Fail{PASIF_ERR_DEVICE};
@@ -2673,7 +2677,11 @@ int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_
     if (compile) {
       std::string err;
       size_t bytes = 0;
-      if (!lane_compile_only(src, &bytes, &err)) {
+      if (!lane_jit_flags_ok(&err)) {
+        set_err(d, "%s", err.c_str());
+        throw Fail{PASIF_ERR_ARGUMENT};
+      }
+      if (!lane_compile_only(src, &bytes, &err, lay.ilp)) {
         set_err(d, "%s", err.c_str());
         throw Fail{PASIF_ERR_DEVICE};
       }
@@ -3110,6 +3118,10 @@ void pool_build_modules(pasif_pool* P) {
     const std::string src = lane_source(M.shapes, int(el * 8), lay);
     std::string err;
     POOL_HIP(hipSetDevice(P->dev));
+    if (!lane_jit_flags_ok(&err)) {
+      P->err = err;
+      throw Fail{PASIF_ERR_ARGUMENT};
+    }
     if (!lane_compile(src, P->dev, &M.kern, &err)) {
       P->err = err;
       throw Fail{PASIF_ERR_DEVICE};

@@ -1826,27 +1826,86 @@ const char* kJitOptions[] = {"--gpu-architecture=gfx950", "-O3", "-std=c++17", "
                              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
 constexpr int kJitOptionCount = sizeof(kJitOptions) / sizeof(kJitOptions[0]);
 
-// the options above, then any from PASIF_JIT_FLAGS (space separated, e.g.
-// "-mllvm -amdgpu-sched-strategy=max-ilp"); they are part of the cache key
-std::vector<std::string> jit_options(bool ilp = false) {
-  std::vector<std::string> o(kJitOptions, kJitOptions + kJitOptionCount);
-  if (ilp) {
-    o.push_back("-mllvm");
-    o.push_back("-amdgpu-sched-strategy=max-ilp");
-  }
-  if (const char* e = std::getenv("PASIF_JIT_FLAGS")) {
-    std::string t;
-    for (const char* c = e;; ++c) {
-      if (*c == ' ' || *c == '\0') {
-        if (!t.empty()) o.push_back(t);
-        t.clear();
-        if (*c == '\0') break;
-      } else {
-        t += *c;
-      }
+// Caller options come from PASIF_JIT_FLAGS (space separated).  hipRTC runs
+// LLVM inside this process, and LLVM ends the process on option errors: an
+// unknown `-mllvm` option exits, an option given twice in one compilation is
+// refused, and the iterative-ilp machine scheduler crashes (SIGSEGV) on the
+// one-lane two-DOF forward program.  So only options compiled cleanly on the
+// generated programs are accepted, each replacing the library's own option of
+// its kind, and anything else is an argument error returned to the caller:
+//   -O0 .. -O3                          (replaces -O3)
+//   -mllvm -amdgpu-sched-strategy=S     S in kJitSchedulers (replaces the
+//                                       library's max-ilp choice)
+const char* const kJitSchedulers[] = {"max-ilp", "max-memory-clause", "iterative-minreg",
+                                      "iterative-maxocc"};
+
+bool jit_user_options(std::string* opt_level, std::string* sched, std::string* err) {
+  const char* e = std::getenv("PASIF_JIT_FLAGS");
+  if (!e) return true;
+  std::vector<std::string> tok;
+  std::string t;
+  for (const char* c = e;; ++c) {
+    if (*c == ' ' || *c == '\t' || *c == '\0') {
+      if (!t.empty()) tok.push_back(t);
+      t.clear();
+      if (*c == '\0') break;
+    } else {
+      t += *c;
     }
   }
-  return o;
+  bool have_o = false, have_s = false;
+  for (size_t i = 0; i < tok.size(); ++i) {
+    const std::string& x = tok[i];
+    if (x == "-O0" || x == "-O1" || x == "-O2" || x == "-O3") {
+      if (have_o) {
+        *err = "PASIF_JIT_FLAGS: more than one optimisation level";
+        return false;
+      }
+      have_o = true;
+      *opt_level = x;
+      continue;
+    }
+    const std::string pre = "-amdgpu-sched-strategy=";
+    if (x == "-mllvm" && i + 1 < tok.size() && tok[i + 1].compare(0, pre.size(), pre) == 0) {
+      const std::string v = tok[i + 1].substr(pre.size());
+      bool ok = false;
+      for (const char* s : kJitSchedulers) ok = ok || v == s;
+      if (!ok) {
+        *err = "PASIF_JIT_FLAGS: scheduler '" + v +
+               "' is not supported (accepted: max-ilp, max-memory-clause, iterative-minreg, "
+               "iterative-maxocc)";
+        return false;
+      }
+      if (have_s) {
+        *err = "PASIF_JIT_FLAGS: -amdgpu-sched-strategy given more than once";
+        return false;
+      }
+      have_s = true;
+      *sched = v;
+      ++i;
+      continue;
+    }
+    *err = "PASIF_JIT_FLAGS: unsupported option '" + x +
+           "' (accepted: -O0..-O3, -mllvm -amdgpu-sched-strategy=<scheduler>)";
+    return false;
+  }
+  return true;
+}
+
+// Options of one lane-kernel compilation: the defaults, the machine scheduler
+// (max-ilp for launches bound by a lone wave's dependency chains, `ilp`), and
+// the caller's replacements; they are part of the cache key.
+bool jit_options(bool ilp, std::vector<std::string>* o, std::string* err) {
+  std::string opt_level = "-O3", sched = ilp ? "max-ilp" : "";
+  if (!jit_user_options(&opt_level, &sched, err)) return false;
+  o->assign(kJitOptions, kJitOptions + kJitOptionCount);
+  for (std::string& x : *o)
+    if (x == "-O3") x = opt_level;
+  if (!sched.empty()) {
+    o->push_back("-mllvm");
+    o->push_back("-amdgpu-sched-strategy=" + sched);
+  }
+  return true;
 }
 
 std::mutex g_jit_mu;
@@ -2144,7 +2203,11 @@ bool rtc_compile(const std::string& src, std::vector<char>* code, std::string* e
     *err = "hipRTC: cannot create the lane kernel program";
     return false;
   }
-  const std::vector<std::string> opt = jit_options(ilp);
+  std::vector<std::string> opt;
+  if (!jit_options(ilp, &opt, err)) {
+    hiprtcDestroyProgram(&prog);
+    return false;
+  }
   std::vector<const char*> optp;
   for (const std::string& x : opt) optp.push_back(x.c_str());
   const hiprtcResult rc = hiprtcCompileProgram(prog, int(optp.size()), optp.data());
@@ -2167,16 +2230,23 @@ bool rtc_compile(const std::string& src, std::vector<char>* code, std::string* e
 }
 }  // namespace
 
-bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* err) {
+bool lane_jit_flags_ok(std::string* err) {
+  std::string o, s;
+  return jit_user_options(&o, &s, err);
+}
+
+bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* err, bool ilp) {
   std::vector<char> code;
-  if (!rtc_compile(src, &code, err)) return false;
+  if (!rtc_compile(src, &code, err, ilp)) return false;
   if (code_bytes) *code_bytes = code.size();
   return true;
 }
 
 bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err, bool ilp) {
+  std::vector<std::string> optv;
+  if (!jit_options(ilp, &optv, err)) return false;
   std::string opts;
-  for (const std::string& x : jit_options(ilp)) opts += x + " ";
+  for (const std::string& x : optv) opts += x + " ";
   const uint64_t key = fnv1a(src + opts);
   std::lock_guard<std::mutex> lk(g_jit_mu);
   auto it = g_jit.find({dev, key});

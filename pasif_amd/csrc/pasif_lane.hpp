@@ -203,6 +203,9 @@ struct LaneKernel {
 bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err,
                   bool ilp = false);
 // hipRTC compilation only (no device needed): size of the code object
-bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* err);
+bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* err,
+                       bool ilp = false);
+// checks the caller's PASIF_JIT_FLAGS against the accepted options
+bool lane_jit_flags_ok(std::string* err);
 
 }  // namespace pasif

@@ -217,3 +217,45 @@ def test_autotune_probe_keeps_step_ratios_and_precision_in_key():
     key64 = AT.signature(p, False)
     p.setPrecision(32)
     assert AT.signature(p, False) != key64
+
+
+_JIT_FLAG_SCRIPT = r"""
+import os, sys
+sys.path.insert(0, sys.argv[1])
+os.environ["LOCAL_RANK"] = "-1"
+from pasif_amd import _lib, workloads as W
+p = W.make_solver(W.two_dof(4096, 200))
+assert p.kernelInfo()["lanes_per_instance"] == [2]
+try:
+    p._laneSource(False, compile=True)
+    print("COMPILED")
+except _lib.PasifError as e:
+    print("PASIF_ERROR", e.code, str(e).splitlines()[0])
+"""
+
+
+@pytest.mark.parametrize("flags,expect", [
+    # the scheduler that crashed LLVM inside hipRTC on this program: refused
+    ("-mllvm -amdgpu-sched-strategy=iterative-ilp", "PASIF_ERROR 1"),
+    # accepted schedulers replace the library's max-ilp (one value, no clash)
+    ("-mllvm -amdgpu-sched-strategy=iterative-maxocc", "COMPILED"),
+    ("-mllvm -amdgpu-sched-strategy=max-ilp", "COMPILED"),
+    ("-O2", "COMPILED"),
+    # LLVM exits the process on an unknown option: refused before compiling
+    ("-mllvm -no-such-llvm-option=1", "PASIF_ERROR 1"),
+    ("-ffast-math", "PASIF_ERROR 1"),
+    ("-mllvm -amdgpu-sched-strategy=max-ilp -mllvm -amdgpu-sched-strategy=max-ilp",
+     "PASIF_ERROR 1"),
+])
+def test_jit_flags_compile_or_raise_never_signal(flags, expect, tmp_path):
+    # 4,096 two-DOF systems on lane pairs (128 waves, one per SIMD): the
+    # launch compiled with the library's max-ilp scheduler.  Run in a child process so that a
+    # crash inside the compiler fails this test instead of ending pytest.
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, PASIF_JIT_FLAGS=flags, PASIF_JIT_CACHE=str(tmp_path))
+    r = subprocess.run([sys.executable, "-c", _JIT_FLAG_SCRIPT, root], env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+    assert r.stdout.strip().splitlines()[-1].startswith(expect), r.stdout
