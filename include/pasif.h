@@ -250,6 +250,32 @@ int pasif_pool_submit_systems(pasif_pool* pool, uint32_t n, const uint32_t* ids,
 /* *running = 1 while the last tick's launch is still executing */
 int pasif_pool_poll(pasif_pool* pool, int* running);
 
+/* ---------------- no-allocation solve (extension) -----------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the forward solve (adjoint == 0, as pasif_get_amplitudes)
+ * or the forward + adjoint solve (adjoint != 0, as pasif_get_gradient) of the
+ * problem already loaded in `d`, with every run buffer (states, progress
+ * records, checkpoints, recomputed segments, kernel argument tables) in a
+ * static device array of PASIF_NOALLOC_SCRATCH_BYTES per GPU, held by the
+ * library until the process ends.  It calls no hipMalloc / hipFree.
+ *  - Set-up (allocating, once per problem and settings): the output-length
+ *    query pasif_get_amplitudes(d, NULL, 0, &n) or pasif_get_gradient(d, 0,
+ *    NULL, 0, &n) packs the problem and uploads it and the excitations.
+ *  - out_q: device buffer of the final states (pasif_get_amplitudes' length),
+ *    out_grad: device buffer of lambda_0 (pasif_get_gradient's length,
+ *    adjoint only), both of the working precision (double for 64 bits,
+ *    float for 32).  Results equal those of the allocating entry points.
+ *  - Returns when the results are in place.  PASIF_ERR_MEMORY if the run
+ *    buffers exceed the scratch array, PASIF_ERR_STATE without the set-up,
+ *    PASIF_ERR_UNSUPPORTED with an amplitude window, the graph solver or a
+ *    pool job.
+ *  - Every call on one GPU shares that GPU's array: calls that overlap in
+ *    time on the same GPU (two threads, two drivers) race and must be
+ *    serialised by the caller. */
+#define PASIF_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pasif_solve_noalloc(pasif_driver* d, int adjoint, void* out_q, void* out_grad);
+
 /* library build string */
 const char* pasif_version(void);
 

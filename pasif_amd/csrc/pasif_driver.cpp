@@ -411,7 +411,16 @@ int guarded(pasif_driver* d, F&& f) {
   }
 }
 
+static_assert(kScratchBytes == PASIF_NOALLOC_SCRATCH_BYTES, "scratch size stated in pasif.h");
+
+// set while pasif_solve_noalloc runs on this thread: any device allocation
+// is then a library bug, reported instead of performed
+thread_local bool t_no_alloc = false;
+
 void dev_alloc(pasif_driver* d, DevBuf& b, size_t bytes) {
+  PASIF_REQUIRE(!t_no_alloc, PASIF_ERR_STATE,
+                "internal error: device allocation of %zu bytes inside pasif_solve_noalloc",
+                bytes);
   b.release();
   if (bytes == 0) bytes = 16;
   if (d->pool && d->pool->arena) {
@@ -1847,14 +1856,38 @@ void fill_program(Program<R>& p, const DevBuf& info, const DevBuf& out, const De
   p.init = init.as<R>();
 }
 
-// Pack (if needed), allocate the run buffers and clear the tile progress:
-// the arguments of a run of the packed tiles.  *work is the number of steps
+// Run buffers of one solve, without the outputs: the driver's own device
+// buffers, allocated on first use and reused by later solves, or -- for
+// pasif_solve_noalloc -- consecutive 256-byte aligned pieces of the library's
+// static scratch array.  A sizing pass only adds up the bytes.
+struct RunMem {
+  pasif_driver* d = nullptr;
+  unsigned char* base = nullptr;  // scratch region (null: the driver's buffers)
+  size_t cap = 0, used = 0;
+  bool sizing = false;
+  void* get(DevBuf& own, size_t bytes) {
+    if (!base && !sizing) {
+      dev_reserve(d, own, bytes);
+      return own.p;
+    }
+    used = (used + 255) & ~size_t(255);
+    void* p = sizing ? nullptr : base + used;
+    used += bytes ? bytes : 16;
+    PASIF_REQUIRE(sizing || used <= cap, PASIF_ERR_MEMORY, "run scratch overflow");
+    return p;
+  }
+  void zero(void* p, size_t bytes) {
+    if (!sizing) PASIF_HIP(hipMemsetAsync(p, 0, bytes, d->stream));
+  }
+};
+
+// Arguments of a run of the packed tiles with the run buffers from `mem`
+// (progress cleared) and the given outputs.  *work is the number of steps
 // of work each tile needs (3T with the adjoint: forward, recompute, reverse).
 template <typename R>
-TileArgs<R> prepare_run(pasif_driver* d, bool adjoint, uint32_t save_steps, uint64_t* work_out,
-                        uint32_t* nslots_out) {
-  ensure_packed(d, adjoint);
-  ensure_excitations<R>(d);
+TileArgs<R> tile_run_args(pasif_driver* d, bool adjoint, uint32_t save_steps, RunMem& mem,
+                          void* out_q, void* out_max, void* traj, void* out_grad,
+                          uint64_t* work_out, uint32_t* nslots_out) {
   Packed& pk = d->pk;
   const uint32_t L = pk.lanes, nt = pk.ntiles;
   const size_t rows = size_t(nt) * L;
@@ -1879,28 +1912,20 @@ TileArgs<R> prepare_run(pasif_driver* d, bool adjoint, uint32_t save_steps, uint
   a.h = h;
   a.h2 = R(double(h) / 2.0);
   a.h6 = R(double(h) / 6.0);
-  dev_reserve(d, d->qstate, rows * sizeof(R));
-  dev_reserve(d, d->progress, size_t(nt) * 4 * sizeof(uint32_t));
-  PASIF_HIP(hipMemsetAsync(d->progress.p, 0, size_t(nt) * 4 * sizeof(uint32_t), d->stream));
-  a.qstate = d->qstate.as<R>();
-  a.progress = d->progress.as<uint32_t>();
+  a.qstate = static_cast<R*>(mem.get(d->qstate, rows * sizeof(R)));
+  a.progress = static_cast<uint32_t*>(mem.get(d->progress, size_t(nt) * 4 * sizeof(uint32_t)));
+  mem.zero(a.progress, size_t(nt) * 4 * sizeof(uint32_t));
   a.window = d->window;
   if (d->window) {
-    dev_reserve(d, d->mxstate, rows * sizeof(R));
-    dev_reserve(d, d->out_max, size_t(pk.out_len_f) * sizeof(R));
-    PASIF_HIP(hipMemsetAsync(d->out_max.p, 0, size_t(pk.out_len_f) * sizeof(R), d->stream));
-    a.mxstate = d->mxstate.as<R>();
-    a.out_max = d->out_max.as<R>();
+    a.mxstate = static_cast<R*>(mem.get(d->mxstate, rows * sizeof(R)));
+    a.out_max = static_cast<R*>(out_max);
   }
-  dev_reserve(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
-  a.out_q = d->out_q.as<R>();
+  a.out_q = static_cast<R*>(out_q);
   a.out_len = pk.out_len_f;
   uint32_t nslots = 0;
   if (save_steps) {
     nslots = pk.tmax / save_steps;
-    dev_reserve(d, d->traj, size_t(nslots) * pk.out_len_f * sizeof(R));
-    PASIF_HIP(hipMemsetAsync(d->traj.p, 0, size_t(nslots) * pk.out_len_f * sizeof(R), d->stream));
-    a.traj = d->traj.as<R>();
+    a.traj = static_cast<R*>(traj);
     a.save_every = save_steps;
     a.nslots = nslots;
   }
@@ -1909,14 +1934,10 @@ TileArgs<R> prepare_run(pasif_driver* d, bool adjoint, uint32_t save_steps, uint
     uint32_t K = d->ck_every ? d->ck_every : 256;
     K = std::max<uint32_t>(1, std::min<uint32_t>(K, std::max<uint32_t>(pk.tmax, 1)));
     const uint32_t nck = std::max<uint32_t>(1, (pk.tmax + K - 1) / K);
-    dev_reserve(d, d->lstate, rows * sizeof(R));
-    dev_reserve(d, d->ckpt, rows * nck * sizeof(R));
-    dev_reserve(d, d->segbuf, rows * (K + 1) * sizeof(R));
-    dev_reserve(d, d->out_grad, size_t(pk.out_len_a) * sizeof(R));
-    a.lstate = d->lstate.as<R>();
-    a.ckpt = d->ckpt.as<R>();
-    a.segbuf = d->segbuf.as<R>();
-    a.out_grad = d->out_grad.as<R>();
+    a.lstate = static_cast<R*>(mem.get(d->lstate, rows * sizeof(R)));
+    a.ckpt = static_cast<R*>(mem.get(d->ckpt, rows * nck * sizeof(R)));
+    a.segbuf = static_cast<R*>(mem.get(d->segbuf, rows * (K + 1) * sizeof(R)));
+    a.out_grad = static_cast<R*>(out_grad);
     a.ck_every = K;
     a.nck = nck;
     a.adjoint = 1;
@@ -1925,6 +1946,22 @@ TileArgs<R> prepare_run(pasif_driver* d, bool adjoint, uint32_t save_steps, uint
   *work_out = work;
   *nslots_out = nslots;
   return a;
+}
+
+template <typename R>
+uint32_t run_outputs(pasif_driver* d, bool adjoint, uint32_t save_steps);
+
+// Pack (if needed), allocate the outputs and run buffers and clear the tile
+// progress: the arguments of a run of the packed tiles.
+template <typename R>
+TileArgs<R> prepare_run(pasif_driver* d, bool adjoint, uint32_t save_steps, uint64_t* work_out,
+                        uint32_t* nslots_out) {
+  ensure_packed(d, adjoint);
+  (void)run_outputs<R>(d, adjoint, save_steps);
+  RunMem mem;
+  mem.d = d;
+  return tile_run_args<R>(d, adjoint, save_steps, mem, d->out_q.p, d->out_max.p, d->traj.p,
+                          adjoint ? d->out_grad.p : nullptr, work_out, nslots_out);
 }
 
 // Copy the results of a finished run back to the host.
@@ -2034,9 +2071,10 @@ void drive(pasif_driver* d, uint64_t launches, F&& launch) {
   d->last_launches = uint32_t(launches);
 }
 
-// Output buffers the lane parts write (shared with the tile part).
+// Output buffers of a solve (written by the lane and the tile parts); the
+// device excitations in the working precision.
 template <typename R>
-uint32_t lane_outputs(pasif_driver* d, bool adjoint, uint32_t save_steps) {
+uint32_t run_outputs(pasif_driver* d, bool adjoint, uint32_t save_steps) {
   ensure_excitations<R>(d);
   Packed& pk = d->pk;
   dev_reserve(d, d->out_q, size_t(pk.out_len_f) * sizeof(R));
@@ -2055,34 +2093,42 @@ uint32_t lane_outputs(pasif_driver* d, bool adjoint, uint32_t save_steps) {
 }
 
 // Run buffers of one lane part and its per-class kernel arguments (budget
-// filled in by the caller).  K is the part's checkpoint interval.
+// filled in by the caller).  K is the part's checkpoint interval.  Buffers
+// come from `mem` (null: the driver's own), outputs are the driver's unless
+// given.
 template <typename R>
 std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
                                      uint32_t save_steps, uint32_t nslots, const void* excit,
-                                     const void* interp) {
+                                     const void* interp, RunMem* mem_in = nullptr,
+                                     void* out_q = nullptr, void* out_grad = nullptr) {
   Packed& pk = d->pk;
+  RunMem own;
+  own.d = d;
+  RunMem& mem = mem_in ? *mem_in : own;
+  if (!out_q) out_q = d->out_q.p;
+  if (!out_grad && adjoint) out_grad = d->out_grad.p;
   const R h = R(1.0 / double(d->sample_rate));
   std::vector<LaneArgs> args(P.cls.size());
   for (size_t c = 0; c < P.cls.size(); ++c) {
     LaneCls& L = *P.cls[c];
     const uint32_t K = adjoint ? L.seg : 1u;
     const size_t st = L.stride;
-    dev_reserve(d, L.qstate, size_t(L.nf) * st * sizeof(R));
-    dev_reserve(d, L.progress, size_t(L.waves) * 4 * sizeof(uint32_t));
-    PASIF_HIP(hipMemsetAsync(L.progress.p, 0, size_t(L.waves) * 4 * sizeof(uint32_t), d->stream));
-    if (d->window) dev_reserve(d, L.mxstate, size_t(L.nf) * st * sizeof(R));
+    LaneArgs& a = args[c];
+    std::memset(&a, 0, sizeof a);
+    a.qstate = mem.get(L.qstate, size_t(L.nf) * st * sizeof(R));
+    a.progress = mem.get(L.progress, size_t(L.waves) * 4 * sizeof(uint32_t));
+    mem.zero(a.progress, size_t(L.waves) * 4 * sizeof(uint32_t));
+    if (d->window) a.mxstate = mem.get(L.mxstate, size_t(L.nf) * st * sizeof(R));
     uint32_t nck = 1;
     if (adjoint) {
       nck = std::max<uint32_t>(1, (L.tmax + K - 1) / K);
-      dev_reserve(d, L.lstate, size_t(std::max<uint32_t>(L.na, 1)) * st * sizeof(R));
+      a.lstate = mem.get(L.lstate, size_t(std::max<uint32_t>(L.na, 1)) * st * sizeof(R));
       if (P.lay.waves == 1)
-        dev_reserve(d, L.ckpt, size_t(L.ck_elems) * sizeof(R));
+        a.ckpt = mem.get(L.ckpt, size_t(L.ck_elems) * sizeof(R));
       else
-        dev_reserve(d, L.ckpt, size_t(nck) * L.nf * st * sizeof(R));
-      dev_reserve(d, L.segbuf, size_t(K + 1) * L.nf * st * sizeof(R));
+        a.ckpt = mem.get(L.ckpt, size_t(nck) * L.nf * st * sizeof(R));
+      a.segbuf = mem.get(L.segbuf, size_t(K + 1) * L.nf * st * sizeof(R));
     }
-    LaneArgs& a = args[c];
-    std::memset(&a, 0, sizeof a);
     a.coef_f = L.coef_f;
     a.coef_a = L.coef_a;
     a.init_f = L.init_f;
@@ -2091,17 +2137,11 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
     a.exc = L.exc;
     a.out_f = L.out_f;
     a.out_a = L.out_a;
-    a.qstate = L.qstate.p;
-    a.lstate = L.lstate.p;
-    a.mxstate = L.mxstate.p;
-    a.progress = L.progress.p;
-    a.ckpt = L.ckpt.p;
     a.ck_off = L.ckoff;
-    a.segbuf = L.segbuf.p;
-    a.out_q = d->out_q.p;
+    a.out_q = out_q;
     a.out_max = d->window ? d->out_max.p : nullptr;
     a.traj = save_steps ? d->traj.p : nullptr;
-    a.out_grad = adjoint ? d->out_grad.p : nullptr;
+    a.out_grad = adjoint ? out_grad : nullptr;
     a.excit = excit;
     a.interp = d->interp_w ? interp : nullptr;
     a.stride = st;
@@ -2156,7 +2196,7 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   if (tile) ta = prepare_run<R>(d, adjoint, save_steps, &twork, &nslots);
   std::vector<std::vector<LaneArgs>> largs;
   if (lane) {
-    nslots = lane_outputs<R>(d, adjoint, save_steps);
+    nslots = run_outputs<R>(d, adjoint, save_steps);
     for (auto& lp : pk.lparts)
       largs.push_back(lane_part_args<R>(d, *lp, adjoint, save_steps, nslots, d->d_exc.p,
                                         d->d_interp.p));
@@ -2189,6 +2229,89 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     return e;
   });
   finish_run<R>(d, adjoint, nslots, traj_out, grad_out);
+}
+
+// pasif_solve_noalloc: the packed problem run with every run buffer and the
+// lane kernels' argument tables carved from the library's static scratch
+// array, results written into the caller's device buffers.  Nothing is
+// allocated or freed: the problem must already be packed for this mode and
+// its excitations uploaded (the output-length query of pasif_get_amplitudes
+// / pasif_get_gradient does both).  Lane and tile parts run one after the
+// other on the driver's stream.
+template <typename R>
+void solve_noalloc_impl(pasif_driver* d, bool adjoint, void* out_q, void* out_grad) {
+  Packed& pk = d->pk;
+  PASIF_REQUIRE(pk.valid && pk.key == pack_key(d, adjoint), PASIF_ERR_STATE,
+                "pasif_solve_noalloc runs a problem already packed for this mode and these "
+                "settings: query the output length first (pasif_get_%s with out == NULL)",
+                adjoint ? "gradient" : "amplitudes");
+  PASIF_REQUIRE(d->ext_exc || (!d->d_exc_dirty && d->d_exc_prec == d->precision),
+                PASIF_ERR_STATE,
+                "the excitations are not on the device in this precision: query the output "
+                "length first (pasif_get_%s with out == NULL)", adjoint ? "gradient" : "amplitudes");
+  PASIF_REQUIRE(d->window == 0, PASIF_ERR_UNSUPPORTED,
+                "pasif_solve_noalloc does not keep the amplitude window (set it to 0)");
+  PASIF_REQUIRE(d->solver != PASIF_SOLVER_GRAPH, PASIF_ERR_UNSUPPORTED,
+                "pasif_solve_noalloc runs the persistent or the plain solver, not the graph one");
+  PASIF_REQUIRE(!d->pool, PASIF_ERR_UNSUPPORTED, "pool jobs run through pasif_pool_tick");
+  PASIF_REQUIRE(out_q != nullptr, PASIF_ERR_ARGUMENT, "out_q (device, %u values) is required",
+                pk.out_len_f);
+  PASIF_REQUIRE(!adjoint || out_grad != nullptr, PASIF_ERR_ARGUMENT,
+                "out_grad (device, %u values) is required for the adjoint", pk.out_len_a);
+  void* base = nullptr;
+  size_t cap = 0;
+  PASIF_HIP(scratch_region(d->dev, &base, &cap));
+  const bool lane = pk.lane, tile = pk.tile;
+  DevBuf unused;
+  TileArgs<R> ta{};
+  uint64_t twork = 0;
+  std::vector<std::vector<LaneArgs>> largs;
+  std::vector<void*> tables;
+  auto plan = [&](RunMem& mem) {
+    largs.clear();
+    tables.clear();
+    uint32_t nslots = 0;
+    if (tile)
+      ta = tile_run_args<R>(d, adjoint, 0, mem, out_q, nullptr, nullptr, out_grad, &twork,
+                            &nslots);
+    if (lane)
+      for (auto& lp : pk.lparts) {
+        largs.push_back(lane_part_args<R>(d, *lp, adjoint, 0, 0, d->d_exc.p, d->d_interp.p,
+                                          &mem, out_q, out_grad));
+        tables.push_back(mem.get(unused, largs.back().size() * sizeof(LaneArgs)));
+      }
+  };
+  RunMem sizing;
+  sizing.d = d;
+  sizing.sizing = true;
+  plan(sizing);
+  PASIF_REQUIRE(sizing.used <= cap, PASIF_ERR_MEMORY,
+                "this problem needs %zu bytes of run buffers; pasif_solve_noalloc holds %zu "
+                "bytes per GPU (solve it with pasif_get_%s instead)",
+                sizing.used, cap, adjoint ? "gradient" : "amplitudes");
+  RunMem mem;
+  mem.d = d;
+  mem.base = static_cast<unsigned char*>(base);
+  mem.cap = cap;
+  plan(mem);
+  const uint64_t lwork = lane ? (adjoint ? 3ull * pk.ltmax : pk.ltmax) : 0;
+  uint64_t launches = 1;
+  const uint32_t budget = launch_budget(d, std::max(twork, lwork), &launches);
+  ta.budget = budget;
+  for (size_t i = 0; i < largs.size(); ++i) {
+    for (LaneArgs& a : largs[i]) a.budget = budget;
+    PASIF_HIP(hipMemcpyAsync(tables[i], largs[i].data(), largs[i].size() * sizeof(LaneArgs),
+                             hipMemcpyHostToDevice, d->stream));
+  }
+  const uint32_t nt = pk.ntiles;
+  drive(d, launches, [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < largs.size() && e == hipSuccess; ++i)
+      if (pk.lparts[i]->blocks)
+        e = lane_launch(*pk.lparts[i], tables[i], uint32_t(largs[i].size()), d->stream);
+    if (tile && e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+    return e;
+  });
 }
 
 void solve(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj,
@@ -2476,10 +2599,19 @@ int pasif_set_modulation_buffer(pasif_driver* d, const double* b, size_t n) {
   });
 }
 
+void ensure_excitations_any(pasif_driver* d) {
+  if (d->dev < 0) return;  // host-only handle: nothing to upload
+  if (d->precision == 64)
+    ensure_excitations<double>(d);
+  else
+    ensure_excitations<float>(d);
+}
+
 int pasif_get_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* len) {
   return guarded(d, [&] {
-    if (!out) {  // length query without solving
+    if (!out) {  // length query without solving; packs and uploads the problem
       ensure_packed(d, false);
+      ensure_excitations_any(d);
       if (len) *len = d->pk.out_len_f;
       return;
     }
@@ -2507,14 +2639,30 @@ int pasif_get_gradient(pasif_driver* d, uint32_t save, double* out, size_t cap, 
   return guarded(d, [&] {
     PASIF_REQUIRE(save == 0, PASIF_ERR_UNSUPPORTED,
                   "partial gradients (save != 0) are not supported; the full schedule runs");
-    if (!out) {
+    if (!out) {  // length query without solving; packs and uploads the problem
       ensure_packed(d, true);
+      ensure_excitations_any(d);
       if (len) *len = d->pk.out_len_a;
       return;
     }
     std::vector<double> g;
     solve(d, true, 0, nullptr, &g);
     copy_out(d, g, out, cap, len);
+  });
+}
+
+int pasif_solve_noalloc(pasif_driver* d, int adjoint, void* out_q, void* out_grad) {
+  return guarded(d, [&] {
+    require_device(d);
+    PASIF_HIP(hipSetDevice(d->dev));
+    struct Scope {
+      Scope() { t_no_alloc = true; }
+      ~Scope() { t_no_alloc = false; }
+    } scope;
+    if (d->precision == 64)
+      solve_noalloc_impl<double>(d, adjoint != 0, out_q, out_grad);
+    else
+      solve_noalloc_impl<float>(d, adjoint != 0, out_q, out_grad);
   });
 }
 
@@ -2948,12 +3096,12 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
     d->pool_largs.clear();
     if (d->pk.lane) {
       if (P->precision == 64) {
-        (void)lane_outputs<double>(d, adjoint != 0, 0);
+        (void)run_outputs<double>(d, adjoint != 0, 0);
         for (auto& lp : d->pk.lparts)
           d->pool_largs.push_back(lane_part_args<double>(d, *lp, adjoint != 0, 0, 0,
                                                          P->bank.p, P->bank_interp.p));
       } else {
-        (void)lane_outputs<float>(d, adjoint != 0, 0);
+        (void)run_outputs<float>(d, adjoint != 0, 0);
         for (auto& lp : d->pk.lparts)
           d->pool_largs.push_back(lane_part_args<float>(d, *lp, adjoint != 0, 0, 0,
                                                         P->bank.p, P->bank_interp.p));

@@ -22,6 +22,9 @@
 // with fused multiply-adds where the reference's compiler contracts them.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "pasif_tiles.hpp"
 
 namespace pasif {
@@ -607,4 +610,36 @@ hipError_t launch_tiles(const TileArgs<float>& a, uint32_t ntiles, int slot_cfg,
   return launch_tiles_impl(a, ntiles, slot_cfg, s);
 }
 
+}  // namespace pasif
+
+// Run scratch of pasif_solve_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host the name is a stand-in, not a device address): scratch_region looks
+// up each device's copy once, with that device current, and every use goes
+// through that address.
+static __device__ __align__(256) unsigned char scratch[pasif::kScratchBytes];
+
+namespace pasif {
+hipError_t scratch_region(int dev, void** ptr, size_t* bytes) {
+  static std::mutex mu;
+  static std::vector<void*> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (dev < 0) return hipErrorInvalidDevice;
+  if (size_t(dev) >= addr.size()) addr.resize(size_t(dev) + 1, nullptr);
+  if (!addr[dev]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (p == nullptr) return hipErrorInvalidSymbol;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  *bytes = kScratchBytes;
+  return hipSuccess;
+}
 }  // namespace pasif

@@ -135,4 +135,9 @@ hipError_t launch_pool(const TileArgs<double>* jobs, const uint32_t* map, uint32
 hipError_t launch_pool(const TileArgs<float>* jobs, const uint32_t* map, uint32_t nblocks,
                        uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s);
 size_t tile_lds_bytes(size_t elem, uint32_t lanes, bool adjoint, uint32_t W, uint32_t nforced);
+// Size of the static device scratch array that holds the run buffers of
+// pasif_solve_noalloc, and the address of `dev`'s copy of it (looked up once
+// per device).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+hipError_t scratch_region(int dev, void** ptr, size_t* bytes);
 }  // namespace pasif
