@@ -262,10 +262,19 @@ class Forcing:
     ``exc`` is ``[n_exc][L]``; ``interp`` is ``[rows][W]`` (rows >= 2)."""
     exc: np.ndarray
     interp: Optional[np.ndarray] = None
+    _cast: dict = field(default_factory=dict, repr=False, compare=False)
 
     @property
     def length(self) -> int:
         return self.exc.shape[1]
+
+    def as_type(self, dt):
+        """Samples and interpolation rows rounded to ``dt`` (once per type)."""
+        dt = np.dtype(dt)
+        if dt not in self._cast:
+            self._cast[dt] = (self.exc.astype(dt, copy=False),
+                              None if self.interp is None else self.interp.astype(dt, copy=False))
+        return self._cast[dt]
 
 
 def excitation_stage_values(forcing: Forcing, step: int, dtype=np.float64) -> np.ndarray:
@@ -275,7 +284,7 @@ def excitation_stage_values(forcing: Forcing, step: int, dtype=np.float64) -> np
     ``Mi[1][j]*e[i] + Mi[0][j]*e[i+1]`` (``src/kernels.cu:124``) with
     ``e[L] := 0``."""
     dt = np.dtype(dtype)
-    exc = forcing.exc.astype(dt, copy=False)
+    exc, M = forcing.as_type(dt)
     n_exc, L = exc.shape
     u = np.zeros((4, n_exc), dtype=dt)
     if forcing.interp is None:
@@ -284,7 +293,6 @@ def excitation_stage_values(forcing: Forcing, step: int, dtype=np.float64) -> np
             if k is not None:
                 u[s] = exc[:, k]
     else:
-        M = forcing.interp.astype(dt, copy=False)
         W = M.shape[1]
         for s in range(4):
             ij = stage_sample_index_interp(step, s, L, W)
