@@ -391,6 +391,12 @@ class Pool:
         d._h = None
         return q[:lq.value], (g[:lg.value] if adjoint else None)
 
+    def drop(self, d):
+        """Remove an unfinished job from the pool (its results are discarded)."""
+        if d._h:
+            self.lib.pasif_destroy(d._h)
+            d._h = None
+
     def stats(self) -> dict:
         j, b, ms, blk = C.c_uint32(0), C.c_double(0), C.c_double(0), C.c_uint32(0)
         self._chk(self.lib.pasif_pool_stats(self.h, C.byref(j), C.byref(b), C.byref(ms),
@@ -422,11 +428,6 @@ class ServeReport:
 class SolveServer:
     def __init__(self, pool: Pool):
         self.pool = pool
-        # admitted jobs by the tick after which their work is covered: a job
-        # of at most T steps per system needs ceil(T (3 with the adjoint) /
-        # steps_per_tick) ticks, so only the jobs due are asked whether they
-        # finished
-        self.due = {}               # tick -> [(job, request)]
 
     def run(self, requests: Sequence[Request], horizon_s: Optional[float] = None,
             drain: bool = True, max_wall_s: float = 600.0,
@@ -440,7 +441,9 @@ class SolveServer:
         submitted, so an overloaded stream does not build requests the pool
         never reaches.  Latency percentiles cover completed requests; requests
         still queued or in flight at the end are reported as ``backlog`` with
-        their waiting time so far in ``p99_all_ms`` (a lower bound)."""
+        their waiting time so far in ``p99_all_ms`` (a lower bound).  Jobs
+        still resident when the run stops (horizon or ``max_wall_s``) are
+        removed from the pool, so a later run starts from an empty pool."""
         if arrivals is None:
             reqs = sorted(requests, key=lambda r: r.arrival)
             arr = np.array([r.arrival for r in reqs], dtype=np.float64)
@@ -461,6 +464,11 @@ class SolveServer:
 
         pending = []                # finished after the last tick, not collected yet
         nres = [0]                  # resident (admitted, not finished) jobs
+        # admitted jobs by the tick after which their work is covered: a job
+        # of at most T steps per system needs ceil(T (3 with the adjoint) /
+        # steps_per_tick) ticks, so only the jobs due are asked whether they
+        # finished
+        due = {}                    # tick -> [(job, request)]
 
         def arrivals_until(now):
             nonlocal i
@@ -493,8 +501,8 @@ class SolveServer:
                 return False
             r.admitted = time.perf_counter() - t0
             work = int(np.max(r.steps)) * (3 if r.adjoint else 1)
-            due = ticks + max(1, -(-work // self.pool.steps_per_tick))
-            self.due.setdefault(due, []).append((job, r))
+            ready = ticks + max(1, -(-work // self.pool.steps_per_tick))
+            due.setdefault(ready, []).append((job, r))
             nres[0] += 1
             admitted += 1
             queue.popleft()
@@ -523,13 +531,13 @@ class SolveServer:
                     if not admit_one():
                         break
                 busy_ms += self.pool.wait()
-                for t in [t for t in self.due if t <= ticks]:
-                    for job, r in self.due.pop(t):
+                for t in [t for t in due if t <= ticks]:
+                    for job, r in due.pop(t):
                         if self.pool.finished(job):
                             pending.append((job, r))
                             nres[0] -= 1
                         else:               # not expected: ask again after the next tick
-                            self.due.setdefault(ticks + 1, []).append((job, r))
+                            due.setdefault(ticks + 1, []).append((job, r))
             elif pending:
                 collect_pending()
             elif i >= n_req and not queue:
@@ -539,8 +547,11 @@ class SolveServer:
                 time.sleep(max(0.0, min(0.01, nxt - now)))
         collect_pending()
         wall = time.perf_counter() - t0
+        for lst in due.values():            # unfinished at the stop: leave the pool
+            for job, _ in lst:
+                self.pool.drop(job)
         lat = np.array([(r.done - r.arrival) * 1e3 for r in done]) if done else np.zeros(1)
-        waiting = [(wall - r.arrival) * 1e3 for lst in self.due.values() for _, r in lst] + \
+        waiting = [(wall - r.arrival) * 1e3 for lst in due.values() for _, r in lst] + \
             [(wall - arr[k]) * 1e3 for k in queue]
         lat_all = np.concatenate([lat, np.array(waiting)]) if waiting else lat
         steps = sum(r.system_steps for r in done)

@@ -23,6 +23,8 @@ class FakePool:
         self.in_flight = False
         self.finished_calls = 0
         self.collected = []
+        self.dropped = []
+        self.tick_s = 0.0
 
     def submit(self, req):
         if len(self.jobs) >= self.slots:
@@ -33,6 +35,9 @@ class FakePool:
 
     def tick(self, wait):
         assert not self.in_flight
+        if self.tick_s:
+            import time
+            time.sleep(self.tick_s)
         for j in self.jobs:
             j.joined = True
             if j.done < j.work:
@@ -50,6 +55,10 @@ class FakePool:
     def finished(self, job):
         self.finished_calls += 1
         return not self.in_flight and job.done >= job.work
+
+    def drop(self, job):
+        self.jobs.remove(job)
+        self.dropped.append(job.req.rid)
 
     def collect(self, job, req):
         assert job.done >= job.work and job.req is req
@@ -79,3 +88,25 @@ def test_server_asks_only_due_jobs_and_completes_every_request():
     # no job waited past that tick: total ticks are what the slot cap forces
     assert rep.ticks <= sum(-(-int(np.max(r.steps)) * (3 if r.adjoint else 1) // 700)
                             for r in reqs)
+
+
+def test_second_run_starts_from_an_empty_pool():
+    # the first run stops at its horizon with long jobs resident: they leave
+    # the pool, and a second run on the same server serves its own requests
+    # against fresh tick numbers
+    rng = np.random.default_rng(5)
+    pool = FakePool(steps_per_tick=100, slots=4)
+    pool.tick_s = 0.005
+    server = S.SolveServer(pool)
+    long_reqs = [S.Request(k, [None], np.zeros(1, dtype=np.int64), np.array([100000]), False)
+                 for k in range(6)]
+    rep1 = server.run(long_reqs, horizon_s=0.05, drain=False, arrivals=np.zeros(6))
+    assert rep1.completed == 0 and rep1.admitted == 4 and rep1.backlog == 6
+    assert sorted(pool.dropped) == [0, 1, 2, 3] and pool.jobs == []
+    pool.tick_s = 0.0
+    reqs = _requests(rng, 20)
+    for r in reqs:
+        r.rid += 100
+    rep2 = server.run(reqs, arrivals=np.zeros(len(reqs)))
+    assert rep2.completed == rep2.admitted == len(reqs) and rep2.backlog == 0
+    assert sorted(pool.collected) == [r.rid for r in reqs]
