@@ -11,6 +11,13 @@ src/kernels.cu:162-195) and its checkpointed schedule:
   [x, v, E], gradient of E(T) w.r.t. k and x0), the gradient matches central
   finite differences of the forward oracle to the scheme's discretisation
   order (relative 2e-4 at dt = 1e-3), and the error falls ~4x when dt halves.
+
+The exact discrete adjoint (``discrete_adjoint_gradient``) is pinned by the
+same linear closed form, by central differences of the forward oracle that
+it matches to finite-difference precision (1e-9 and below, where the
+reference recursion keeps its O(dt^2) gap), by checkpoint invariance, and on
+a forced chain whose adjoint system comes from ``pasif_amd.adjoint`` (K_ii and
+Lambda_i gradients against central differences).
 """
 import numpy as np
 import pytest
@@ -163,3 +170,91 @@ def test_duffing_gradient_error_is_second_order():
         errs.append(abs(lam0[3] - dk))
     rate = np.log2(errs[0] / errs[1])
     assert 1.5 < rate < 2.6, (errs, rate)
+
+
+# ------------------------------------------------------------------ exact mode
+def test_exact_discrete_adjoint_linear_closed_form():
+    # for a state-independent adjoint the exact recursion is T(hA')^T too
+    rng = np.random.default_rng(12)
+    fwd = _fwd_linear(rng, 3)
+    Ap = rng.normal(0, 0.5, (4, 4))
+    lamT = rng.normal(0, 1, 4)
+    adj = _adj_from_arrays(Ap, 3, lamT=lamT)
+    sr, T = 50, 23
+    lay = O.cartesian_layout(3, 1, T)
+    lam0 = O.discrete_adjoint_gradient(fwd, adj, None, sr, lay, np.full((1, 4), T),
+                                       checkpoint_every=5)
+    want = np.linalg.matrix_power(_T(Ap / sr), T) @ lamT
+    np.testing.assert_allclose(lam0[0], want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("sr", [250, 1000])
+def test_exact_discrete_adjoint_is_the_derivative_of_the_computed_map(sr):
+    # Duffing with forcing: central differences (h = 1e-5) of the forward
+    # oracle agree with the exact mode to finite-difference precision
+    # (~1e-11), while the reference recursion keeps its O(h^2) gap (9e-7 at
+    # 250 Hz, 6e-8 at 1 kHz); the exact mode does not depend on the
+    # checkpoint interval
+    k, T = 1.2, 2 * sr
+    fwd, F = duffing_forward(k, sr=sr, T=T)
+    adj = duffing_adjoint_hand(k)
+    lay = O.cartesian_layout(3, 1, T)
+    st = np.full((1, 4), T)
+    ex = [O.discrete_adjoint_gradient(fwd, adj, F, sr, lay, st, checkpoint_every=c)[0]
+          for c in (1, 64, 300, T)]
+    for e in ex[1:]:
+        np.testing.assert_array_equal(e, ex[0])
+    ref = O.adjoint_gradient(fwd, adj, F, sr, lay, st, checkpoint_every=64)[0]
+
+    def E(kk, x0=0.0):
+        f2, F2 = duffing_forward(kk, x0=x0, sr=sr, T=T)
+        return O.rk4_forward(f2, F2, sr, lay).q[0, 2]
+
+    h = 1e-5
+    dk = (E(k + h) - E(k - h)) / (2 * h)
+    dx0 = (E(k, h) - E(k, -h)) / (2 * h)
+    assert abs(ex[0][3] - dk) / abs(dk) < 1e-9
+    assert abs(ex[0][0] - dx0) / abs(dx0) < 1e-9
+    assert abs(ref[3] - dk) / abs(dk) > 30 * abs(ex[0][3] - dk) / abs(dk)
+
+
+def test_exact_mode_with_assembled_chain_jacobian_matches_fd():
+    # the adjoint assembler's system is the transposed Jacobian: with it the
+    # exact mode's K_ii and Lambda_i gradients match central differences of
+    # the forward oracle on a forced 3-DOF chain
+    from pasif_amd import adjoint as ADJ
+    from pasif_amd import workloads as W
+    w = W.chains(n_systems=1, d=3, steps=400, steps_per_sample=10)
+    J = ADJ.energy_gradient_inputs(w)
+    fwd = O.assemble_forward(w.vecM, w.vecB, w.vecK, w.vecGamma, w.vecLambda, w.vecForce,
+                             w.vecQ0)
+    adj = O.assemble_adjoint(*J.as_args(), n_fwd=fwd.n)
+    F = O.Forcing(w.excitations, w.interp)
+    lay = O.mapped_layout(fwd.sizes, [0], [w.numsteps])
+    g = O.discrete_adjoint_gradient(fwd, adj, F, w.sample_rate, lay,
+                                    np.full((1, adj.n), w.numsteps), checkpoint_every=32)[0]
+    d = 3
+
+    def E(dK=None, dL=None):
+        K = w.vecK[0].copy().tocoo()
+        L = ADJ.coo_tensor(list(w.vecLambda[0].dimensions))
+        L.val = np.array(w.vecLambda[0].val, dtype=float).copy()
+        L.indices = np.array(w.vecLambda[0].indices).copy()
+        if dK is not None:
+            i, eps = dK
+            msk = (K.row == d + i) & (K.col == i)
+            K.data = K.data.copy()
+            K.data[msk] += eps
+        if dL is not None:
+            t, eps = dL
+            L.val[t] += eps
+        f2 = O.assemble_forward(w.vecM, w.vecB, [K], w.vecGamma, [L], w.vecForce, w.vecQ0)
+        return O.rk4_forward(f2, F, w.sample_rate, lay).q[0, 2 * d]
+
+    h = 1e-6
+    for i in range(d):
+        fd = (E(dK=(i, h)) - E(dK=(i, -h))) / (2 * h)
+        assert abs(g[2 * d + 1 + i] - fd) <= 1e-6 * abs(fd) + 1e-12, (i, g[2 * d + 1 + i], fd)
+    for t in range(d):
+        fd = (E(dL=(t, h)) - E(dL=(t, -h))) / (2 * h)
+        assert abs(g[3 * d + 1 + t] - fd) <= 1e-6 * abs(fd) + 1e-12, (t, g[3 * d + 1 + t], fd)
