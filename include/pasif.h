@@ -179,6 +179,25 @@ int pasif_set_lane_bias(pasif_driver* d, int bias);
 /* lanes per instance of each lane-kernel class of the packed problem
  * (out == NULL: count query) */
 int pasif_lane_groups(pasif_driver* d, int adjoint, uint32_t* out, size_t cap, size_t* len);
+/* Adjoint recursion (extension).  0 (default): the reference's recursion
+ * (src/__GpuDriver.cu:746-783): RK4 on the adjoint system with set points
+ * x_n, (x_n + x_{n-1})/2, x_{n-1} -- a continuous adjoint, within O(h^2) of
+ * the derivative of the computed forward map.  1: the exact discrete adjoint
+ * -- the reverse-mode derivative of the computed RK4 step, the adjoint system
+ * evaluated at each step's forward stage inputs y4, y3, y2, y1 (recomputed
+ * and stored per segment); it equals that derivative to round-off when the
+ * adjoint system is the transposed Jacobian of the forward one.  Mode 1 runs
+ * on the tile kernel (PASIF_ERR_UNSUPPORTED with PASIF_KERNEL_LANE). */
+int pasif_set_adjoint_mode(pasif_driver* d, int mode);
+/* Memory plan (extension): device bytes the run buffers and outputs of a
+ * solve of the loaded problem take with the current settings (*run_bytes),
+ * of which checkpoints (*ckpt_bytes), and the device memory a solve may use
+ * now (*free_bytes: free memory plus the run buffers the handle already
+ * holds; 0 on a host-only handle).  A solve whose run buffers exceed 80 % of
+ * that is refused with PASIF_ERR_MEMORY before anything is allocated, as the
+ * reference's planner keeps to 0.8 x free (src/__GpuDriver.cu:896-938). */
+int pasif_run_footprint(pasif_driver* d, int adjoint, size_t* run_bytes, size_t* ckpt_bytes,
+                        size_t* free_bytes);
 /* Diagnostics: the generated HIP source of the lane kernel for the loaded
  * problem (NUL-terminated; out == NULL: length query), and with compile != 0
  * its hipRTC compilation for gfx950, which needs no GPU. */

@@ -490,6 +490,13 @@ class PASIf(_GpuDriver):
         self._setLaneGroup(lanes_per_system)
         self._setLaneBias(group_bias)
 
+    def setAdjointMode(self, mode: str = "reference"):
+        """``"reference"``: the reference's adjoint recursion (continuous
+        adjoint, ``src/__GpuDriver.cu:746-783``).  ``"exact"``: the exact
+        discrete adjoint, the derivative of the computed RK4 map (tile
+        kernel)."""
+        self._setAdjointMode(mode)
+
     def kernelInfo(self, adjoint: bool = False) -> dict:
         """Kernel the loaded problem runs on, its structure classes and the
         checkpoint interval of the adjoint."""

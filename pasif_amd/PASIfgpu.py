@@ -266,6 +266,21 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
         return {"kernel": name, "classes": ncls.value, "checkpoint_every": ck.value,
                 "lanes_per_instance": groups}
 
+    def _setAdjointMode(self, mode: str = "reference"):
+        modes = {"reference": 0, "exact": 1}
+        if mode not in modes:
+            raise ValueError(f"adjoint mode {mode!r}: 'reference' or 'exact'")
+        self._chk(self._pl.pasif_set_adjoint_mode(self._h, modes[mode]))
+
+    def _runFootprint(self, adjoint: bool = False) -> dict:
+        """Device bytes of a solve's run buffers and outputs (``run``), the
+        checkpoints among them (``ckpt``) and the memory a solve may use now
+        (``free``) for the loaded problem and settings."""
+        r, c, f = C.c_size_t(0), C.c_size_t(0), C.c_size_t(0)
+        self._chk(self._pl.pasif_run_footprint(self._h, int(adjoint), C.byref(r), C.byref(c),
+                                               C.byref(f)))
+        return {"run": r.value, "ckpt": c.value, "free": f.value}
+
     def _laneSource(self, adjoint: bool = False, compile: bool = False) -> str:
         """Generated HIP source of the lane kernel (optionally compiled with
         hipRTC for gfx950, which needs no GPU)."""

@@ -72,6 +72,9 @@ _SIGS = {
     "pasif_kernel_info": (C.c_int, [_h, C.c_int, C.POINTER(C.c_int), _u32p, _u32p]),
     "pasif_lane_source": (C.c_int, [_h, C.c_int, C.c_char_p, C.c_size_t,
                                     C.POINTER(C.c_size_t), C.c_int]),
+    "pasif_set_adjoint_mode": (C.c_int, [_h, C.c_int]),
+    "pasif_run_footprint": (C.c_int, [_h, C.c_int, C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "pasif_solve_noalloc": (C.c_int, [_h, C.c_int, C.c_void_p, C.c_void_p]),
     "pasif_version": (C.c_char_p, []),
 }

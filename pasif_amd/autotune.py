@@ -53,6 +53,25 @@ def probe_caps(steps, probe_steps: int) -> np.ndarray:
     return np.where(full > 0, np.maximum(1, np.ceil(full * scale)), 0).astype(np.int64)
 
 
+def full_size_bytes(probe: dict, probe_tmax: int, full_tmax: int) -> float:
+    """Run-buffer bytes of the full-size solve estimated from a probe's
+    footprint: checkpoint storage grows with the step counts (ceil(T/k)
+    slots per tile or wave, the probe's steps being the full ones times one
+    factor), the rest does not.  Rounds up: one slot more per checkpoint
+    block than the scaled count."""
+    scale = full_tmax / max(1, probe_tmax)
+    return float(probe["run"] - probe["ckpt"]) + probe["ckpt"] * scale * 1.02
+
+
+def fits(probe: dict, probe_tmax: int, full_tmax: int, frac: float = 0.8) -> bool:
+    """The full-size solve of a candidate stays within ``frac`` of the
+    device memory a solve may use (the reference's 0.8 x free planner,
+    src/__GpuDriver.cu:896-938).  Host-only handles (free == 0) always fit."""
+    if probe["free"] == 0:
+        return True
+    return full_size_bytes(probe, probe_tmax, full_tmax) <= frac * probe["free"]
+
+
 def _load_cache() -> dict:
     try:
         with open(CACHE) as fh:
@@ -86,8 +105,9 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
         steps_full = p._sys_steps
         exc = p._sys_exc
         n = p.numberOfSystems
-        cap = probe_caps(steps_full if steps_full is not None else np.full(n, p.numsteps),
-                         probe_steps)
+        full = steps_full if steps_full is not None else np.full(n, p.numsteps)
+        cap = probe_caps(full, probe_steps)
+        full_tmax, probe_tmax = int(np.max(full)) if n else 0, int(np.max(cap)) if n else 0
         p.setSystemExcitations(exc, cap)
         cks = list(checkpoints) if checkpoints is not None else (list(CHECKPOINTS) if adjoint
                                                                  else [0])
@@ -106,6 +126,12 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
                             p.setTuning(ln, sl, ck)
                             (p.getGradient if adjoint else p.getAmplitudes)()
                             ms = p._lastTiming()["kernel_ms"]
+                            # the full-size solve of this candidate must fit
+                            if not fits(p._runFootprint(adjoint), probe_tmax, full_tmax):
+                                if verbose:
+                                    print(f"[autotune] lanes {ln} slots {sl} ck {ck}: the "
+                                          f"full-size run buffers do not fit")
+                                continue
                         except Exception as e:  # configuration cannot hold the systems
                             if verbose:
                                 print(f"[autotune] lanes {ln} slots {sl} ck {ck}: {e}")
@@ -114,7 +140,8 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
                         if verbose:
                             print(f"[autotune] lanes {ln} slots {sl} ck {ck}: {ms:.3f} ms")
             if not trials:
-                raise RuntimeError("no tile configuration fits this problem")
+                raise RuntimeError("no tile configuration fits this problem (on the device "
+                                   "and, at full size, in 80 % of its free memory)")
             ms, ln, sl, ck = min(trials)
             # the lane kernel's group sizes: the cost model's choice, and every
             # class one step narrower / wider where that fits
@@ -126,6 +153,8 @@ def autotune(p, adjoint: bool = False, probe_steps: int = 2000,
                         p._setLaneBias(b)
                         (p.getGradient if adjoint else p.getAmplitudes)()
                         mb = p._lastTiming()["kernel_ms"]
+                        if not fits(p._runFootprint(adjoint), probe_tmax, full_tmax):
+                            continue
                     except Exception as e:
                         if verbose:
                             print(f"[autotune] lane bias {b}: {e}")

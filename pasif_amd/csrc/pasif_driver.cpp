@@ -201,12 +201,18 @@ struct Packed {
   DevBuf hdr;
   DevBuf f_info, f_out, f_src, f_steps, f_exc, f_fslot, f_force, f_val, f_idx, f_init;
   DevBuf a_info, a_out, a_src, a_steps, a_exc, a_fslot, a_force, a_val, a_idx, a_init;
+  // adjoint: checkpoint interval, first checkpoint slot of each tile (a tile
+  // holds ceil(its steps / interval) slots of `lanes` values), slots in all
+  DevBuf ck_tile;
+  uint32_t ck_every = 0;
+  uint64_t ck_slots = 0;
   uint32_t max_forced = 0;
   size_t program_bytes = 0;
   void release() {
     DevBuf* all[] = {&hdr,     &f_info,  &f_out,  &f_src,   &f_steps, &f_exc, &f_fslot,
                      &f_force, &f_val,   &f_idx,  &f_init,  &a_info,  &a_out, &a_src,
-                     &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init};
+                     &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init,
+                     &ck_tile};
     for (DevBuf* b : all) b->release();
     lparts.clear();
     lshapes.clear();
@@ -245,6 +251,9 @@ struct pasif_driver {
   uint32_t layout_systems = 0;
 
   int precision = 64;
+  // adjoint recursion: 0 the reference's (continuous adjoint, interpolated
+  // set points), 1 the exact discrete adjoint of the forward RK4 map
+  int adj_mode = 0;
   int solver = PASIF_SOLVER_PERSISTENT;
   uint32_t tile_lanes = 0, slot_cfg = 0, ck_every = 0, window = 0;
   int kernel_path = PASIF_KERNEL_AUTO;
@@ -277,7 +286,7 @@ struct pasif_driver {
   // destroyed by its owner while a tick in flight still ran it: the pool
   // deletes it once the tick is over
   bool orphan = false;
-  uint32_t pool_ck = 0, pool_nck = 0;
+  uint32_t pool_ck = 0;
   // a pool job's lane kernel arguments, one vector per lane part (ticks
   // copy them into the pool's per-module tables)
   std::vector<std::vector<LaneArgs>> pool_largs;
@@ -994,6 +1003,21 @@ void pack_impl(pasif_driver* d, bool adjoint) {
   upload(d, pk.f_init, initf);
   pk.program_bytes = pk.f_info.bytes * 4 + pk.f_force.bytes * 2 + pk.f_val.bytes + pk.f_idx.bytes;
   if (adjoint) {
+    // checkpoint blocks sized by each tile's own longest instance
+    uint32_t K = d->ck_every ? d->ck_every : 256;
+    K = std::max<uint32_t>(1, std::min<uint32_t>(K, std::max<uint32_t>(tmax, 1)));
+    std::vector<uint32_t> ck_tile(ntiles);
+    uint64_t slots = 0;
+    for (uint32_t t = 0; t < ntiles; ++t) {
+      ck_tile[t] = uint32_t(slots);
+      slots += std::max<uint32_t>(1, (hdr[t].steps + K - 1) / K);
+    }
+    PASIF_REQUIRE(slots <= 0xFFFFFFFFull, PASIF_ERR_UNSUPPORTED,
+                  "%llu checkpoint slots (more than 2^32): use a longer checkpoint interval",
+                  (unsigned long long)slots);
+    upload(d, pk.ck_tile, ck_tile);
+    pk.ck_every = K;
+    pk.ck_slots = slots;
     std::vector<R> inita(d->adj.init.begin(), d->adj.init.end());
     upload(d, pk.a_info, ha.info);
     upload(d, pk.a_out, ha.out);
@@ -1037,7 +1061,8 @@ uint32_t lane_group_code(uint32_t lanes) {
 }
 
 uint64_t pack_key(pasif_driver* d, bool adjoint) {
-  return (uint64_t(d->precision) << 1) ^ (uint64_t(adjoint) << 8) ^ (uint64_t(d->tile_lanes) << 10) ^
+  return (uint64_t(d->precision) << 1) ^ (uint64_t(adjoint) << 8) ^
+         (uint64_t(adjoint && d->adj_mode) << 9) ^ (uint64_t(d->tile_lanes) << 10) ^
          (uint64_t(d->slot_cfg) << 24) ^ (uint64_t(d->kernel_path) << 28) ^
          (uint64_t(d->ck_every) << 32) ^ (uint64_t(lane_group_code(d->lane_group)) << 56) ^
          (uint64_t(uint32_t(d->lane_bias + 8) & 15u) << 59);
@@ -1789,7 +1814,11 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
   d->pk.valid = false;
   d->pk.release();
   if (!d->pool) d->lane_cls.clear();
-  if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool) {
+  // the exact discrete adjoint runs on the tile kernel only
+  const bool exact = adjoint && d->adj_mode != 0;
+  PASIF_REQUIRE(!(exact && d->kernel_path == PASIF_KERNEL_LANE), PASIF_ERR_UNSUPPORTED,
+                "the exact discrete adjoint runs on the tile kernel (kernel 'auto' or 'tile')");
+  if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool && !exact) {
     if (d->precision == 64)
       (void)lane_plan<double>(d, adjoint);
     else
@@ -1864,8 +1893,10 @@ struct RunMem {
   pasif_driver* d = nullptr;
   unsigned char* base = nullptr;  // scratch region (null: the driver's buffers)
   size_t cap = 0, used = 0;
+  size_t ckpt = 0;                // bytes of checkpoint buffers among `used`
   bool sizing = false;
-  void* get(DevBuf& own, size_t bytes) {
+  void* get(DevBuf& own, size_t bytes, bool checkpoint = false) {
+    if (checkpoint) ckpt += bytes;
     if (!base && !sizing) {
       dev_reserve(d, own, bytes);
       return own.p;
@@ -1931,15 +1962,14 @@ TileArgs<R> tile_run_args(pasif_driver* d, bool adjoint, uint32_t save_steps, Ru
   }
   uint64_t work = pk.tmax;
   if (adjoint) {
-    uint32_t K = d->ck_every ? d->ck_every : 256;
-    K = std::max<uint32_t>(1, std::min<uint32_t>(K, std::max<uint32_t>(pk.tmax, 1)));
-    const uint32_t nck = std::max<uint32_t>(1, (pk.tmax + K - 1) / K);
+    const uint32_t K = pk.ck_every;
     a.lstate = static_cast<R*>(mem.get(d->lstate, rows * sizeof(R)));
-    a.ckpt = static_cast<R*>(mem.get(d->ckpt, rows * nck * sizeof(R)));
-    a.segbuf = static_cast<R*>(mem.get(d->segbuf, rows * (K + 1) * sizeof(R)));
+    a.ckpt = static_cast<R*>(mem.get(d->ckpt, size_t(pk.ck_slots) * L * sizeof(R), true));
+    a.ck_tile = pk.ck_tile.as<uint32_t>();
+    a.exact = d->adj_mode != 0 ? 1u : 0u;
+    a.segbuf = static_cast<R*>(mem.get(d->segbuf, rows * (K + 1) * (a.exact ? 4 : 1) * sizeof(R)));
     a.out_grad = static_cast<R*>(out_grad);
     a.ck_every = K;
-    a.nck = nck;
     a.adjoint = 1;
     work = 3ull * pk.tmax;
   }
@@ -2124,9 +2154,9 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
       nck = std::max<uint32_t>(1, (L.tmax + K - 1) / K);
       a.lstate = mem.get(L.lstate, size_t(std::max<uint32_t>(L.na, 1)) * st * sizeof(R));
       if (P.lay.waves == 1)
-        a.ckpt = mem.get(L.ckpt, size_t(L.ck_elems) * sizeof(R));
+        a.ckpt = mem.get(L.ckpt, size_t(L.ck_elems) * sizeof(R), true);
       else
-        a.ckpt = mem.get(L.ckpt, size_t(nck) * L.nf * st * sizeof(R));
+        a.ckpt = mem.get(L.ckpt, size_t(nck) * L.nf * st * sizeof(R), true);
       a.segbuf = mem.get(L.segbuf, size_t(K + 1) * L.nf * st * sizeof(R));
     }
     a.coef_f = L.coef_f;
@@ -2182,12 +2212,74 @@ void ensure_aux(pasif_driver* d) {
   PASIF_HIP(hipEventCreateWithFlags(&d->join, hipEventDisableTiming));
 }
 
+// Device bytes the run buffers of a solve of the packed problem take
+// (outputs included; the packed program itself is not), and of those the
+// checkpoints: a sizing pass over the same requests a solve makes.
+template <typename R>
+void run_footprint(pasif_driver* d, bool adjoint, uint32_t save_steps, size_t* total,
+                   size_t* ckpt) {
+  Packed& pk = d->pk;
+  RunMem m;
+  m.d = d;
+  m.sizing = true;
+  DevBuf unused;
+  uint64_t work = 0;
+  uint32_t nslots = save_steps ? pk.tmax / save_steps : 0;
+  m.get(unused, size_t(pk.out_len_f) * sizeof(R));
+  if (d->window) m.get(unused, size_t(pk.out_len_f) * sizeof(R));
+  if (nslots) m.get(unused, size_t(nslots) * pk.out_len_f * sizeof(R));
+  if (adjoint) m.get(unused, size_t(pk.out_len_a) * sizeof(R));
+  if (pk.tile)
+    (void)tile_run_args<R>(d, adjoint, save_steps, m, nullptr, nullptr, nullptr, nullptr, &work,
+                           &nslots);
+  for (auto& lp : pk.lparts) {
+    auto args = lane_part_args<R>(d, *lp, adjoint, save_steps, nslots, nullptr, nullptr, &m,
+                                  nullptr, nullptr);
+    m.get(unused, args.size() * sizeof(LaneArgs));
+  }
+  *total = m.used;
+  *ckpt = m.ckpt;
+}
+
+// Bytes of run buffers (and outputs) the driver holds now: a solve reuses
+// them, so they count as available to it.
+size_t held_run_bytes(const pasif_driver* d) {
+  size_t b = 0;
+  const DevBuf* own[] = {&d->qstate, &d->lstate, &d->mxstate, &d->progress, &d->out_q,
+                         &d->out_max, &d->traj, &d->out_grad, &d->ckpt, &d->segbuf};
+  for (const DevBuf* x : own) b += x->bytes;
+  for (const auto& lp : d->pk.lparts)
+    for (const auto& c : lp->cls)
+      b += c->qstate.bytes + c->lstate.bytes + c->mxstate.bytes + c->progress.bytes +
+           c->ckpt.bytes + c->segbuf.bytes;
+  return b;
+}
+
+// The memory plan of a solve: its run buffers must fit in 80 % of the device
+// memory free now plus what the driver already holds for them (the 0.8 x
+// free rule of the reference's planner, src/__GpuDriver.cu:896-938), or the
+// solve is refused before anything is allocated.
+template <typename R>
+void check_run_memory(pasif_driver* d, bool adjoint, uint32_t save_steps) {
+  if (d->dev < 0 || d->pool) return;
+  size_t need = 0, ck = 0, freeb = 0, totalb = 0;
+  run_footprint<R>(d, adjoint, save_steps, &need, &ck);
+  PASIF_HIP(hipMemGetInfo(&freeb, &totalb));
+  const double avail = 0.8 * double(freeb + held_run_bytes(d));
+  PASIF_REQUIRE(double(need) <= avail, PASIF_ERR_MEMORY,
+                "the solve needs %zu bytes of run buffers (%zu of them checkpoints), more than "
+                "80%% of the free device memory (%.0f bytes available); lengthen the checkpoint "
+                "interval or solve fewer systems at once",
+                need, ck, avail);
+}
+
 // Run the packed problem to completion: the lane part, the tile part, or
 // both side by side on two streams.  save_steps > 0 records a trajectory.
 template <typename R>
 void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
                 std::vector<double>* grad_out) {
   ensure_packed(d, adjoint);
+  check_run_memory<R>(d, adjoint, save_steps);
   Packed& pk = d->pk;
   const bool lane = pk.lane, tile = pk.tile;
   uint64_t twork = 0;
@@ -2651,6 +2743,37 @@ int pasif_get_gradient(pasif_driver* d, uint32_t save, double* out, size_t cap, 
   });
 }
 
+int pasif_set_adjoint_mode(pasif_driver* d, int mode) {
+  return guarded(d, [&] {
+    PASIF_REQUIRE(mode == 0 || mode == 1, PASIF_ERR_ARGUMENT,
+                  "adjoint mode %d (0 reference recursion, 1 exact discrete adjoint)", mode);
+    d->adj_mode = mode;
+  });
+}
+
+int pasif_run_footprint(pasif_driver* d, int adjoint, size_t* run_bytes, size_t* ckpt_bytes,
+                        size_t* free_bytes) {
+  return guarded(d, [&] {
+    ensure_packed(d, adjoint != 0);
+    size_t total = 0, ck = 0;
+    if (d->precision == 64)
+      run_footprint<double>(d, adjoint != 0, 0, &total, &ck);
+    else
+      run_footprint<float>(d, adjoint != 0, 0, &total, &ck);
+    if (run_bytes) *run_bytes = total;
+    if (ckpt_bytes) *ckpt_bytes = ck;
+    if (free_bytes) {
+      *free_bytes = 0;
+      if (d->dev >= 0) {
+        size_t f = 0, t = 0;
+        PASIF_HIP(hipSetDevice(d->dev));
+        PASIF_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f + held_run_bytes(d);
+      }
+    }
+  });
+}
+
 int pasif_solve_noalloc(pasif_driver* d, int adjoint, void* out_q, void* out_grad) {
   return guarded(d, [&] {
     require_device(d);
@@ -3081,11 +3204,9 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
       if (P->precision == 64) {
         TileArgs<double> a = prepare_run<double>(d, adjoint != 0, 0, &work, &nslots);
         d->pool_ck = a.ck_every;
-        d->pool_nck = a.nck;
       } else {
         TileArgs<float> a = prepare_run<float>(d, adjoint != 0, 0, &work, &nslots);
         d->pool_ck = a.ck_every;
-        d->pool_nck = a.nck;
       }
       PASIF_REQUIRE(d->pk.lanes == P->lanes, PASIF_ERR_UNSUPPORTED,
                     "a system of this request needs %u lanes, the pool runs %u", d->pk.lanes,
@@ -3372,7 +3493,7 @@ void pool_tick_impl(pasif_pool* P) {
         a.segbuf = d->segbuf.as<R>();
         a.out_grad = d->out_grad.as<R>();
         a.ck_every = d->pool_ck;
-        a.nck = d->pool_nck;
+        a.ck_tile = d->pk.ck_tile.as<uint32_t>();
         a.adjoint = 1;
       }
       a.budget = P->budget;

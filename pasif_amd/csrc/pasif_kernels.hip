@@ -218,12 +218,16 @@ __device__ __forceinline__ R interp_at(const R* slot, uint32_t b, const R* wts, 
 // stage-1 input y = st must already be published in buffer 0 of `yb`.
 // Adjoint steps (ADJ) read set points x_n / xbar / x_{n-1} from elements
 // 0 / 1 / 2 of `xb`; forward steps read them from the stage buffer itself.
+// `rec` (forward steps, owner lanes; null: none) receives the step's stage
+// inputs y1 = st, y2, y3, y4 at rec[0], rec[L], rec[2L], rec[3L].
 template <bool ADJ, typename R, class Prog>
 __device__ __forceinline__ void rk4_step(const Prog& P, unsigned char* yb, const unsigned char* xb,
                                          R* part, uint32_t lane, bool split, bool owner,
                                          uint32_t nseg, uint32_t yoff, R& st, bool forced, R u0,
-                                         R u1, R u3, R force, bool active, R h, R h2, R h6) {
+                                         R u1, R u3, R force, bool active, R h, R h2, R h6,
+                                         R* rec = nullptr, uint32_t L = 0) {
   R acc = R(0);
+  if (!ADJ && rec) rec[0] = st;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     constexpr int dummy = 0;
@@ -263,6 +267,61 @@ __device__ __forceinline__ void rk4_step(const Prog& P, unsigned char* yb, const
         if (active) st = qn;
         y = st;
       }
+      if (!ADJ && rec && s < 3) rec[(s + 1) * L] = y;
+      *reinterpret_cast<R*>(yb + yoff + ((s + 1) & 1) * sizeof(R)) = y;
+    }
+    sync_lds();
+  }
+}
+
+// One step of the exact discrete adjoint (reverse mode of rk4_step's
+// forward map) for every adjoint row of the tile.  The owner's state is
+// `lam`; h6 * lam must already be published in buffer 0 of `yb`, and the
+// step's forward stage inputs y1..y4 in elements 0..3 of `xb`:
+//   a4 = h6 lam;             mu4 = g(a4, y4)
+//   a3 = 2 h6 lam + h mu4;   mu3 = g(a3, y3)
+//   a2 = 2 h6 lam + h2 mu3;  mu2 = g(a2, y2)
+//   a1 = h6 lam + h2 mu2;    mu1 = g(a1, y1)
+//   lam += ((mu4 + mu3) + mu2) + mu1
+// and h6 * lam is published again for the next step.
+template <typename R, class Prog>
+__device__ __forceinline__ void rk4_exact_adjoint_step(const Prog& P, unsigned char* yb,
+                                                       const unsigned char* xb, R* part,
+                                                       uint32_t lane, bool split, bool owner,
+                                                       uint32_t nseg, uint32_t yoff, R& lam,
+                                                       bool active, R h, R h2, R h6) {
+  R acc = R(0);
+  const R h3 = R(2) * h6;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    R p;
+    if (s == 0) p = P.template eval<0, 3>(yb, xb);
+    else if (s == 1) p = P.template eval<1, 2>(yb, xb);
+    else if (s == 2) p = P.template eval<0, 1>(yb, xb);
+    else p = P.template eval<1, 0>(yb, xb);
+    if (split) {
+      part[lane] = p;
+      sync_lds();
+      if (owner)
+        for (uint32_t j = 1; j < nseg; ++j) p += part[lane + j];
+    }
+    if (owner) {
+      R y;
+      if (s == 0) {
+        acc = p;
+        y = fmaR(h, p, h3 * lam);
+      } else if (s == 1) {
+        acc = acc + p;
+        y = fmaR(h2, p, h3 * lam);
+      } else if (s == 2) {
+        acc = acc + p;
+        y = fmaR(h2, p, h6 * lam);
+      } else {
+        acc = acc + p;
+        const R ln = lam + acc;
+        if (active) lam = ln;
+        y = h6 * lam;
+      }
       *reinterpret_cast<R*>(yb + yoff + ((s + 1) & 1) * sizeof(R)) = y;
     }
     sync_lds();
@@ -298,7 +357,10 @@ __device__ __forceinline__ void run_tile(const TileArgs<R>& a, const size_t tile
   const uint32_t T = uni(H->steps);
   const uint32_t K = adjoint ? a.ck_every : 1u;
   const size_t rb = tile * L;
-  const size_t segb = tile * (K + 1);
+  const bool exact = adjoint && a.exact != 0u;
+  // segment buffer of the tile: post-step states [K+1][L], or (exact) the
+  // four stage inputs of each step [K+1][4][L]
+  const size_t segb = tile * (K + 1) * (exact ? 4u : 1u);
   const uint32_t W = a.interp_w;
 
   uint32_t* pst = a.progress + 4 * tile;
@@ -372,8 +434,8 @@ __device__ __forceinline__ void run_tile(const TileArgs<R>& a, const size_t tile
       P.load(a.fwd, H->cnt_f, tile, L, lane);
       const uint32_t s0 = seg * K;
       if (recompute && n == s0 && fown) {
-        q = a.ckpt[(tile * a.nck + seg) * L + frow];
-        a.segbuf[segb * L + frow] = q;
+        q = a.ckpt[size_t(a.ck_tile[tile] + seg) * L + frow];
+        if (!exact) a.segbuf[segb * L + frow] = q;
       }
       if (fown) *reinterpret_cast<R*>(yb + fyoff) = q;
       sync_lds();
@@ -411,13 +473,15 @@ __device__ __forceinline__ void run_tile(const TileArgs<R>& a, const size_t tile
           hi += 1u;
         }
         if (adjoint && !recompute && fown && (n % K) == 0u)
-          a.ckpt[(tile * a.nck + n / K) * L + frow] = q;
+          a.ckpt[size_t(a.ck_tile[tile] + n / K) * L + frow] = q;
         const bool active = n < fsteps;
         rk4_step<false>(P, yb, xbb, part, lane, fsplit, fown, fnseg, fyoff, q, forced, u0, u1,
-                        u3, fforce, active, a.h, a.h2, a.h6);
+                        u3, fforce, active, a.h, a.h2, a.h6,
+                        (exact && recompute && fown)
+                            ? a.segbuf + (segb + 4u * (n - s0)) * L + frow : nullptr, L);
         if (fown) {
           if (recompute) {
-            a.segbuf[(segb + (n + 1u - s0)) * L + frow] = q;
+            if (!exact) a.segbuf[(segb + (n + 1u - s0)) * L + frow] = q;
           } else {
             if (a.save_every && (n % a.save_every) == 0u) {
               const uint32_t slot = n / a.save_every;
@@ -454,15 +518,31 @@ __device__ __forceinline__ void run_tile(const TileArgs<R>& a, const size_t tile
       // ---------------- adjoint (reverse) ----------------
       P.load(a.adj, H->cnt_a, tile, L, lane);
       const uint32_t s0 = seg * K;
-      if (aown) *reinterpret_cast<R*>(yb + ayoff) = lam;
+      if (aown) *reinterpret_cast<R*>(yb + ayoff) = exact ? a.h6 * lam : lam;
       R xcur = R(0), xprev = R(0);
-      if (lane < rows_f) {
+      if (lane < rows_f && !exact) {
         xcur = a.segbuf[(segb + (n - s0)) * L + lane];
         if (n > s0) xprev = a.segbuf[(segb + (n - 1u - s0)) * L + lane];
       }
       sync_lds();
       const uint32_t nend = (n - s0 < budget) ? s0 : n - budget;
       budget -= n - nend;
+      if (exact) {
+        for (; n > nend; --n) {
+          if (lane < rows_f) {
+            // stage inputs of the forward step n-1 -> n
+            const R* y = a.segbuf + (segb + 4u * (n - 1u - s0)) * L + lane;
+            const R y1 = y[0], y2 = y[L], y3 = y[2 * L], y4 = y[3 * L];
+            xb[4 * lane] = y1;
+            xb[4 * lane + 1] = y2;
+            xb[4 * lane + 2] = y3;
+            xb[4 * lane + 3] = y4;
+          }
+          sync_lds();
+          rk4_exact_adjoint_step(P, yb, xbb, part, lane, asplit, aown, anseg, ayoff, lam,
+                                 n <= asteps, a.h, a.h2, a.h6);
+        }
+      }
       for (; n > nend; --n) {
         if (lane < rows_f) {
           const R xnext = (n - 1u > s0) ? a.segbuf[(segb + (n - 2u - s0)) * L + lane] : R(0);

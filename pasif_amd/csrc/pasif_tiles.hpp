@@ -116,10 +116,11 @@ struct TileArgs {
   uint32_t nslots;
   R* out_grad;
   // adjoint scratch, [tile][...][lanes] by forward row
-  R* ckpt;                 // [tile][nck][lanes]
-  R* segbuf;               // [tile][ck_every+1][lanes]
+  R* ckpt;                 // per tile ceil(steps/ck_every) slots of [lanes], from ck_tile
+  const uint32_t* ck_tile; // [tile] first checkpoint slot of the tile (sized by its steps)
+  R* segbuf;               // [tile][ck_every+1][lanes]; exact: [tile][4 (ck_every+1)][lanes]
   uint32_t ck_every;
-  uint32_t nck;
+  uint32_t exact;          // adjoint: 0 the reference's recursion, 1 the exact discrete adjoint
   uint32_t adjoint;        // run the adjoint program after the forward pass
 };
 
