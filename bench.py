@@ -167,6 +167,44 @@ def cpu_oracle(w, one_core_sample: int) -> dict:
             "one_core_matches_all_core": bool(np.array_equal(q1, o[:q1.size]))}
 
 
+def duffing_cpu() -> dict:
+    """The CPU-only configuration in full on the host: 64 Duffing oscillators
+    x'' + 0.1 x' + k x + 0.5 x^3 = sin(w t) (k ~ U[0.5, 2], w = linspace(0.5,
+    2, 64), seed 0), x0 = v0 = 0, dt = 1e-3, 10,000 steps, FP64 oracle: max
+    |x| over the last 2,000 steps and E(T) = int b v^2 dt per system."""
+    import oracle as O
+    from pasif_amd import workloads as W
+    w = W.duffing()
+    asm = O.assemble_forward(w.vecM, w.vecB, w.vecK, w.vecGamma, w.vecLambda, w.vecForce,
+                             w.vecQ0)
+    lay = O.mapped_layout(asm.sizes, w.exc_of_system, np.full(w.n_systems, w.numsteps))
+    t0 = time.perf_counter()
+    r = O.rk4_forward(asm, O.Forcing(w.excitations), w.sample_rate, lay, window=2000)
+    sec = time.perf_counter() - t0
+    return {"w": w, "q": r.q.reshape(-1), "mx": r.maxabs.reshape(-1),
+            "value": w.n_systems * w.numsteps / sec, "seconds": sec}
+
+
+def duffing_section(cpu: dict) -> dict:
+    """The same 64 Duffing oscillators on the GPU (fused window maximum)
+    against the full CPU run."""
+    from oracle import parity as PAR
+    from pasif_amd import workloads as W
+    w = cpu["w"]
+    p = W.make_solver(w)
+    q, mx = p.getMaxAmplitudes(window=2000)
+    sizes = w.sizes()
+    return {"workload": "64 Duffing oscillators x 10,000 steps, dt 1e-3, FP64: max |x| over "
+                        "the last 2,000 steps and E(T)",
+            "cpu_oracle": {"value": cpu["value"], "unit": "system-steps/s", "cores": 1,
+                           "seconds": cpu["seconds"]},
+            "gpu_kernel_ms": p._lastTiming()["kernel_ms"],
+            "max_abs_x": {"gpu": mx[0::3].tolist()[:4], "oracle": cpu["mx"][0::3].tolist()[:4]},
+            "parity_states": PAR.report(q, cpu["q"], sizes).max_rel,
+            "parity_window_max": PAR.report(mx, cpu["mx"], sizes).max_rel,
+            "tolerance_fp64": PAR.TOL64}
+
+
 # ------------------------------------------------------------------ sections
 def fwd_adj_section(args) -> dict:
     """Forward + checkpointed adjoint on chains of 16 DOF (energy gradient
@@ -435,9 +473,10 @@ def main():
 
     w = W.two_dof(n_systems=args.systems, steps=args.rk_steps)
     sizes = w.sizes()
-    cpu = None
+    cpu = duff = None
     if not args.no_cpu:
-        _progress("cpu oracle (all cores, then one core)")
+        _progress("cpu oracle (all cores, then one core; Duffing configuration in full)")
+        duff = duffing_cpu()
         cpu = cpu_oracle(w, min(args.cpu_sample, w.n_systems))
         _progress(f"cpu oracle: {cpu['all_core']['seconds']:.1f} s on "
                   f"{cpu['all_core']['cores']} cores, {cpu['one_core']['seconds']:.1f} s one core")
@@ -551,6 +590,8 @@ def main():
                          "sample": f"first {cpu['one_core']['systems']} systems x all "
                                    f"{w.numsteps} steps ({cpu['one_core']['seconds']:.1f} s)"}}
     del p
+    if duff is not None:
+        out["duffing"] = duffing_section(duff)
 
     if not args.no_adjoint:
         _progress("fwd_adj")
