@@ -410,7 +410,7 @@ def serving_section(args) -> dict:
         return pool
 
     arr = S.poisson_arrivals(rng, args.serving_rps, args.serving_s)
-    reqs = S.LazyRequests(lib, len(arr), 4000, n_signals, tmin=1e3, tmax=1e5, max_systems=256)
+    reqs = S.RequestStream(lib, len(arr), 4000, n_signals, tmin=1e3, tmax=1e5, max_systems=256)
     pool = new_pool(5000)
     rep = S.SolveServer(pool).run(reqs, horizon_s=args.serving_s, drain=False,
                                   max_wall_s=args.serving_s + 15, arrivals=arr)

@@ -65,7 +65,7 @@ def main():
 
     for li, load in enumerate(a.loads):
         arr = S.poisson_arrivals(rng, load, a.duration)
-        reqs = S.LazyRequests(lib, len(arr), a.seed * 1000 + li, n_signals, tmin=a.tmin,
+        reqs = S.RequestStream(lib, len(arr), a.seed * 1000 + li, n_signals, tmin=a.tmin,
                               tmax=a.tmax, max_systems=a.max_systems)
         pool = new_pool(a.steps_per_tick)
         rep = S.SolveServer(pool).run(reqs, horizon_s=a.duration, drain=False,

@@ -266,6 +266,23 @@ int pasif_pool_register(pasif_pool* pool, const pasif_driver* src, uint32_t* fir
 int pasif_pool_submit_systems(pasif_pool* pool, uint32_t n, const uint32_t* ids,
                               const int32_t* exc_of_system, const uint32_t* steps_of_system,
                               int adjoint, pasif_driver** job);
+/* Many requests in one call: request r is systems sys_off[r] ..
+ * sys_off[r+1]-1 of ids / exc_of_system / steps_of_system, forward + adjoint
+ * when adjoint[r] != 0.  Requests are admitted in order until one does not
+ * fit the slots or the HBM budget; *admitted counts those admitted (their
+ * handles in jobs[0 .. *admitted-1], the rest NULL) and the call still
+ * returns PASIF_OK.  Any other failure returns its code, with the requests
+ * before it admitted. */
+int pasif_pool_submit_batch(pasif_pool* pool, uint32_t n_req, const uint32_t* sys_off,
+                            const uint32_t* ids, const int32_t* exc_of_system,
+                            const uint32_t* steps_of_system, const int32_t* adjoint,
+                            pasif_driver** jobs, uint32_t* admitted);
+/* pasif_pool_collect for n finished jobs, then pasif_destroy of each handle:
+ * job i's final states go to q[q_off[i] .. q_off[i+1]) (exactly that many),
+ * its lambda_0 (adjoint jobs) to grad[g_off[i] .. g_off[i+1]); grad may be
+ * NULL when no job is an adjoint job. */
+int pasif_pool_collect_batch(pasif_pool* pool, uint32_t n, pasif_driver* const* jobs, double* q,
+                             const size_t* q_off, double* grad, const size_t* g_off);
 /* *running = 1 while the last tick's launch is still executing */
 int pasif_pool_poll(pasif_pool* pool, int* running);
 

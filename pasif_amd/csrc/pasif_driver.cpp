@@ -46,10 +46,27 @@ struct Fail {
 // is only handed out again after the job that held it left the pool, i.e.
 // after the last launch that could read it completed.
 struct Arena {
+  // Free blocks of a chunk, indexed by offset (to merge neighbours when a
+  // block comes back) and by size (best fit in O(log n): with thousands of
+  // resident jobs the free list is long and a first-fit scan dominated
+  // admission).
   struct Chunk {
     char* base;
     size_t cap;
-    std::map<size_t, size_t> free_;  // offset -> bytes
+    std::map<size_t, size_t> free_;           // offset -> bytes
+    std::multimap<size_t, size_t> by_size;    // bytes -> offset
+    void add(size_t off, size_t len) {
+      free_.emplace(off, len);
+      by_size.emplace(len, off);
+    }
+    void drop_size(size_t len, size_t off) {
+      auto r = by_size.equal_range(len);
+      for (auto it = r.first; it != r.second; ++it)
+        if (it->second == off) {
+          by_size.erase(it);
+          return;
+        }
+    }
   };
   std::vector<Chunk> chunks;
   size_t chunk_bytes = size_t(1) << 30;
@@ -57,21 +74,22 @@ struct Arena {
   static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~size_t(255); }
   void* take(size_t bytes) {
     const size_t need = round(bytes);
-    for (Chunk& c : chunks)
-      for (auto it = c.free_.begin(); it != c.free_.end(); ++it)
-        if (it->second >= need) {
-          const size_t off = it->first, rest = it->second - need;
-          c.free_.erase(it);
-          if (rest) c.free_.emplace(off + need, rest);
-          return c.base + off;
-        }
+    for (Chunk& c : chunks) {
+      auto it = c.by_size.lower_bound(need);
+      if (it == c.by_size.end()) continue;
+      const size_t len = it->first, off = it->second;
+      c.by_size.erase(it);
+      c.free_.erase(off);
+      if (len > need) c.add(off + need, len - need);
+      return c.base + off;
+    }
     Chunk c{};
     c.cap = std::max(chunk_bytes, need);
     void* base = nullptr;
     if (hipMalloc(&base, c.cap) != hipSuccess) return nullptr;
     c.base = static_cast<char*>(base);
     reserved += c.cap;
-    if (c.cap > need) c.free_.emplace(need, c.cap - need);
+    if (c.cap > need) c.add(need, c.cap - need);
     chunks.push_back(std::move(c));
     return chunks.back().base;
   }
@@ -83,6 +101,7 @@ struct Arena {
       auto nx = c.free_.lower_bound(off);
       if (nx != c.free_.end() && off + len == nx->first) {
         len += nx->second;
+        c.drop_size(nx->second, nx->first);
         nx = c.free_.erase(nx);
       }
       if (nx != c.free_.begin()) {
@@ -90,10 +109,11 @@ struct Arena {
         if (pv->first + pv->second == off) {
           off = pv->first;
           len += pv->second;
+          c.drop_size(pv->second, pv->first);
           c.free_.erase(pv);
         }
       }
-      c.free_.emplace(off, len);
+      c.add(off, len);
       return;
     }
   }
@@ -371,6 +391,9 @@ struct pasif_pool {
   std::vector<int32_t> reg_lane;
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  // pinned staging of batched collects (grows, freed with the pool)
+  unsigned char* pinned = nullptr;
+  size_t pinned_bytes = 0;
 };
 
 namespace {
@@ -3298,9 +3321,15 @@ void pool_build_modules(pasif_pool* P) {
   for (size_t c = 0; c < all.size(); ++c) {
     const auto opt = group_options(all[c], bits);
     if (!opt.empty()) {
-      // the shortest step among the sizes within 2x of the least work: a
-      // tick lasts `budget` steps of the slowest class, and a request waits
-      // for its slowest system
+      // the shortest step among the sizes within `slack` x the least work
+      // (default 2): a tick lasts `budget` steps of the slowest class, and
+      // a request waits for its slowest system; PASIF_POOL_WORK_SLACK=1
+      // takes the least work (throughput over latency)
+      double slack = 2.0;
+      if (const char* e = std::getenv("PASIF_POOL_WORK_SLACK")) {
+        const double v = std::strtod(e, nullptr);
+        if (v >= 1.0 && v <= 16.0) slack = v;
+      }
       double wmin = 0;
       for (size_t k = 0; k < opt.size(); ++k) {
         const double wk = opt[k].second * opt[k].first;
@@ -3308,7 +3337,7 @@ void pool_build_modules(pasif_pool* P) {
       }
       size_t b = opt.size();
       for (size_t k = 0; k < opt.size(); ++k)
-        if (opt[k].second * opt[k].first <= 2.0 * wmin &&
+        if (opt[k].second * opt[k].first <= slack * wmin &&
             (b == opt.size() || opt[k].second < opt[b].second))
           b = k;
       (void)lane_group(all[c], opt[b].first);
@@ -3763,59 +3792,99 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
   });
 }
 
+}  // extern "C"
+
+namespace {
+// One request of registered systems: validated, concatenated, packed and
+// admitted (throws Fail; PASIF_ERR_MEMORY when slots or budget refuse it).
+pasif_driver* pool_submit_one(pasif_pool* P, uint32_t n, const uint32_t* ids,
+                              const int32_t* exc_of_system, const uint32_t* steps_of_system,
+                              int adjoint) {
+  POOL_REQUIRE(n > 0 && ids && exc_of_system && steps_of_system, PASIF_ERR_ARGUMENT,
+               "a request needs at least one system with its excitation and step count");
+  POOL_REQUIRE(P->bank_n > 0, PASIF_ERR_STATE, "no signal bank: pasif_pool_set_bank first");
+  for (uint32_t i = 0; i < n; ++i) {
+    POOL_REQUIRE(ids[i] < P->reg.size(), PASIF_ERR_ARGUMENT,
+                 "system id " + std::to_string(ids[i]) + " is not registered");
+    POOL_REQUIRE(!adjoint || P->reg[ids[i]].has_adj, PASIF_ERR_ARGUMENT,
+                 "system id " + std::to_string(ids[i]) + " has no registered adjoint");
+    POOL_REQUIRE(exc_of_system[i] >= -1 && exc_of_system[i] < int32_t(P->bank_n),
+                 PASIF_ERR_ARGUMENT,
+                 "excitation " + std::to_string(exc_of_system[i]) + " not in the bank");
+    POOL_REQUIRE(steps_of_system[i] > 0, PASIF_ERR_ARGUMENT, "step counts must be positive");
+  }
+  if (!P->mods_built) {
+    if (P->precision == 64)
+      pool_build_modules<double>(P);
+    else
+      pool_build_modules<float>(P);
+  }
+  const auto c0 = std::chrono::steady_clock::now();
+  std::unique_ptr<pasif_driver> d(new_bank_job(P, 0));
+  d->lane_cls.assign(n, -1);
+  for (uint32_t i = 0; i < n; ++i) d->lane_cls[i] = P->reg_lane[ids[i]];
+  std::vector<const RowTerms*> parts(n);
+  for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].f;
+  concat_blocks(parts, false, {}, d->fwd);
+  d->fwd.nset = d->fwd.n;
+  if (adjoint) {
+    for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].a;
+    concat_blocks(parts, true, d->fwd.off, d->adj);
+    d->adj.nset = d->fwd.n;
+  }
+  d->mapped = true;
+  d->map_exc.assign(exc_of_system, exc_of_system + n);
+  d->map_steps.assign(steps_of_system, steps_of_system + n);
+  d->layout_systems = n;
+  d->sys_ready = true;
+  d->adj_ready = adjoint != 0;
+  const auto c1 = std::chrono::steady_clock::now();
+  admit_job(P, d.get(), adjoint);
+  pasif_driver* job = d.release();
+  P->handles.insert(job);
+  if (P->timing) {
+    const auto c2 = std::chrono::steady_clock::now();
+    P->tm[0] += std::chrono::duration<double, std::milli>(c1 - c0).count();
+    P->tm[3] += std::chrono::duration<double, std::milli>(c2 - c1).count();
+    ++P->tm_n;
+  }
+  return job;
+}
+}  // namespace
+
+extern "C" {
+
 int pasif_pool_submit_systems(pasif_pool* P, uint32_t n, const uint32_t* ids,
                               const int32_t* exc_of_system, const uint32_t* steps_of_system,
                               int adjoint, pasif_driver** job) {
   return pool_guarded(P, [&] {
     POOL_REQUIRE(job != nullptr, PASIF_ERR_ARGUMENT, "null job pointer");
     *job = nullptr;
-    POOL_REQUIRE(n > 0 && ids && exc_of_system && steps_of_system, PASIF_ERR_ARGUMENT,
-                 "a request needs at least one system with its excitation and step count");
-    POOL_REQUIRE(P->bank_n > 0, PASIF_ERR_STATE, "no signal bank: pasif_pool_set_bank first");
-    for (uint32_t i = 0; i < n; ++i) {
-      POOL_REQUIRE(ids[i] < P->reg.size(), PASIF_ERR_ARGUMENT,
-                   "system id " + std::to_string(ids[i]) + " is not registered");
-      POOL_REQUIRE(!adjoint || P->reg[ids[i]].has_adj, PASIF_ERR_ARGUMENT,
-                   "system id " + std::to_string(ids[i]) + " has no registered adjoint");
-      POOL_REQUIRE(exc_of_system[i] >= -1 && exc_of_system[i] < int32_t(P->bank_n),
-                   PASIF_ERR_ARGUMENT,
-                   "excitation " + std::to_string(exc_of_system[i]) + " not in the bank");
-      POOL_REQUIRE(steps_of_system[i] > 0, PASIF_ERR_ARGUMENT, "step counts must be positive");
-    }
-    if (!P->mods_built) {
-      if (P->precision == 64)
-        pool_build_modules<double>(P);
-      else
-        pool_build_modules<float>(P);
-    }
-    const auto c0 = std::chrono::steady_clock::now();
-    std::unique_ptr<pasif_driver> d(new_bank_job(P, 0));
-    d->lane_cls.assign(n, -1);
-    for (uint32_t i = 0; i < n; ++i) d->lane_cls[i] = P->reg_lane[ids[i]];
-    std::vector<const RowTerms*> parts(n);
-    for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].f;
-    concat_blocks(parts, false, {}, d->fwd);
-    d->fwd.nset = d->fwd.n;
-    if (adjoint) {
-      for (uint32_t i = 0; i < n; ++i) parts[i] = &P->reg[ids[i]].a;
-      concat_blocks(parts, true, d->fwd.off, d->adj);
-      d->adj.nset = d->fwd.n;
-    }
-    d->mapped = true;
-    d->map_exc.assign(exc_of_system, exc_of_system + n);
-    d->map_steps.assign(steps_of_system, steps_of_system + n);
-    d->layout_systems = n;
-    d->sys_ready = true;
-    d->adj_ready = adjoint != 0;
-    const auto c1 = std::chrono::steady_clock::now();
-    admit_job(P, d.get(), adjoint);
-    *job = d.release();
-    P->handles.insert(*job);
-    if (P->timing) {
-      const auto c2 = std::chrono::steady_clock::now();
-      P->tm[0] += std::chrono::duration<double, std::milli>(c1 - c0).count();
-      P->tm[3] += std::chrono::duration<double, std::milli>(c2 - c1).count();
-      ++P->tm_n;
+    *job = pool_submit_one(P, n, ids, exc_of_system, steps_of_system, adjoint);
+  });
+}
+
+int pasif_pool_submit_batch(pasif_pool* P, uint32_t n_req, const uint32_t* sys_off,
+                            const uint32_t* ids, const int32_t* exc_of_system,
+                            const uint32_t* steps_of_system, const int32_t* adjoint,
+                            pasif_driver** jobs, uint32_t* admitted) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(admitted != nullptr && jobs != nullptr && (n_req == 0 || (sys_off && adjoint)),
+                 PASIF_ERR_ARGUMENT, "null argument");
+    *admitted = 0;
+    for (uint32_t r = 0; r < n_req; ++r) jobs[r] = nullptr;
+    for (uint32_t r = 0; r < n_req; ++r) {
+      POOL_REQUIRE(sys_off[r + 1] > sys_off[r], PASIF_ERR_ARGUMENT,
+                   "request " + std::to_string(r) + " has no system");
+      const uint32_t b = sys_off[r];
+      try {
+        jobs[r] = pool_submit_one(P, sys_off[r + 1] - b, ids + b, exc_of_system + b,
+                                  steps_of_system + b, adjoint[r]);
+      } catch (const Fail& f) {
+        if (f.code == PASIF_ERR_MEMORY) return;   // full: the rest waits
+        throw;
+      }
+      *admitted = r + 1;
     }
   });
 }
@@ -3929,6 +3998,103 @@ int pasif_pool_collect(pasif_pool* P, pasif_driver* d, double* q, size_t capq, s
   });
 }
 
+int pasif_pool_collect_batch(pasif_pool* P, uint32_t n, pasif_driver* const* jobs, double* q,
+                             const size_t* q_off, double* grad, const size_t* g_off) {
+  return pool_guarded(P, [&] {
+    POOL_REQUIRE(n == 0 || (jobs && q && q_off && g_off), PASIF_ERR_ARGUMENT, "null argument");
+    if (n == 0) return;
+    POOL_HIP(hipSetDevice(P->dev));
+    const size_t el = P->precision == 64 ? 8 : 4;
+    // every job: finished, its slots sized right; then its progress records
+    // and outputs copied into one pinned buffer with a single wait
+    std::vector<size_t> at(n);
+    struct Piece {
+      const void* src;
+      size_t bytes, off;
+    };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    auto add = [&](const void* src, size_t bytes) {
+      pieces.push_back({src, bytes, total});
+      total += (bytes + 15) & ~size_t(15);
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+      size_t k = 0;
+      for (; k < P->jobs.size(); ++k)
+        if (P->jobs[k].d == jobs[i]) break;
+      POOL_REQUIRE(k < P->jobs.size(), PASIF_ERR_ARGUMENT,
+                   "job " + std::to_string(i) + " is not in this pool");
+      const auto& j = P->jobs[k];
+      POOL_REQUIRE(!P->in_flight || j.finished, PASIF_ERR_STATE, "wait for the tick first");
+      POOL_REQUIRE(j.done >= j.work, PASIF_ERR_STATE,
+                   "job " + std::to_string(i) + " not finished");
+      const pasif_driver* d = j.d;
+      POOL_REQUIRE(q_off[i + 1] - q_off[i] == d->pk.out_len_f, PASIF_ERR_ARGUMENT,
+                   "job " + std::to_string(i) + ": " + std::to_string(d->pk.out_len_f) +
+                       " final states, slot of " + std::to_string(q_off[i + 1] - q_off[i]));
+      if (j.adjoint)
+        POOL_REQUIRE(grad && g_off[i + 1] - g_off[i] == d->pk.out_len_a, PASIF_ERR_ARGUMENT,
+                     "job " + std::to_string(i) + ": gradient slot size");
+      at[i] = pieces.size();
+      if (d->pk.tile) add(d->progress.p, size_t(d->pk.ntiles) * 4 * sizeof(uint32_t));
+      for (const auto& lp : d->pk.lparts)
+        for (const auto& L : lp->cls) add(L->progress.p, size_t(L->waves) * 4 * sizeof(uint32_t));
+      add(d->out_q.p, size_t(d->pk.out_len_f) * el);
+      if (j.adjoint) add(d->out_grad.p, size_t(d->pk.out_len_a) * el);
+    }
+    if (P->pinned_bytes < total) {
+      if (P->pinned) (void)hipHostFree(P->pinned);
+      P->pinned = nullptr;
+      P->pinned_bytes = 0;
+      const size_t want = std::max(total, size_t(1) << 20) * 2;
+      POOL_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), want, hipHostMallocDefault));
+      P->pinned_bytes = want;
+    }
+    for (const Piece& c : pieces)
+      POOL_HIP(hipMemcpyAsync(P->pinned + c.off, c.src, c.bytes, hipMemcpyDeviceToHost,
+                              P->upload));
+    POOL_HIP(hipStreamSynchronize(P->upload));
+    for (uint32_t i = 0; i < n; ++i) {
+      size_t k = 0;
+      for (; k < P->jobs.size(); ++k)
+        if (P->jobs[k].d == jobs[i]) break;
+      const bool adj = P->jobs[k].adjoint;
+      const pasif_driver* d = P->jobs[k].d;
+      size_t x = at[i];
+      auto prog = [&](uint32_t recs) {
+        return reinterpret_cast<const uint32_t*>(P->pinned + pieces[x++].off);
+      };
+      if (d->pk.tile) {
+        const uint32_t* pr = prog(d->pk.ntiles);
+        for (uint32_t t = 0; t < d->pk.ntiles; ++t) {
+          const bool done = adj ? pr[4 * t + 1] == kPhaseDone
+                                : pr[4 * t] >= d->pk.tmax || pr[4 * t + 1] == kPhaseDone;
+          POOL_REQUIRE(done, PASIF_ERR_STATE, "job tiles did not finish");
+        }
+      }
+      for (const auto& lp : d->pk.lparts)
+        for (const auto& L : lp->cls) {
+          const uint32_t* pr = prog(L->waves);
+          for (uint32_t w = 0; w < L->waves; ++w)
+            POOL_REQUIRE(pr[4 * w + 1] == kPhaseDone, PASIF_ERR_STATE,
+                         "job lane workgroups did not finish");
+        }
+      auto widen = [&](const Piece& c, double* out, size_t len) {
+        if (el == 8) {
+          std::memcpy(out, P->pinned + c.off, len * sizeof(double));
+        } else {
+          const float* f = reinterpret_cast<const float*>(P->pinned + c.off);
+          for (size_t e = 0; e < len; ++e) out[e] = double(f[e]);
+        }
+      };
+      widen(pieces[x++], q + q_off[i], d->pk.out_len_f);
+      if (adj) widen(pieces[x++], grad + g_off[i], d->pk.out_len_a);
+      pool_remove_job(P, k);
+      pasif_destroy(jobs[i]);
+    }
+  });
+}
+
 int pasif_pool_stats(const pasif_pool* P, uint32_t* jobs, double* used_bytes,
                      double* last_kernel_ms, uint32_t* last_blocks) {
   if (!P) return PASIF_ERR_ARGUMENT;
@@ -3963,6 +4129,8 @@ int pasif_pool_destroy(pasif_pool* P) {
   for (pasif_driver* d : P->handles) pool_detach(d);
   P->handles.clear();
   P->arena.reset();
+  if (P->pinned) (void)hipHostFree(P->pinned);
+  P->pinned = nullptr;
   for (int m = 0; m < 2; ++m) {
     if (P->aux[m]) {
       (void)hipStreamSynchronize(P->aux[m]);

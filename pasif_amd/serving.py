@@ -57,6 +57,11 @@ _POOL_SIGS = {
     "pasif_pool_submit_systems": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _i32p, _u32p,
                                             C.c_int, C.POINTER(C.c_void_p)]),
     "pasif_pool_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "pasif_pool_submit_batch": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _u32p, _i32p, _u32p,
+                                          _i32p, C.POINTER(C.c_void_p), _u32p]),
+    "pasif_pool_collect_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p),
+                                           _f64p, C.POINTER(C.c_size_t), _f64p,
+                                           C.POINTER(C.c_size_t)]),
 }
 
 
@@ -177,10 +182,70 @@ class Request:
     result: Optional[np.ndarray] = None
     grad: Optional[np.ndarray] = None
     keys: Optional[np.ndarray] = None     # library positions of the systems
+    nq: int = 0                           # output lengths (final states, gradient)
+    ng: int = 0
+
+    def __post_init__(self):
+        if self.systems is not None and not self.nq and \
+                all(hasattr(x, "n") for x in self.systems):
+            self.nq = sum(int(x.n) for x in self.systems)
+            self.ng = sum(int(x.na) for x in self.systems) if self.adjoint else 0
 
     @property
     def system_steps(self) -> int:
         return int(np.sum(self.steps))
+
+
+class RequestStream:
+    """``n`` random requests of library systems, the distributions of
+    ``draw_request``, drawn vectorised in blocks of ``block`` requests (each
+    block from its own seed (seed, block) on first access, so an overloaded
+    stream never builds the blocks the pool does not reach); the k-th is
+    served as a :class:`Request` of array slices."""
+
+    def __init__(self, lib: SystemLibrary, n: int, seed: int, n_signals: int, tmin=1e3,
+                 tmax=1e5, max_systems: int = 256, adjoint_frac: float = 0.7,
+                 block: int = 4096):
+        self.lib, self.n, self.seed, self.block = lib, n, seed, block
+        self.kw = dict(n_signals=n_signals, tmin=tmin, tmax=tmax, max_systems=max_systems,
+                       adjoint_frac=adjoint_frac)
+        self.lib_n = np.array([e.n for e in lib.flat], dtype=np.int64)
+        self.lib_na = np.array([getattr(e, "na", 0) for e in lib.flat], dtype=np.int64)
+        self._blocks = {}
+
+    def __len__(self):
+        return self.n
+
+    def _draw(self, b: int) -> dict:
+        idx, probs, nvar, ents, keys = _draw_tables(self.lib)
+        kw = self.kw
+        rng = np.random.default_rng((self.seed, b))
+        m = min(self.block, self.n - b * self.block)
+        cnt = rng.integers(1, kw["max_systems"] + 1, m)
+        off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+        tot = int(off[-1])
+        di = rng.choice(idx, size=tot, p=probs)
+        var = (rng.uniform(size=tot) * nvar[di]).astype(np.int64)
+        first = np.concatenate([[0], np.cumsum([len(k) for k in keys])])[:-1]
+        k = np.concatenate(keys)[first[di] + var]
+        steps = np.exp(rng.uniform(np.log(kw["tmin"]), np.log(kw["tmax"]), tot)).astype(np.int64)
+        exc = rng.integers(0, kw["n_signals"], tot)
+        adj = rng.uniform(size=m) < kw["adjoint_frac"]
+        cq = np.concatenate([[0], np.cumsum(self.lib_n[k])])
+        cg = np.concatenate([[0], np.cumsum(self.lib_na[k])])
+        nq = cq[off[1:]] - cq[off[:-1]]
+        ng = np.where(adj, cg[off[1:]] - cg[off[:-1]], 0)
+        return {"off": off, "keys": k, "steps": steps, "exc": exc, "adj": adj, "nq": nq,
+                "ng": ng}
+
+    def __getitem__(self, k: int) -> Request:
+        b, i = divmod(int(k), self.block)
+        blk = self._blocks.get(b)
+        if blk is None:
+            blk = self._blocks[b] = self._draw(b)
+        a, e = int(blk["off"][i]), int(blk["off"][i + 1])
+        return Request(int(k), None, blk["exc"][a:e], blk["steps"][a:e], bool(blk["adj"][i]),
+                       keys=blk["keys"][a:e], nq=int(blk["nq"][i]), ng=int(blk["ng"][i]))
 
 
 def _draw_tables(lib: SystemLibrary):
@@ -356,6 +421,59 @@ class Pool:
         d._h = j
         return d
 
+    def submit_batch(self, reqs: Sequence[Request]) -> list:
+        """Admit requests of registered systems in order with one C call;
+        returns the jobs of those admitted (a prefix of ``reqs``: the first
+        that does not fit the slots or the HBM budget stops the batch)."""
+        n = len(reqs)
+        if n == 0:
+            return []
+        cnt = np.array([len(r.keys) for r in reqs], dtype=np.int64)
+        off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(cnt)]), dtype=np.uint32)
+        ids = np.ascontiguousarray(self.ids[np.concatenate([r.keys for r in reqs])],
+                                   dtype=np.uint32)
+        exc = np.ascontiguousarray(np.concatenate([r.exc_ids for r in reqs]), dtype=np.int32)
+        steps = np.ascontiguousarray(np.concatenate([r.steps for r in reqs]), dtype=np.uint32)
+        adj = np.ascontiguousarray([int(r.adjoint) for r in reqs], dtype=np.int32)
+        jobs = (C.c_void_p * n)()
+        got = C.c_uint32(0)
+        rc = self.lib.pasif_pool_submit_batch(
+            self.h, n, off.ctypes.data_as(_u32p), ids.ctypes.data_as(_u32p),
+            exc.ctypes.data_as(_i32p), steps.ctypes.data_as(_u32p), adj.ctypes.data_as(_i32p),
+            jobs, C.byref(got))
+        out = []
+        for i in range(got.value):
+            d = object.__new__(_Driver)
+            d._pl = self.lib
+            d._h = C.c_void_p(jobs[i])
+            out.append(d)
+        self._chk(rc)
+        return out
+
+    def collect_batch(self, jobs: Sequence, reqs: Sequence[Request]):
+        """Collect finished jobs with one C call (their handles are destroyed):
+        sets ``result`` / ``grad`` of each request."""
+        n = len(jobs)
+        if n == 0:
+            return
+        nq = np.array([r.nq for r in reqs], dtype=np.int64)
+        ng = np.array([r.ng if r.adjoint else 0 for r in reqs], dtype=np.int64)
+        q_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(nq)]), dtype=np.uint64)
+        g_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(ng)]), dtype=np.uint64)
+        q = np.empty(int(q_off[-1]))
+        g = np.empty(max(int(g_off[-1]), 1))
+        hs = (C.c_void_p * n)(*[j._h for j in jobs])
+        szp = C.POINTER(C.c_size_t)
+        rc = self.lib.pasif_pool_collect_batch(self.h, n, hs, q.ctypes.data_as(_f64p),
+                                               q_off.ctypes.data_as(szp), g.ctypes.data_as(_f64p),
+                                               g_off.ctypes.data_as(szp))
+        for j in jobs:
+            j._h = None
+        self._chk(rc)
+        for i, r in enumerate(reqs):
+            r.result = q[q_off[i]:q_off[i + 1]]
+            r.grad = g[g_off[i]:g_off[i + 1]] if r.adjoint else None
+
     def set_slots(self, max_tiles: int):
         self._chk(self.lib.pasif_pool_set_slots(self.h, int(max_tiles)))
 
@@ -382,8 +500,8 @@ class Pool:
     def collect(self, d, req: "Request"):
         adjoint = req.adjoint
         lq, lg = C.c_size_t(0), C.c_size_t(0)
-        q = np.empty(sum(s.n for s in req.systems))
-        g = np.empty(sum(s.na for s in req.systems) if adjoint else 0)
+        q = np.empty(req.nq)
+        g = np.empty(req.ng if adjoint else 0)
         self._chk(self.lib.pasif_pool_collect(self.h, d._h, q.ctypes.data_as(_f64p), q.size,
                                               C.byref(lq), g.ctypes.data_as(_f64p), g.size,
                                               C.byref(lg)))
@@ -476,14 +594,53 @@ class SolveServer:
                 queue.append(i)
                 i += 1
 
+        batched = by_id and hasattr(self.pool, "submit_batch")
+
         def collect_pending():
             nonlocal blocked
-            for job, r in pending:
-                r.result, r.grad = self.pool.collect(job, r)
-                r.done = time.perf_counter() - t0
-                done.append(r)
-                blocked = False
+            if not pending:
+                return
+            if batched:
+                self.pool.collect_batch([j for j, _ in pending], [r for _, r in pending])
+                now = time.perf_counter() - t0
+                for _, r in pending:
+                    r.done = now
+                    done.append(r)
+            else:
+                for job, r in pending:
+                    r.result, r.grad = self.pool.collect(job, r)
+                    r.done = time.perf_counter() - t0
+                    done.append(r)
+            blocked = False
             pending.clear()
+
+        def admit_batch(limit: int = 256) -> bool:
+            """Admit queued requests in one call; False when nothing was
+            admitted (none queued, or the pool is full)."""
+            nonlocal admitted, blocked, admit_s
+            if blocked or not queue:
+                return False
+            ta = time.perf_counter()
+            take = [queue[k] for k in range(min(limit, len(queue)))]
+            batch = []
+            for k in take:
+                r = reqs[k]
+                r.arrival = float(arr[k])
+                batch.append(r)
+            jobs = self.pool.submit_batch(batch)
+            admit_s += time.perf_counter() - ta
+            now = time.perf_counter() - t0
+            for job, r in zip(jobs, batch):
+                r.admitted = now
+                work = int(np.max(r.steps)) * (3 if r.adjoint else 1)
+                ready = ticks + max(1, -(-work // self.pool.steps_per_tick))
+                due.setdefault(ready, []).append((job, r))
+                nres[0] += 1
+                admitted += 1
+                queue.popleft()
+            if len(jobs) < len(batch):
+                blocked = True
+            return len(jobs) > 0
 
         def admit_one() -> bool:
             """Admit the oldest queued request; False if none or refused."""
@@ -515,9 +672,10 @@ class SolveServer:
             if horizon_s is not None and now > horizon_s and not drain:
                 break
             arrivals_until(now)
+            admit = admit_batch if batched else admit_one
             if not nres[0]:
                 collect_pending()
-                while admit_one():
+                while admit():
                     pass
             if nres[0]:
                 self.pool.tick(wait=False)
@@ -528,7 +686,7 @@ class SolveServer:
                 collect_pending()
                 while self.pool.running():
                     arrivals_until(time.perf_counter() - t0)
-                    if not admit_one():
+                    if not admit():
                         break
                 busy_ms += self.pool.wait()
                 for t in [t for t in due if t <= ticks]:

@@ -322,3 +322,48 @@ def test_job_handles_leave_and_outlive_the_pool():
     pool.close()
     assert pool.lib.pasif_destroy(jobs[3]._h) == 0
     jobs[3]._h = None
+
+
+def test_batch_submit_prefix_and_batch_collect_equal_single_calls():
+    """pasif_pool_submit_batch admits requests in order until the slot cap
+    refuses one (the rest wait); pasif_pool_collect_batch returns every job's
+    results bitwise equal to the same request submitted and collected alone."""
+    print("[gpu case] serving batch submit / collect", flush=True)
+    rng = np.random.default_rng(47)
+    lib = S.SystemLibrary(dofs=(1, 2, 8, 16), variants=2)
+    bank = S.signal_bank(rng, 5, 2000, steps_per_sample=10)
+    from pasif_amd.workloads import linear_interp_matrix
+    interp = linear_interp_matrix(10)
+
+    def new_pool(slots=0):
+        pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=400, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        if slots:
+            pool.set_slots(slots)
+        return pool
+
+    stream = S.RequestStream(lib, 12, 5, 5, tmin=100, tmax=1200, max_systems=12)
+    reqs = [stream[k] for k in range(12)]
+    pool = new_pool(slots=6)
+    jobs = pool.submit_batch(reqs)
+    assert 0 < len(jobs) < len(reqs)          # the slot cap stops the batch
+    rest = reqs[len(jobs):]
+    first = reqs[:len(jobs)]
+    while not all(pool.finished(j) for j in jobs):
+        pool.tick(wait=True)
+    pool.collect_batch(jobs, first)
+    jobs2 = pool.submit_batch(rest)
+    assert len(jobs2) > 0
+    while not all(pool.finished(j) for j in jobs2):
+        pool.tick(wait=True)
+    pool.collect_batch(jobs2, rest[:len(jobs2)])
+    for r in reqs[:len(jobs) + len(jobs2)]:
+        solo = new_pool()
+        j = solo.submit_ids(r)
+        while not solo.finished(j):
+            solo.tick(wait=True)
+        q, g = solo.collect(j, r)
+        np.testing.assert_array_equal(r.result, q)
+        if r.adjoint:
+            np.testing.assert_array_equal(r.grad, g)
