@@ -20,7 +20,8 @@ Beside it, in the same line:
 * ``cpu_baseline``: the FP64 NumPy oracle on the host, every core (all 4,096
   systems, sharded) and one core (256 systems);
 * ``fwd_adj``: forward + checkpointed adjoint on 8,192 chains of 16 DOF x
-  20,000 steps, persistent vs plain, roofline and parity on 256 chains;
+  20,000 steps, persistent vs plain, roofline and parity on 256 chains, and
+  the differentiable form (torch.autograd.Function) on the same chains;
 * ``batching``: the 100,000-system heterogeneous batch, FP64 and FP32,
   forward and forward + adjoint, persistent vs plain, parity on a stratified
   sample;
@@ -293,6 +294,51 @@ def fwd_adj_section(args) -> dict:
     if par:
         sec["parity"] = par
     return sec
+
+
+def differentiable_section(args) -> dict:
+    """The differentiable form (``autograd.rk4_solve``, a torch.autograd
+    Function) on the 16-DOF chains: loss = sum(q(T)), gradients w.r.t. q0 and
+    the 16 K_ii + 16 Lambda_i of every chain; one forward + backward, timed
+    after a warm-up, against the raw driver calls it makes on the same
+    problem (getAmplitudes + getGradient).  The host time is what the
+    Function spends re-assembling the systems and their adjoints for the
+    current coefficients."""
+    import torch
+    from pasif_amd import workloads as W
+    from pasif_amd.autograd import DifferentiableProblem, rk4_solve
+    w = W.chains(n_systems=args.adj_systems, d=16, steps=args.adj_steps, steps_per_sample=10)
+    d = 16
+    params = [[("K", d + i, i) for i in range(d)] + [("Lambda", t) for t in range(d)]
+              for _ in range(w.n_systems)]
+    out = {"workload": f"chains {w.n_systems} x 16 DOF x {w.numsteps} steps, loss sum(q(T)), "
+                       f"gradient w.r.t. q0 and {32 * w.n_systems} coefficients"}
+    for mode in ("reference", "exact"):
+        prob = DifferentiableProblem(w, params, checkpoint_every=256, mode=mode)
+        q0 = torch.tensor(prob.q00(), dtype=torch.float64, requires_grad=True)
+        th = torch.tensor(prob.theta0(), dtype=torch.float64, requires_grad=True)
+
+        def step():
+            qT = rk4_solve(prob, q0, th)
+            qT.sum().backward()
+        step()                                       # warm-up (packing, compilation)
+        prob.timing = {"host_s": 0.0, "solves": 0}
+        t0 = time.perf_counter()
+        step()
+        wall = time.perf_counter() - t0
+        p = prob.solver
+        t1 = time.perf_counter()
+        p.getAmplitudes()
+        kq = p._lastTiming()["kernel_ms"]
+        p.getGradient()
+        kg = p._lastTiming()["kernel_ms"]
+        raw = time.perf_counter() - t1
+        out[mode] = {"kernel": p.kernelInfo(adjoint=True)["kernel"], "function_s": wall,
+                     "host_assembly_s": prob.timing["host_s"], "raw_calls_s": raw,
+                     "raw_kernel_ms": {"forward": kq, "forward_adjoint": kg},
+                     "function_vs_raw": wall / raw}
+        _progress(f"differentiable {mode}: {wall:.2f} s (raw {raw:.2f} s)")
+    return out
 
 
 def batching_section(args) -> dict:
@@ -596,6 +642,8 @@ def main():
     if not args.no_adjoint:
         _progress("fwd_adj")
         out["fwd_adj"] = fwd_adj_section(args)
+        _progress("differentiable form")
+        out["fwd_adj"]["differentiable"] = differentiable_section(args)
     if not args.no_batching:
         _progress("batching")
         out["batching"] = batching_section(args)

@@ -1460,6 +1460,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
 
 template <typename R>
 bool lane_plan(pasif_driver* d, bool adjoint) {
+  const auto tp0 = std::chrono::steady_clock::now();
   Packed& pk = d->pk;
   const uint32_t S = uint32_t(d->fwd.size.size());
   const double eps = std::numeric_limits<R>::epsilon();
@@ -1599,7 +1600,13 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   lk.seg = K;
   std::vector<uint32_t> klass(shapes.size());
   for (size_t c = 0; c < shapes.size(); ++c) klass[c] = uint32_t(c);
+  const auto tp1 = std::chrono::steady_clock::now();
   LanePart* part = lane_build<R>(d, adjoint, 0, shapes, klass, lk);
+  if (std::getenv("PASIF_PACK_TIMING"))
+    fprintf(stderr, "[pasif] lane plan %.1f ms, lane build %.1f ms\n",
+            std::chrono::duration<double, std::milli>(tp1 - tp0).count(),
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1)
+                .count());
   const size_t lds = lane_lds_bytes(shapes, int(sizeof(R) * 8), lk);
   PASIF_REQUIRE(lds <= (lk.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
                 "lane kernel LDS %zu bytes", lds);

@@ -3,9 +3,12 @@ forward kernel and the hand-written adjoint kernel):
 
 * gradients of ``L = sum(w * q(T))`` w.r.t. q0 and theta (K diagonal and
   Lambda coefficients of chain systems) equal the oracle adjoint of the same
-  assembled adjoint system within 1e-10 relative;
-* they agree with central finite differences of the forward (h = 1e-6)
-  within 2e-4 relative (the adjoint scheme is second order in dt).
+  assembled adjoint system within 1e-10 relative, in both modes (the
+  reference recursion: ``adjoint_gradient``; the exact discrete adjoint, the
+  default: ``discrete_adjoint_gradient``);
+* they agree with central finite differences of the forward (h = 1e-6):
+  the exact mode to 1e-6 relative (finite-difference precision), the
+  reference recursion within 2e-4 (its scheme is second order in dt).
 """
 import numpy as np
 import pytest
@@ -18,15 +21,16 @@ from tests.test_gpu_forward import TOL64, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def test_autograd_matches_oracle_and_finite_differences():
-    print("[gpu case] autograd chains 4 x d4 x 600", flush=True)
+@pytest.mark.parametrize("mode,fd_tol", [("reference", 2e-4), ("exact", 1e-6)])
+def test_autograd_matches_oracle_and_finite_differences(mode, fd_tol):
+    print(f"[gpu case] autograd chains 4 x d4 x 600, {mode} adjoint", flush=True)
     import torch
     from pasif_amd.autograd import DifferentiableProblem, rk4_solve
     w = W.chains(n_systems=4, d=4, steps=600, steps_per_sample=10)
     d = 4
     params = [[("K", d + i, i) for i in range(d)] + [("Lambda", t) for t in range(d)]
               for _ in range(w.n_systems)]
-    prob = DifferentiableProblem(w, params, checkpoint_every=64)
+    prob = DifferentiableProblem(w, params, checkpoint_every=64, mode=mode)
     rng = np.random.default_rng(3)
     q0 = torch.tensor(prob.q00() + rng.normal(0, 0.05, prob.q00().size), dtype=torch.float64,
                       requires_grad=True)
@@ -45,8 +49,9 @@ def test_autograd_matches_oracle_and_finite_differences():
     fwd = O.assemble_forward(w.vecM, vB, vK, vG, vL, w.vecForce, vQ)
     adj = O.assemble_adjoint(*J.as_args(), n_fwd=fwd.n)
     lay = O.mapped_layout(fwd.sizes, w.exc_of_system, np.full(w.n_systems, w.numsteps))
-    lam0 = O.adjoint_gradient(fwd, adj, O.Forcing(w.excitations, w.interp), w.sample_rate, lay,
-                              np.full((1, adj.n), w.numsteps), checkpoint_every=64)[0]
+    grad_fn = O.adjoint_gradient if mode == "reference" else O.discrete_adjoint_gradient
+    lam0 = grad_fn(fwd, adj, O.Forcing(w.excitations, w.interp), w.sample_rate, lay,
+                   np.full((1, adj.n), w.numsteps), checkpoint_every=64)[0]
     aoff = np.concatenate([[0], np.cumsum(J.sizes)])
     oq = np.concatenate([lam0[aoff[s]:aoff[s] + prob.sizes[s]] for s in range(w.n_systems)])
     ot = np.concatenate([lam0[aoff[s] + prob.sizes[s]:aoff[s + 1]] for s in range(w.n_systems)])
@@ -65,9 +70,9 @@ def test_autograd_matches_oracle_and_finite_differences():
         e = np.zeros_like(th0)
         e[j] = h
         fd = (L(qq, th0 + e) - L(qq, th0 - e)) / (2 * h)
-        assert abs(gt[j] - fd) <= 2e-4 * abs(fd) + 1e-9, (j, gt[j], fd)
+        assert abs(gt[j] - fd) <= fd_tol * abs(fd) + 1e-9, (j, gt[j], fd)
     for i in (0, 4, 9, 20):
         e = np.zeros_like(qq)
         e[i] = h
         fd = (L(qq + e, th0) - L(qq - e, th0)) / (2 * h)
-        assert abs(gq[i] - fd) <= 2e-4 * abs(fd) + 1e-9, (i, gq[i], fd)
+        assert abs(gq[i] - fd) <= fd_tol * abs(fd) + 1e-9, (i, gq[i], fd)
