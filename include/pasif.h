@@ -41,8 +41,10 @@ enum {
 
 enum { PASIF_SOLVER_PERSISTENT = 0, PASIF_SOLVER_PLAIN = 1, PASIF_SOLVER_GRAPH = 2 };
 enum { PASIF_KERNEL_AUTO = 0, PASIF_KERNEL_TILE = 1, PASIF_KERNEL_LANE = 2 };
-/* reported by pasif_kernel_info when both families share a solve */
+/* reported by pasif_kernel_info when several families share a solve */
 enum { PASIF_KERNEL_MIXED = 3 };
+/* row-parallel path: one workgroup per instance (extension) */
+enum { PASIF_KERNEL_BIG = 4 };
 
 /* __GpuDriver(excitationSet_, sampleRate_, numsteps_, dCompute_, dSystem_,
  * dSolver_, GPUId_)  (src/__GpuDriver.cu:25-157).  excitations is
@@ -151,15 +153,22 @@ int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* l
 int pasif_last_timing(const pasif_driver* d, double* kernel_ms, uint32_t* launches,
                       uint32_t* tiles, uint32_t* lanes);
 /* Kernel family.  PASIF_KERNEL_TILE: rows of packed systems across the lanes
- * of a workgroup, stage values exchanged through LDS (any system size).
+ * of a workgroup, stage values exchanged through LDS (systems of up to 1024
+ * lanes of row segments).
  * PASIF_KERNEL_LANE: one lane per (system, excitation) with the state and
  * coefficients in registers, code generated per sparsity structure and
  * compiled at the first solve (small systems: PASIF_ERR_UNSUPPORTED when a
  * system does not fit; at most 8 distinct structures).  PASIF_KERNEL_AUTO
  * (default): the lane kernel for the structures that fit it (the 8 with the
- * most work), the tile kernel for the rest, launched side by side. */
+ * most work), the tile kernel for the rest, launched side by side, and the
+ * row-parallel path for systems that fit neither.  PASIF_KERNEL_BIG: every
+ * instance on the row-parallel path -- one workgroup of 1024 threads per
+ * (system, excitation), rows strided over its threads, state and stage
+ * vectors in device memory, a barrier between stages; any system size, no
+ * trajectory or amplitude window (PASIF_ERR_UNSUPPORTED), not for pool jobs
+ * or pasif_solve_noalloc. */
 int pasif_set_kernel(pasif_driver* d, int kernel);
-/* what the packed problem runs on: kernel (TILE, LANE or MIXED), structure classes
+/* what the packed problem runs on: kernel (TILE, LANE, BIG or MIXED), structure classes
  * (lane kernel; 0 for tiles) and the checkpoint interval the adjoint uses
  * (the lane kernel shortens it until a segment fits its LDS) */
 int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* classes,

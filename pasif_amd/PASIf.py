@@ -479,10 +479,13 @@ class PASIf(_GpuDriver):
         self._setTuning(tile_lanes, slot_config, checkpoint_every)
 
     def setKernel(self, kernel: str = "auto", lanes_per_system: int = 0, group_bias: int = 0):
-        """"tile" (any system size), "lane" (one lane -- or a group of lanes
+        """"tile" (rows across a workgroup's lanes, systems of up to 1024
+        lanes of row segments), "lane" (one lane -- or a group of lanes
         exchanging values by lane rotations -- per system, state in
-        registers) or "auto" (lane kernel for the structures that fit it,
-        tiles for the rest).  ``lanes_per_system`` 0 lets a cost model pick
+        registers), "big" (one workgroup per system with every vector in
+        device memory: any size, no trajectory or amplitude window) or
+        "auto" (lane kernel for the structures that fit it, tiles for the
+        rest, "big" for systems that fit neither).  ``lanes_per_system`` 0 lets a cost model pick
         the group size per structure (``group_bias`` moves every choice that
         many steps along the sizes that fit; the autotuner probes it); a
         power of two up to 64 requests that size."""
@@ -493,8 +496,8 @@ class PASIf(_GpuDriver):
     def setAdjointMode(self, mode: str = "reference"):
         """``"reference"``: the reference's adjoint recursion (continuous
         adjoint, ``src/__GpuDriver.cu:746-783``).  ``"exact"``: the exact
-        discrete adjoint, the derivative of the computed RK4 map (tile
-        kernel)."""
+        discrete adjoint, the derivative of the computed RK4 map (tile and
+        big kernels)."""
         self._setAdjointMode(mode)
 
     def kernelInfo(self, adjoint: bool = False) -> dict:

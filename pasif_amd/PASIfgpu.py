@@ -233,7 +233,7 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
     def _getMaxAmplitudes(self) -> np.ndarray:
         return self._get(self._pl.pasif_get_max_amplitudes)
 
-    _KERNELS = {"auto": 0, "tile": 1, "lane": 2}
+    _KERNELS = {"auto": 0, "tile": 1, "lane": 2, "big": 4}
 
     def _setKernel(self, kernel: str = "auto"):
         if kernel not in self._KERNELS:
@@ -261,7 +261,7 @@ class __GpuDriver:  # noqa: N801  (the reference's class name)
         k, ncls, ck = C.c_int(0), C.c_uint32(0), C.c_uint32(0)
         self._chk(self._pl.pasif_kernel_info(self._h, int(adjoint), C.byref(k), C.byref(ncls),
                                              C.byref(ck)))
-        name = {1: "tile", 2: "lane", 3: "mixed"}.get(k.value, "?")
+        name = {1: "tile", 2: "lane", 3: "mixed", 4: "big"}.get(k.value, "?")
         groups = self._laneGroups(adjoint) if k.value in (2, 3) else []
         return {"kernel": name, "classes": ncls.value, "checkpoint_every": ck.value,
                 "lanes_per_instance": groups}

@@ -226,18 +226,28 @@ struct Packed {
   DevBuf ck_tile;
   uint32_t ck_every = 0;
   uint64_t ck_slots = 0;
+  // row-parallel path (systems larger than a workgroup's lanes): instances,
+  // packed sides, per-instance scratch sizes (elements)
+  bool big = false;
+  uint32_t nbig = 0;
+  DevBuf b_inst, b_rf, b_vf, b_xf, b_ff, b_if, b_ra, b_va, b_xa, b_ia;
+  uint64_t b_vec_f = 0, b_vec_a = 0, b_ck = 0, b_seg = 0;
+  uint32_t b_tmax = 0, b_every = 1;
   uint32_t max_forced = 0;
   size_t program_bytes = 0;
   void release() {
     DevBuf* all[] = {&hdr,     &f_info,  &f_out,  &f_src,   &f_steps, &f_exc, &f_fslot,
                      &f_force, &f_val,   &f_idx,  &f_init,  &a_info,  &a_out, &a_src,
                      &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init,
-                     &ck_tile};
+                     &ck_tile, &b_inst,  &b_rf,    &b_vf,    &b_xf,    &b_ff,  &b_if,
+                     &b_ra,    &b_va,    &b_xa,    &b_ia};
     for (DevBuf* b : all) b->release();
     lparts.clear();
     lshapes.clear();
     lane = false;
     tile = false;
+    big = false;
+    nbig = 0;
     ltmax = 0;
     valid = false;
   }
@@ -284,6 +294,8 @@ struct pasif_driver {
   int lane_bias = 0;
   // lane class of every system in the packed problem, -1: tile kernel
   std::vector<int32_t> lane_cls;
+  // systems on the row-parallel path (too large for a tile), by system
+  std::vector<uint8_t> big_sys;
   // second stream and fork / join events when lane and tile kernels run together
   hipStream_t aux = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
@@ -296,6 +308,8 @@ struct pasif_driver {
   bool d_exc_dirty = true;
   // run buffers
   DevBuf qstate, lstate, mxstate, progress, out_q, out_max, traj, out_grad, ckpt, segbuf;
+  // run buffers of the row-parallel path
+  DevBuf bvec_f, bvec_a, bsetp, bckpt, bseg, bprog;
   // results of the last solve (host, double)
   std::vector<double> res_q, res_max;
   bool have_max = false;
@@ -828,9 +842,11 @@ void validate_program(pasif_driver* d, const HostProgram<R>& hp, const std::vect
 
 // Instances of the problem; `part` keeps those of the lane kernel's systems
 // (kPartLane), of the tile kernel's (kPartTile) or all (kPartAll).
-enum { kPartAll = 0, kPartLane = 1, kPartTile = 2 };
+enum { kPartAll = 0, kPartLane = 1, kPartTile = 2, kPartBig = 3 };
 bool in_part(const pasif_driver* d, uint32_t s, int part) {
   if (part == kPartAll) return true;
+  const bool big = s < d->big_sys.size() && d->big_sys[s] != 0;
+  if (part == kPartBig || big) return part == kPartBig && big;
   const bool lane = s < d->lane_cls.size() && d->lane_cls[s] >= 0;
   return part == kPartLane ? lane : !lane;
 }
@@ -1071,6 +1087,119 @@ void pack_impl(pasif_driver* d, bool adjoint) {
            "program\n",
            inst.size(), ntiles, L, cfg, tmax,
            pk.program_bytes / 1048576.0);
+}
+
+// Row-parallel packing: every instance with its own copy of its system's
+// terms (local rows, kinds in order per row), force and initial values in
+// the working precision, and its scratch offsets.
+template <typename R>
+void pack_big(pasif_driver* d, bool adjoint) {
+  Packed& pk = d->pk;
+  std::vector<Instance> inst = make_instances(d, adjoint, kPartBig);
+  const double eps = std::numeric_limits<R>::epsilon();
+  struct Side {
+    std::vector<uint32_t> rptr, idx;
+    std::vector<R> val, force, init;
+  };
+  Side sf, sa;
+  const uint32_t K0 = d->ck_every ? d->ck_every : 256;
+  uint64_t vf = 0, va = 0, ck = 0, sg = 0;
+  uint32_t tmax = 0, K = 1;
+  for (const Instance& in : inst) tmax = std::max(tmax, in.steps);
+  if (adjoint) K = std::max<uint32_t>(1, std::min<uint32_t>(K0, std::max<uint32_t>(tmax, 1)));
+  std::vector<BigInst> bi(inst.size());
+  auto add_side = [&](const RowTerms& rt, uint32_t s, bool adj, const Instance& in, Side& out,
+                      uint64_t& rptr_off, uint64_t& term_off, uint64_t& row_off) {
+    const uint32_t off = rt.off[s], n = rt.size[s];
+    const uint32_t xo = adj ? rt.xoff[s] : off;
+    rptr_off = out.rptr.size();
+    term_off = out.val.size();
+    row_off = out.init.size();
+    uint32_t t = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+      const uint32_t g = off + r;
+      for (int k = 0; k < 4; ++k) {
+        out.rptr.push_back(t);
+        const int nf = k + 1;
+        for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e, ++t) {
+          out.val.push_back(R(rt.val[k][e]));
+          uint32_t w[4] = {0, 0, 0, 0};
+          w[0] = rt.idx[k][size_t(e) * nf] - off;
+          for (int a = 1; a < nf; ++a) w[a] = rt.idx[k][size_t(e) * nf + a] - xo;
+          for (int a = 0; a < 4; ++a) out.idx.push_back(w[a]);
+        }
+      }
+      const double f = adj ? 0.0 : rt.force[g];
+      out.force.push_back((std::fabs(f) > eps && in.exc >= 0) ? R(f) : R(0));
+      out.init.push_back(R(rt.init[g]));
+    }
+    out.rptr.push_back(t);
+  };
+  for (size_t i = 0; i < inst.size(); ++i) {
+    const Instance& in = inst[i];
+    BigInst& b = bi[i];
+    add_side(d->fwd, in.sys, false, in, sf, b.rptr_f, b.term_f, b.row_f);
+    b.n_f = d->fwd.size[in.sys];
+    b.steps = in.steps;
+    b.exc = in.exc;
+    b.out_f = in.out_f;
+    b.scr_f = vf;
+    vf += 4ull * b.n_f;
+    if (adjoint) {
+      add_side(d->adj, in.sys, true, in, sa, b.rptr_a, b.term_a, b.row_a);
+      b.n_a = d->adj.size[in.sys];
+      b.out_a = in.out_a;
+      b.scr_a = va;
+      va += 4ull * b.n_a;
+      b.ck = ck;
+      ck += uint64_t(std::max<uint32_t>(1, (in.steps + K - 1) / K)) * b.n_f;
+      b.seg = sg;
+      sg += uint64_t(d->adj_mode ? 4ull * K : K + 1ull) * b.n_f;
+    }
+  }
+  PASIF_REQUIRE(sf.val.size() < 0xFFFFFFFFull && sa.val.size() < 0xFFFFFFFFull,
+                PASIF_ERR_UNSUPPORTED, "more than 2^32 terms");
+  upload(d, pk.b_inst, bi);
+  upload(d, pk.b_rf, sf.rptr);
+  upload(d, pk.b_vf, sf.val);
+  upload(d, pk.b_xf, sf.idx);
+  upload(d, pk.b_ff, sf.force);
+  upload(d, pk.b_if, sf.init);
+  if (adjoint) {
+    upload(d, pk.b_ra, sa.rptr);
+    upload(d, pk.b_va, sa.val);
+    upload(d, pk.b_xa, sa.idx);
+    upload(d, pk.b_ia, sa.init);
+  }
+  if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
+  pk.big = true;
+  pk.nbig = uint32_t(inst.size());
+  pk.b_vec_f = vf;
+  pk.b_vec_a = va;
+  pk.b_ck = ck;
+  pk.b_seg = sg;
+  pk.b_tmax = tmax;
+  pk.b_every = K;
+  pk.out_len_f = d->mapped ? d->fwd.n : d->fwd.n * d->n_exc;
+  pk.out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
+  if (d->d_compute)
+    printf("[pasif] %zu instances on the row-parallel path, T max %u\n", inst.size(), tmax);
+}
+
+// Whether system s fits one tile under some slot configuration the tile
+// packer may choose (the same test pack_impl applies to every system).
+bool fits_tile(const pasif_driver* d, uint32_t s, bool adjoint) {
+  const uint32_t cands[3] = {3, 1, 2};
+  for (uint32_t cand : cands) {
+    if (d->slot_cfg && cand != d->slot_cfg) continue;
+    const SlotConfig c = slot_of(cand);
+    const uint32_t lf = system_lanes(d->fwd, s, c);
+    const uint32_t la = adjoint ? system_lanes(d->adj, s, c) : 0u;
+    if (lf == 0 || (adjoint && la == 0)) continue;
+    const uint32_t l = std::max(lf, la);
+    if (l <= kMaxLanes && (l <= 256 || cand == 1)) return true;
+  }
+  return false;
 }
 
 // 0 for an automatic group size, log2(lanes) + 1 otherwise (3 bits)
@@ -1843,12 +1972,20 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
   if (d->pk.valid && d->pk.key == key) return;
   d->pk.valid = false;
   d->pk.release();
-  if (!d->pool) d->lane_cls.clear();
-  // the exact discrete adjoint runs on the tile kernel only
+  if (!d->pool) {
+    d->lane_cls.clear();
+    d->big_sys.clear();
+  }
+  // the exact discrete adjoint runs on the tile and row-parallel kernels only
   const bool exact = adjoint && d->adj_mode != 0;
   PASIF_REQUIRE(!(exact && d->kernel_path == PASIF_KERNEL_LANE), PASIF_ERR_UNSUPPORTED,
                 "the exact discrete adjoint runs on the tile kernel (kernel 'auto' or 'tile')");
-  if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool && !exact) {
+  PASIF_REQUIRE(!(d->pool && d->kernel_path == PASIF_KERNEL_BIG), PASIF_ERR_UNSUPPORTED,
+                "pool jobs run on the tile and lane kernels");
+  const uint32_t S = uint32_t(d->fwd.size.size());
+  if (d->kernel_path == PASIF_KERNEL_BIG) {
+    d->big_sys.assign(S, 1);
+  } else if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool && !exact) {
     if (d->precision == 64)
       (void)lane_plan<double>(d, adjoint);
     else
@@ -1859,9 +1996,27 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
     else
       pool_lane_parts<float>(d, adjoint);
   }
-  // the tile kernel takes every system the lane kernel did not
-  bool rest = !d->pk.lane;
-  for (size_t s = 0; s < d->lane_cls.size() && !rest; ++s) rest = d->lane_cls[s] < 0;
+  // under the automatic choice, systems the lane kernel did not take that do
+  // not fit a tile run on the row-parallel path (not for pool jobs)
+  if (!d->pool && d->kernel_path == PASIF_KERNEL_AUTO) {
+    d->big_sys.assign(S, 0);
+    bool any = false;
+    for (uint32_t s = 0; s < S; ++s)
+      if (in_part(d, s, kPartTile) && !fits_tile(d, s, adjoint)) {
+        d->big_sys[s] = 1;
+        any = true;
+      }
+    if (!any) d->big_sys.clear();
+  }
+  if (!d->big_sys.empty()) {
+    if (d->precision == 64)
+      pack_big<double>(d, adjoint);
+    else
+      pack_big<float>(d, adjoint);
+  }
+  // the tile kernel takes every other system
+  bool rest = false;
+  for (uint32_t s = 0; s < S && !rest; ++s) rest = in_part(d, s, kPartTile);
   if (rest) {
     if (d->precision == 64)
       pack_impl<double>(d, adjoint);
@@ -1873,7 +2028,7 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
     d->pk.tmax = 0;
   }
   Packed& pk = d->pk;
-  pk.tmax = std::max(pk.tmax, pk.ltmax);
+  pk.tmax = std::max(pk.tmax, std::max(pk.ltmax, pk.b_tmax));
   if (pk.lane && !pk.tile) {
     pk.ntiles = 0;  // reported: the lane launch's workgroups
     for (const auto& lp : pk.lparts) pk.ntiles += lp->blocks;
@@ -2005,6 +2160,50 @@ TileArgs<R> tile_run_args(pasif_driver* d, bool adjoint, uint32_t save_steps, Ru
   }
   *work_out = work;
   *nslots_out = nslots;
+  return a;
+}
+
+// Arguments of a run of the row-parallel instances with the run buffers
+// from `mem` (progress cleared) and the given outputs; *work as for tiles.
+template <typename R>
+BigArgs<R> big_run_args(pasif_driver* d, bool adjoint, RunMem& mem, void* out_q,
+                        void* out_grad, uint64_t* work_out) {
+  Packed& pk = d->pk;
+  BigArgs<R> a{};
+  a.inst = pk.b_inst.as<BigInst>();
+  a.fwd = {pk.b_rf.as<uint32_t>(), pk.b_vf.as<R>(), pk.b_xf.as<uint32_t>(), pk.b_ff.as<R>(),
+           pk.b_if.as<R>()};
+  if (adjoint)
+    a.adj = {pk.b_ra.as<uint32_t>(), pk.b_va.as<R>(), pk.b_xa.as<uint32_t>(), nullptr,
+             pk.b_ia.as<R>()};
+  a.excit = d->ext_exc ? static_cast<const R*>(d->ext_exc) : d->d_exc.as<R>();
+  a.exc_len = d->exc_len;
+  a.interp = d->interp_w ? (d->ext_interp ? static_cast<const R*>(d->ext_interp)
+                                          : d->d_interp.as<R>())
+                         : nullptr;
+  a.interp_w = d->interp_w;
+  const R h = R(1.0 / double(d->sample_rate));
+  a.h = h;
+  a.h2 = R(double(h) / 2.0);
+  a.h6 = R(double(h) / 6.0);
+  const size_t el = sizeof(R);
+  a.vec_f = static_cast<R*>(mem.get(d->bvec_f, size_t(pk.b_vec_f) * el));
+  a.progress = static_cast<uint32_t*>(mem.get(d->bprog, size_t(pk.nbig) * 4 * sizeof(uint32_t)));
+  mem.zero(a.progress, size_t(pk.nbig) * 4 * sizeof(uint32_t));
+  a.out_q = static_cast<R*>(out_q);
+  a.ck_every = pk.b_every;
+  uint64_t work = pk.b_tmax;
+  if (adjoint) {
+    a.vec_a = static_cast<R*>(mem.get(d->bvec_a, size_t(pk.b_vec_a) * el));
+    a.setp = static_cast<R*>(mem.get(d->bsetp, size_t(pk.b_vec_f) * el));
+    a.ckpt = static_cast<R*>(mem.get(d->bckpt, size_t(pk.b_ck) * el, true));
+    a.segbuf = static_cast<R*>(mem.get(d->bseg, size_t(pk.b_seg) * el));
+    a.out_grad = static_cast<R*>(out_grad);
+    a.adjoint = 1;
+    a.exact = d->adj_mode != 0 ? 1u : 0u;
+    work = 3ull * pk.b_tmax;
+  }
+  *work_out = work;
   return a;
 }
 
@@ -2262,6 +2461,7 @@ void run_footprint(pasif_driver* d, bool adjoint, uint32_t save_steps, size_t* t
   if (pk.tile)
     (void)tile_run_args<R>(d, adjoint, save_steps, m, nullptr, nullptr, nullptr, nullptr, &work,
                            &nslots);
+  if (pk.big) (void)big_run_args<R>(d, adjoint, m, nullptr, nullptr, &work);
   for (auto& lp : pk.lparts) {
     auto args = lane_part_args<R>(d, *lp, adjoint, save_steps, nslots, nullptr, nullptr, &m,
                                   nullptr, nullptr);
@@ -2276,7 +2476,9 @@ void run_footprint(pasif_driver* d, bool adjoint, uint32_t save_steps, size_t* t
 size_t held_run_bytes(const pasif_driver* d) {
   size_t b = 0;
   const DevBuf* own[] = {&d->qstate, &d->lstate, &d->mxstate, &d->progress, &d->out_q,
-                         &d->out_max, &d->traj, &d->out_grad, &d->ckpt, &d->segbuf};
+                         &d->out_max, &d->traj,   &d->out_grad, &d->ckpt,  &d->segbuf,
+                         &d->bvec_f, &d->bvec_a,  &d->bsetp,    &d->bckpt, &d->bseg,
+                         &d->bprog};
   for (const DevBuf* x : own) b += x->bytes;
   for (const auto& lp : d->pk.lparts)
     for (const auto& c : lp->cls)
@@ -2309,13 +2511,23 @@ template <typename R>
 void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<double>* traj_out,
                 std::vector<double>* grad_out) {
   ensure_packed(d, adjoint);
-  check_run_memory<R>(d, adjoint, save_steps);
   Packed& pk = d->pk;
-  const bool lane = pk.lane, tile = pk.tile;
-  uint64_t twork = 0;
+  const bool lane = pk.lane, tile = pk.tile, big = pk.big;
+  PASIF_REQUIRE(!big || (save_steps == 0 && d->window == 0), PASIF_ERR_UNSUPPORTED,
+                "systems on the row-parallel path (too large for a tile) record neither a "
+                "trajectory nor an amplitude window");
+  check_run_memory<R>(d, adjoint, save_steps);
+  uint64_t twork = 0, bwork = 0;
   uint32_t nslots = 0;
   TileArgs<R> ta{};
   if (tile) ta = prepare_run<R>(d, adjoint, save_steps, &twork, &nslots);
+  BigArgs<R> ba{};
+  if (big) {
+    (void)run_outputs<R>(d, adjoint, save_steps);
+    RunMem mem;
+    mem.d = d;
+    ba = big_run_args<R>(d, adjoint, mem, d->out_q.p, adjoint ? d->out_grad.p : nullptr, &bwork);
+  }
   std::vector<std::vector<LaneArgs>> largs;
   if (lane) {
     nslots = run_outputs<R>(d, adjoint, save_steps);
@@ -2325,13 +2537,15 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
   }
   const uint64_t lwork = lane ? (adjoint ? 3ull * pk.ltmax : pk.ltmax) : 0;
   uint64_t launches = 1;
-  const uint32_t budget = launch_budget(d, std::max(twork, lwork), &launches);
+  const uint32_t budget = launch_budget(d, std::max(std::max(twork, lwork), bwork), &launches);
   ta.budget = budget;
+  ba.budget = budget;
   for (size_t i = 0; i < largs.size(); ++i) {
     for (LaneArgs& a : largs[i]) a.budget = budget;
     upload(d, pk.lparts[i]->args, largs[i]);
   }
   if (lane && tile) ensure_aux(d);
+  const uint32_t nb = pk.nbig;
   const uint32_t nt = pk.ntiles;
   drive(d, launches, [&]() -> hipError_t {
     hipError_t e = hipSuccess;
@@ -2344,6 +2558,7 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
       if (pk.lparts[i]->blocks)
         e = lane_launch(*pk.lparts[i], pk.lparts[i]->args.p, uint32_t(largs[i].size()), ls);
     if (tile && e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
+    if (big && e == hipSuccess) e = launch_big(ba, nb, d->stream);
     if (lane && tile) {
       if (e == hipSuccess) e = hipEventRecord(d->join, d->aux);
       if (e == hipSuccess) e = hipStreamWaitEvent(d->stream, d->join, 0);
@@ -2376,6 +2591,8 @@ void solve_noalloc_impl(pasif_driver* d, bool adjoint, void* out_q, void* out_gr
   PASIF_REQUIRE(d->solver != PASIF_SOLVER_GRAPH, PASIF_ERR_UNSUPPORTED,
                 "pasif_solve_noalloc runs the persistent or the plain solver, not the graph one");
   PASIF_REQUIRE(!d->pool, PASIF_ERR_UNSUPPORTED, "pool jobs run through pasif_pool_tick");
+  PASIF_REQUIRE(!pk.big, PASIF_ERR_UNSUPPORTED,
+                "pasif_solve_noalloc runs the tile and lane kernels, not the row-parallel path");
   PASIF_REQUIRE(out_q != nullptr, PASIF_ERR_ARGUMENT, "out_q (device, %u values) is required",
                 pk.out_len_f);
   PASIF_REQUIRE(!adjoint || out_grad != nullptr, PASIF_ERR_ARGUMENT,
@@ -2897,7 +3114,7 @@ int pasif_get_max_amplitudes(pasif_driver* d, double* out, size_t cap, size_t* l
 int pasif_set_kernel(pasif_driver* d, int kernel) {
   return guarded(d, [&] {
     PASIF_REQUIRE(kernel == PASIF_KERNEL_AUTO || kernel == PASIF_KERNEL_TILE ||
-                      kernel == PASIF_KERNEL_LANE,
+                      kernel == PASIF_KERNEL_LANE || kernel == PASIF_KERNEL_BIG,
                   PASIF_ERR_ARGUMENT, "unknown kernel %d", kernel);
     d->kernel_path = kernel;
   });
@@ -2907,14 +3124,20 @@ int pasif_kernel_info(pasif_driver* d, int adjoint, int* kernel, uint32_t* class
                       uint32_t* checkpoint_every) {
   return guarded(d, [&] {
     ensure_packed(d, adjoint != 0);
-    if (kernel)
-      *kernel = d->pk.lane ? (d->pk.tile ? PASIF_KERNEL_MIXED : PASIF_KERNEL_LANE)
+    const Packed& pk = d->pk;
+    if (kernel) {
+      const int parts = int(pk.lane) + int(pk.tile) + int(pk.big);
+      *kernel = parts > 1  ? PASIF_KERNEL_MIXED
+                : pk.lane  ? PASIF_KERNEL_LANE
+                : pk.big   ? PASIF_KERNEL_BIG
                            : PASIF_KERNEL_TILE;
-    if (classes) *classes = d->pk.lane ? uint32_t(d->pk.lshapes.size()) : 0u;
+    }
+    if (classes) *classes = pk.lane ? uint32_t(pk.lshapes.size()) : 0u;
     if (checkpoint_every)
       *checkpoint_every = !adjoint ? 0u
-                          : d->pk.lane ? d->pk.lseg
-                                       : (d->ck_every ? d->ck_every : 256u);
+                          : pk.lane ? pk.lseg
+                          : (pk.big && !pk.tile) ? pk.b_every
+                                                 : (d->ck_every ? d->ck_every : 256u);
   });
 }
 

@@ -690,6 +690,296 @@ hipError_t launch_tiles(const TileArgs<float>& a, uint32_t ntiles, int slot_cfg,
   return launch_tiles_impl(a, ntiles, slot_cfg, s);
 }
 
+// ---------------------------------------------------------------- big path
+// Systems too large for a workgroup's lanes: one workgroup per instance, rows
+// strided over its threads, every vector of the instance in global memory
+// and a barrier between stages (written by some threads, read by all).
+// The arithmetic is the tile kernel's: kinds summed in order, each from zero
+// in entry order, then the force; the same RK4 update, adjoint recursions
+// and checkpoint schedule.
+
+// u of a forced row at (step n, stage), read from the excitation: zero-order
+// hold e[n] (stages 1-3) / e[n+1] (stage 4) when 0 < s < L (src/kernels.cu:
+// 83-86), or the interpolation on the half-step grid (src/kernels.cu:114-124)
+// with e[L] read as 0.
+template <typename R>
+__device__ __forceinline__ R big_force(const BigArgs<R>& a, int32_t exc, uint32_t n, int stage) {
+  if (exc < 0) return R(0);
+  const R* E = a.excit + static_cast<size_t>(exc) * a.exc_len;
+  const uint32_t Lx = a.exc_len;
+  const uint32_t n1 = n + (stage == 3 ? 1u : 0u);
+  if (a.interp_w == 0u) return (n1 > 0u && n1 < Lx) ? E[n1] : R(0);
+  const uint32_t W = a.interp_w;
+  const uint32_t hs = 2u * n1 + ((stage == 1 || stage == 2) ? 1u : 0u);
+  const uint32_t i = hs / W, j = hs - i * W;
+  if (!(i > 0u && i < Lx)) return R(0);
+  const R ei = E[i];
+  const R ei1 = (i + 1u < Lx) ? E[i + 1u] : R(0);
+  return fmaR(a.interp[W + j], ei, a.interp[j] * ei1);
+}
+
+// Right-hand side of row r: S factors from `S`, set-point factors from `X`.
+template <typename R>
+__device__ __forceinline__ R big_row(const uint32_t* rp, const R* val, const uint32_t* idx,
+                                     uint32_t r, const R* S, const R* X) {
+  R part[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    R t = R(0);
+    const uint32_t b = rp[4u * r + k], e = rp[4u * r + k + 1u];
+    for (uint32_t q = b; q < e; ++q) {
+      const uint32_t* ix = idx + 4u * q;
+      if (k == 0) {
+        t = fmaR(val[q], S[ix[0]], t);
+      } else {
+        R v = val[q] * S[ix[0]];
+        for (int f = 1; f < k; ++f) v = v * X[ix[f]];
+        t = fmaR(v, X[ix[k]], t);
+      }
+    }
+    part[k] = t;
+  }
+  return ((part[0] + part[1]) + part[2]) + part[3];
+}
+
+template <typename R, bool ADJ>
+__global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
+  const BigInst I = a.inst[blockIdx.x];
+  const uint32_t tid = threadIdx.x, NT = blockDim.x;
+  const uint32_t nf = I.n_f, na = I.n_a, T = I.steps;
+  const bool adjoint = ADJ && a.adjoint != 0u;
+  const bool exact = adjoint && a.exact != 0u;
+  const uint32_t K = adjoint ? a.ck_every : 1u;
+  const uint32_t* rpf = a.fwd.rptr + I.rptr_f;
+  const R* vf = a.fwd.val + I.term_f;
+  const uint32_t* xf = a.fwd.idx + 4u * I.term_f;
+  const R* ff = a.fwd.force + I.row_f;
+  R* q = a.vec_f + I.scr_f;
+  R* y0 = q + nf;
+  R* y1 = y0 + nf;
+  R* acc = y1 + nf;
+  uint32_t* pst = a.progress + 4u * blockIdx.x;
+  uint32_t n = pst[0], phase = pst[1], seg = pst[2];
+  uint32_t budget = a.budget;
+  const R h = a.h, h2 = a.h2, h6 = a.h6;
+
+  if (n == 0u && phase == kPhaseForward) {
+    for (uint32_t r = tid; r < nf; r += NT) {
+      q[r] = a.fwd.init[I.row_f + r];
+      if (T == 0u) a.out_q[I.out_f + r] = q[r];
+    }
+  }
+  __syncthreads();
+  while (budget > 0u && phase != kPhaseDone) {
+    if (phase != kPhaseBackward) {
+      // ---------------- forward / recompute ----------------
+      const bool recompute = phase == kPhaseRecompute;
+      const uint32_t s0 = seg * K;
+      if (recompute && n == s0) {
+        for (uint32_t r = tid; r < nf; r += NT) {
+          q[r] = a.ckpt[I.ck + static_cast<size_t>(seg) * nf + r];
+          if (!exact) a.segbuf[I.seg + r] = q[r];
+        }
+        __syncthreads();
+      }
+      const uint32_t e = recompute ? min(T, s0 + K) : T;
+      const uint32_t nend = (e - n < budget) ? e : n + budget;
+      budget -= nend - n;
+      for (; n < nend; ++n) {
+        if (adjoint && !recompute && (n % K) == 0u)
+          for (uint32_t r = tid; r < nf; r += NT)
+            a.ckpt[I.ck + static_cast<size_t>(n / K) * nf + r] = q[r];
+        R* rec = (exact && recompute)
+                     ? a.segbuf + I.seg + static_cast<size_t>(4u * (n - s0)) * nf : nullptr;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const R* Y = s == 0 ? q : (s == 2 ? y1 : y0);
+          R* Yn = s == 1 ? y1 : y0;
+          R u = R(0);
+          if (I.exc >= 0) u = big_force(a, I.exc, n, s);
+          for (uint32_t r = tid; r < nf; r += NT) {
+            const R p = big_row(rpf, vf, xf, r, Y, Y);
+            const R k = fmaR(ff[r], u, p);
+            const R qr = q[r];
+            if (s == 0) {
+              acc[r] = k;
+              const R y = fmaR(h2, k, qr);
+              Yn[r] = y;
+              if (rec) {
+                rec[r] = qr;
+                rec[nf + r] = y;
+              }
+            } else if (s == 1) {
+              acc[r] = fmaR(R(2), k, acc[r]);
+              const R y = fmaR(h2, k, qr);
+              Yn[r] = y;
+              if (rec) rec[2u * nf + r] = y;
+            } else if (s == 2) {
+              acc[r] = fmaR(R(2), k, acc[r]);
+              const R y = fmaR(h, k, qr);
+              Yn[r] = y;
+              if (rec) rec[3u * nf + r] = y;
+            } else {
+              acc[r] = acc[r] + k;
+            }
+          }
+          __syncthreads();
+        }
+        for (uint32_t r = tid; r < nf; r += NT) {
+          const R qn = fmaR(h6, acc[r], q[r]);
+          q[r] = qn;
+          if (recompute) {
+            if (!exact) a.segbuf[I.seg + static_cast<size_t>(n + 1u - s0) * nf + r] = qn;
+          } else if (n + 1u == T) {
+            a.out_q[I.out_f + r] = qn;
+          }
+        }
+        __syncthreads();
+      }
+      if (n == e) {
+        if (recompute) {
+          phase = kPhaseBackward;
+        } else if (adjoint && T > 0u) {
+          seg = (T - 1u) / K;
+          n = seg * K;
+          phase = kPhaseRecompute;
+          for (uint32_t r = tid; r < na; r += NT) a.vec_a[I.scr_a + r] = a.adj.init[I.row_a + r];
+        } else {
+          phase = kPhaseDone;
+          if (adjoint)
+            for (uint32_t r = tid; r < na; r += NT)
+              a.out_grad[I.out_a + r] = a.adj.init[I.row_a + r];
+        }
+      }
+      __syncthreads();
+    } else if (ADJ) {
+      // ---------------- adjoint (reverse) ----------------
+      const uint32_t* rpa = a.adj.rptr + I.rptr_a;
+      const R* va = a.adj.val + I.term_a;
+      const uint32_t* xa = a.adj.idx + 4u * I.term_a;
+      R* lam = a.vec_a + I.scr_a;
+      R* z0 = lam + na;
+      R* z1 = z0 + na;
+      R* za = z1 + na;
+      R* X = a.setp + I.scr_f;
+      const uint32_t s0 = seg * K;
+      const uint32_t nend = (n - s0 < budget) ? s0 : n - budget;
+      budget -= n - nend;
+      for (; n > nend; --n) {
+        for (uint32_t r = tid; r < nf; r += NT) {
+          if (exact) {
+            const R* y = a.segbuf + I.seg + static_cast<size_t>(4u * (n - 1u - s0)) * nf;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) X[v * nf + r] = y[v * nf + r];
+          } else {
+            const R xn = a.segbuf[I.seg + static_cast<size_t>(n - s0) * nf + r];
+            const R xp = a.segbuf[I.seg + static_cast<size_t>(n - 1u - s0) * nf + r];
+            X[r] = xn;
+            X[nf + r] = R(0.5) * (xn + xp);
+            X[2u * nf + r] = xp;
+          }
+        }
+        if (exact)
+          for (uint32_t r = tid; r < na; r += NT) z0[r] = h6 * lam[r];
+        __syncthreads();
+        const bool active = n <= T;
+        if (!exact) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const R* Y = s == 0 ? lam : (s == 2 ? z1 : z0);
+            R* Yn = s == 1 ? z1 : z0;
+            const R* Xs = X + (s == 0 ? 0u : (s == 3 ? 2u : 1u)) * nf;
+            for (uint32_t r = tid; r < na; r += NT) {
+              const R p = big_row(rpa, va, xa, r, Y, Xs);
+              const R l = lam[r];
+              if (s == 0) {
+                za[r] = p;
+                Yn[r] = fmaR(h2, p, l);
+              } else if (s == 1) {
+                za[r] = fmaR(R(2), p, za[r]);
+                Yn[r] = fmaR(h2, p, l);
+              } else if (s == 2) {
+                za[r] = fmaR(R(2), p, za[r]);
+                Yn[r] = fmaR(h, p, l);
+              } else {
+                za[r] = za[r] + p;
+              }
+            }
+            __syncthreads();
+          }
+          for (uint32_t r = tid; r < na; r += NT) {
+            const R ln = fmaR(h6, za[r], lam[r]);
+            if (active) lam[r] = ln;
+          }
+        } else {
+          const R h3 = R(2) * h6;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const R* Y = (s & 1) ? z1 : z0;
+            R* Yn = (s & 1) ? z0 : z1;
+            const R* Xs = X + static_cast<uint32_t>(3 - s) * nf;
+            for (uint32_t r = tid; r < na; r += NT) {
+              const R p = big_row(rpa, va, xa, r, Y, Xs);
+              const R l = lam[r];
+              if (s == 0) {
+                za[r] = p;
+                Yn[r] = fmaR(h, p, h3 * l);
+              } else if (s == 1) {
+                za[r] = za[r] + p;
+                Yn[r] = fmaR(h2, p, h3 * l);
+              } else if (s == 2) {
+                za[r] = za[r] + p;
+                Yn[r] = fmaR(h2, p, h6 * l);
+              } else {
+                za[r] = za[r] + p;
+              }
+            }
+            __syncthreads();
+          }
+          for (uint32_t r = tid; r < na; r += NT) {
+            const R ln = lam[r] + za[r];
+            if (active) lam[r] = ln;
+          }
+        }
+        __syncthreads();
+      }
+      if (n == s0) {
+        if (seg == 0u) {
+          phase = kPhaseDone;
+          for (uint32_t r = tid; r < na; r += NT) a.out_grad[I.out_a + r] = lam[r];
+        } else {
+          seg -= 1u;
+          n = seg * K;
+          phase = kPhaseRecompute;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0u) {
+    pst[0] = n;
+    pst[1] = phase;
+    pst[2] = seg;
+  }
+}
+
+template <typename R>
+static hipError_t launch_big_impl(const BigArgs<R>& a, uint32_t ninst, hipStream_t s) {
+  if (ninst == 0u) return hipSuccess;
+  if (a.adjoint)
+    hipLaunchKernelGGL((big_kernel<R, true>), dim3(ninst), dim3(kBigThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((big_kernel<R, false>), dim3(ninst), dim3(kBigThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_big(const BigArgs<double>& a, uint32_t ninst, hipStream_t s) {
+  return launch_big_impl(a, ninst, s);
+}
+hipError_t launch_big(const BigArgs<float>& a, uint32_t ninst, hipStream_t s) {
+  return launch_big_impl(a, ninst, s);
+}
+
 }  // namespace pasif
 
 // Run scratch of pasif_solve_noalloc: one copy per GPU, held until the

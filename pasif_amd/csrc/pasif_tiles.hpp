@@ -124,6 +124,54 @@ struct TileArgs {
   uint32_t adjoint;        // run the adjoint program after the forward pass
 };
 
+// ---------------------------------------------------------------------------
+// Row-parallel path for systems larger than a workgroup's lanes: one
+// workgroup per instance, rows strided over its threads, every vector of the
+// instance in global memory (L2-resident), a barrier between stages.
+//
+// Terms of a side, per instance, row-major and kind-ordered: the terms of
+// row r and kind k are [rptr[4r+k], rptr[4r+k+1]) of val / idx, with idx[4e]
+// the S factor row and idx[4e+1..k] the set-point factor rows (local rows).
+template <typename R>
+struct BigSide {
+  const uint32_t* rptr;   // [rows * 4 + 1]
+  const R* val;           // [terms]
+  const uint32_t* idx;    // [terms][4]
+  const R* force;         // [rows] (forward side; 0 where unforced)
+  const R* init;          // [rows] initial state / terminal adjoint state
+};
+
+struct BigInst {
+  uint64_t rptr_f, term_f, rptr_a, term_a;  // offsets into the packed sides
+  uint64_t row_f, row_a;                     // offsets of force / init rows
+  uint64_t scr_f, scr_a, ck, seg;            // offsets into the scratch arrays
+  uint32_t n_f, n_a, steps, out_f, out_a;
+  int32_t exc;
+};
+
+template <typename R>
+struct BigArgs {
+  const BigInst* inst;
+  BigSide<R> fwd, adj;      // packed sides (base pointers; per-instance offsets)
+  const R* excit;           // [n_exc][exc_len]
+  uint32_t exc_len;
+  const R* interp;          // [2][interp_w] or null (zero-order hold)
+  uint32_t interp_w;
+  R h, h2, h6;
+  R* vec_f;                 // per instance [4][n_f]: q, y0, y1, acc
+  R* vec_a;                 // per instance [4][n_a]: lam, y0, y1, acc
+  R* setp;                  // per instance [4][n_f]: set points of the reverse step
+  R* ckpt;                  // per instance [ceil(steps/K)][n_f]
+  R* segbuf;                // per instance [K+1][n_f] or (exact) [K][4][n_f]
+  uint32_t* progress;       // [inst][4]: step, phase, segment
+  R* out_q;
+  R* out_grad;
+  uint32_t budget;
+  uint32_t ck_every;
+  uint32_t adjoint;
+  uint32_t exact;
+};
+
 }  // namespace pasif
 
 namespace pasif {
@@ -136,6 +184,10 @@ hipError_t launch_pool(const TileArgs<double>* jobs, const uint32_t* map, uint32
 hipError_t launch_pool(const TileArgs<float>* jobs, const uint32_t* map, uint32_t nblocks,
                        uint32_t lanes, int slot_cfg, size_t lds, hipStream_t s);
 size_t tile_lds_bytes(size_t elem, uint32_t lanes, bool adjoint, uint32_t W, uint32_t nforced);
+// row-parallel path: one workgroup of kBigThreads per instance
+constexpr uint32_t kBigThreads = 1024;
+hipError_t launch_big(const BigArgs<double>& a, uint32_t ninst, hipStream_t s);
+hipError_t launch_big(const BigArgs<float>& a, uint32_t ninst, hipStream_t s);
 // Size of the static device scratch array that holds the run buffers of
 // pasif_solve_noalloc, and the address of `dev`'s copy of it (looked up once
 // per device).
