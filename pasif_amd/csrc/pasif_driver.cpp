@@ -185,7 +185,7 @@ struct LaneCls {
   // program arrays: views into the part's one program buffer
   const void *coef_f = nullptr, *coef_a = nullptr, *init_f = nullptr, *init_a = nullptr,
              *steps = nullptr, *exc = nullptr, *out_f = nullptr, *out_a = nullptr,
-             *ckoff = nullptr;
+             *na_out = nullptr, *ckoff = nullptr;
   DevBuf qstate, lstate, mxstate, progress, ckpt, segbuf;  // run
 };
 
@@ -213,6 +213,8 @@ struct Packed {
   std::vector<std::unique_ptr<LanePart>> lparts;
   // the driver's own lane plan (diagnostics): its classes, layout, checkpoint
   std::vector<LaneShape> lshapes;
+  std::vector<uint32_t> lgroups;  // lanes per instance of each class in lshapes
+  std::vector<std::pair<std::vector<LaneShape>, LaneLayout>> lmods;  // per module
   uint32_t lseg = 1;
   LaneLayout llay;
   uint32_t lanes = 0, ntiles = 0, slot_cfg = 0, tile_target = 0;
@@ -244,6 +246,8 @@ struct Packed {
     for (DevBuf* b : all) b->release();
     lparts.clear();
     lshapes.clear();
+    lgroups.clear();
+    lmods.clear();
     lane = false;
     tile = false;
     big = false;
@@ -296,9 +300,14 @@ struct pasif_driver {
   std::vector<int32_t> lane_cls;
   // systems on the row-parallel path (too large for a tile), by system
   std::vector<uint8_t> big_sys;
+  // structure of every system in the lane plan before classes were merged
+  std::vector<uint32_t> lane_struct;
   // second stream and fork / join events when lane and tile kernels run together
-  hipStream_t aux = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  // side streams (lane modules beside each other and beside the tiles), the
+  // fork event and one join event per side stream
+  static constexpr int kSide = 3;
+  hipStream_t aux[kSide] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[kSide] = {nullptr, nullptr, nullptr};
   uint32_t graph_slice = 256, graph_launches = 16;
 
   Packed pk;
@@ -1231,7 +1240,14 @@ uint32_t lane_max_live() {
   }
   return kLaneMaxLive;
 }
-constexpr uint32_t kLaneMaxClasses = 8;
+// structure classes the lane kernel takes (the rest run on tiles), and per
+// module (one generated kernel)
+constexpr uint32_t kLaneMaxClasses = 64;
+constexpr uint32_t kLaneModuleClasses = 8;
+// Multi-wave classes compile one kernel body per wave (about 5 s of hipRTC
+// time each): the lane kernel takes them, heaviest first, up to this many
+// bodies in all; the rest run on tiles.
+constexpr uint32_t kLaneMultiWaveBodies = 16;
 // multi-wave: waves per workgroup and values one wave's share may keep live
 constexpr uint32_t kLaneMaxWaves = 16;
 constexpr uint32_t kLaneMaxWaveLive = 100;
@@ -1305,17 +1321,54 @@ void shape_key(const LaneSide& sd, std::vector<uint32_t>& key) {
   for (uint32_t r : sd.forced) key.push_back(r);
 }
 
+bool same_term(const LaneTerm& x, const LaneTerm& y) {
+  if (x.kind != y.kind) return false;
+  for (uint32_t a = 0; a <= x.kind; ++a)
+    if (x.f[a] != y.f[a]) return false;
+  return true;
+}
+
+// Position of merged term t in a class row (the class may be the union of
+// several structures: a system fills its own terms, the others stay zero).
+uint32_t term_slot(const std::vector<LaneTerm>& row, const LaneTerm& t) {
+  for (uint32_t m = 0; m < row.size(); ++m)
+    if (same_term(row[m], t)) return m;
+  throw Fail{PASIF_ERR_UNSUPPORTED};
+}
+
+// The union of two structures of one size: every row's terms of `a` in
+// order, then those of `b` that `a` lacks; the rows forced in either.
+LaneSide side_union(const LaneSide& a, const LaneSide& b) {
+  LaneSide u = a;
+  u.n = std::max(a.n, b.n);
+  u.rows.resize(u.n);
+  for (uint32_t r = 0; r < b.n; ++r)
+    for (const LaneTerm& t : b.rows[r]) {
+      bool have = false;
+      for (const LaneTerm& x : u.rows[r]) have = have || same_term(x, t);
+      if (!have) u.rows[r].push_back(t);
+    }
+  std::vector<uint32_t> f;
+  std::set_union(a.forced.begin(), a.forced.end(), b.forced.begin(), b.forced.end(),
+                 std::back_inserter(f));
+  u.forced = f;
+  return u;
+}
+
 // Coefficients of system s in the class's slot order (merged terms row by
-// row, kinds ascending, each the sum of its entries in entry order; then the
-// force of each forced row).
-void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint,
+// row, each the sum of its entries in entry order, at its place in the
+// class row, zero for class terms the system lacks; then the force of each
+// forced row of the class, zero where this system's is below eps).
+void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint, double eps,
                 std::vector<double>& out) {
   out.clear();
   const uint32_t off = rt.off[s];
+  const uint32_t n = rt.size[s];
+  if (n > sd.n) throw Fail{PASIF_ERR_UNSUPPORTED};  // not a system of this class
   std::vector<LaneTerm> mt;
   std::vector<uint32_t> which;
   std::vector<double> v;
-  for (uint32_t r = 0; r < sd.n; ++r) {
+  for (uint32_t r = 0; r < n; ++r) {
     merged_row(rt, s, r, adjoint, mt, which);
     v.assign(mt.size(), 0.0);
     // merged terms appear in the order of their first entries
@@ -1331,9 +1384,16 @@ void lane_coefs(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint
           v[m] += rt.val[k][e];
         }
       }
-    out.insert(out.end(), v.begin(), v.end());
+    const size_t base = out.size();
+    out.resize(base + sd.rows[r].size(), 0.0);
+    for (uint32_t m = 0; m < mt.size(); ++m) out[base + term_slot(sd.rows[r], mt[m])] = v[m];
   }
-  for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
+  // rows a padded adjoint class has beyond this system's
+  for (uint32_t r = n; r < sd.n; ++r) out.resize(out.size() + sd.rows[r].size(), 0.0);
+  for (uint32_t r : sd.forced) {
+    const double f = rt.force[off + r];
+    out.push_back(std::fabs(f) > eps ? f : 0.0);
+  }
 }
 
 // lane_coefs through a merge map cached from one system of a class: the raw
@@ -1351,7 +1411,7 @@ struct CoefMap {
 };
 
 void lane_coefs_mapped(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool adjoint,
-                       CoefMap& cm, std::vector<double>& out) {
+                       double eps, CoefMap& cm, std::vector<double>& out) {
   const uint32_t off = rt.off[s];
   const uint32_t xo = adjoint ? rt.xoff[s] : off;
   const uint32_t n = rt.size[s];
@@ -1363,9 +1423,13 @@ void lane_coefs_mapped(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool 
     cm.rel.clear();
     cm.slot.clear();
     cm.first.clear();
+    if (n > sd.n) throw Fail{PASIF_ERR_UNSUPPORTED};  // not a system of this class
     uint32_t base = 0;
+    std::vector<uint32_t> pos;
     for (uint32_t r = 0; r < n; ++r) {
       merged_row(rt, s, r, adjoint, mt, which);
+      pos.resize(mt.size());
+      for (uint32_t m = 0; m < mt.size(); ++m) pos[m] = term_slot(sd.rows[r], mt[m]);
       const uint32_t g = off + r;
       uint32_t q = 0, fresh = 0;
       for (int k = 0; k < 4; ++k) {
@@ -1374,13 +1438,14 @@ void lane_coefs_mapped(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool 
         for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e, ++q) {
           cm.rel.push_back(rt.idx[k][size_t(e) * nf] - off);
           for (int a = 1; a < nf; ++a) cm.rel.push_back(rt.idx[k][size_t(e) * nf + a] - xo);
-          cm.slot.push_back(base + which[q]);
+          cm.slot.push_back(base + pos[which[q]]);
           cm.first.push_back(which[q] == fresh ? 1 : 0);
           if (which[q] == fresh) ++fresh;
         }
       }
-      base += uint32_t(mt.size());
+      base += uint32_t(sd.rows[r].size());
     }
+    for (uint32_t r = n; r < sd.n; ++r) base += uint32_t(sd.rows[r].size());
     cm.nslots = base;
     cm.valid = true;
   } else {
@@ -1403,7 +1468,7 @@ void lane_coefs_mapped(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool 
       }
     }
     if (!same) {
-      lane_coefs(rt, s, sd, adjoint, out);
+      lane_coefs(rt, s, sd, adjoint, eps, out);
       return;
     }
   }
@@ -1419,7 +1484,10 @@ void lane_coefs_mapped(const RowTerms& rt, uint32_t s, const LaneSide& sd, bool 
           out[cm.slot[q]] += rt.val[k][e];
       }
   }
-  for (uint32_t r : sd.forced) out.push_back(rt.force[off + r]);
+  for (uint32_t r : sd.forced) {
+    const double f = rt.force[off + r];
+    out.push_back(std::fabs(f) > eps ? f : 0.0);
+  }
 }
 
 // Lanes per instance (powers of two up to a wave) at which a class fits the
@@ -1437,6 +1505,46 @@ std::vector<std::pair<uint32_t, double>> group_options(const LaneShape& sh, int 
 
 struct ClassLoad {
   double inst = 0, steps_sum = 0, steps_max = 0;
+};
+
+// Per-step cost of a class (coefficient slots and state values of both
+// sides), the unit of the merge decisions below.
+double class_cost(const LaneShape& sh) {
+  double c = sh.f.coefs() + 2.0 * sh.f.n;
+  if (sh.adjoint) c += sh.a.coefs() + 2.0 * sh.a.n + sh.f.n;
+  return c;
+}
+
+// Coefficient slots of the union of two sides of one size.
+uint32_t union_coefs(const LaneSide& a, const LaneSide& b) {
+  size_t t = 0;
+  for (uint32_t r = a.n; r < b.n; ++r) t += b.rows[r].size();
+  for (uint32_t r = 0; r < a.n; ++r) {
+    t += a.rows[r].size();
+    if (r >= b.n) continue;
+    for (const LaneTerm& x : b.rows[r]) {
+      bool have = false;
+      for (const LaneTerm& y : a.rows[r]) have = have || same_term(x, y);
+      t += have ? 0 : 1;
+    }
+  }
+  size_t i = 0, j = 0, f = 0;
+  while (i < a.forced.size() || j < b.forced.size()) {
+    if (j == b.forced.size() || (i < a.forced.size() && a.forced[i] < b.forced[j])) ++i;
+    else if (i == a.forced.size() || b.forced[j] < a.forced[i]) ++j;
+    else ++i, ++j;
+    ++f;
+  }
+  return uint32_t(t + f);
+}
+
+// one generated lane kernel of a plan: its classes (indices into the plan's
+// classes), their shapes and loads, and its layout
+struct LaneModule {
+  std::vector<uint32_t> cls;
+  std::vector<LaneShape> shapes;
+  std::vector<ClassLoad> load;
+  LaneLayout lay;
 };
 
 // Lanes per instance of every class of a single-wave lane kernel.  A class's
@@ -1640,125 +1748,325 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     loads[c].steps_sum += in.steps;
     loads[c].steps_max = std::max(loads[c].steps_max, double(in.steps));
   }
+  // Structures of one size merge into union classes (a system's coefficient
+  // is zero where it lacks a term of the union) while more classes of one
+  // layout (one wave per instance group, or a workgroup of waves per
+  // instance) remain than one module takes: cheapest merge first, as long as
+  // a merge adds at most half the work of the two classes it joins and the
+  // union still fits that layout.  Among the 256 heaviest classes.
+  d->lane_struct = cls_of;
+  std::vector<uint32_t> into(all.size());  // class a class merged into (itself if kept)
+  std::iota(into.begin(), into.end(), 0u);
+  {
+    const size_t C = all.size();
+    std::vector<uint32_t> cand;
+    for (uint32_t c = 0; c < C; ++c)
+      if (need[c] >= 1) cand.push_back(c);
+    std::stable_sort(cand.begin(), cand.end(),
+                     [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
+    if (cand.size() > 256) cand.resize(256);
+    // classes of each layout (0: one wave, 1: several) among the candidates
+    size_t live[2] = {0, 0};
+    for (uint32_t c : cand) live[need[c] == 1 ? 0 : 1]++;
+    std::vector<double> cost(C), steps(C);
+    for (uint32_t c = 0; c < C; ++c) {
+      cost[c] = class_cost(all[c]);
+      steps[c] = loads[c].steps_sum;
+    }
+    const size_t M = cand.size();
+    // added work of merging candidates x and y (infinite: not mergeable)
+    const double kNo = std::numeric_limits<double>::infinity();
+    std::vector<double> gain(M * M, kNo);
+    std::vector<char> alive(M, 1);
+    auto eval = [&](size_t x, size_t y) {
+      const LaneShape& A = all[cand[x]];
+      const LaneShape& B = all[cand[y]];
+      // the forward sides must match in size; an adjoint side pads to the larger
+      if (A.f.n != B.f.n || A.adjoint != B.adjoint) return kNo;
+      const int la = need[cand[x]] == 1 ? 0 : 1, lb = need[cand[y]] == 1 ? 0 : 1;
+      if (la != lb || live[la] <= kLaneModuleClasses) return kNo;
+      double cu = union_coefs(A.f, B.f) + 2.0 * A.f.n;
+      if (A.adjoint) cu += union_coefs(A.a, B.a) + 2.0 * A.a.n + A.f.n;
+      const double wa = steps[cand[x]] * cost[cand[x]], wb = steps[cand[y]] * cost[cand[y]];
+      const double added = steps[cand[x]] * (cu - cost[cand[x]]) + steps[cand[y]] * (cu - cost[cand[y]]);
+      return added <= 0.5 * (wa + wb) ? added / std::max(wa + wb, 1e-300) : kNo;
+    };
+    for (size_t x = 0; x < M; ++x)
+      for (size_t y = x + 1; y < M; ++y) gain[x * M + y] = eval(x, y);
+    while (live[0] > kLaneModuleClasses || live[1] > kLaneModuleClasses) {
+      size_t bx = M, by = M;
+      double best = kNo;
+      for (size_t x = 0; x < M; ++x)
+        for (size_t y = x + 1; y < M; ++y)
+          if (alive[x] && alive[y] && gain[x * M + y] < best) {
+            best = gain[x * M + y];
+            bx = x;
+            by = y;
+          }
+      if (bx == M) break;
+      const uint32_t cx = cand[bx], cy = cand[by];
+      LaneShape U = all[cx];
+      U.f = side_union(all[cx].f, all[cy].f);
+      if (U.adjoint) U.a = side_union(all[cx].a, all[cy].a);
+      const int lay = need[cx] == 1 ? 0 : 1;
+      uint32_t wu = 0;  // waves the union needs
+      if (lay == 0) {
+        wu = (d->lane_group == 1 ? U.live_values() <= lane_max_live()
+                                 : !group_options(U, bits).empty())
+                 ? 1u
+                 : 0u;
+      } else {
+        for (uint32_t w = 2; w <= kLaneMaxWaves && !wu; ++w) {
+          LaneShape t = U;
+          if (lane_partition(t, w) <= kLaneMaxWaveLive) wu = w;
+        }
+      }
+      const bool fits = wu != 0;
+      if (std::getenv("PASIF_LANE_DEBUG"))
+        fprintf(stderr, "[pasif] merge classes %u (n %u/%u) + %u (n %u/%u): +%.1f %% work%s\n", cx,
+                all[cx].f.n, all[cx].a.n, cy, all[cy].f.n, all[cy].a.n, 100.0 * best,
+                fits ? "" : ", union does not fit a lane: kept apart");
+      if (!fits) {
+        gain[bx * M + by] = kNo;
+        continue;
+      }
+      all[cx] = U;
+      need[cx] = wu;
+      cost[cx] = class_cost(U);
+      steps[cx] += steps[cy];
+      loads[cx].inst += loads[cy].inst;
+      loads[cx].steps_sum += loads[cy].steps_sum;
+      loads[cx].steps_max = std::max(loads[cx].steps_max, loads[cy].steps_max);
+      work[cx] += work[cy];
+      into[cy] = cx;
+      alive[by] = 0;
+      const size_t before = live[lay]--;
+      if (before - 1 == kLaneModuleClasses) {
+        // this layout is down to one module: no further merges in it
+        for (size_t x = 0; x < M; ++x)
+          for (size_t y = x + 1; y < M; ++y)
+            if (alive[x] && alive[y] && (need[cand[x]] == 1 ? 0 : 1) == lay) gain[x * M + y] = kNo;
+        continue;
+      }
+      for (size_t z = 0; z < M; ++z) {
+        if (!alive[z] || z == bx) continue;
+        const double g = eval(std::min(bx, z), std::max(bx, z));
+        gain[std::min(bx, z) * M + std::max(bx, z)] = g;
+      }
+    }
+    for (uint32_t s2 = 0; s2 < S; ++s2) {
+      uint32_t c = cls_of[s2];
+      while (into[c] != c) c = into[c];
+      cls_of[s2] = c;
+    }
+    if (std::getenv("PASIF_LANE_DEBUG"))
+      for (uint32_t c = 0; c < C; ++c)
+        if (into[c] == c)
+          fprintf(stderr, "[pasif] class %u: n %u/%u, %.0f instances, needs %u wave(s)%s\n", c,
+                  all[c].f.n, all[c].a.n, loads[c].inst, need[c],
+                  need[c] ? "" : " (too large for lanes)");
+  }
   // the lane kernel takes the classes that fit, heaviest first, up to kLaneMaxClasses
   std::vector<uint32_t> order;
-  for (uint32_t c = 0; c < all.size(); ++c)
+  size_t kept = 0;
+  for (uint32_t c = 0; c < all.size(); ++c) {
+    if (into[c] != c) continue;
+    ++kept;
     if (need[c]) order.push_back(c);
+  }
   std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
   if (order.size() > kLaneMaxClasses) order.resize(kLaneMaxClasses);
   const char* why = nullptr;
-  if (order.size() < all.size()) {
+  if (order.size() < kept) {
     bool too_large = false;
-    for (uint32_t c = 0; c < all.size(); ++c) too_large = too_large || !need[c];
+    for (uint32_t c = 0; c < all.size(); ++c) too_large = too_large || (into[c] == c && !need[c]);
     why = too_large ? "a system needs more registers than a workgroup of lanes holds"
                     : "the systems have too many distinct structures";
     PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED, "lane kernel: %s", why);
     why = nullptr;
   }
   if (order.empty()) return false;
-  std::vector<LaneShape> shapes;
-  std::vector<ClassLoad> sel_load;
-  std::vector<int32_t> sel_of(all.size(), -1);
-  for (uint32_t c : order) {
-    sel_of[c] = int32_t(shapes.size());
-    shapes.push_back(all[c]);
-    sel_load.push_back(loads[c]);
-  }
-  for (uint32_t s = 0; s < S; ++s)
-    d->lane_cls[s] = sel_of[cls_of[s]] >= 0 ? lane_code(0, uint32_t(sel_of[cls_of[s]])) : -1;
-  // Layout: one wave per instance group when every class fits a lane's
-  // registers; otherwise the rows of each instance spread over the fewest
-  // waves (up to kLaneMaxWaves) whose share fits.
-  LaneLayout lay;
+  // Modules: the classes that fit one wave per instance group, and those
+  // that spread an instance over a workgroup of waves, each set split into
+  // modules of at most kLaneModuleClasses classes (dealt out in order of
+  // work, so the modules carry similar work); every module is one kernel,
+  // compiled on its own and launched beside the others.
   const size_t el = sizeof(R);
   const uint32_t Kreq = d->ck_every ? d->ck_every : 256;
-  if (!why) {
-    uint32_t waves = 1;
-    for (uint32_t c : order) waves = std::max(waves, need[c]);
-    if (waves > 1) {
-      {
-        for (LaneShape& sh : shapes) (void)lane_partition(sh, waves);
-        lay.waves = waves;
-        // instances per workgroup: enough workgroups for every CU, then the
-        // longest checkpoint interval (up to the requested one) that fits LDS
-        const uint64_t total = make_instances(d, adjoint, kPartLane).size();
-        lay.lw = total >= 256ull * 64 ? 64 : (total >= 256ull * 32 ? 32 : 16);
-        for (;; lay.lw /= 2) {
-          lay.seg = 1;
-          if (adjoint) {
-            uint32_t lo = 1, hi = Kreq;
-            lay.seg = 0;
-            while (lo <= hi) {
-              const uint32_t mid = lo + (hi - lo) / 2;
-              LaneLayout t = lay;
-              t.seg = mid;
-              if (lane_lds_bytes(shapes, int(el * 8), t) <= kMaxLdsBytes) {
-                lay.seg = mid;
-                lo = mid + 1;
-              } else {
-                hi = mid - 1;
-              }
-            }
-          }
-          const bool fits = lay.seg >= 1 && lane_lds_bytes(shapes, int(el * 8), lay) <= kMaxLdsBytes;
-          if (fits && (!adjoint || lay.seg >= std::min<uint32_t>(Kreq, 8) || lay.lw == 16)) break;
-          if (lay.lw == 16) {
-            if (!fits) why = "the systems' stage buffers do not fit LDS";
-            break;
-          }
-        }
-      }
-    } else {
-      choose_groups(shapes, sel_load, bits, device_simds(d->dev), d->lane_group, d->lane_bias);
-      if (adjoint) lay.seg = single_wave_segs(shapes, sel_load, el, Kreq, d->dev);
-      lay.occ = single_wave_occupancy(shapes, sel_load, bits, d->dev);
-      double waves = 0;
-      for (size_t c = 0; c < shapes.size(); ++c)
-        waves += std::ceil(sel_load[c].inst * shapes[c].group / double(kLaneWave));
-      lay.ilp = waves <= device_simds(d->dev);
+  std::vector<uint32_t> one, many;
+  uint32_t bodies = 0;
+  for (uint32_t c : order) {
+    if (need[c] == 1) {
+      one.push_back(c);
+    } else if (forced_lane || bodies + need[c] <= kLaneMultiWaveBodies) {
+      many.push_back(c);
+      bodies += need[c];
     }
   }
-  if (why) {
-    PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED, "lane kernel: %s", why);
+  std::vector<LaneModule> mods;
+  auto deal = [&](const std::vector<uint32_t>& cls, const LaneLayout& base) {
+    const size_t nm = (cls.size() + kLaneModuleClasses - 1) / kLaneModuleClasses;
+    const size_t first = mods.size();
+    for (size_t m = 0; m < nm; ++m) mods.push_back(LaneModule{{}, {}, {}, base});
+    for (size_t i = 0; i < cls.size(); ++i) {
+      LaneModule& M = mods[first + i % nm];
+      M.cls.push_back(cls[i]);
+      M.shapes.push_back(all[cls[i]]);
+      M.load.push_back(loads[cls[i]]);
+    }
+  };
+  if (!one.empty()) {
+    // group sizes, checkpoint intervals and the ILP choice over every
+    // single-wave class together (their modules share the GPU)
+    std::vector<LaneShape> sh;
+    std::vector<ClassLoad> ld;
+    for (uint32_t c : one) {
+      sh.push_back(all[c]);
+      ld.push_back(loads[c]);
+    }
+    choose_groups(sh, ld, bits, device_simds(d->dev), d->lane_group, d->lane_bias);
+    if (adjoint) (void)single_wave_segs(sh, ld, el, Kreq, d->dev);
+    double waves = 0;
+    for (size_t c = 0; c < sh.size(); ++c)
+      waves += std::ceil(ld[c].inst * sh[c].group / double(kLaneWave));
+    for (size_t i = 0; i < one.size(); ++i) all[one[i]] = sh[i];
+    LaneLayout lay;
+    lay.ilp = waves <= device_simds(d->dev);
+    const uint32_t by_waves =
+        uint32_t(std::min(8.0, std::max(1.0, std::ceil(waves / device_simds(d->dev)))));
+    const size_t m0 = mods.size();
+    deal(one, lay);
+    for (size_t m = m0; m < mods.size(); ++m) {
+      LaneModule& M = mods[m];
+      uint32_t live = 0;
+      M.lay.seg = adjoint ? Kreq : 1u;
+      for (const LaneShape& x : M.shapes) {
+        live = std::max(live, lane_live(x));
+        if (adjoint) M.lay.seg = std::min(M.lay.seg, x.seg ? x.seg : Kreq);
+      }
+      // waves per SIMD: the registers of this module's widest class, the
+      // residency of every single-wave module together
+      const uint32_t by_regs = std::max<uint32_t>(1, 512 / (live * uint32_t(bits / 32) + 48));
+      M.lay.occ = std::min(by_regs, by_waves);
+    }
+  }
+  if (!many.empty()) {
+    const size_t m0 = mods.size();
+    deal(many, LaneLayout{});
+    double total = 0;
+    for (uint32_t c : many) total += loads[c].inst;
+    for (size_t m = m0; m < mods.size(); ++m) {
+      LaneModule& M = mods[m];
+      LaneLayout& lay = M.lay;
+      uint32_t waves = 2;
+      for (uint32_t c : M.cls) waves = std::max(waves, need[c]);
+      for (LaneShape& x : M.shapes) (void)lane_partition(x, waves);
+      lay.waves = waves;
+      // instances per workgroup: enough workgroups for every CU, then the
+      // longest checkpoint interval (up to the requested one) that fits LDS
+      lay.lw = total >= 256.0 * 64 ? 64 : (total >= 256.0 * 32 ? 32 : 16);
+      bool fits = false;
+      for (;; lay.lw /= 2) {
+        lay.seg = 1;
+        if (adjoint) {
+          uint32_t lo = 1, hi = Kreq;
+          lay.seg = 0;
+          while (lo <= hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            LaneLayout t = lay;
+            t.seg = mid;
+            if (lane_lds_bytes(M.shapes, int(el * 8), t) <= kMaxLdsBytes) {
+              lay.seg = mid;
+              lo = mid + 1;
+            } else {
+              hi = mid - 1;
+            }
+          }
+        }
+        fits = lay.seg >= 1 && lane_lds_bytes(M.shapes, int(el * 8), lay) <= kMaxLdsBytes;
+        if (fits && (!adjoint || lay.seg >= std::min<uint32_t>(Kreq, 8) || lay.lw == 16)) break;
+        if (lay.lw == 16) break;
+      }
+      if (!fits) {
+        // these classes stay on tiles
+        PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED,
+                      "lane kernel: the systems' stage buffers do not fit LDS");
+        M.cls.clear();
+        M.shapes.clear();
+      }
+    }
+    mods.erase(std::remove_if(mods.begin(), mods.end(),
+                              [](const LaneModule& M) { return M.cls.empty(); }),
+               mods.end());
+  }
+  if (mods.empty()) {
     d->lane_cls.assign(S, -1);
     return false;
   }
-  const uint32_t K = adjoint ? lay.seg : 1;
-  LaneLayout lk = lay;
-  lk.seg = K;
-  std::vector<uint32_t> klass(shapes.size());
-  for (size_t c = 0; c < shapes.size(); ++c) klass[c] = uint32_t(c);
+  // lane_cls of every system before the parts are built (a part gathers its
+  // instances through it)
+  std::vector<int32_t> code_of(all.size(), -1);
+  for (size_t m = 0; m < mods.size(); ++m)
+    for (size_t k = 0; k < mods[m].cls.size(); ++k)
+      code_of[mods[m].cls[k]] = lane_code(int(m), uint32_t(k));
+  for (uint32_t s = 0; s < S; ++s) d->lane_cls[s] = code_of[cls_of[s]];
   const auto tp1 = std::chrono::steady_clock::now();
-  LanePart* part = lane_build<R>(d, adjoint, 0, shapes, klass, lk);
-  if (std::getenv("PASIF_PACK_TIMING"))
-    fprintf(stderr, "[pasif] lane plan %.1f ms, lane build %.1f ms\n",
-            std::chrono::duration<double, std::milli>(tp1 - tp0).count(),
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1)
-                .count());
-  const size_t lds = lane_lds_bytes(shapes, int(sizeof(R) * 8), lk);
-  PASIF_REQUIRE(lds <= (lk.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
-                "lane kernel LDS %zu bytes", lds);
-  part->lds = lds;
+  std::vector<std::string> srcs;
+  pk.lshapes.clear();
+  pk.lgroups.clear();
+  pk.lmods.clear();
+  pk.lseg = 0;
+  for (size_t m = 0; m < mods.size(); ++m) {
+    LaneModule& M = mods[m];
+    LaneLayout lk = M.lay;
+    lk.seg = adjoint ? lk.seg : 1u;
+    std::vector<uint32_t> klass(M.shapes.size());
+    for (size_t c = 0; c < M.shapes.size(); ++c) klass[c] = uint32_t(c);
+    LanePart* part = lane_build<R>(d, adjoint, int(m), M.shapes, klass, lk);
+    const size_t lds = lane_lds_bytes(M.shapes, int(sizeof(R) * 8), lk);
+    PASIF_REQUIRE(lds <= (lk.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
+                  "lane kernel LDS %zu bytes", lds);
+    part->lds = lds;
+    srcs.push_back(lane_source(M.shapes, int(sizeof(R) * 8), lk));
+    for (const LaneShape& x : M.shapes) {
+      pk.lshapes.push_back(x);
+      pk.lgroups.push_back(lk.waves > 1 ? 1u : x.group);
+    }
+    pk.lmods.push_back({M.shapes, lk});
+    pk.lseg = pk.lseg ? std::min(pk.lseg, lk.seg) : lk.seg;
+    M.lay = lk;
+  }
   if (d->dev >= 0) {
-    const std::string src = lane_source(shapes, int(sizeof(R) * 8), lk);
     std::string err;
     if (!lane_jit_flags_ok(&err)) {
       set_err(d, "%s", err.c_str());
       throw Fail{PASIF_ERR_ARGUMENT};
     }
-    if (!lane_compile(src, d->dev, &part->kern, &err, lk.ilp)) {
+    std::vector<LaneKernel*> outs;
+    std::vector<bool> ilp;
+    for (size_t m = 0; m < mods.size(); ++m) {
+      outs.push_back(&pk.lparts[pk.lparts.size() - mods.size() + m]->kern);
+      ilp.push_back(mods[m].lay.ilp);
+    }
+    if (!lane_compile_many(srcs, ilp, d->dev, outs, &err)) {
       set_err(d, "%s", err.c_str());
       throw Fail{PASIF_ERR_DEVICE};
     }
   }
-  pk.lshapes = shapes;
-  pk.lseg = K;
-  pk.llay = lk;
+  if (std::getenv("PASIF_PACK_TIMING"))
+    fprintf(stderr, "[pasif] lane plan %.1f ms, lane build and compile %.1f ms\n",
+            std::chrono::duration<double, std::milli>(tp1 - tp0).count(),
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1)
+                .count());
+  pk.llay = mods[0].lay;
   if (d->d_compute)
-    printf("[pasif] lane kernel: %zu structure class(es), %u workgroups of %u wave(s), T max %u, "
-           "checkpoint %u\n",
-           shapes.size(), part->blocks, lk.waves, part->tmax, K);
+    for (size_t m = 0; m < mods.size(); ++m) {
+      const LanePart& P = *pk.lparts[pk.lparts.size() - mods.size() + m];
+      printf("[pasif] lane module %zu: %zu structure class(es), %u workgroups of %u wave(s), "
+             "T max %u, checkpoint %u\n",
+             m, mods[m].shapes.size(), P.blocks, mods[m].lay.waves, P.tmax, mods[m].lay.seg);
+    }
   return true;
 }
 
@@ -1801,7 +2109,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
   const uint32_t out_len_a = adjoint ? (d->mapped ? d->adj.n : d->adj.n * d->n_exc) : 0;
   // classes and their fields' places
   struct Fields {
-    size_t cf, ca, qf, qa, stp, ex, of, oa, ck;
+    size_t cf, ca, qf, qa, stp, ex, of, oa, na, ck;
   };
   std::vector<Fields> fld(shapes.size());
   for (size_t c = 0; c < shapes.size(); ++c) {
@@ -1844,6 +2152,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     f.ex = place(&L->exc, st * sizeof(int32_t));
     f.of = place(&L->out_f, st * sizeof(uint32_t));
     f.oa = place(&L->out_a, st * sizeof(uint32_t));
+    f.na = place(&L->na_out, st * sizeof(uint32_t));
     P->cls.push_back(std::move(L));
   }
   if (host.size() < top) host.resize(top);
@@ -1868,9 +2177,13 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     int32_t* ex = reinterpret_cast<int32_t*>(host.data() + f.ex);
     uint32_t* of = reinterpret_cast<uint32_t*>(host.data() + f.of);
     uint32_t* oa = reinterpret_cast<uint32_t*>(host.data() + f.oa);
+    uint32_t* nao = reinterpret_cast<uint32_t*>(host.data() + f.na);
     std::fill(ex, ex + st, -1);
     std::vector<double> lv;
-    CoefMap map_f, map_a;
+    // coefficient maps per original structure (a class may be the union of
+    // several)
+    std::map<uint32_t, CoefMap> map_f, map_a;
+    const double eps = std::numeric_limits<R>::epsilon();
     // lane slot r of position p of a group side -> the system's row
     auto slot_row = [](const GSide& gs, uint32_t r, uint32_t p) {
       return r < gs.RL ? gs.band[r] + p * gs.stride[r] : gs.shared[r - gs.RL];
@@ -1880,11 +2193,15 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
       // every output index the kernel writes stays inside the output arrays
       PASIF_REQUIRE(uint64_t(in.out_f) + sh.f.n <= out_len_f, PASIF_ERR_UNSUPPORTED,
                     "lane plan: output index out of range");
-      PASIF_REQUIRE(!adjoint || uint64_t(in.out_a) + sh.a.n <= out_len_a,
+      // adjoint rows of this instance's own system (a merged class may pad
+      // its adjoint side beyond them; padded rows are never stored)
+      const uint32_t own_a = adjoint ? d->adj.size[in.sys] : 0u;
+      PASIF_REQUIRE(!adjoint || (own_a <= sh.a.n && uint64_t(in.out_a) + own_a <= out_len_a),
                     PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
       PASIF_REQUIRE(in.exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
                     "lane plan: excitation out of range");
-      lane_coefs_mapped(d->fwd, in.sys, sh.f, false, map_f, cv);
+      const uint32_t sk = in.sys < d->lane_struct.size() ? d->lane_struct[in.sys] : 0u;
+      lane_coefs_mapped(d->fwd, in.sys, sh.f, false, eps, map_f[sk], cv);
       PASIF_REQUIRE(cv.size() == sh.f.coefs(), PASIF_ERR_UNSUPPORTED,
                     "lane plan: structure mismatch");
       if (grp) {
@@ -1900,7 +2217,7 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
         for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
       }
       if (adjoint) {
-        lane_coefs_mapped(d->adj, in.sys, sh.a, true, map_a, cv);
+        lane_coefs_mapped(d->adj, in.sys, sh.a, true, eps, map_a[sk], cv);
         PASIF_REQUIRE(cv.size() == sh.a.coefs(), PASIF_ERR_UNSUPPORTED,
                       "lane plan: structure mismatch");
         if (grp) {
@@ -1908,19 +2225,22 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
             const size_t col = size_t(i) * G + p;
             lane_group_coefs(sh.ga, cv, p, lv);
             for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + col] = R(lv[t]);
-            for (uint32_t r = 0; r < L->na; ++r)
-              qa[size_t(r) * st + col] = R(d->adj.init[d->adj.off[in.sys] + slot_row(sh.ga, r, p)]);
+            for (uint32_t r = 0; r < L->na; ++r) {
+              const uint32_t row = slot_row(sh.ga, r, p);
+              qa[size_t(r) * st + col] = row < own_a ? R(d->adj.init[d->adj.off[in.sys] + row]) : R(0);
+            }
           }
         } else {
           for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
           for (uint32_t r = 0; r < L->na; ++r)
-            qa[size_t(r) * st + i] = R(d->adj.init[d->adj.off[in.sys] + r]);
+            qa[size_t(r) * st + i] = r < own_a ? R(d->adj.init[d->adj.off[in.sys] + r]) : R(0);
         }
       }
       stp[i] = in.steps;
       ex[i] = in.exc;
       of[i] = in.out_f;
       oa[i] = in.out_a;
+      nao[i] = own_a;
       L->tmax = std::max(L->tmax, in.steps);
     }
     tmax = std::max(tmax, L->tmax);
@@ -2396,6 +2716,7 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
     a.exc = L.exc;
     a.out_f = L.out_f;
     a.out_a = L.out_a;
+    a.na_out = L.na_out;
     a.ck_off = L.ckoff;
     a.out_q = out_q;
     a.out_max = d->window ? d->out_max.p : nullptr;
@@ -2434,11 +2755,13 @@ hipError_t lane_launch(const LanePart& P, const void* dargs, uint32_t n, hipStre
                                st, params, nullptr);
 }
 
-void ensure_aux(pasif_driver* d) {
-  if (d->aux) return;
-  PASIF_HIP(hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking));
-  PASIF_HIP(hipEventCreateWithFlags(&d->fork, hipEventDisableTiming));
-  PASIF_HIP(hipEventCreateWithFlags(&d->join, hipEventDisableTiming));
+void ensure_aux(pasif_driver* d, int n) {
+  if (!d->fork) PASIF_HIP(hipEventCreateWithFlags(&d->fork, hipEventDisableTiming));
+  for (int i = 0; i < n; ++i) {
+    if (d->aux[i]) continue;
+    PASIF_HIP(hipStreamCreateWithFlags(&d->aux[i], hipStreamNonBlocking));
+    PASIF_HIP(hipEventCreateWithFlags(&d->join[i], hipEventDisableTiming));
+  }
 }
 
 // Device bytes the run buffers of a solve of the packed problem take
@@ -2544,24 +2867,40 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
     for (LaneArgs& a : largs[i]) a.budget = budget;
     upload(d, pk.lparts[i]->args, largs[i]);
   }
-  if (lane && tile) ensure_aux(d);
+  // lane modules with work, each on a stream of its own where one is free:
+  // the driver's stream when no tile or row-parallel launch runs on it, then
+  // up to three side streams (modules beyond that share them in turn)
+  std::vector<size_t> lmods;
+  for (size_t i = 0; i < largs.size(); ++i)
+    if (pk.lparts[i]->blocks) lmods.push_back(i);
+  const bool main_busy = tile || big;
+  const int nside = int(std::min<size_t>(pasif_driver::kSide,
+                                         lmods.size() - (main_busy || lmods.empty() ? 0 : 1)));
+  if (nside > 0) ensure_aux(d, nside);
+  auto lane_stream = [&](size_t j) -> int {  // -1: the driver's stream
+    if (!main_busy && j == 0) return -1;
+    return int((j - (main_busy ? 0 : 1)) % size_t(nside));
+  };
   const uint32_t nb = pk.nbig;
   const uint32_t nt = pk.ntiles;
   drive(d, launches, [&]() -> hipError_t {
     hipError_t e = hipSuccess;
-    hipStream_t ls = tile ? d->aux : d->stream;
-    if (lane && tile) {
+    if (nside > 0) {
       e = hipEventRecord(d->fork, d->stream);
-      if (e == hipSuccess) e = hipStreamWaitEvent(d->aux, d->fork, 0);
+      for (int k = 0; k < nside && e == hipSuccess; ++k)
+        e = hipStreamWaitEvent(d->aux[k], d->fork, 0);
     }
-    for (size_t i = 0; i < largs.size() && e == hipSuccess; ++i)
-      if (pk.lparts[i]->blocks)
-        e = lane_launch(*pk.lparts[i], pk.lparts[i]->args.p, uint32_t(largs[i].size()), ls);
+    for (size_t j = 0; j < lmods.size() && e == hipSuccess; ++j) {
+      const LanePart& P = *pk.lparts[lmods[j]];
+      const int k = lane_stream(j);
+      e = lane_launch(P, P.args.p, uint32_t(largs[lmods[j]].size()),
+                      k < 0 ? d->stream : d->aux[k]);
+    }
     if (tile && e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
     if (big && e == hipSuccess) e = launch_big(ba, nb, d->stream);
-    if (lane && tile) {
-      if (e == hipSuccess) e = hipEventRecord(d->join, d->aux);
-      if (e == hipSuccess) e = hipStreamWaitEvent(d->stream, d->join, 0);
+    for (int k = 0; k < nside; ++k) {
+      if (e == hipSuccess) e = hipEventRecord(d->join[k], d->aux[k]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(d->stream, d->join[k], 0);
     }
     return e;
   });
@@ -2768,12 +3107,13 @@ int pasif_destroy(pasif_driver* d) {
   if (d->dev >= 0) (void)hipSetDevice(d->dev);
   if (d->stream) (void)hipStreamSynchronize(d->stream);
   d->pk.release();
-  if (d->aux) {
-    (void)hipStreamSynchronize(d->aux);
-    (void)hipStreamDestroy(d->aux);
+  for (int k = 0; k < pasif_driver::kSide; ++k) {
+    if (!d->aux[k]) continue;
+    (void)hipStreamSynchronize(d->aux[k]);
+    (void)hipStreamDestroy(d->aux[k]);
+    if (d->join[k]) (void)hipEventDestroy(d->join[k]);
   }
   if (d->fork) (void)hipEventDestroy(d->fork);
-  if (d->join) (void)hipEventDestroy(d->join);
   if (!d->pool) {
     if (d->ev0) (void)hipEventDestroy(d->ev0);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
@@ -3162,7 +3502,7 @@ int pasif_lane_groups(pasif_driver* d, int adjoint, uint32_t* out, size_t cap, s
     ensure_packed(d, adjoint != 0);
     std::vector<uint32_t> g;
     if (d->pk.lane)
-      for (const LaneShape& sh : d->pk.lshapes) g.push_back(d->pk.llay.waves > 1 ? 1u : sh.group);
+      g = d->pk.lgroups;
     if (len) *len = g.size();
     if (out) {
       PASIF_REQUIRE(cap >= g.size(), PASIF_ERR_ARGUMENT, "buffer holds %zu, need %zu", cap, g.size());
@@ -3174,8 +3514,11 @@ int pasif_lane_groups(pasif_driver* d, int adjoint, uint32_t* out, size_t cap, s
 int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_t* len,
                       int compile) {
   return guarded(d, [&] {
+    // the lane modules of the automatic plan (every class on lanes when the
+    // lane kernel is forced; with tiles forced, the plan the lane kernel
+    // would take)
     const int saved = d->kernel_path;
-    d->kernel_path = PASIF_KERNEL_LANE;
+    if (saved != PASIF_KERNEL_AUTO) d->kernel_path = PASIF_KERNEL_LANE;
     const int dev = d->dev;
     d->dev = -1;  // plan on the host only; the compilation below needs no GPU
     d->pk.release();
@@ -3187,11 +3530,19 @@ int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_
     });
     d->dev = dev;
     d->kernel_path = saved;
-    const std::vector<LaneShape> shapes = d->pk.lshapes;
-    const LaneLayout lay = d->pk.llay;
+    const auto mods = d->pk.lmods;
     d->pk.release();
     if (rc != PASIF_OK) throw Fail{rc};
-    const std::string src = lane_source(shapes, d->precision, lay);
+    // one kernel per module, in module order
+    std::string src;
+    std::vector<std::string> srcs;
+    std::vector<bool> ilp;
+    for (size_t m = 0; m < mods.size(); ++m) {
+      srcs.push_back(lane_source(mods[m].first, d->precision, mods[m].second));
+      ilp.push_back(mods[m].second.ilp);
+      if (mods.size() > 1) src += "// ---- lane module " + std::to_string(m) + "\n";
+      src += srcs.back();
+    }
     if (len) *len = src.size() + 1;
     if (out) {
       PASIF_REQUIRE(cap > src.size(), PASIF_ERR_ARGUMENT, "source buffer holds %zu, need %zu",
@@ -3200,12 +3551,12 @@ int pasif_lane_source(pasif_driver* d, int adjoint, char* out, size_t cap, size_
     }
     if (compile) {
       std::string err;
-      size_t bytes = 0;
       if (!lane_jit_flags_ok(&err)) {
         set_err(d, "%s", err.c_str());
         throw Fail{PASIF_ERR_ARGUMENT};
       }
-      if (!lane_compile_only(src, &bytes, &err, lay.ilp)) {
+      if (!lane_compile_many(srcs, ilp, -1, std::vector<LaneKernel*>(srcs.size(), nullptr),
+                             &err)) {
         set_err(d, "%s", err.c_str());
         throw Fail{PASIF_ERR_DEVICE};
       }

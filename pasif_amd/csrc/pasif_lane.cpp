@@ -13,6 +13,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +22,7 @@
 #include <mutex>
 #include <sstream>
 #include <sys/stat.h>
+#include <thread>
 #include <unistd.h>
 
 namespace pasif {
@@ -756,6 +758,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   const bool forced = exc >= 0;
   const u32 fout = ((GP(const u32))a.out_f)[i];
   const u32 aout = ((GP(const u32))a.out_a)[i];
+  const u32 nao = ((GP(const u32))a.na_out)[i];
   GP(const R) cf = (GP(const R))a.coef_f + col;
   GP(const R) ca = (GP(const R))a.coef_a + col;
   GP(const R) qi = (GP(const R))a.init_f + col;
@@ -1013,7 +1016,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           if (adjoint && valid)
 #pragma unroll
             for (int r = 0; r < S::NA; ++r)
-              if (orow_a(r) >= 0) outg[aout + orow_a(r)] = li[r * st];
+              if (orow_a(r) >= 0 && (u32)orow_a(r) < nao) outg[aout + orow_a(r)] = li[r * st];
         }
       }
     } else if (S::ADJ) {
@@ -1059,7 +1062,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
           if (valid)
 #pragma unroll
             for (int r = 0; r < S::NA; ++r)
-              if (orow_a(r) >= 0) outg[aout + orow_a(r)] = lam[r];
+              if (orow_a(r) >= 0 && (u32)orow_a(r) < nao) outg[aout + orow_a(r)] = lam[r];
         } else {
           sg -= 1u;
           n = sg * K;
@@ -1171,6 +1174,7 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
   const bool forced = exc >= 0;
   const u32 fout = ((GP(const u32))a.out_f)[i];
   const u32 aout = ((GP(const u32))a.out_a)[i];
+  const u32 nao = ((GP(const u32))a.na_out)[i];
   GP(const R) cf = (GP(const R))a.coef_f + i;
   GP(const R) ca = (GP(const R))a.coef_a + i;
   GP(const R) qi = (GP(const R))a.init_f + i;
@@ -1376,7 +1380,8 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
           phase = kDone;
           if (adjoint && valid)
 #pragma unroll
-            for (int r = 0; r < SW::RA; ++r) outg[aout + SW::ra(r)] = li[SW::ra(r) * st];
+            for (int r = 0; r < SW::RA; ++r)
+              if ((u32)SW::ra(r) < nao) outg[aout + SW::ra(r)] = li[SW::ra(r) * st];
         }
       }
       bar();
@@ -1415,7 +1420,8 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
           phase = kDone;
           if (valid)
 #pragma unroll
-            for (int r = 0; r < SW::RA; ++r) outg[aout + SW::ra(r)] = lam[r];
+            for (int r = 0; r < SW::RA; ++r)
+              if ((u32)SW::ra(r) < nao) outg[aout + SW::ra(r)] = lam[r];
         } else {
           sg -= 1u;
           n = sg * K;
@@ -2242,62 +2248,120 @@ bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* 
   return true;
 }
 
-bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err, bool ilp) {
-  std::vector<std::string> optv;
-  if (!jit_options(ilp, &optv, err)) return false;
-  std::string opts;
-  for (const std::string& x : optv) opts += x + " ";
-  const uint64_t key = fnv1a(src + opts);
-  std::lock_guard<std::mutex> lk(g_jit_mu);
-  auto it = g_jit.find({dev, key});
-  if (it != g_jit.end()) {
-    *out = it->second;
-    return true;
-  }
-  std::vector<char> code;
+namespace {
+std::string jit_cache_path(uint64_t key) {
   const std::string dir = cache_dir();
+  if (dir.empty()) return std::string();
   char name[64];
   snprintf(name, sizeof name, "lane_%016llx.co", (unsigned long long)key);
-  const std::string path = dir.empty() ? std::string() : dir + "/" + name;
-  if (!path.empty()) {
-    if (FILE* f = fopen(path.c_str(), "rb")) {
-      fseek(f, 0, SEEK_END);
-      const long sz = ftell(f);
-      fseek(f, 0, SEEK_SET);
-      if (sz > 0) {
-        code.resize(size_t(sz));
-        if (fread(code.data(), 1, code.size(), f) != code.size()) code.clear();
-      }
-      fclose(f);
+  return dir + "/" + name;
+}
+
+void read_cached(const std::string& path, std::vector<char>* code) {
+  if (path.empty()) return;
+  if (FILE* f = fopen(path.c_str(), "rb")) {
+    fseek(f, 0, SEEK_END);
+    const long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    if (sz > 0) {
+      code->resize(size_t(sz));
+      if (fread(code->data(), 1, code->size(), f) != code->size()) code->clear();
     }
+    fclose(f);
   }
-  if (code.empty()) {
-    if (!rtc_compile(src, &code, err, ilp)) return false;
-    if (!path.empty()) {
-      mkdirs(dir);
-      const std::string tmp = path + ".tmp" + std::to_string(long(getpid()));
-      if (FILE* f = fopen(tmp.c_str(), "wb")) {
-        const bool ok = fwrite(code.data(), 1, code.size(), f) == code.size();
-        fclose(f);
-        if (ok) std::rename(tmp.c_str(), path.c_str());
-        else std::remove(tmp.c_str());
-      }
+}
+
+void write_cached(const std::string& path, const std::vector<char>& code) {
+  if (path.empty()) return;
+  mkdirs(cache_dir());
+  const std::string tmp = path + ".tmp" + std::to_string(long(getpid()));
+  if (FILE* f = fopen(tmp.c_str(), "wb")) {
+    const bool ok = fwrite(code.data(), 1, code.size(), f) == code.size();
+    fclose(f);
+    if (ok)
+      std::rename(tmp.c_str(), path.c_str());
+    else
+      std::remove(tmp.c_str());
+  }
+}
+}  // namespace
+
+bool lane_compile_many(const std::vector<std::string>& srcs, const std::vector<bool>& ilp,
+                       int dev, const std::vector<LaneKernel*>& outs, std::string* err) {
+  const size_t n = srcs.size();
+  std::vector<uint64_t> keys(n, 0);
+  std::vector<std::vector<char>> code(n);
+  std::vector<char> done(n, 0);
+  std::vector<size_t> todo;
+  bool mllvm = false;
+  std::lock_guard<std::mutex> lk(g_jit_mu);
+  for (size_t i = 0; i < n; ++i) {
+    std::vector<std::string> optv;
+    if (!jit_options(ilp[i], &optv, err)) return false;
+    std::string opts;
+    for (const std::string& x : optv) {
+      opts += x + " ";
+      mllvm = mllvm || x == "-mllvm";
     }
+    keys[i] = fnv1a(srcs[i] + opts);
+    if (dev >= 0) {
+      auto it = g_jit.find({dev, keys[i]});
+      if (it != g_jit.end()) {
+        *outs[i] = it->second;
+        done[i] = 1;
+        continue;
+      }
+      read_cached(jit_cache_path(keys[i]), &code[i]);
+    }
+    if (code[i].empty()) todo.push_back(i);
   }
-  LaneKernel k;
-  hipError_t e = hipModuleLoadData(&k.mod, code.data());
-  if (e != hipSuccess) {
-    *err = std::string("loading the lane kernel module failed: ") + hipGetErrorName(e);
-    return false;
+  std::vector<std::string> errs(n);
+  std::vector<char> ok(n, 1);
+  auto compile = [&](size_t j) {
+    const size_t i = todo[j];
+    ok[i] = rtc_compile(srcs[i], &code[i], &errs[i], ilp[i]) ? 1 : 0;
+  };
+  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+  const size_t nt = mllvm ? 1 : std::min<size_t>({todo.size(), hw, 16});
+  if (nt > 1) {
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nt; ++t)
+      pool.emplace_back([&] {
+        for (size_t j; (j = next.fetch_add(1)) < todo.size();) compile(j);
+      });
+    for (std::thread& t : pool) t.join();
+  } else {
+    for (size_t j = 0; j < todo.size(); ++j) compile(j);
   }
-  e = hipModuleGetFunction(&k.fn, k.mod, "pasif_lane_kernel");
-  if (e != hipSuccess) {
-    *err = std::string("lane kernel not found in its module: ") + hipGetErrorName(e);
-    return false;
+  for (size_t i : todo)
+    if (!ok[i]) {
+      *err = errs[i];
+      return false;
+    }
+  if (dev < 0) return true;
+  for (size_t i : todo) write_cached(jit_cache_path(keys[i]), code[i]);
+  for (size_t i = 0; i < n; ++i) {
+    if (done[i]) continue;
+    LaneKernel k;
+    hipError_t e = hipModuleLoadData(&k.mod, code[i].data());
+    if (e != hipSuccess) {
+      *err = std::string("loading the lane kernel module failed: ") + hipGetErrorName(e);
+      return false;
+    }
+    e = hipModuleGetFunction(&k.fn, k.mod, "pasif_lane_kernel");
+    if (e != hipSuccess) {
+      *err = std::string("lane kernel not found in its module: ") + hipGetErrorName(e);
+      return false;
+    }
+    g_jit[{dev, keys[i]}] = k;
+    *outs[i] = k;
   }
-  g_jit[{dev, key}] = k;
-  *out = k;
   return true;
+}
+
+bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err, bool ilp) {
+  return lane_compile_many({src}, {ilp}, dev, {out}, err);
 }
 
 }  // namespace pasif

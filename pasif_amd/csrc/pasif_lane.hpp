@@ -31,6 +31,8 @@ namespace pasif {
   X(const void*, exc)      /* i32 [stride], -1 = unforced */                \
   X(const void*, out_f)    /* u32 [stride] output base, forward rows */     \
   X(const void*, out_a)    /* u32 [stride] output base, adjoint rows */     \
+  X(const void*, na_out)   /* u32 [stride] adjoint rows the instance owns    \
+                              (a merged class may pad its adjoint) */       \
   X(void*, qstate)         /* [NF][stride] state between launches */        \
   X(void*, lstate)         /* [NA][stride] */                               \
   X(void*, mxstate)        /* [NF][stride] running max |q| */               \
@@ -202,6 +204,13 @@ struct LaneKernel {
 // Returns false with a message in *err on failure.
 bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string* err,
                   bool ilp = false);
+// Several kernels at once (one per module of a lane plan): those in neither
+// cache are compiled on worker threads, one program each (one after the
+// other when the options carry -mllvm flags, which LLVM keeps in
+// process-wide state).  dev < 0 compiles only (no cache, no module load;
+// outs may hold null).
+bool lane_compile_many(const std::vector<std::string>& srcs, const std::vector<bool>& ilp,
+                       int dev, const std::vector<LaneKernel*>& outs, std::string* err);
 // hipRTC compilation only (no device needed): size of the code object
 bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* err,
                        bool ilp = false);

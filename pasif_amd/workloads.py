@@ -238,13 +238,70 @@ DOF_CHOICES = np.array([1, 2, 4, 8, 16, 32, 64])
 DOF_PROBS = np.array([0.3, 0.25, 0.15, 0.1, 0.1, 0.06, 0.04])
 
 
+def structure_variant(d: int, v: int):
+    """Sparsity variant ``v`` of the ``d``-DOF chain: which of its 2d
+    nearest-neighbour quadratic couplings (Gamma) it keeps and up to three
+    extra springs between DOF two or three apart (next-nearest couplings),
+    drawn from a generator seeded by ``(d, v)``.  The parameters of the
+    energy gradient (K diagonal, Lambda) are those of the plain chain in
+    every variant.  Variant 0 is the plain chain; a chain of 1 DOF has 2
+    variants, of 2 DOF 4."""
+    if v == 0:
+        return np.ones(2 * d, dtype=bool), []
+    if d == 1:          # two copies of one monomial: all or none
+        return np.zeros(2, dtype=bool), []
+    if d == 2:
+        keep = np.ones(4, dtype=bool)
+        keep[v - 1] = False
+        return keep, []
+    r = np.random.default_rng([d, v])
+    keep = r.uniform(size=2 * d) < 0.7
+    starts = r.integers(0, d, size=6)
+    gaps = r.integers(2, 4, size=6)
+    pairs = {(int(i), int(i + g)) for i, g in zip(starts, gaps) if i + g < d}
+    return keep, sorted(pairs)[:3]
+
+
+def variant_count(d: int, per_size: int) -> int:
+    """Distinct variants ``structure_variant`` gives a ``d``-DOF chain."""
+    return min(per_size, 2 if d == 1 else (4 if d == 2 else per_size))
+
+
+def _apply_variant(d, K, G, keep, springs, s):
+    """K with the extra springs (stiffness 0.5 s each, symmetric) and the
+    Gamma tensor without the nearest-neighbour couplings ``keep`` drops (the
+    energy-probe entries after them stay)."""
+    if springs:
+        K = K.tocoo()
+        rows, cols, vals = [K.row], [K.col], [K.data]
+        for i, j in springs:
+            k = 0.5 * s
+            rows.append(np.array([d + i, d + j, d + i, d + j]))
+            cols.append(np.array([i, j, j, i]))
+            vals.append(np.array([k, k, -k, -k]))
+        K = coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=K.shape)
+        K.sum_duplicates()
+    if not keep.all():
+        idx = np.asarray(G.indices).reshape(-1, 3)
+        val = np.asarray(G.val)
+        k = np.ones(len(val), dtype=bool)
+        k[: len(keep)] = keep
+        G = _tensor(list(G.dimensions), val[k], idx[k].reshape(-1))
+    return K, G
+
+
 def heterogeneous(n_systems: int = 100_000, tmin: float = 1e3, tmax: float = 1e5,
                   dt: float = 1e-3, steps_per_sample: int = 10, seed: int = 3,
-                  n_signals: int = 64, dof_choices=None, dof_probs=None) -> Workload:
+                  n_signals: int = 64, dof_choices=None, dof_probs=None,
+                  variants_per_size: int = 1) -> Workload:
     """Systems of uneven size and step count: DOF from {1,..,64} with the
     configured probabilities (or ``dof_choices`` / ``dof_probs``), steps
     log-uniform on [tmin, tmax], chain structure at each size, one of
-    ``n_signals`` band-limited forcings each."""
+    ``n_signals`` band-limited forcings each.  ``variants_per_size`` > 1
+    draws each system's sparsity uniformly from that many variants of its
+    size (``structure_variant``): 12 gives 66 distinct structures over the
+    seven sizes."""
     rng = np.random.default_rng(seed)
     choices = DOF_CHOICES if dof_choices is None else np.asarray(dof_choices)
     probs = DOF_PROBS if dof_probs is None else np.asarray(dof_probs, dtype=np.float64)
@@ -254,19 +311,29 @@ def heterogeneous(n_systems: int = 100_000, tmin: float = 1e3, tmax: float = 1e5
     n_samples = int(steps.max()) // steps_per_sample + 1
     exc = np.stack([band_limited(rng, n_samples) for _ in range(n_signals)])
     vM, vB, vK, vG, vL, vF, vQ = [], [], [], [], [], [], []
-    cache = {}
     scales = rng.uniform(0.9, 1.1, n_systems)
+    variants = np.zeros(n_systems, dtype=np.int64)
+    if variants_per_size > 1:
+        nv = np.array([variant_count(int(d), variants_per_size) for d in dofs])
+        variants = (rng.uniform(size=n_systems) * nv).astype(np.int64)
+    shapes = {}
     for s in range(n_systems):
         d = int(dofs[s])
         gv = rng.normal(0.0, 0.05, 2 * d)
         lv = rng.uniform(0.0, 1.0, d)
         M, B, K, G, L, f, q0, _, _ = chain_system(d, scales[s], gv, lv)
+        v = int(variants[s])
+        if v:
+            key = (d, v)
+            if key not in shapes:
+                shapes[key] = structure_variant(d, v)
+            K, G = _apply_variant(d, K, G, *shapes[key], scales[s])
         vM.append(M); vB.append(B); vK.append(K); vG.append(G); vL.append(L)
         vF.append(f); vQ.append(q0)
     return Workload("heterogeneous", vM, vB, vK, vG, vL, vF, vQ, exc, sr, int(steps.max()),
                     exc_of_system=rng.integers(0, n_signals, n_systems),
                     steps_of_system=steps, interp=linear_interp_matrix(steps_per_sample),
-                    meta={"dofs": dofs})
+                    meta={"dofs": dofs, "variants": variants})
 
 
 def make_solver(w: Workload, **kw):
