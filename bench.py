@@ -427,6 +427,60 @@ def batching_section(args) -> dict:
             "fwd_adj_fp32_vs_fp64_oracle": PAR.report(sample_a(g32), G["g64"], sza).max_rel,
             "fwd_adj_fp32_vs_fp32_oracle": PAR.report(sample_a(g32), G["g32"], sza).max_rel,
             "tolerance_fp64": PAR.TOL64}
+    if not args.no_structures:
+        sec["structures"] = structures_leg(args, sec)
+    return sec
+
+
+def structures_leg(args, batch: dict) -> dict:
+    """The same batch with 66 sparsity structures: each system's drawn from
+    12 variants of its size (nearest-neighbour Gamma couplings dropped at
+    random, up to three extra next-nearest springs).  The lane planner merges
+    them into union classes; FP64 forward and forward + adjoint, persistent,
+    default tuning, against the 7-structure batch's autotuned times above;
+    errors against the FP64 oracle on a stratified sample."""
+    from oracle import parity as PAR
+    from pasif_amd import adjoint as ADJ
+    from pasif_amd import workloads as W
+    t0 = time.perf_counter()
+    w = W.heterogeneous(n_systems=args.batch_systems, tmin=1e3, tmax=1e5, steps_per_sample=10,
+                        variants_per_size=12)
+    p = W.make_solver(w)
+    sys_steps = int(np.sum(w.steps_of_system))
+    info = p.kernelInfo()
+    sec = {"workload": "heterogeneous batch above, 66 sparsity structures (12 variants per "
+                       "DOF size)", "setup_s": time.perf_counter() - t0,
+           "fwd_classes": info["classes"], "fwd_kernel": info["kernel"]}
+    _progress(f"structures: workload built, plan {info}")
+    p.getAmplitudes()
+    q, wall, t = _timed(p, p.getAmplitudes)
+    sec["fwd_fp64"] = {"kernel_ms": t["kernel_ms"], "value": sys_steps / wall,
+                       "vs_7_structures": t["kernel_ms"] / batch["fwd_fp64"]["kernel_ms"]}
+    blocks = ADJ.energy_gradient_blocks(w)
+    p.setJacobianBlocks(blocks)
+    info = p.kernelInfo(True)
+    sec["adj_classes"], sec["adj_kernel"] = info["classes"], info["kernel"]
+    p.getGradient()
+    g, wall, t = _timed(p, p.getGradient)
+    sec["fwd_adj_fp64"] = {"kernel_ms": t["kernel_ms"], "value": sys_steps / wall,
+                           "vs_7_structures": t["kernel_ms"] / batch["fwd_adj_fp64"]["kernel_ms"]}
+    _progress(f"structures: forward {sec['fwd_fp64']['kernel_ms']:.0f} ms, forward+adjoint "
+              f"{sec['fwd_adj_fp64']['kernel_ms']:.0f} ms")
+    gold = os.path.join(GOLD, "fullsize_structures.npz")
+    if os.path.exists(gold) and args.batch_systems == 100_000:
+        G = np.load(gold)
+        ids = G["ids"]
+        sizes = w.sizes()
+        off_f = np.concatenate([[0], np.cumsum(sizes)])
+        off_a = np.concatenate([[0], np.cumsum(blocks.sizes)])
+        qs = np.concatenate([q[off_f[i]:off_f[i + 1]] for i in ids])
+        gs = np.concatenate([g[off_a[i]:off_a[i + 1]] for i in ids])
+        sec["parity"] = {
+            "sample": f"{len(ids)} systems: per DOF size the two with the most steps and two at "
+                      f"random (tests/golden/fullsize_structures.npz)",
+            "fwd_fp64": PAR.report(qs, G["q64"], sizes[ids]).as_dict(),
+            "fwd_adj_fp64": PAR.report(gs, G["g64"], blocks.sizes[ids]).as_dict(),
+            "tolerance_fp64": PAR.TOL64}
     return sec
 
 
@@ -502,6 +556,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-structures", action="store_true")
     ap.add_argument("--adj-systems", type=int, default=8192)
     ap.add_argument("--adj-steps", type=int, default=20_000)
     ap.add_argument("--batch-systems", type=int, default=100_000)

@@ -6,7 +6,11 @@ distributions (DOF {1,..,64}, steps log-uniform on [1e3, 1e5], dt 1e-3,
 chain structure, interpolated band-limited forcing); 30 % forward-only, 70 %
 forward+adjoint (energy gradient), FP64, admitted against a 200 GB HBM
 budget and a resident-tile (slot) cap; library systems are registered with
-the pool once and requests travel as system ids.  For every offered load it prints one JSON line: p50 / p99 / mean
+the pool once and requests travel as system ids.  With ``--fresh-frac f`` a
+fraction f of the requests instead carry their own arrays: systems with
+coefficients drawn afresh and one of 66 sparsity structures (12 variants per
+DOF size), none registered, run by the pool's tile kernel beside the lane
+modules.  For every offered load it prints one JSON line: p50 / p99 / mean
 latency (arrival to results on the host) of completed requests, the backlog
 left at the end with p99 over all arrived requests (waiting time so far for
 the unfinished ones, a lower bound), sustained system-steps/s and GPU busy
@@ -44,6 +48,9 @@ def main():
     ap.add_argument("--slots", type=int, default=32768,
                     help="resident tile cap (0: HBM budget only)")
     ap.add_argument("--plain-sample", type=int, default=3)
+    ap.add_argument("--fresh-frac", type=float, default=0.0,
+                    help="fraction of requests carrying fresh, unregistered systems")
+    ap.add_argument("--fresh-systems", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=4)
     a = ap.parse_args()
 
@@ -54,6 +61,7 @@ def main():
     n_signals = 32
     bank = S.signal_bank(rng, n_signals, a.tmax, steps_per_sample=10)
     interp = linear_interp_matrix(10)
+    fresh = S.FreshSystems(a.fresh_systems) if a.fresh_frac > 0 else None
 
     def new_pool(steps_per_tick):
         pool = S.Pool(lanes=a.lanes, slot_config=1, checkpoint_every=256,
@@ -67,6 +75,10 @@ def main():
         arr = S.poisson_arrivals(rng, load, a.duration)
         reqs = S.RequestStream(lib, len(arr), a.seed * 1000 + li, n_signals, tmin=a.tmin,
                               tmax=a.tmax, max_systems=a.max_systems)
+        if fresh is not None:
+            reqs = S.MixedRequests(reqs, S.FreshRequestStream(
+                fresh, len(arr), a.seed * 1000 + li + 500, n_signals, tmin=a.tmin, tmax=a.tmax,
+                max_systems=a.max_systems), a.fresh_frac, seed=a.seed + li)
         pool = new_pool(a.steps_per_tick)
         rep = S.SolveServer(pool).run(reqs, horizon_s=a.duration, drain=False,
                                       max_wall_s=a.max_wall, arrivals=arr)
@@ -79,7 +91,8 @@ def main():
                           "p99_all_ms": rep.p99_all_ms, "admit_ms_mean": rep.admit_ms_mean,
                           "system_steps_per_s": rep.system_steps_per_s, "ticks": rep.ticks,
                           "gpu_busy_frac": rep.gpu_busy_frac, "wall_s": rep.duration_s,
-                          "fwd_adj_fraction": 0.7, "dtype": "f64"}), flush=True)
+                          "fwd_adj_fraction": 0.7, "fresh_fraction": a.fresh_frac,
+                          "dtype": "f64"}), flush=True)
 
     # the plain kernel run per request: one launch per step, each request alone
     rates = []

@@ -104,6 +104,38 @@ def _tens(t, order):
     return ix, v
 
 
+def chain_arrays(d: int, s: float, gv: np.ndarray, lv: np.ndarray, adjoint: bool = True,
+                 variant: int = 0) -> SystemArrays:
+    """Driver arrays of one chain of ``d`` DOF (``workloads.chain_system``,
+    sparsity variant ``variant`` of ``workloads.structure_variant``) with the
+    energy-gradient adjoint (dE(T)/dK_ii, dE(T)/dLambda_i)."""
+    M, B, K, G, L, f, q0, _, _ = W.chain_system(int(d), s, gv, lv)
+    if variant:
+        K, G = W.apply_variant(int(d), K, G, *W.structure_variant(int(d), int(variant)), s)
+    r = assemble_arrays([M], [B], [K], [G], [L], [f], [q0])
+    csr = r["csr"]
+    e = SystemArrays(n=M.shape[0], csr=(csr.data.copy(), csr.indices.astype(np.int64),
+                                        csr.indptr.astype(np.int64)),
+                     gamma=_tens(r["gamma"], 3), lam=_tens(r["lam"], 4),
+                     force=r["force"], q0=r["q0"])
+    if adjoint:
+        n = M.shape[0]
+        params = [("K", d + i, i) for i in range(d)] + \
+                 [("Lambda", t) for t in range(len(np.asarray(L.val).reshape(-1)))]
+        term = np.zeros(n)
+        term[-1] = 1.0
+        Mp, Bp, Kp, Gp, Lp, fp, lT, Pp = ADJ.system_jacobian(M, B, K, G, L, params, term)
+        ra = assemble_arrays([Mp], [Bp], [Kp], [Gp], [Lp], [fp], [lT], [Pp], adjoint=True)
+        ac = ra["csr"]
+        e.na = Mp.shape[0]
+        e.acsr = (ac.data.copy(), ac.indices.astype(np.int64), ac.indptr.astype(np.int64))
+        e.agamma = _tens(ra["gamma"], 3)
+        e.alam = _tens(ra["lam"], 4)
+        e.apsi = _tens(ra["psi"], 5)
+        e.lamT = ra["q0"]
+    return e
+
+
 class SystemLibrary:
     """Pre-assembled chain systems of each DOF size with the energy-gradient
     adjoint (dE(T)/dK_ii, dE(T)/dLambda_i), ``variants`` per size."""
@@ -118,32 +150,7 @@ class SystemLibrary:
                 s = rng.uniform(0.9, 1.1)
                 gv = rng.normal(0.0, 0.05, 2 * d)
                 lv = rng.uniform(0.0, 1.0, d)
-                M, B, K, G, L, f, q0, _, _ = W.chain_system(int(d), s, gv, lv)
-                r = assemble_arrays([M], [B], [K], [G], [L], [f], [q0])
-                csr = r["csr"]
-                e = SystemArrays(n=M.shape[0], csr=(csr.data.copy(), csr.indices.astype(np.int64),
-                                                    csr.indptr.astype(np.int64)),
-                                 gamma=_tens(r["gamma"], 3), lam=_tens(r["lam"], 4),
-                                 force=r["force"], q0=r["q0"])
-                if adjoint:
-                    n = M.shape[0]
-                    params = [("K", d + i, i) for i in range(d)] + \
-                             [("Lambda", t) for t in range(len(np.asarray(L.val).reshape(-1)))]
-                    term = np.zeros(n)
-                    term[-1] = 1.0
-                    Mp, Bp, Kp, Gp, Lp, fp, lT, Pp = ADJ.system_jacobian(M, B, K, G, L, params,
-                                                                         term)
-                    ra = assemble_arrays([Mp], [Bp], [Kp], [Gp], [Lp], [fp], [lT], [Pp],
-                                         adjoint=True)
-                    ac = ra["csr"]
-                    e.na = Mp.shape[0]
-                    e.acsr = (ac.data.copy(), ac.indices.astype(np.int64),
-                              ac.indptr.astype(np.int64))
-                    e.agamma = _tens(ra["gamma"], 3)
-                    e.alam = _tens(ra["lam"], 4)
-                    e.apsi = _tens(ra["psi"], 5)
-                    e.lamT = ra["q0"]
-                lst.append(e)
+                lst.append(chain_arrays(int(d), s, gv, lv, adjoint))
             self.entries[int(d)] = lst
         self.flat: List[SystemArrays] = [e for d in sorted(self.entries)
                                          for e in self.entries[d]]
@@ -278,6 +285,68 @@ def draw_request(lib: SystemLibrary, rng, rid: int, n_signals: int, tmin=1e3, tm
     k = np.array([keys[a][b] for a, b in zip(di.tolist(), var.tolist())], dtype=np.int64)
     return Request(rid, systems, rng.integers(0, n_signals, n), steps,
                    bool(rng.uniform() < adjoint_frac), keys=k)
+
+
+class FreshSystems:
+    """``n`` systems none of which a pool has registered: each a chain of a
+    DOF size drawn with the heterogeneous probabilities, coefficients drawn
+    afresh (K scale, Gamma, Lambda) and its sparsity one of ``variants``
+    structure variants of its size (``workloads.structure_variant``).
+    Requests of them carry the full arrays; the pool packs them into tiles."""
+
+    def __init__(self, n: int, seed: int = 9, variants: int = 12, adjoint: bool = True):
+        rng = np.random.default_rng(seed)
+        probs = np.asarray(W.DOF_PROBS, dtype=np.float64)
+        self.dofs = rng.choice(np.asarray(W.DOF_CHOICES), size=n, p=probs / probs.sum())
+        self.systems: List[SystemArrays] = []
+        for d in self.dofs.tolist():
+            v = int(rng.integers(0, W.variant_count(int(d), variants)))
+            s = rng.uniform(0.9, 1.1)
+            gv = rng.normal(0.0, 0.05, 2 * d)
+            lv = rng.uniform(0.0, 1.0, d)
+            self.systems.append(chain_arrays(int(d), s, gv, lv, adjoint, variant=v))
+
+
+class FreshRequestStream:
+    """``n`` random requests of :class:`FreshSystems` (1..max_systems systems
+    drawn uniformly from the bank, log-uniform steps, uniform signals,
+    forward+adjoint with probability ``adjoint_frac``), each drawn from its
+    own seed (seed, k) on first access."""
+
+    def __init__(self, bank: FreshSystems, n: int, seed: int, n_signals: int, tmin=1e3,
+                 tmax=1e5, max_systems: int = 256, adjoint_frac: float = 0.7):
+        self.bank, self.n, self.seed = bank, n, seed
+        self.kw = dict(n_signals=n_signals, tmin=tmin, tmax=tmax, max_systems=max_systems,
+                       adjoint_frac=adjoint_frac)
+
+    def __len__(self):
+        return self.n
+
+    def __getitem__(self, k: int) -> Request:
+        kw = self.kw
+        rng = np.random.default_rng((self.seed, int(k)))
+        m = int(rng.integers(1, kw["max_systems"] + 1))
+        pick = rng.integers(0, len(self.bank.systems), m)
+        steps = np.exp(rng.uniform(np.log(kw["tmin"]), np.log(kw["tmax"]), m)).astype(np.int64)
+        exc = rng.integers(0, kw["n_signals"], m)
+        adj = bool(rng.uniform() < kw["adjoint_frac"])
+        return Request(int(k), [self.bank.systems[i] for i in pick.tolist()], exc, steps, adj)
+
+
+class MixedRequests:
+    """Request k from ``fresh`` with probability ``fresh_frac`` (decided by
+    its own seed), from ``registered`` otherwise."""
+
+    def __init__(self, registered, fresh, fresh_frac: float, seed: int):
+        self.a, self.b = registered, fresh
+        n = min(len(registered), len(fresh))
+        self.is_fresh = np.random.default_rng(seed).uniform(size=n) < fresh_frac
+
+    def __len__(self):
+        return len(self.is_fresh)
+
+    def __getitem__(self, k: int) -> Request:
+        return self.b[k] if self.is_fresh[k] else self.a[k]
 
 
 # --------------------------------------------------------------- pool
@@ -620,8 +689,14 @@ class SolveServer:
             nonlocal admitted, blocked, admit_s
             if blocked or not queue:
                 return False
+            if reqs[queue[0]].keys is None:     # carries its arrays: one submit
+                return admit_one()
             ta = time.perf_counter()
-            take = [queue[k] for k in range(min(limit, len(queue)))]
+            take = []
+            for k in range(min(limit, len(queue))):
+                if reqs[queue[k]].keys is None:
+                    break
+                take.append(queue[k])
             batch = []
             for k in take:
                 r = reqs[k]

@@ -267,7 +267,7 @@ def variant_count(d: int, per_size: int) -> int:
     return min(per_size, 2 if d == 1 else (4 if d == 2 else per_size))
 
 
-def _apply_variant(d, K, G, keep, springs, s):
+def apply_variant(d, K, G, keep, springs, s):
     """K with the extra springs (stiffness 0.5 s each, symmetric) and the
     Gamma tensor without the nearest-neighbour couplings ``keep`` drops (the
     energy-probe entries after them stay)."""
@@ -327,7 +327,7 @@ def heterogeneous(n_systems: int = 100_000, tmin: float = 1e3, tmax: float = 1e5
             key = (d, v)
             if key not in shapes:
                 shapes[key] = structure_variant(d, v)
-            K, G = _apply_variant(d, K, G, *shapes[key], scales[s])
+            K, G = apply_variant(d, K, G, *shapes[key], scales[s])
         vM.append(M); vB.append(B); vK.append(K); vG.append(G); vL.append(L)
         vF.append(f); vQ.append(q0)
     return Workload("heterogeneous", vM, vB, vK, vG, vL, vF, vQ, exc, sr, int(steps.max()),

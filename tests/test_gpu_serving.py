@@ -367,3 +367,72 @@ def test_batch_submit_prefix_and_batch_collect_equal_single_calls():
         np.testing.assert_array_equal(r.result, q)
         if r.adjoint:
             np.testing.assert_array_equal(r.grad, g)
+
+
+def test_fresh_request_alone_equals_mix_with_registered_and_matches_oracle():
+    # requests that carry their own coefficients and unregistered sparsity
+    # structures (tile pool kernel) share the launches of registered-id
+    # requests (lane modules); the server admits both kinds from one stream
+    print("[gpu case] fresh-coefficient requests beside registered ones", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(43)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    fresh = S.FreshSystems(24, seed=5)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        # 512-lane tiles: the adjoint of a 64-DOF variant spans 384 lanes
+        pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=150, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        return pool
+
+    fs = S.FreshRequestStream(fresh, 8, 77, 6, tmin=200, tmax=1200, max_systems=10,
+                              adjoint_frac=1.0)
+    target = fs[0]
+    assert target.keys is None and target.adjoint
+    reg = [S.draw_request(lib, rng, 100 + i, 6, tmin=200, tmax=1200, max_systems=12,
+                          adjoint_frac=0.5) for i in range(4)]
+    alone = new_pool()
+    q1, g1 = _run_alone(alone, target)
+    alone.close()
+    pool = new_pool()
+    order = [(reg[0], True), (fs[1], False), (target, False), (reg[1], True), (fs[2], False),
+             (reg[2], True), (reg[3], True)]
+    jobs = [((pool.submit_ids(r) if by_id else pool.submit(r)), r) for r, by_id in order]
+    assert all(j is not None for j, _ in jobs)
+    results = {}
+    while jobs:
+        pool.tick(wait=True)
+        keep = []
+        for j, r in jobs:
+            if pool.finished(j):
+                results[r.rid] = pool.collect(j, r)
+            else:
+                keep.append((j, r))
+        jobs = keep
+    q2, g2 = results[target.rid]
+    np.testing.assert_array_equal(q1, q2)
+    np.testing.assert_array_equal(g1, g2)
+    oq, og = _oracle(target, bank, 1000, interp)
+    assert rel_err(q1, oq) < TOL64
+    assert rel_err(g1, og) < TOL64
+    oq, og = _oracle(reg[0], bank, 1000, interp)
+    rq, rg = results[reg[0].rid]
+    assert rel_err(rq, oq) < TOL64
+    if reg[0].adjoint:
+        assert rel_err(rg, og) < TOL64
+    pool.close()
+    # the server admits a mixed stream (registered ids in batches, fresh
+    # requests one submit each) and completes it
+    arr = np.sort(rng.uniform(0, 0.5, 24))
+    stream = S.MixedRequests(S.RequestStream(lib, 24, 5, 6, tmin=200, tmax=1200,
+                                             max_systems=12),
+                             S.FreshRequestStream(fresh, 24, 6, 6, tmin=200, tmax=1200,
+                                                  max_systems=12), 0.4, seed=3)
+    assert 0 < int(stream.is_fresh.sum()) < 24
+    pool = new_pool()
+    rep = S.SolveServer(pool).run(stream, arrivals=arr, max_wall_s=120)
+    pool.close()
+    assert rep.completed == rep.arrived == 24 and rep.backlog == 0

@@ -110,3 +110,49 @@ def test_second_run_starts_from_an_empty_pool():
     rep2 = server.run(reqs, arrivals=np.zeros(len(reqs)))
     assert rep2.completed == rep2.admitted == len(reqs) and rep2.backlog == 0
     assert sorted(pool.collected) == [r.rid for r in reqs]
+
+
+class FakeBatchPool(FakePool):
+    """FakePool with registered systems: id requests are admitted in
+    batches, requests carrying arrays one submit each."""
+    ids = np.arange(4)
+
+    def __init__(self, steps_per_tick, slots):
+        super().__init__(steps_per_tick, slots)
+        self.single, self.batched = [], []
+
+    def submit(self, req):
+        assert req.keys is None
+        j = super().submit(req)
+        if j is not None:
+            self.single.append(req.rid)
+        return j
+
+    def submit_batch(self, reqs):
+        out = []
+        for r in reqs:
+            assert r.keys is not None
+            j = FakePool.submit(self, r)
+            if j is None:
+                break
+            out.append(j)
+            self.batched.append(r.rid)
+        return out
+
+    def collect_batch(self, jobs, reqs):
+        for j, r in zip(jobs, reqs):
+            self.collect(j, r)
+
+
+def test_server_admits_id_batches_and_array_requests_from_one_stream():
+    rng = np.random.default_rng(8)
+    reqs = _requests(rng, 40)
+    fresh = set(int(k) for k in rng.choice(40, 13, replace=False))
+    for r in reqs:
+        r.keys = None if r.rid in fresh else np.zeros(len(r.steps), dtype=np.int64)
+    pool = FakeBatchPool(steps_per_tick=500, slots=6)
+    rep = S.SolveServer(pool).run(reqs, arrivals=np.zeros(len(reqs)))
+    assert rep.completed == rep.admitted == 40 and rep.backlog == 0
+    assert sorted(pool.single) == sorted(fresh)
+    assert sorted(pool.batched) == sorted(set(range(40)) - fresh)
+    assert sorted(pool.collected) == list(range(40))
