@@ -1649,7 +1649,7 @@ uint32_t single_wave_occupancy(const std::vector<LaneShape>& shapes,
 // than half the free device memory, the LDS share grows (fewer resident
 // waves, longer intervals) until they fit.  Returns the shortest interval.
 uint32_t single_wave_segs(std::vector<LaneShape>& shapes, const std::vector<ClassLoad>& load,
-                          size_t el, uint32_t Kreq, int dev) {
+                          size_t el, uint32_t Kreq, int dev, bool exact = false) {
   double waves = 0;
   for (size_t c = 0; c < shapes.size(); ++c)
     waves += std::ceil(load[c].inst * shapes[c].group / double(kLaneWave));
@@ -1671,9 +1671,11 @@ uint32_t single_wave_segs(std::vector<LaneShape>& shapes, const std::vector<Clas
       // checkpoint: the whole forward state
       const size_t step = size_t(lane_seg_rows(sh)) * kLaneWave * el;
       const size_t ckstep = size_t(G > 1 ? sh.gf.nl() : sh.f.n) * kLaneWave * el;
-      const size_t b = std::min<size_t>(kLaneMaxLds, std::max(budget, stage + 6 * step));
+      // K + 1 post-step states (and 3 K stage inputs for the exact adjoint)
+      const size_t per = exact ? 4 : 1;
+      const size_t b = std::min<size_t>(kLaneMaxLds, std::max(budget, stage + (6 * per) * step));
       const size_t fit = b > stage ? (b - stage) / step : 0;
-      sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+      sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? (fit - 1) / per : 0)));
       kmin = std::min(kmin, sh.seg);
       // a checkpoint of every wave's lanes every seg steps of its longest instance
       bytes += (load[c].steps_sum * G / double(kLaneWave) / sh.seg + load[c].inst * G / kLaneWave) *
@@ -1702,6 +1704,7 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   const uint32_t S = uint32_t(d->fwd.size.size());
   const double eps = std::numeric_limits<R>::epsilon();
   const bool forced_lane = d->kernel_path == PASIF_KERNEL_LANE;
+  const bool exact = adjoint && d->adj_mode != 0;
   d->lane_cls.assign(S, -1);
   // structure classes of every system
   std::map<std::vector<uint32_t>, uint32_t> cls_of_key;
@@ -1899,7 +1902,8 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   for (uint32_t c : order) {
     if (need[c] == 1) {
       one.push_back(c);
-    } else if (forced_lane || bodies + need[c] <= kLaneMultiWaveBodies) {
+    } else if (!exact && (forced_lane || bodies + need[c] <= kLaneMultiWaveBodies)) {
+      // (the exact discrete adjoint runs on single-wave kernels only)
       many.push_back(c);
       bodies += need[c];
     }
@@ -1926,13 +1930,14 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
       ld.push_back(loads[c]);
     }
     choose_groups(sh, ld, bits, device_simds(d->dev), d->lane_group, d->lane_bias);
-    if (adjoint) (void)single_wave_segs(sh, ld, el, Kreq, d->dev);
+    if (adjoint) (void)single_wave_segs(sh, ld, el, Kreq, d->dev, exact);
     double waves = 0;
     for (size_t c = 0; c < sh.size(); ++c)
       waves += std::ceil(ld[c].inst * sh[c].group / double(kLaneWave));
     for (size_t i = 0; i < one.size(); ++i) all[one[i]] = sh[i];
     LaneLayout lay;
     lay.ilp = waves <= device_simds(d->dev);
+    lay.exact = exact;
     const uint32_t by_waves =
         uint32_t(std::min(8.0, std::max(1.0, std::ceil(waves / device_simds(d->dev)))));
     const size_t m0 = mods.size();
@@ -2000,6 +2005,17 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
                               [](const LaneModule& M) { return M.cls.empty(); }),
                mods.end());
   }
+  // a module whose LDS does not fit leaves its classes to the tiles
+  for (LaneModule& M : mods) {
+    const size_t lds = lane_lds_bytes(M.shapes, bits, M.lay);
+    if (lds <= (M.lay.waves > 1 ? kMaxLdsBytes : kLaneMaxLds)) continue;
+    PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED, "lane kernel LDS %zu bytes", lds);
+    M.cls.clear();
+    M.shapes.clear();
+  }
+  mods.erase(std::remove_if(mods.begin(), mods.end(),
+                            [](const LaneModule& M) { return M.cls.empty(); }),
+             mods.end());
   if (mods.empty()) {
     d->lane_cls.assign(S, -1);
     return false;
@@ -2296,21 +2312,21 @@ void ensure_packed(pasif_driver* d, bool adjoint) {
     d->lane_cls.clear();
     d->big_sys.clear();
   }
-  // the exact discrete adjoint runs on the tile and row-parallel kernels only
+  // the exact discrete adjoint runs on single-wave lane kernels, tiles and
+  // the row-parallel path (not on pool jobs' lane modules)
   const bool exact = adjoint && d->adj_mode != 0;
-  PASIF_REQUIRE(!(exact && d->kernel_path == PASIF_KERNEL_LANE), PASIF_ERR_UNSUPPORTED,
-                "the exact discrete adjoint runs on the tile kernel (kernel 'auto' or 'tile')");
   PASIF_REQUIRE(!(d->pool && d->kernel_path == PASIF_KERNEL_BIG), PASIF_ERR_UNSUPPORTED,
                 "pool jobs run on the tile and lane kernels");
   const uint32_t S = uint32_t(d->fwd.size.size());
+  if (d->pool && exact) d->lane_cls.clear();  // the pool's lane modules run the reference recursion
   if (d->kernel_path == PASIF_KERNEL_BIG) {
     d->big_sys.assign(S, 1);
-  } else if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool && !exact) {
+  } else if (d->kernel_path != PASIF_KERNEL_TILE && !d->pool) {
     if (d->precision == 64)
       (void)lane_plan<double>(d, adjoint);
     else
       (void)lane_plan<float>(d, adjoint);
-  } else if (d->pool && !d->lane_cls.empty()) {
+  } else if (d->pool && !d->lane_cls.empty() && !exact) {
     if (d->precision == 64)
       pool_lane_parts<double>(d, adjoint);
     else
@@ -2706,7 +2722,7 @@ std::vector<LaneArgs> lane_part_args(pasif_driver* d, LanePart& P, bool adjoint,
         a.ckpt = mem.get(L.ckpt, size_t(L.ck_elems) * sizeof(R), true);
       else
         a.ckpt = mem.get(L.ckpt, size_t(nck) * L.nf * st * sizeof(R), true);
-      a.segbuf = mem.get(L.segbuf, size_t(K + 1) * L.nf * st * sizeof(R));
+      a.segbuf = mem.get(L.segbuf, lane_seg_values(K, P.lay.exact) * L.nf * st * sizeof(R));
     }
     a.coef_f = L.coef_f;
     a.coef_a = L.coef_a;

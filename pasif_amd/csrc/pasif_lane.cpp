@@ -710,14 +710,48 @@ __device__ __forceinline__ void rk4(R* q, F rhs, R h, R h2, R h6, bool active) {
   }
 }
 
+// rk4 that hands the inputs of stages 2..4 to rec(s, y) before evaluating
+// them (the exact discrete adjoint's recompute keeps them)
+template <int N, bool ALL, class K, class F, class Rec>
+__device__ __forceinline__ void rk4rec(R* q, F rhs, Rec rec, R h, R h2, R h6, bool active) {
+  R y[N], k[N], acc[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) y[r] = q[r];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s > 0) rec(s, y);
+    rhs(k, y, s);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      if (s == 0) {
+        acc[r] = k[r];
+        if (!K::sink(r)) y[r] = fmaR(h2, k[r], q[r]);
+      } else if (s == 1) {
+        acc[r] = fmaR(R(2), k[r], acc[r]);
+        if (!K::sink(r)) y[r] = fmaR(h2, k[r], q[r]);
+      } else if (s == 2) {
+        acc[r] = fmaR(R(2), k[r], acc[r]);
+        if (!K::sink(r)) y[r] = fmaR(h, k[r], q[r]);
+      } else {
+        acc[r] = acc[r] + k[r];
+        if (K::sink(r)) acc[r] = gsum<K::G>(acc[r]);
+        const R qn = fmaR(h6, acc[r], q[r]);
+        q[r] = (ALL || active) ? qn : q[r];
+      }
+    }
+  }
+}
+
 // One wave of one class through its phases; same schedule and resumable
 // state as the tile kernel.  S::G lanes integrate one instance (1: one
 // instance per lane); per-lane arrays (coefficients, states, checkpoints)
 // are indexed by the lane's column, per-instance ones (steps, excitation,
 // output bases) by the instance.  LDS: eb [STAGE][WAVE / G] staged samples
 // of the current chunk of steps, one column per instance (the next chunk is
-// prefetched into registers), then seg [K+1][NF][WAVE] (adjoint) the
-// recomputed segment.
+// prefetched into registers), then seg [K+1][NSEG][WAVE] (adjoint) the
+// recomputed segment's post-step states and, for the exact discrete
+// adjoint, seg3 [K][3][NSEG][WAVE] each step's stage inputs y2, y3, y4 (HBM
+// mirror sb likewise, by forward slot).
 template <class S>
 __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   constexpr int NF = S::NF;
@@ -744,6 +778,8 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   R* seg = smem + (STAGE + FT) * IPW;
   const bool adjoint = S::ADJ && a.adjoint != 0ull;
   const u32 K = adjoint ? (u32)a.ck_every : 1u;
+  constexpr bool EX = EXACT != 0 && S::ADJ;
+  R* seg3 = seg + (K + 1u) * NSEG * WAVE;
   const u32 W = (u32)a.interp_w;
   const u32 Lx = (u32)a.exc_len;
   const u32 SE = (u32)a.save_every;
@@ -769,6 +805,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
   // checkpoints: the wave's block [nck_w][NF][WAVE] (sized by its longest instance)
   GP(R) ck = (GP(R))a.ckpt + (adjoint ? ((GP(const u64))a.ck_off)[wave] : 0ull) + lane;
   GP(R) sb = (GP(R))a.segbuf + col;
+  GP(R) sb3 = sb + (u64)(K + 1u) * NF * st;
   GP(R) outq = (GP(R))a.out_q;
   GP(R) outm = (GP(R))a.out_max;
   GP(R) outg = (GP(R))a.out_grad;
@@ -853,6 +890,12 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
 #pragma unroll
             for (int r = 0; r < NF; ++r)
               if (S::segr(r) >= 0) seg[(m * NSEG + S::segr(r)) * WAVE + lane] = sb[((u64)m * NF + r) * st];
+          if constexpr (EX)
+            for (u32 m = 0; m < 3u * (n - s0); ++m)
+#pragma unroll
+              for (int r = 0; r < NF; ++r)
+                if (S::segr(r) >= 0)
+                  seg3[(m * NSEG + S::segr(r)) * WAVE + lane] = sb3[((u64)m * NF + r) * st];
         }
         seg_here = true;
       }
@@ -897,9 +940,23 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             for (int r = 0; r < NF; ++r) ck[((u64)(n / K) * NF + r) * WAVE] = q[r];
             next_ck += K;
           }
-          rk4<NF, ALL, S>(q, [&](R* k, const R* y, int s) {
-            S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
-          }, h, h2, h6, n < fsteps);
+          if constexpr (EX && MODE == mRec) {
+            const u32 mb = 3u * (n - s0);
+            rk4rec<NF, ALL, S>(q, [&](R* k, const R* y, int s) {
+              S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
+            }, [&](int s, const R* y) {
+              const u32 m = mb + (u32)(s - 1);
+#pragma unroll
+              for (int r = 0; r < NF; ++r) {
+                if (S::segr(r) >= 0) seg3[(m * NSEG + S::segr(r)) * WAVE + lane] = y[r];
+                if (mirror) sb3[((u64)m * NF + r) * st] = y[r];
+              }
+            }, h, h2, h6, n < fsteps);
+          } else {
+            rk4<NF, ALL, S>(q, [&](R* k, const R* y, int s) {
+              S::rhs_f(k, y, c, s == 0 ? u0 : (s == 3 ? u3 : u1));
+            }, h, h2, h6, n < fsteps);
+          }
           if constexpr (MODE == mRec) {
             const u32 m = n + 1u - s0;
 #pragma unroll
@@ -1030,6 +1087,12 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
 #pragma unroll
           for (int r = 0; r < NF; ++r)
             if (S::segr(r) >= 0) seg[(m * NSEG + S::segr(r)) * WAVE + lane] = sb[((u64)m * NF + r) * st];
+        if constexpr (EX)
+          for (u32 m = 0; m < 3u * (n - s0); ++m)
+#pragma unroll
+            for (int r = 0; r < NF; ++r)
+              if (S::segr(r) >= 0)
+                seg3[(m * NSEG + S::segr(r)) * WAVE + lane] = sb3[((u64)m * NF + r) * st];
         seg_here = true;
       }
       __builtin_amdgcn_s_waitcnt(0);
@@ -1038,6 +1101,55 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
       auto back_to = [&](auto all_t, u32 stop) {
         constexpr bool ALL = decltype(all_t)::value;
         for (; n > stop; --n) {
+          if constexpr (EX) {
+            // exact discrete adjoint of step n-1 -> n from its stage inputs
+            // y1 (the post-step state of step n-1) .. y4:
+            //   a4 = h6 lam,            mu4 = g(a4, y4)
+            //   a3 = 2 h6 lam + h mu4,  mu3 = g(a3, y3)
+            //   a2 = 2 h6 lam + h2 mu3, mu2 = g(a2, y2)
+            //   a1 = h6 lam + h2 mu2,   mu1 = g(a1, y1)
+            //   lam += ((mu4 + mu3) + mu2) + mu1
+            R y1[NX], y2[NX], y3[NX], y4[NX];
+            const u32 mb = 3u * (n - 1u - s0);
+            S::xload(y4, y1, seg3 + (mb + 2u) * (NSEG * WAVE), seg + (n - 1u - s0) * (NSEG * WAVE), lane);
+            S::xload(y3, y2, seg3 + (mb + 1u) * (NSEG * WAVE), seg3 + mb * (NSEG * WAVE), lane);
+            const R h3 = R(2) * h6;
+            R e[S::NE > 0 ? S::NE : 1];
+            R av[NA], mu[NA], ac[NA];
+#pragma unroll
+            for (int r = 0; r < NA; ++r) av[r] = h6 * lam[r];
+            S::eff_a(e, y4, c);
+            S::lin_a(mu, av, e);
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+              ac[r] = mu[r];
+              av[r] = fmaR(h, mu[r], h3 * lam[r]);
+            }
+            S::eff_a(e, y3, c);
+            S::lin_a(mu, av, e);
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+              ac[r] = ac[r] + mu[r];
+              av[r] = fmaR(h2, mu[r], h3 * lam[r]);
+            }
+            S::eff_a(e, y2, c);
+            S::lin_a(mu, av, e);
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+              ac[r] = ac[r] + mu[r];
+              av[r] = fmaR(h2, mu[r], h6 * lam[r]);
+            }
+            S::eff_a(e, y1, c);
+            S::lin_a(mu, av, e);
+            const bool act = ALL || n <= fsteps;
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+              ac[r] = ac[r] + mu[r];
+              const R ln = lam[r] + ac[r];
+              lam[r] = act ? ln : lam[r];
+            }
+            continue;
+          }
           R xc[NX], xb[NX], xp[NX];
           S::xload(xc, xp, seg + (n - s0) * (NSEG * WAVE), seg + (n - 1u - s0) * (NSEG * WAVE), lane);
 #pragma unroll
@@ -1981,7 +2093,8 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
   if (mw) dims = mw_dims(shapes, lay.waves);
   o << "#define LW " << (mw ? lay.lw : kLaneWave) << "u\n#define NYMAX " << dims.nymax
     << "u\n#define NFMAX " << dims.nfmax << "u\n#define SEGK " << lay.seg << "u\n#define SEGN "
-    << (dims.adjoint ? size_t(lay.seg + 1) * dims.nfmax * lay.lw : 0) << "u\n";
+    << (dims.adjoint ? size_t(lay.seg + 1) * dims.nfmax * lay.lw : 0) << "u\n#define EXACT "
+    << (lay.exact && !mw ? 1 : 0) << "\n";
   if (bits == 64) {
     o << "__device__ __forceinline__ R fmaR(R a, R b, R c) { return __builtin_fma(a, b, c); }\n"
          "__device__ __forceinline__ R absR(R a) { return __builtin_fabs(a); }\n"
@@ -2193,7 +2306,7 @@ size_t lane_lds_bytes(const std::vector<LaneShape>& shapes, int bits, const Lane
     const uint32_t G = s.group > 1 ? s.group : 1u;
     const uint32_t K = s.seg ? s.seg : lay.seg;
     size_t c = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G);
-    if (s.adjoint) c += size_t(K + 1) * lane_seg_rows(s) * kLaneWave;
+    if (s.adjoint) c += lane_seg_values(K, lay.exact) * lane_seg_rows(s) * kLaneWave;
     v = std::max(v, c);
   }
   return v * el;

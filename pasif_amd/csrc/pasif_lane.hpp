@@ -178,7 +178,16 @@ struct LaneLayout {
   // wave's dependency chains bound them, so they are compiled with the
   // scheduler that maximises instruction-level parallelism
   bool ilp = false;
+  // single-wave adjoint kernels: the exact discrete adjoint (the recompute
+  // keeps every step's stage inputs y2..y4 beside the post-step states)
+  bool exact = false;
 };
+// Values per lane and set-point row that a checkpoint segment of K steps
+// keeps: K + 1 post-step states, and three stage inputs per step for the
+// exact discrete adjoint.
+inline size_t lane_seg_values(uint32_t K, bool exact) {
+  return size_t(K) + 1 + (exact ? 3 * size_t(K) : 0);
+}
 
 // Split the rows of `sh` over `waves` waves (largest term cost first onto
 // the least loaded wave) and return the largest per-wave live estimate

@@ -1,7 +1,9 @@
-"""The exact discrete adjoint (``setAdjointMode("exact")``, tile kernel) on
-the GPU: against the oracle's ``discrete_adjoint_gradient`` (FP64, 1e-10 per
-output, ``oracle.parity``), bitwise invariant under the checkpoint interval
-and the solver, with uneven step counts; and against central finite
+"""The exact discrete adjoint (``setAdjointMode("exact")``, lane and tile
+kernels) on the GPU: against the oracle's ``discrete_adjoint_gradient``
+(FP64, 1e-10 per output, ``oracle.parity``), bitwise invariant under the
+checkpoint interval and the solver (the plain solver resumes every launch
+inside a recompute or reverse segment), with uneven step counts; and against
+central finite
 differences of the GPU forward at the differentiation configuration's shape
 (16-DOF chains, 20,000 steps at dt = 1e-3, interpolated band-limited
 forcing, h = 1e-6, 32 random parameters among K_ii and Lambda_i), where it
@@ -47,28 +49,29 @@ def test_exact_adjoint_matches_oracle_and_is_schedule_invariant():
     p = W.make_solver(w)
     p.setJacobian(*J.as_args())
     p.setAdjointMode("exact")
-    assert p.kernelInfo(adjoint=True)["kernel"] == "tile"
     ref = oracle_exact(w, J)
-    outs = []
-    for k in (1, 7, 64, 500):
-        p.setTuning(checkpoint_every=k)
-        g = p.getGradient()
-        r = PAR.report(g, ref.reshape(-1), J.sizes)
-        assert r.max_rel < PAR.TOL64, (k, r.as_dict())
-        outs.append(g)
-    for g in outs[1:]:
-        np.testing.assert_array_equal(g, outs[0])
-    p.setSolver("plain")
-    np.testing.assert_array_equal(p.getGradient(), outs[-1])
-    p.setSolver("persistent")
-    # the reference recursion on the same handle differs at O(dt^2)
-    p.setAdjointMode("reference")
-    gr = p.getGradient()
-    assert not np.array_equal(gr, outs[0])
-    p.setKernel("lane")
-    with pytest.raises(Exception):
+    for kernel in ("auto", "tile"):
+        p.setKernel(kernel)
+        assert p.kernelInfo(adjoint=True)["kernel"] == ("lane" if kernel == "auto" else "tile")
+        outs = []
+        for k in (1, 7, 64, 500):
+            p.setTuning(checkpoint_every=k)
+            g = p.getGradient()
+            r = PAR.report(g, ref.reshape(-1), J.sizes)
+            assert r.max_rel < PAR.TOL64, (kernel, k, r.as_dict())
+            outs.append(g)
+        for g in outs[1:]:
+            np.testing.assert_array_equal(g, outs[0])
+        p.setSolver("plain")
+        np.testing.assert_array_equal(p.getGradient(), outs[-1])
+        p.setSolver("graph")
+        np.testing.assert_array_equal(p.getGradient(), outs[-1])
+        p.setSolver("persistent")
+        # the reference recursion on the same handle differs at O(dt^2)
+        p.setAdjointMode("reference")
+        gr = p.getGradient()
+        assert not np.array_equal(gr, outs[0])
         p.setAdjointMode("exact")
-        p.getGradient()
 
 
 def _perturbed(d, s, gv, lv, which, idx, eps):
