@@ -21,6 +21,7 @@
 #include <memory>
 #include <map>
 #include <set>
+#include <thread>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -302,6 +303,18 @@ struct pasif_driver {
   std::vector<uint8_t> big_sys;
   // structure of every system in the lane plan before classes were merged
   std::vector<uint32_t> lane_struct;
+  // the last lane plan of each mode (forward, adjoint) by the signature of
+  // everything it depends on but coefficient values: a reload of new values
+  // into the same structure reuses its classes, layouts and kernels
+  struct LanePlanCache {
+    bool valid = false;
+    uint64_t sig = 0;
+    std::vector<int32_t> lane_cls;
+    std::vector<uint32_t> lane_struct;
+    std::vector<std::vector<LaneShape>> shapes;
+    std::vector<LaneLayout> lays;
+    std::vector<LaneKernel> kerns;
+  } lcache[2];
   // second stream and fork / join events when lane and tile kernels run together
   // side streams (lane modules beside each other and beside the tiles), the
   // fork event and one join event per side stream
@@ -1507,6 +1520,65 @@ struct ClassLoad {
   double inst = 0, steps_sum = 0, steps_max = 0;
 };
 
+// FNV-1a over 32-bit words
+struct Sig {
+  uint64_t h = 1469598103934665603ull;
+  void word(uint64_t w) {
+    for (int b = 0; b < 8; ++b) {
+      h ^= (w >> (8 * b)) & 0xFFu;
+      h *= 1099511628211ull;
+    }
+  }
+  // eight bytes at a time (a signature, not a checksum of the bytes' order
+  // within a word)
+  template <typename T>
+  void vec(const std::vector<T>& v) {
+    word(v.size());
+    const size_t nb = v.size() * sizeof(T);
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(v.data());
+    size_t i = 0;
+    for (; i + 8 <= nb; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, p + i, 8);
+      h = (h ^ w) * 1099511628211ull;
+      h ^= h >> 29;
+    }
+    for (; i < nb; ++i) h = (h ^ p[i]) * 1099511628211ull;
+  }
+};
+
+// What a lane plan depends on besides coefficient values and initial
+// states: the systems' structure (term indices, sizes, force pattern), the
+// instances (excitations, step counts), the settings and the device.
+uint64_t lane_plan_sig(const pasif_driver* d, bool adjoint, int bits) {
+  Sig g;
+  g.word(pack_key(const_cast<pasif_driver*>(d), adjoint));
+  g.word(uint64_t(bits));
+  g.word(uint64_t(int64_t(d->dev)));
+  g.word(lane_max_live());
+  if (const char* f = std::getenv("PASIF_JIT_FLAGS"))
+    for (const char* c = f; *c; ++c) g.word(uint64_t(uint8_t(*c)));
+  auto rows = [&](const RowTerms& rt) {
+    g.word(rt.n);
+    g.vec(rt.off);
+    g.vec(rt.size);
+    g.vec(rt.xoff);
+    for (int k = 0; k < 4; ++k) {
+      g.vec(rt.ptr[k]);
+      g.vec(rt.idx[k]);
+    }
+  };
+  rows(d->fwd);
+  g.vec(d->fwd.force);
+  if (adjoint) rows(d->adj);
+  g.word(d->mapped ? 1 : 0);
+  g.vec(d->map_exc);
+  g.vec(d->map_steps);
+  g.word(d->n_exc);
+  g.word(d->numsteps);
+  return g.h;
+}
+
 // Per-step cost of a class (coefficient slots and state values of both
 // sides), the unit of the merge decisions below.
 double class_cost(const LaneShape& sh) {
@@ -1705,6 +1777,38 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   const double eps = std::numeric_limits<R>::epsilon();
   const bool forced_lane = d->kernel_path == PASIF_KERNEL_LANE;
   const bool exact = adjoint && d->adj_mode != 0;
+  const int bits = int(sizeof(R) * 8);
+  auto& cache = d->lcache[adjoint ? 1 : 0];
+  const uint64_t sig = d->pool ? 0 : lane_plan_sig(d, adjoint, bits);
+  if (!d->pool && cache.valid && cache.sig == sig) {
+    // the same structure with new values: the cached classes, layouts and
+    // kernels, parts rebuilt with the current coefficients
+    d->lane_cls = cache.lane_cls;
+    d->lane_struct = cache.lane_struct;
+    if (cache.lays.empty()) return false;
+    pk.lshapes.clear();
+    pk.lgroups.clear();
+    pk.lmods.clear();
+    pk.lseg = 0;
+    for (size_t m = 0; m < cache.lays.size(); ++m) {
+      const auto& shapes = cache.shapes[m];
+      const LaneLayout& lk = cache.lays[m];
+      std::vector<uint32_t> klass(shapes.size());
+      std::iota(klass.begin(), klass.end(), 0u);
+      LanePart* part = lane_build<R>(d, adjoint, int(m), shapes, klass, lk);
+      part->lds = lane_lds_bytes(shapes, bits, lk);
+      if (m < cache.kerns.size()) part->kern = cache.kerns[m];
+      for (const LaneShape& x : shapes) {
+        pk.lshapes.push_back(x);
+        pk.lgroups.push_back(lk.waves > 1 ? 1u : x.group);
+      }
+      pk.lmods.push_back({shapes, lk});
+      pk.lseg = pk.lseg ? std::min(pk.lseg, lk.seg) : lk.seg;
+    }
+    pk.llay = cache.lays[0];
+    return true;
+  }
+  cache.valid = false;
   d->lane_cls.assign(S, -1);
   // structure classes of every system
   std::map<std::vector<uint32_t>, uint32_t> cls_of_key;
@@ -1729,7 +1833,6 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
   // what each class needs: 1 = one lane, w >= 2 = a workgroup of w waves, 0 = too large
   std::vector<uint32_t> need(all.size(), 0);
   std::vector<double> work(all.size(), 0.0);
-  const int bits = int(sizeof(R) * 8);
   for (size_t c = 0; c < all.size(); ++c) {
     LaneShape& sh = all[c];
     // lane_group == 1: one lane per instance only (workgroups of waves for
@@ -1889,7 +1992,18 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
     PASIF_REQUIRE(!forced_lane, PASIF_ERR_UNSUPPORTED, "lane kernel: %s", why);
     why = nullptr;
   }
-  if (order.empty()) return false;
+  if (order.empty()) {
+    if (!d->pool) {
+      cache.lane_cls = d->lane_cls;
+      cache.lane_struct = d->lane_struct;
+      cache.shapes.clear();
+      cache.lays.clear();
+      cache.kerns.clear();
+      cache.sig = sig;
+      cache.valid = true;
+    }
+    return false;
+  }
   // Modules: the classes that fit one wave per instance group, and those
   // that spread an instance over a workgroup of waves, each set split into
   // modules of at most kLaneModuleClasses classes (dealt out in order of
@@ -2018,6 +2132,15 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
              mods.end());
   if (mods.empty()) {
     d->lane_cls.assign(S, -1);
+    if (!d->pool) {
+      cache.lane_cls = d->lane_cls;
+      cache.lane_struct = d->lane_struct;
+      cache.shapes.clear();
+      cache.lays.clear();
+      cache.kerns.clear();
+      cache.sig = sig;
+      cache.valid = true;
+    }
     return false;
   }
   // lane_cls of every system before the parts are built (a part gathers its
@@ -2076,6 +2199,20 @@ bool lane_plan(pasif_driver* d, bool adjoint) {
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1)
                 .count());
   pk.llay = mods[0].lay;
+  if (!d->pool) {
+    cache.lane_cls = d->lane_cls;
+    cache.lane_struct = d->lane_struct;
+    cache.shapes.clear();
+    cache.lays.clear();
+    cache.kerns.clear();
+    for (size_t m = 0; m < mods.size(); ++m) {
+      cache.shapes.push_back(mods[m].shapes);
+      cache.lays.push_back(mods[m].lay);
+      if (d->dev >= 0) cache.kerns.push_back(pk.lparts[pk.lparts.size() - mods.size() + m]->kern);
+    }
+    cache.sig = sig;
+    cache.valid = true;
+  }
   if (d->d_compute)
     for (size_t m = 0; m < mods.size(); ++m) {
       const LanePart& P = *pk.lparts[pk.lparts.size() - mods.size() + m];
@@ -2106,7 +2243,6 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
   std::unique_ptr<LanePart> P(new LanePart());
   P->lay = lay;
   uint32_t block = 0, tmax = 0;
-  std::vector<double> cv;
   // the part's program arrays, 256-byte aligned in one host buffer (a pool
   // job reuses the pool's): one upload for the whole part.  Every class's
   // fields are placed first, the buffer is cleared once, then the fields are
@@ -2195,15 +2331,12 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     uint32_t* oa = reinterpret_cast<uint32_t*>(host.data() + f.oa);
     uint32_t* nao = reinterpret_cast<uint32_t*>(host.data() + f.na);
     std::fill(ex, ex + st, -1);
-    std::vector<double> lv;
-    // coefficient maps per original structure (a class may be the union of
-    // several)
-    std::map<uint32_t, CoefMap> map_f, map_a;
     const double eps = std::numeric_limits<R>::epsilon();
     // lane slot r of position p of a group side -> the system's row
     auto slot_row = [](const GSide& gs, uint32_t r, uint32_t p) {
       return r < gs.RL ? gs.band[r] + p * gs.stride[r] : gs.shared[r - gs.RL];
     };
+    // bounds and per-instance words, in order
     for (uint32_t i = 0; i < L->count; ++i) {
       const Instance& in = *mem[i];
       // every output index the kernel writes stays inside the output arrays
@@ -2216,42 +2349,6 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
                     PASIF_ERR_UNSUPPORTED, "lane plan: gradient index out of range");
       PASIF_REQUIRE(in.exc < int32_t(d->n_exc), PASIF_ERR_UNSUPPORTED,
                     "lane plan: excitation out of range");
-      const uint32_t sk = in.sys < d->lane_struct.size() ? d->lane_struct[in.sys] : 0u;
-      lane_coefs_mapped(d->fwd, in.sys, sh.f, false, eps, map_f[sk], cv);
-      PASIF_REQUIRE(cv.size() == sh.f.coefs(), PASIF_ERR_UNSUPPORTED,
-                    "lane plan: structure mismatch");
-      if (grp) {
-        for (uint32_t p = 0; p < G; ++p) {
-          const size_t col = size_t(i) * G + p;
-          lane_group_coefs(sh.gf, cv, p, lv);
-          for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + col] = R(lv[t]);
-          for (uint32_t r = 0; r < L->nf; ++r)
-            qf[size_t(r) * st + col] = R(d->fwd.init[d->fwd.off[in.sys] + slot_row(sh.gf, r, p)]);
-        }
-      } else {
-        for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + i] = R(cv[t]);
-        for (uint32_t r = 0; r < L->nf; ++r) qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
-      }
-      if (adjoint) {
-        lane_coefs_mapped(d->adj, in.sys, sh.a, true, eps, map_a[sk], cv);
-        PASIF_REQUIRE(cv.size() == sh.a.coefs(), PASIF_ERR_UNSUPPORTED,
-                      "lane plan: structure mismatch");
-        if (grp) {
-          for (uint32_t p = 0; p < G; ++p) {
-            const size_t col = size_t(i) * G + p;
-            lane_group_coefs(sh.ga, cv, p, lv);
-            for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + col] = R(lv[t]);
-            for (uint32_t r = 0; r < L->na; ++r) {
-              const uint32_t row = slot_row(sh.ga, r, p);
-              qa[size_t(r) * st + col] = row < own_a ? R(d->adj.init[d->adj.off[in.sys] + row]) : R(0);
-            }
-          }
-        } else {
-          for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
-          for (uint32_t r = 0; r < L->na; ++r)
-            qa[size_t(r) * st + i] = r < own_a ? R(d->adj.init[d->adj.off[in.sys] + r]) : R(0);
-        }
-      }
       stp[i] = in.steps;
       ex[i] = in.exc;
       of[i] = in.out_f;
@@ -2259,6 +2356,75 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
       nao[i] = own_a;
       L->tmax = std::max(L->tmax, in.steps);
     }
+    // coefficients and initial states of instances [i0, i1) (instances
+    // write disjoint columns: ranges run on threads of their own, each with
+    // its own coefficient maps); false on a structure mismatch
+    auto fill = [&](uint32_t i0, uint32_t i1) -> bool {
+      std::map<uint32_t, CoefMap> map_f, map_a;
+      std::vector<double> cv, lv;
+      try {
+        for (uint32_t i = i0; i < i1; ++i) {
+          const Instance& in = *mem[i];
+          const uint32_t own_a = nao[i];
+          const uint32_t sk = in.sys < d->lane_struct.size() ? d->lane_struct[in.sys] : 0u;
+          lane_coefs_mapped(d->fwd, in.sys, sh.f, false, eps, map_f[sk], cv);
+          if (cv.size() != sh.f.coefs()) return false;
+          if (grp) {
+            for (uint32_t p = 0; p < G; ++p) {
+              const size_t col = size_t(i) * G + p;
+              lane_group_coefs(sh.gf, cv, p, lv);
+              for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + col] = R(lv[t]);
+              for (uint32_t r = 0; r < L->nf; ++r)
+                qf[size_t(r) * st + col] =
+                    R(d->fwd.init[d->fwd.off[in.sys] + slot_row(sh.gf, r, p)]);
+            }
+          } else {
+            for (uint32_t t = 0; t < TF; ++t) cf[size_t(t) * st + i] = R(cv[t]);
+            for (uint32_t r = 0; r < L->nf; ++r)
+              qf[size_t(r) * st + i] = R(d->fwd.init[d->fwd.off[in.sys] + r]);
+          }
+          if (!adjoint) continue;
+          lane_coefs_mapped(d->adj, in.sys, sh.a, true, eps, map_a[sk], cv);
+          if (cv.size() != sh.a.coefs()) return false;
+          if (grp) {
+            for (uint32_t p = 0; p < G; ++p) {
+              const size_t col = size_t(i) * G + p;
+              lane_group_coefs(sh.ga, cv, p, lv);
+              for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + col] = R(lv[t]);
+              for (uint32_t r = 0; r < L->na; ++r) {
+                const uint32_t row = slot_row(sh.ga, r, p);
+                qa[size_t(r) * st + col] =
+                    row < own_a ? R(d->adj.init[d->adj.off[in.sys] + row]) : R(0);
+              }
+            }
+          } else {
+            for (uint32_t t = 0; t < TA; ++t) ca[size_t(t) * st + i] = R(cv[t]);
+            for (uint32_t r = 0; r < L->na; ++r)
+              qa[size_t(r) * st + i] = r < own_a ? R(d->adj.init[d->adj.off[in.sys] + r]) : R(0);
+          }
+        }
+      } catch (const Fail&) {
+        return false;
+      }
+      return true;
+    };
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t nth = std::min<uint32_t>(std::min(hw, 16u), (L->count + 1023) / 1024);
+    bool ok = true;
+    if (nth > 1) {
+      std::vector<std::thread> th;
+      std::vector<char> res(nth, 1);
+      for (uint32_t t = 0; t < nth; ++t) {
+        const uint32_t i0 = uint32_t(uint64_t(L->count) * t / nth);
+        const uint32_t i1 = uint32_t(uint64_t(L->count) * (t + 1) / nth);
+        th.emplace_back([&, t, i0, i1] { res[t] = fill(i0, i1) ? 1 : 0; });
+      }
+      for (std::thread& x : th) x.join();
+      for (char r : res) ok = ok && r;
+    } else {
+      ok = fill(0, L->count);
+    }
+    PASIF_REQUIRE(ok, PASIF_ERR_UNSUPPORTED, "lane plan: structure mismatch");
     tmax = std::max(tmax, L->tmax);
     if (adjoint && lay.waves == 1) {
       // wave w runs instances [w*ipw, (w+1)*ipw), the first with the most steps

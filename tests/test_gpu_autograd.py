@@ -76,3 +76,40 @@ def test_autograd_matches_oracle_and_finite_differences(mode, fd_tol):
         e[i] = h
         fd = (L(qq + e, th0) - L(qq - e, th0)) / (2 * h)
         assert abs(gq[i] - fd) <= fd_tol * abs(fd) + 1e-9, (i, gq[i], fd)
+
+
+def test_value_plan_reloads_equal_full_reloads():
+    # after the first solves, new coefficients and terminal conditions go
+    # through the value plan (no per-system objects; the driver keeps its
+    # lane plan); three steps of a descent give the same results bit for bit
+    # as the full path
+    print("[gpu case] autograd value-plan reloads vs full reloads, chains 16 x d8 x 500",
+          flush=True)
+    import torch
+    from pasif_amd.autograd import DifferentiableProblem, rk4_solve
+    w = W.chains(n_systems=16, d=8, steps=500, steps_per_sample=10)
+    d = 8
+    params = [[("K", d + i, i) for i in range(d)] + [("Lambda", t) for t in range(d)]
+              for _ in range(w.n_systems)]
+    runs = []
+    for fast in (True, False):
+        prob = DifferentiableProblem(w, params, checkpoint_every=64)
+        assert prob.plan.ok
+        prob.plan.ok = fast
+        th = torch.tensor(prob.theta0(), dtype=torch.float64, requires_grad=True)
+        q0 = torch.tensor(prob.q00(), dtype=torch.float64, requires_grad=True)
+        out = []
+        for it in range(3):
+            qT = rk4_solve(prob, q0, th)
+            (qT * qT).sum().backward()
+            out.append((qT.detach().numpy().copy(), q0.grad.numpy().copy(),
+                        th.grad.numpy().copy()))
+            with torch.no_grad():
+                th -= 1e-3 * th.grad
+                q0 -= 1e-3 * q0.grad
+            th.grad = None
+            q0.grad = None
+        runs.append(out)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
