@@ -234,6 +234,9 @@ struct Packed {
   bool big = false;
   uint32_t nbig = 0;
   DevBuf b_inst, b_rf, b_vf, b_xf, b_ff, b_if, b_ra, b_va, b_xa, b_ia;
+  DevBuf b_sgf, b_spf, b_sga, b_spa;  // segment lists and their (row, kind) ranges
+  uint64_t b_part = 0;                 // segment partial sums, all instances
+  uint32_t b_part_max = 0;             // most segments of one instance's side
   uint64_t b_vec_f = 0, b_vec_a = 0, b_ck = 0, b_seg = 0;
   uint32_t b_tmax = 0, b_every = 1;
   uint32_t max_forced = 0;
@@ -243,7 +246,8 @@ struct Packed {
                      &f_force, &f_val,   &f_idx,  &f_init,  &a_info,  &a_out, &a_src,
                      &a_steps, &a_exc,   &a_fslot, &a_force, &a_val,  &a_idx, &a_init,
                      &ck_tile, &b_inst,  &b_rf,    &b_vf,    &b_xf,    &b_ff,  &b_if,
-                     &b_ra,    &b_va,    &b_xa,    &b_ia};
+                     &b_ra,    &b_va,    &b_xa,    &b_ia,    &b_sgf,   &b_spf, &b_sga,
+                     &b_spa};
     for (DevBuf* b : all) b->release();
     lparts.clear();
     lshapes.clear();
@@ -253,6 +257,8 @@ struct Packed {
     tile = false;
     big = false;
     nbig = 0;
+    b_part = 0;
+    b_part_max = 0;
     ltmax = 0;
     valid = false;
   }
@@ -331,7 +337,7 @@ struct pasif_driver {
   // run buffers
   DevBuf qstate, lstate, mxstate, progress, out_q, out_max, traj, out_grad, ckpt, segbuf;
   // run buffers of the row-parallel path
-  DevBuf bvec_f, bvec_a, bsetp, bckpt, bseg, bprog;
+  DevBuf bvec_f, bvec_a, bsetp, bckpt, bseg, bprog, bpart;
   // results of the last solve (host, double)
   std::vector<double> res_q, res_max;
   bool have_max = false;
@@ -1120,7 +1126,7 @@ void pack_big(pasif_driver* d, bool adjoint) {
   std::vector<Instance> inst = make_instances(d, adjoint, kPartBig);
   const double eps = std::numeric_limits<R>::epsilon();
   struct Side {
-    std::vector<uint32_t> rptr, idx;
+    std::vector<uint32_t> rptr, idx, seg, sptr;
     std::vector<R> val, force, init;
   };
   Side sf, sa;
@@ -1130,18 +1136,34 @@ void pack_big(pasif_driver* d, bool adjoint) {
   for (const Instance& in : inst) tmax = std::max(tmax, in.steps);
   if (adjoint) K = std::max<uint32_t>(1, std::min<uint32_t>(K0, std::max<uint32_t>(tmax, 1)));
   std::vector<BigInst> bi(inst.size());
+  // every (row, kind) also as segments of at most kBigSegTerms terms, so a
+  // row with many terms is summed by several threads
   auto add_side = [&](const RowTerms& rt, uint32_t s, bool adj, const Instance& in, Side& out,
-                      uint64_t& rptr_off, uint64_t& term_off, uint64_t& row_off) {
+                      uint64_t& rptr_off, uint64_t& term_off, uint64_t& row_off,
+                      uint64_t& sptr_off, uint64_t& seg_off, uint32_t& nseg, uint32_t& split) {
     const uint32_t off = rt.off[s], n = rt.size[s];
     const uint32_t xo = adj ? rt.xoff[s] : off;
     rptr_off = out.rptr.size();
     term_off = out.val.size();
     row_off = out.init.size();
+    sptr_off = out.sptr.size();
+    seg_off = out.seg.size() / 3;
+    nseg = 0;
+    split = 0;
     uint32_t t = 0;
     for (uint32_t r = 0; r < n; ++r) {
       const uint32_t g = off + r;
       for (int k = 0; k < 4; ++k) {
         out.rptr.push_back(t);
+        out.sptr.push_back(nseg);
+        const uint32_t cnt = rt.ptr[k][g + 1] - rt.ptr[k][g];
+        if (cnt > kBigSegTerms) split = 1;
+        for (uint32_t b = 0; b < cnt; b += kBigSegTerms) {
+          out.seg.push_back(t + b);
+          out.seg.push_back(t + std::min(cnt, b + kBigSegTerms));
+          out.seg.push_back(uint32_t(k));
+          ++nseg;
+        }
         const int nf = k + 1;
         for (uint32_t e = rt.ptr[k][g]; e < rt.ptr[k][g + 1]; ++e, ++t) {
           out.val.push_back(R(rt.val[k][e]));
@@ -1156,11 +1178,17 @@ void pack_big(pasif_driver* d, bool adjoint) {
       out.init.push_back(R(rt.init[g]));
     }
     out.rptr.push_back(t);
+    out.sptr.push_back(nseg);
   };
+  uint64_t part = 0;
   for (size_t i = 0; i < inst.size(); ++i) {
     const Instance& in = inst[i];
     BigInst& b = bi[i];
-    add_side(d->fwd, in.sys, false, in, sf, b.rptr_f, b.term_f, b.row_f);
+    add_side(d->fwd, in.sys, false, in, sf, b.rptr_f, b.term_f, b.row_f, b.sptr_f, b.sseg_f,
+             b.nseg_f, b.split_f);
+    b.part_f = part;
+    part += b.nseg_f;
+    pk.b_part_max = std::max(pk.b_part_max, b.nseg_f);
     b.n_f = d->fwd.size[in.sys];
     b.steps = in.steps;
     b.exc = in.exc;
@@ -1168,7 +1196,11 @@ void pack_big(pasif_driver* d, bool adjoint) {
     b.scr_f = vf;
     vf += 4ull * b.n_f;
     if (adjoint) {
-      add_side(d->adj, in.sys, true, in, sa, b.rptr_a, b.term_a, b.row_a);
+      add_side(d->adj, in.sys, true, in, sa, b.rptr_a, b.term_a, b.row_a, b.sptr_a, b.sseg_a,
+               b.nseg_a, b.split_a);
+      b.part_a = part;
+      part += b.nseg_a;
+      pk.b_part_max = std::max(pk.b_part_max, b.nseg_a);
       b.n_a = d->adj.size[in.sys];
       b.out_a = in.out_a;
       b.scr_a = va;
@@ -1187,11 +1219,16 @@ void pack_big(pasif_driver* d, bool adjoint) {
   upload(d, pk.b_xf, sf.idx);
   upload(d, pk.b_ff, sf.force);
   upload(d, pk.b_if, sf.init);
+  upload(d, pk.b_sgf, sf.seg);
+  upload(d, pk.b_spf, sf.sptr);
+  pk.b_part = part;
   if (adjoint) {
     upload(d, pk.b_ra, sa.rptr);
     upload(d, pk.b_va, sa.val);
     upload(d, pk.b_xa, sa.idx);
     upload(d, pk.b_ia, sa.init);
+    upload(d, pk.b_sga, sa.seg);
+    upload(d, pk.b_spa, sa.sptr);
   }
   if (d->dev >= 0) PASIF_HIP(hipStreamSynchronize(d->stream));
   pk.big = true;
@@ -2674,10 +2711,10 @@ BigArgs<R> big_run_args(pasif_driver* d, bool adjoint, RunMem& mem, void* out_q,
   BigArgs<R> a{};
   a.inst = pk.b_inst.as<BigInst>();
   a.fwd = {pk.b_rf.as<uint32_t>(), pk.b_vf.as<R>(), pk.b_xf.as<uint32_t>(), pk.b_ff.as<R>(),
-           pk.b_if.as<R>()};
+           pk.b_if.as<R>(), pk.b_sgf.as<uint32_t>(), pk.b_spf.as<uint32_t>()};
   if (adjoint)
     a.adj = {pk.b_ra.as<uint32_t>(), pk.b_va.as<R>(), pk.b_xa.as<uint32_t>(), nullptr,
-             pk.b_ia.as<R>()};
+             pk.b_ia.as<R>(), pk.b_sga.as<uint32_t>(), pk.b_spa.as<uint32_t>()};
   a.excit = d->ext_exc ? static_cast<const R*>(d->ext_exc) : d->d_exc.as<R>();
   a.exc_len = d->exc_len;
   a.interp = d->interp_w ? (d->ext_interp ? static_cast<const R*>(d->ext_interp)
@@ -2690,6 +2727,10 @@ BigArgs<R> big_run_args(pasif_driver* d, bool adjoint, RunMem& mem, void* out_q,
   a.h6 = R(double(h) / 6.0);
   const size_t el = sizeof(R);
   a.vec_f = static_cast<R*>(mem.get(d->bvec_f, size_t(pk.b_vec_f) * el));
+  // the partial sums of one side at a time live in LDS when the largest fits
+  a.part_lds = size_t(pk.b_part_max) * el <= kBigPartLds ? 1u : 0u;
+  a.part = a.part_lds ? nullptr
+                      : static_cast<R*>(mem.get(d->bpart, size_t(std::max<uint64_t>(pk.b_part, 1)) * el));
   a.progress = static_cast<uint32_t*>(mem.get(d->bprog, size_t(pk.nbig) * 4 * sizeof(uint32_t)));
   mem.zero(a.progress, size_t(pk.nbig) * 4 * sizeof(uint32_t));
   a.out_q = static_cast<R*>(out_q);
@@ -2983,7 +3024,7 @@ size_t held_run_bytes(const pasif_driver* d) {
   const DevBuf* own[] = {&d->qstate, &d->lstate, &d->mxstate, &d->progress, &d->out_q,
                          &d->out_max, &d->traj,   &d->out_grad, &d->ckpt,  &d->segbuf,
                          &d->bvec_f, &d->bvec_a,  &d->bsetp,    &d->bckpt, &d->bseg,
-                         &d->bprog};
+                         &d->bprog,  &d->bpart};
   for (const DevBuf* x : own) b += x->bytes;
   for (const auto& lp : d->pk.lparts)
     for (const auto& c : lp->cls)
@@ -3079,7 +3120,8 @@ void solve_impl(pasif_driver* d, bool adjoint, uint32_t save_steps, std::vector<
                       k < 0 ? d->stream : d->aux[k]);
     }
     if (tile && e == hipSuccess) e = launch_tiles(ta, nt, int(pk.slot_cfg), d->stream);
-    if (big && e == hipSuccess) e = launch_big(ba, nb, d->stream);
+    if (big && e == hipSuccess)
+      e = launch_big(ba, nb, ba.part_lds ? size_t(pk.b_part_max) * sizeof(R) : 0, d->stream);
     for (int k = 0; k < nside; ++k) {
       if (e == hipSuccess) e = hipEventRecord(d->join[k], d->aux[k]);
       if (e == hipSuccess) e = hipStreamWaitEvent(d->stream, d->join[k], 0);

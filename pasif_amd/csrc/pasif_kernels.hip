@@ -718,28 +718,90 @@ __device__ __forceinline__ R big_force(const BigArgs<R>& a, int32_t exc, uint32_
   return fmaR(a.interp[W + j], ei, a.interp[j] * ei1);
 }
 
-// Right-hand side of row r: S factors from `S`, set-point factors from `X`.
+// Sum of terms [b, e) of kind k, in entry order from zero, four at a time:
+// the four terms' index words and values are requested together, then their
+// factors, so a segment of 8 terms costs four dependent memory round trips
+// instead of two per term.  One copy (not inlined into every stage).
+template <typename R>
+__device__ __noinline__ R big_terms(const R* val, const uint32_t* idx, uint32_t b, uint32_t e,
+                                    uint32_t k, const R* S, const R* X) {
+  R t = R(0);
+  for (uint32_t j0 = b; j0 < e; j0 += 4u) {
+    const uint32_t c = min(4u, e - j0);
+    uint4 ix[4];
+    R v[4], f0[4], f1[4], f2[4], f3[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+      if (j < c) {
+        ix[j] = *reinterpret_cast<const uint4*>(idx + 4u * (j0 + j));
+        v[j] = val[j0 + j];
+      }
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+      if (j < c) {
+        f0[j] = S[ix[j].x];
+        f1[j] = k >= 1u ? X[ix[j].y] : R(0);
+        f2[j] = k >= 2u ? X[ix[j].z] : R(0);
+        f3[j] = k >= 3u ? X[ix[j].w] : R(0);
+      }
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+      if (j < c) {
+        if (k == 0u) {
+          t = fmaR(v[j], f0[j], t);
+        } else {
+          R w = v[j] * f0[j];
+          if (k >= 2u) w = w * f1[j];
+          if (k >= 3u) w = w * f2[j];
+          t = fmaR(w, k == 1u ? f1[j] : (k == 2u ? f2[j] : f3[j]), t);
+        }
+      }
+  }
+  return t;
+}
+
+// Right-hand side of row r (every kind at most kBigSegTerms terms): S
+// factors from `S`, set-point factors from `X`; kinds added in order.
 template <typename R>
 __device__ __forceinline__ R big_row(const uint32_t* rp, const R* val, const uint32_t* idx,
                                      uint32_t r, const R* S, const R* X) {
   R part[4];
 #pragma unroll
+  for (int k = 0; k < 4; ++k)
+    part[k] = big_terms(val, idx, rp[4u * r + k], rp[4u * r + k + 1u], uint32_t(k), S, X);
+  return ((part[0] + part[1]) + part[2]) + part[3];
+}
+
+// Sum of the terms of segment g (one kind of one row, at most kBigSegTerms
+// terms, in entry order from zero).
+template <typename R>
+__device__ __forceinline__ R big_seg(const uint32_t* sg, const R* val, const uint32_t* idx,
+                                     uint32_t g, const R* S, const R* X) {
+  return big_terms(val, idx, sg[3u * g], sg[3u * g + 1u], sg[3u * g + 2u], S, X);
+}
+
+// Right-hand side of row r from its segments' partial sums: each kind the
+// sum of its segments in order, kinds added in order.
+template <typename R>
+__device__ __forceinline__ R big_row_parts(const uint32_t* sp, const R* part, uint32_t r) {
+  R pk[4];
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     R t = R(0);
-    const uint32_t b = rp[4u * r + k], e = rp[4u * r + k + 1u];
-    for (uint32_t q = b; q < e; ++q) {
-      const uint32_t* ix = idx + 4u * q;
-      if (k == 0) {
-        t = fmaR(val[q], S[ix[0]], t);
-      } else {
-        R v = val[q] * S[ix[0]];
-        for (int f = 1; f < k; ++f) v = v * X[ix[f]];
-        t = fmaR(v, X[ix[k]], t);
-      }
+    const uint32_t g1 = sp[4u * r + k + 1u];
+    uint32_t g = sp[4u * r + k];
+    // eight partial sums requested together, added in order
+    for (; g + 8u <= g1; g += 8u) {
+      R x[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j) x[j] = part[g + j];
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j) t = t + x[j];
     }
-    part[k] = t;
+    for (; g < g1; ++g) t = t + part[g];
+    pk[k] = t;
   }
-  return ((part[0] + part[1]) + part[2]) + part[3];
+  return ((pk[0] + pk[1]) + pk[2]) + pk[3];
 }
 
 template <typename R, bool ADJ>
@@ -754,6 +816,12 @@ __global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
   const R* vf = a.fwd.val + I.term_f;
   const uint32_t* xf = a.fwd.idx + 4u * I.term_f;
   const R* ff = a.fwd.force + I.row_f;
+  const uint32_t* sgf = a.fwd.seg + 3u * I.sseg_f;
+  const uint32_t* spf = a.fwd.sptr + I.sptr_f;
+  extern __shared__ unsigned char big_smem[];
+  R* const lds_part = reinterpret_cast<R*>(big_smem);
+  R* pf = a.part_lds ? lds_part : a.part + I.part_f;
+  const bool splitf = I.split_f != 0u;
   R* q = a.vec_f + I.scr_f;
   R* y0 = q + nf;
   R* y1 = y0 + nf;
@@ -797,8 +865,13 @@ __global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
           R* Yn = s == 1 ? y1 : y0;
           R u = R(0);
           if (I.exc >= 0) u = big_force(a, I.exc, n, s);
+          if (splitf) {
+            // rows with many terms: their segments' sums on every thread first
+            for (uint32_t g = tid; g < I.nseg_f; g += NT) pf[g] = big_seg(sgf, vf, xf, g, Y, Y);
+            __syncthreads();
+          }
           for (uint32_t r = tid; r < nf; r += NT) {
-            const R p = big_row(rpf, vf, xf, r, Y, Y);
+            const R p = splitf ? big_row_parts(spf, pf, r) : big_row(rpf, vf, xf, r, Y, Y);
             const R k = fmaR(ff[r], u, p);
             const R qr = q[r];
             if (s == 0) {
@@ -857,6 +930,10 @@ __global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
       const uint32_t* rpa = a.adj.rptr + I.rptr_a;
       const R* va = a.adj.val + I.term_a;
       const uint32_t* xa = a.adj.idx + 4u * I.term_a;
+      const uint32_t* sga = a.adj.seg + 3u * I.sseg_a;
+      const uint32_t* spa = a.adj.sptr + I.sptr_a;
+      R* pa = a.part_lds ? lds_part : a.part + I.part_a;
+      const bool splita = I.split_a != 0u;
       R* lam = a.vec_a + I.scr_a;
       R* z0 = lam + na;
       R* z1 = z0 + na;
@@ -889,8 +966,12 @@ __global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
             const R* Y = s == 0 ? lam : (s == 2 ? z1 : z0);
             R* Yn = s == 1 ? z1 : z0;
             const R* Xs = X + (s == 0 ? 0u : (s == 3 ? 2u : 1u)) * nf;
+            if (splita) {
+              for (uint32_t g = tid; g < I.nseg_a; g += NT) pa[g] = big_seg(sga, va, xa, g, Y, Xs);
+              __syncthreads();
+            }
             for (uint32_t r = tid; r < na; r += NT) {
-              const R p = big_row(rpa, va, xa, r, Y, Xs);
+              const R p = splita ? big_row_parts(spa, pa, r) : big_row(rpa, va, xa, r, Y, Xs);
               const R l = lam[r];
               if (s == 0) {
                 za[r] = p;
@@ -918,8 +999,12 @@ __global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
             const R* Y = (s & 1) ? z1 : z0;
             R* Yn = (s & 1) ? z0 : z1;
             const R* Xs = X + static_cast<uint32_t>(3 - s) * nf;
+            if (splita) {
+              for (uint32_t g = tid; g < I.nseg_a; g += NT) pa[g] = big_seg(sga, va, xa, g, Y, Xs);
+              __syncthreads();
+            }
             for (uint32_t r = tid; r < na; r += NT) {
-              const R p = big_row(rpa, va, xa, r, Y, Xs);
+              const R p = splita ? big_row_parts(spa, pa, r) : big_row(rpa, va, xa, r, Y, Xs);
               const R l = lam[r];
               if (s == 0) {
                 za[r] = p;
@@ -964,20 +1049,22 @@ __global__ void __launch_bounds__(kBigThreads) big_kernel(BigArgs<R> a) {
 }
 
 template <typename R>
-static hipError_t launch_big_impl(const BigArgs<R>& a, uint32_t ninst, hipStream_t s) {
+static hipError_t launch_big_impl(const BigArgs<R>& a, uint32_t ninst, size_t lds,
+                                  hipStream_t s) {
   if (ninst == 0u) return hipSuccess;
+  if (lds > kBigPartLds) return hipErrorInvalidValue;
   if (a.adjoint)
-    hipLaunchKernelGGL((big_kernel<R, true>), dim3(ninst), dim3(kBigThreads), 0, s, a);
+    hipLaunchKernelGGL((big_kernel<R, true>), dim3(ninst), dim3(kBigThreads), lds, s, a);
   else
-    hipLaunchKernelGGL((big_kernel<R, false>), dim3(ninst), dim3(kBigThreads), 0, s, a);
+    hipLaunchKernelGGL((big_kernel<R, false>), dim3(ninst), dim3(kBigThreads), lds, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_big(const BigArgs<double>& a, uint32_t ninst, hipStream_t s) {
-  return launch_big_impl(a, ninst, s);
+hipError_t launch_big(const BigArgs<double>& a, uint32_t ninst, size_t lds, hipStream_t s) {
+  return launch_big_impl(a, ninst, lds, s);
 }
-hipError_t launch_big(const BigArgs<float>& a, uint32_t ninst, hipStream_t s) {
-  return launch_big_impl(a, ninst, s);
+hipError_t launch_big(const BigArgs<float>& a, uint32_t ninst, size_t lds, hipStream_t s) {
+  return launch_big_impl(a, ninst, lds, s);
 }
 
 }  // namespace pasif

@@ -134,19 +134,31 @@ struct TileArgs {
 // the S factor row and idx[4e+1..k] the set-point factor rows (local rows).
 template <typename R>
 struct BigSide {
-  const uint32_t* rptr;   // [rows * 4 + 1]
+  const uint32_t* rptr;   // [rows * 4 + 1] first term of each (row, kind)
   const R* val;           // [terms]
   const uint32_t* idx;    // [terms][4]
   const R* force;         // [rows] (forward side; 0 where unforced)
   const R* init;          // [rows] initial state / terminal adjoint state
+  // segments of at most kBigSegTerms terms of one (row, kind): [segs][3]
+  // (first term, end, kind) and the first segment of each (row, kind)
+  const uint32_t* seg;
+  const uint32_t* sptr;   // [rows * 4 + 1]
 };
+
+constexpr uint32_t kBigSegTerms = 8;
+// segment partial sums kept in LDS up to this many bytes per workgroup
+constexpr size_t kBigPartLds = 64 * 1024;
 
 struct BigInst {
   uint64_t rptr_f, term_f, rptr_a, term_a;  // offsets into the packed sides
   uint64_t row_f, row_a;                     // offsets of force / init rows
   uint64_t scr_f, scr_a, ck, seg;            // offsets into the scratch arrays
+  uint64_t sptr_f, sseg_f, sptr_a, sseg_a;   // offsets of the segment lists
+  uint64_t part_f, part_a;                   // offsets of the segment partial sums
   uint32_t n_f, n_a, steps, out_f, out_a;
   int32_t exc;
+  uint32_t nseg_f, nseg_a;                   // segments of each side
+  uint32_t split_f, split_a;                 // some (row, kind) has several segments
 };
 
 template <typename R>
@@ -163,6 +175,8 @@ struct BigArgs {
   R* setp;                  // per instance [4][n_f]: set points of the reverse step
   R* ckpt;                  // per instance [ceil(steps/K)][n_f]
   R* segbuf;                // per instance [K+1][n_f] or (exact) [K][4][n_f]
+  R* part;                  // per instance: segment partial sums of each side
+  uint32_t part_lds;        // partial sums in LDS (every side's segments fit)
   uint32_t* progress;       // [inst][4]: step, phase, segment
   R* out_q;
   R* out_grad;
@@ -186,8 +200,9 @@ hipError_t launch_pool(const TileArgs<float>* jobs, const uint32_t* map, uint32_
 size_t tile_lds_bytes(size_t elem, uint32_t lanes, bool adjoint, uint32_t W, uint32_t nforced);
 // row-parallel path: one workgroup of kBigThreads per instance
 constexpr uint32_t kBigThreads = 1024;
-hipError_t launch_big(const BigArgs<double>& a, uint32_t ninst, hipStream_t s);
-hipError_t launch_big(const BigArgs<float>& a, uint32_t ninst, hipStream_t s);
+// lds: bytes of dynamic LDS for the partial sums (0 when they stay in device memory)
+hipError_t launch_big(const BigArgs<double>& a, uint32_t ninst, size_t lds, hipStream_t s);
+hipError_t launch_big(const BigArgs<float>& a, uint32_t ninst, size_t lds, hipStream_t s);
 // Size of the static device scratch array that holds the run buffers of
 // pasif_solve_noalloc, and the address of `dev`'s copy of it (looked up once
 // per device).

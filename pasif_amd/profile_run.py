@@ -27,7 +27,10 @@ def main(argv=None):
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--solver", default="persistent")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "lane"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "lane", "big"])
+    ap.add_argument("--adjoint-mode", default="reference", choices=["reference", "exact"])
+    ap.add_argument("--variants", type=int, default=1,
+                    help="heterogeneous: sparsity variants per DOF size")
     ap.add_argument("--lanes", type=int, default=0,
                     help="lanes per system in the lane kernel (0: chosen per structure)")
     ap.add_argument("--checkpoint-every", type=int, default=0,
@@ -56,14 +59,15 @@ def main(argv=None):
     elif a.workload == "chains":
         w = W.chains(n_systems=a.systems, d=a.dof, steps=a.steps)
     else:
-        w = W.heterogeneous(n_systems=a.systems, tmax=a.steps)
+        w = W.heterogeneous(n_systems=a.systems, tmax=a.steps, variants_per_size=a.variants)
     p = W.make_solver(w)
     p.setPrecision(a.precision)
     p.setSolver(a.solver)
     p.setKernel(a.kernel, a.lanes)
     if a.adjoint:
-        from .adjoint import energy_gradient_inputs
-        p.setJacobian(*energy_gradient_inputs(w).as_args())
+        from .adjoint import energy_gradient_blocks
+        p.setJacobianBlocks(energy_gradient_blocks(w))
+        p.setAdjointMode(a.adjoint_mode)
     if a.checkpoint_every:
         p.setTuning(checkpoint_every=a.checkpoint_every)
     for _ in range(a.repeat):
