@@ -52,6 +52,11 @@ def main():
                     help="fraction of requests carrying fresh, unregistered systems")
     ap.add_argument("--fresh-systems", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--group-systems", type=int, default=4096,
+                    help="most systems in one admitted group")
+    ap.add_argument("--no-group", action="store_true",
+                    help="admit one pool job per request instead of one per admission "
+                         "batch and adjoint flag")
     a = ap.parse_args()
 
     from pasif_amd import serving as S
@@ -80,8 +85,10 @@ def main():
                 fresh, len(arr), a.seed * 1000 + li + 500, n_signals, tmin=a.tmin, tmax=a.tmax,
                 max_systems=a.max_systems), a.fresh_frac, seed=a.seed + li)
         pool = new_pool(a.steps_per_tick)
-        rep = S.SolveServer(pool).run(reqs, horizon_s=a.duration, drain=False,
+        rep = S.SolveServer(pool, group=not a.no_group, group_systems=a.group_systems).run(reqs, horizon_s=a.duration, drain=False,
                                       max_wall_s=a.max_wall, arrivals=arr)
+        end_stats = pool.stats()
+        refusals = dict(pool.refusals)
         pool.close()
         print(json.dumps({"metric": "serving latency", "offered_rps": load,
                           "duration_s": a.duration, "arrived": rep.arrived,
@@ -92,6 +99,9 @@ def main():
                           "system_steps_per_s": rep.system_steps_per_s, "ticks": rep.ticks,
                           "gpu_busy_frac": rep.gpu_busy_frac, "wall_s": rep.duration_s,
                           "fwd_adj_fraction": 0.7, "fresh_fraction": a.fresh_frac,
+                          "grouped": not a.no_group, "group_systems": a.group_systems,
+                          "pool_at_stop": end_stats,
+                          "refused_groups": refusals,
                           "dtype": "f64"}), flush=True)
 
     # the plain kernel run per request: one launch per step, each request alone

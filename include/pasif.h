@@ -256,11 +256,16 @@ int pasif_pool_collect(pasif_pool* pool, pasif_driver* job, double* q, size_t ca
                        double* grad, size_t capg, size_t* leng);
 int pasif_pool_stats(const pasif_pool* pool, uint32_t* jobs, double* used_bytes,
                      double* last_kernel_ms, uint32_t* last_blocks);
-/* Slot cap: at most max_tiles tiles resident (0 = no cap; the HBM budget
- * still applies).  A submit that would exceed it returns PASIF_ERR_MEMORY
- * like an exhausted budget (retry after collecting).  A job is always
- * admitted into an empty pool. */
+/* Slot cap: at most max_tiles tiles and lane workgroups that ticks still
+ * launch (0 = no cap; the HBM budget still applies).  A lane workgroup
+ * leaves the count once the completed ticks cover the work of its longest
+ * system, a finished job's tiles once it has finished (its memory stays
+ * with it until it is collected).  A submit that would exceed the cap
+ * returns PASIF_ERR_MEMORY like an exhausted budget (retry after a tick or
+ * a collect).  A job is always admitted into an empty pool. */
 int pasif_pool_set_slots(pasif_pool* pool, uint32_t max_tiles);
+/* slots held now (tiles and lane workgroups still launched) and the cap */
+int pasif_pool_slots(const pasif_pool* pool, uint32_t* used, uint32_t* max_tiles);
 /* Register every system of `src` (a handle, host-only or not, whose system
  * -- and, optionally, adjoint -- was allocated) with the pool: ids
  * first_id .. first_id + count - 1 in the handle's system order.  Requests

@@ -183,6 +183,12 @@ struct LaneCls {
   // single-wave adjoint: checkpoint elements, each wave's block sized by its
   // longest instance (offsets in ckoff)
   uint64_t ck_elems = 0;
+  // the longest instance of each workgroup (instances are sorted by step
+  // count, descending, so this is its first; 0 past the last instance) and
+  // the workgroups a pool tick still launches: a prefix, since a workgroup
+  // whose longest instance's work the completed ticks cover has finished
+  std::vector<uint32_t> wave_t;
+  uint32_t live = 0;
   // program arrays: views into the part's one program buffer
   const void *coef_f = nullptr, *coef_a = nullptr, *init_f = nullptr, *init_a = nullptr,
              *steps = nullptr, *exc = nullptr, *out_f = nullptr, *out_a = nullptr,
@@ -2328,6 +2334,11 @@ LanePart* lane_build(pasif_driver* d, bool adjoint, int part,
     L->klass = klass[c];
     L->seg = adjoint ? ((lay.waves == 1 && sh.seg) ? sh.seg : lay.seg) : 1u;
     block += L->waves;
+    const uint32_t ipb = grp ? kLaneWave / G : unit;
+    L->wave_t.assign(L->waves, 0u);
+    for (uint32_t w = 0; w < L->waves; ++w)
+      if (size_t(w) * ipb < mem.size()) L->wave_t[w] = mem[size_t(w) * ipb]->steps;
+    L->live = L->waves;
     const size_t st = L->stride;
     const uint32_t TF = grp ? sh.gf.coefs() : sh.f.coefs();
     const uint32_t TA = adjoint ? (grp ? sh.ga.coefs() : sh.a.coefs()) : 0;
@@ -4277,22 +4288,39 @@ void pool_tick_impl(pasif_pool* P) {
   // jobs whose work the completed ticks covered leave the launches: their
   // buffers are then untouched by the GPU and may be collected while a
   // later tick runs
+  // (finished jobs hold no slots: the slot cap counts the workgroups ticks
+  // still launch)
   for (auto& j : P->jobs)
-    if (j.done >= j.work) j.finished = true;
+    if (j.done >= j.work && !j.finished) {
+      j.finished = true;
+      P->tiles_used -= j.tiles;
+      j.tiles = 0;
+    }
   for (auto& j : P->jobs) {
     if (j.finished) continue;
     pasif_driver* d = j.d;
+    // a lane workgroup of longest instance T has finished once the completed
+    // ticks cover its T (3 T with the adjoint: forward, recompute, reverse)
+    // steps of work; only the unfinished prefix of each class is launched
+    const uint64_t mult = j.adjoint ? 3u : 1u;
+    uint32_t held = d->pk.tile ? d->pk.ntiles : 0u;
     for (size_t i = 0; i < d->pk.lparts.size(); ++i) {
-      const LanePart& lp = *d->pk.lparts[i];
+      LanePart& lp = *d->pk.lparts[i];
       auto& M = P->mods[size_t(lp.module)];
-      for (const LaneArgs& a0 : d->pool_largs[i]) {
-        LaneArgs a = a0;
-        a.block_begin = M.blocks + a0.block_begin;
+      for (size_t c = 0; c < lp.cls.size(); ++c) {
+        LaneCls& L = *lp.cls[c];
+        while (L.live > 0 && mult * L.wave_t[L.live - 1] <= j.done) --L.live;
+        held += L.live;
+        if (!L.live) continue;
+        LaneArgs a = d->pool_largs[i][c];
+        a.block_begin = M.blocks;
         a.budget = P->budget;
         M.table.push_back(a);
+        M.blocks += L.live;
       }
-      M.blocks += lp.blocks;
     }
+    P->tiles_used -= j.tiles - held;
+    j.tiles = held;
     if (!d->pk.tile) continue;
     // the job's buffers already exist: re-derive its arguments without
     // clearing the progress (prepare_run only clears at admission)
@@ -4907,6 +4935,13 @@ int pasif_pool_stats(const pasif_pool* P, uint32_t* jobs, double* used_bytes,
   if (used_bytes) *used_bytes = P->used;
   if (last_kernel_ms) *last_kernel_ms = P->last_ms;
   if (last_blocks) *last_blocks = P->last_blocks;
+  return PASIF_OK;
+}
+
+int pasif_pool_slots(const pasif_pool* P, uint32_t* used, uint32_t* max_tiles) {
+  if (!P) return PASIF_ERR_ARGUMENT;
+  if (used) *used = P->tiles_used;
+  if (max_tiles) *max_tiles = P->max_tiles;
   return PASIF_OK;
 }
 

@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import time
-from collections import deque
+from collections import Counter, deque
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -52,6 +52,7 @@ _POOL_SIGS = {
                                      C.POINTER(C.c_size_t), _f64p, C.c_size_t,
                                      C.POINTER(C.c_size_t)]),
     "pasif_pool_stats": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _u32p]),
+    "pasif_pool_slots": (C.c_int, [C.c_void_p, _u32p, _u32p]),
     "pasif_pool_set_slots": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pasif_pool_register": (C.c_int, [C.c_void_p, C.c_void_p, _u32p, _u32p]),
     "pasif_pool_submit_systems": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _i32p, _u32p,
@@ -358,6 +359,13 @@ class Pool:
                  gpu: int = 0):
         self.lib = _load()
         self.ids = None
+        self.refusals = Counter()       # refused group submissions by reason
+        # slots and HBM bytes a registered system took in admitted groups, by
+        # adjoint flag (running estimates: a group predicted not to fit is
+        # not packed)
+        self._slots_per_system = {False: None, True: None}
+        self._bytes_per_system = {False: None, True: None}
+        self.hbm_budget = float(hbm_budget)
         h = C.c_void_p()
         rc = self.lib.pasif_pool_create(C.byref(h), gpu, precision, lanes, slot_config,
                                         checkpoint_every, steps_per_tick, float(hbm_budget))
@@ -519,6 +527,98 @@ class Pool:
         self._chk(rc)
         return out
 
+    def slots(self):
+        """(slots held now, slot cap): tiles and lane workgroups ticks still
+        launch; cap 0 = none."""
+        u, m = C.c_uint32(0), C.c_uint32(0)
+        self._chk(self.lib.pasif_pool_slots(self.h, C.byref(u), C.byref(m)))
+        return u.value, m.value
+
+    def group_fit(self, reqs: Sequence[Request]) -> int:
+        """How many leading requests of ``reqs`` (one adjoint flag) the free
+        slots and HBM budget are expected to take as one group, from the
+        slots and bytes per system earlier groups took (no limit before any
+        group was admitted; at least one into an empty pool)."""
+        if not reqs:
+            return 0
+        flag = bool(reqs[0].adjoint)
+        used, cap = self.slots()
+        free = float("inf")
+        per = self._slots_per_system[flag]
+        if cap and per:
+            free = (cap - used) / (1.05 * per)
+        per_b = self._bytes_per_system[flag]
+        if per_b:
+            free = min(free, (self.hbm_budget - self.stats()["used_bytes"]) / (1.05 * per_b))
+        n, tot = 0, 0
+        for r in reqs:
+            tot += len(r.steps)
+            if tot > free:
+                break
+            n += 1
+        return max(n, 1) if used == 0 else n
+
+    def submit_group(self, reqs: Sequence[Request]):
+        """Admit requests of registered systems (one adjoint flag) as one job:
+        the packer sorts all their systems by step count together, so a wave
+        holds systems of similar length instead of one request's spread.  None
+        when the slots or the HBM budget do not take them all."""
+        used0 = self.slots()[0]
+        bytes0 = self.stats()["used_bytes"]
+        ids = np.ascontiguousarray(self.ids[np.concatenate([r.keys for r in reqs])],
+                                   dtype=np.uint32)
+        exc = np.ascontiguousarray(np.concatenate([r.exc_ids for r in reqs]), dtype=np.int32)
+        steps = np.ascontiguousarray(np.concatenate([r.steps for r in reqs]), dtype=np.uint32)
+        j = C.c_void_p()
+        rc = self.lib.pasif_pool_submit_systems(
+            self.h, len(ids), ids.ctypes.data_as(_u32p), exc.ctypes.data_as(_i32p),
+            steps.ctypes.data_as(_u32p), int(reqs[0].adjoint), C.byref(j))
+        if rc == 4:
+            self.refusals[self.lib.pasif_pool_last_error(self.h).decode().split(":")[0]] += 1
+            return None
+        self._chk(rc)
+        flag = bool(reqs[0].adjoint)
+        for est, now, before in ((self._slots_per_system, self.slots()[0], used0),
+                                 (self._bytes_per_system, self.stats()["used_bytes"], bytes0)):
+            per = (now - before) / len(ids)
+            est[flag] = per if est[flag] is None else 0.7 * est[flag] + 0.3 * per
+        d = object.__new__(_Driver)
+        d._pl = self.lib
+        d._h = j
+        return d
+
+    def collect_groups(self, jobs: Sequence, groups: Sequence[Sequence[Request]]):
+        """Collect finished jobs (each a group of requests, in submission
+        order) with one C call; sets ``result`` / ``grad`` of every request."""
+        n = len(jobs)
+        if n == 0:
+            return
+        nq = np.array([sum(r.nq for r in g) for g in groups], dtype=np.int64)
+        ng = np.array([sum(r.ng for r in g) if g[0].adjoint else 0 for g in groups],
+                      dtype=np.int64)
+        q_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(nq)]), dtype=np.uint64)
+        g_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(ng)]), dtype=np.uint64)
+        q = np.empty(int(q_off[-1]))
+        g = np.empty(max(int(g_off[-1]), 1))
+        hs = (C.c_void_p * n)(*[j._h for j in jobs])
+        szp = C.POINTER(C.c_size_t)
+        rc = self.lib.pasif_pool_collect_batch(self.h, n, hs, q.ctypes.data_as(_f64p),
+                                               q_off.ctypes.data_as(szp), g.ctypes.data_as(_f64p),
+                                               g_off.ctypes.data_as(szp))
+        for j in jobs:
+            j._h = None
+        self._chk(rc)
+        for i, grp in enumerate(groups):
+            a, b = int(q_off[i]), int(g_off[i])
+            for r in grp:
+                r.result = q[a:a + r.nq]
+                a += r.nq
+                if r.adjoint:
+                    r.grad = g[b:b + r.ng]
+                    b += r.ng
+                else:
+                    r.grad = None
+
     def collect_batch(self, jobs: Sequence, reqs: Sequence[Request]):
         """Collect finished jobs with one C call (their handles are destroyed):
         sets ``result`` / ``grad`` of each request."""
@@ -613,8 +713,17 @@ class ServeReport:
 
 
 class SolveServer:
-    def __init__(self, pool: Pool):
+    """``group=True`` (default): queued requests of registered systems are
+    admitted together, as pool jobs of one adjoint flag and at most
+    ``group_systems`` systems each (``Pool.submit_group``), so their systems
+    are sorted by step count across requests and a wave's systems run
+    similar lengths; every request of a group completes when the group does.
+    ``group=False``: one job per request (``submit_batch``)."""
+
+    def __init__(self, pool: Pool, group: bool = True, group_systems: int = 4096):
         self.pool = pool
+        self.group = group
+        self.group_systems = group_systems
 
     def run(self, requests: Sequence[Request], horizon_s: Optional[float] = None,
             drain: bool = True, max_wall_s: float = 600.0,
@@ -645,7 +754,7 @@ class SolveServer:
         ticks = 0
         busy_ms = 0.0
         admitted = 0
-        blocked = False             # a submit was refused; retry after a collect
+        blocked = False             # a submit was refused; retry after a tick or a collect
         admit_s = 0.0
         by_id = self.pool.ids is not None
 
@@ -664,19 +773,27 @@ class SolveServer:
                 i += 1
 
         batched = by_id and hasattr(self.pool, "submit_batch")
+        grouped = batched and self.group and hasattr(self.pool, "submit_group")
+        # due / pending entries are (job, [requests]): one request per job, or
+        # a group of them admitted as one job
 
         def collect_pending():
             nonlocal blocked
             if not pending:
                 return
             if batched:
-                self.pool.collect_batch([j for j, _ in pending], [r for _, r in pending])
+                if hasattr(self.pool, "collect_groups"):
+                    self.pool.collect_groups([j for j, _ in pending], [m for _, m in pending])
+                else:
+                    self.pool.collect_batch([j for j, _ in pending], [m[0] for _, m in pending])
                 now = time.perf_counter() - t0
-                for _, r in pending:
-                    r.done = now
-                    done.append(r)
+                for _, m in pending:
+                    for r in m:
+                        r.done = now
+                        done.append(r)
             else:
-                for job, r in pending:
+                for job, m in pending:
+                    r = m[0]
                     r.result, r.grad = self.pool.collect(job, r)
                     r.done = time.perf_counter() - t0
                     done.append(r)
@@ -692,30 +809,79 @@ class SolveServer:
             if reqs[queue[0]].keys is None:     # carries its arrays: one submit
                 return admit_one()
             ta = time.perf_counter()
-            take = []
+            # the queued requests, each fetched once (a stream may build a new
+            # object per access)
+            take, obj = [], {}
             for k in range(min(limit, len(queue))):
-                if reqs[queue[k]].keys is None:
+                r = reqs[queue[k]]
+                if r.keys is None:
                     break
+                r.arrival = float(arr[queue[k]])
                 take.append(queue[k])
-            batch = []
-            for k in take:
-                r = reqs[k]
-                r.arrival = float(arr[k])
-                batch.append(r)
-            jobs = self.pool.submit_batch(batch)
+                obj[queue[k]] = r
+            entries = []
+            if grouped:
+                # per adjoint flag, groups of consecutive queued requests of
+                # at most group_systems systems (a single larger request is a
+                # group of its own), cut to what the pool is expected to take;
+                # a group the pool refuses is halved until it fits, and the
+                # flag's admission stops at the first cut
+                got = set()
+                fit = getattr(self.pool, "group_fit", None)
+                for flag in (False, True):
+                    members = [k for k in take if bool(obj[k].adjoint) == flag]
+                    stop = False
+                    while members and not stop:
+                        n, tot = 0, 0
+                        for k in members:
+                            tot += len(obj[k].steps)
+                            if n and tot > self.group_systems:
+                                break
+                            n += 1
+                        chunk = members[:n]
+                        if fit is not None:
+                            m = fit([obj[k] for k in chunk])
+                            stop = m < len(chunk)
+                            chunk = chunk[:m]
+                        job = None
+                        while chunk:
+                            job = self.pool.submit_group([obj[k] for k in chunk])
+                            if job is not None:
+                                break
+                            stop = True
+                            chunk = chunk[:len(chunk) // 2]
+                        if not chunk:
+                            break
+                        entries.append((job, [obj[k] for k in chunk]))
+                        got.update(chunk)
+                        members = members[len(chunk):]
+                for _ in take:
+                    queue.popleft()
+                for k in reversed(take):
+                    if k not in got:
+                        queue.appendleft(k)
+                if len(got) < len(take):
+                    blocked = True
+            else:
+                batch = [obj[k] for k in take]
+                jobs = self.pool.submit_batch(batch)
+                entries = [(j, [r]) for j, r in zip(jobs, batch)]
+                for _ in jobs:
+                    queue.popleft()
+                if len(jobs) < len(batch):
+                    blocked = True
             admit_s += time.perf_counter() - ta
             now = time.perf_counter() - t0
-            for job, r in zip(jobs, batch):
-                r.admitted = now
-                work = int(np.max(r.steps)) * (3 if r.adjoint else 1)
+            for job, members in entries:
+                work = max(int(np.max(r.steps)) for r in members) * \
+                    (3 if members[0].adjoint else 1)
                 ready = ticks + max(1, -(-work // self.pool.steps_per_tick))
-                due.setdefault(ready, []).append((job, r))
+                due.setdefault(ready, []).append((job, members))
                 nres[0] += 1
-                admitted += 1
-                queue.popleft()
-            if len(jobs) < len(batch):
-                blocked = True
-            return len(jobs) > 0
+                for r in members:
+                    r.admitted = now
+                admitted += len(members)
+            return len(entries) > 0
 
         def admit_one() -> bool:
             """Admit the oldest queued request; False if none or refused."""
@@ -734,7 +900,7 @@ class SolveServer:
             r.admitted = time.perf_counter() - t0
             work = int(np.max(r.steps)) * (3 if r.adjoint else 1)
             ready = ticks + max(1, -(-work // self.pool.steps_per_tick))
-            due.setdefault(ready, []).append((job, r))
+            due.setdefault(ready, []).append((job, [r]))
             nres[0] += 1
             admitted += 1
             queue.popleft()
@@ -764,6 +930,7 @@ class SolveServer:
                     if not admit():
                         break
                 busy_ms += self.pool.wait()
+                blocked = False             # the tick may have freed slots
                 for t in [t for t in due if t <= ticks]:
                     for job, r in due.pop(t):
                         if self.pool.finished(job):
@@ -784,8 +951,8 @@ class SolveServer:
             for job, _ in lst:
                 self.pool.drop(job)
         lat = np.array([(r.done - r.arrival) * 1e3 for r in done]) if done else np.zeros(1)
-        waiting = [(wall - r.arrival) * 1e3 for lst in due.values() for _, r in lst] + \
-            [(wall - arr[k]) * 1e3 for k in queue]
+        waiting = [(wall - r.arrival) * 1e3 for lst in due.values() for _, m in lst
+                   for r in m] + [(wall - arr[k]) * 1e3 for k in queue]
         lat_all = np.concatenate([lat, np.array(waiting)]) if waiting else lat
         steps = sum(r.system_steps for r in done)
         return ServeReport(

@@ -369,6 +369,55 @@ def test_batch_submit_prefix_and_batch_collect_equal_single_calls():
             np.testing.assert_array_equal(r.grad, g)
 
 
+def test_group_submit_equals_single_requests():
+    """Requests admitted together as one job per adjoint flag
+    (``Pool.submit_group``) and split back at ``collect_groups`` get results
+    bitwise equal to each request submitted and collected alone."""
+    print("[gpu case] serving group submit / collect", flush=True)
+    rng = np.random.default_rng(48)
+    lib = S.SystemLibrary(dofs=(1, 2, 8, 16), variants=2)
+    bank = S.signal_bank(rng, 5, 2000, steps_per_sample=10)
+    from pasif_amd.workloads import linear_interp_matrix
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=400, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        return pool
+
+    stream = S.RequestStream(lib, 14, 6, 5, tmin=100, tmax=1200, max_systems=12)
+    reqs = [stream[k] for k in range(14)]
+    groups = [[r for r in reqs if bool(r.adjoint) == f] for f in (False, True)]
+    groups = [g for g in groups if g]
+    assert len(groups) == 2
+    pool = new_pool()
+    jobs = [pool.submit_group(g) for g in groups]
+    assert all(j is not None for j in jobs)
+    # slots held fall as workgroups finish (step counts 100..1200 against
+    # 400 steps per tick), to none once both jobs have finished
+    held = [pool.slots()[0]]
+    while not all(pool.finished(j) for j in jobs):
+        pool.tick(wait=True)
+        held.append(pool.slots()[0])
+    pool.tick(wait=True)
+    held.append(pool.slots()[0])
+    assert all(b <= a for a, b in zip(held, held[1:])), held
+    assert held[0] > 0 and held[-1] == 0 and len(set(held)) > 2, held
+    pool.collect_groups(jobs, groups)
+    for r in reqs:
+        solo = new_pool()
+        j = solo.submit_ids(r)
+        while not solo.finished(j):
+            solo.tick(wait=True)
+        q, g = solo.collect(j, r)
+        np.testing.assert_array_equal(r.result, q)
+        if r.adjoint:
+            np.testing.assert_array_equal(r.grad, g)
+        else:
+            assert r.grad is None
+
+
 def test_fresh_request_alone_equals_mix_with_registered_and_matches_oracle():
     # requests that carry their own coefficients and unregistered sparsity
     # structures (tile pool kernel) share the launches of registered-id

@@ -156,3 +156,116 @@ def test_server_admits_id_batches_and_array_requests_from_one_stream():
     assert sorted(pool.single) == sorted(fresh)
     assert sorted(pool.batched) == sorted(set(range(40)) - fresh)
     assert sorted(pool.collected) == list(range(40))
+
+
+class FakeGroupPool(FakeBatchPool):
+    """FakeBatchPool that takes id requests as groups (one job per group):
+    a group is refused when the systems resident with it would pass
+    ``max_systems``."""
+
+    def __init__(self, steps_per_tick, slots, max_systems):
+        super().__init__(steps_per_tick, slots)
+        self.max_systems = max_systems
+        self.groups = []
+
+    def submit_group(self, reqs):
+        assert len({bool(r.adjoint) for r in reqs}) == 1
+        assert all(r.keys is not None for r in reqs)
+        resident = sum(sum(len(r.steps) for r in j.req) for j in self.jobs
+                       if isinstance(j.req, list))
+        if len(self.jobs) >= self.slots or \
+                resident + sum(len(r.steps) for r in reqs) > self.max_systems:
+            return None
+        work = max(int(np.max(r.steps)) for r in reqs) * (3 if reqs[0].adjoint else 1)
+        j = _Job(list(reqs), work)
+        self.jobs.append(j)
+        self.groups.append([r.rid for r in reqs])
+        return j
+
+    def collect_groups(self, jobs, groups):
+        for j, g in zip(jobs, groups):
+            members = j.req if isinstance(j.req, list) else [j.req]
+            assert j.done >= j.work and all(x is y for x, y in zip(members, g))
+            assert len(members) == len(g)
+            self.jobs.remove(j)
+            for r in g:
+                r.result = np.full(r.nq if hasattr(r, "nq") else 1, float(r.rid))
+                self.collected.append(r.rid)
+
+
+def test_server_groups_id_requests_by_adjoint_flag_and_halves_refused_groups():
+    rng = np.random.default_rng(11)
+    reqs = _requests(rng, 120)
+    fresh = set(int(k) for k in rng.choice(120, 9, replace=False))
+    for r in reqs:
+        r.keys = None if r.rid in fresh else np.zeros(len(r.steps), dtype=np.int64)
+    pool = FakeGroupPool(steps_per_tick=500, slots=6, max_systems=40)
+    rep = S.SolveServer(pool).run(reqs, arrivals=np.zeros(len(reqs)))
+    assert rep.completed == rep.admitted == 120 and rep.backlog == 0
+    assert sorted(pool.single) == sorted(fresh)
+    grouped = [k for g in pool.groups for k in g]
+    assert sorted(grouped) == sorted(set(range(120)) - fresh)
+    assert sorted(pool.collected) == list(range(120))
+    # a group never passes the system cap and holds one adjoint flag
+    for g in pool.groups:
+        assert sum(len(reqs[k].steps) for k in g) <= 40
+        assert len({reqs[k].adjoint for k in g}) == 1
+    assert any(len(g) > 1 for g in pool.groups)
+    for r in reqs:
+        if r.rid not in fresh:
+            assert r.done >= r.admitted >= r.arrival
+    # group=False keeps one job per request
+    pool2 = FakeGroupPool(steps_per_tick=500, slots=6, max_systems=40)
+    for r in reqs:
+        r.result = None
+    rep2 = S.SolveServer(pool2, group=False).run(reqs, arrivals=np.zeros(len(reqs)))
+    assert rep2.completed == 120 and pool2.groups == []
+    assert sorted(pool2.batched) == sorted(set(range(120)) - fresh)
+
+
+def test_groups_stay_within_group_systems():
+    rng = np.random.default_rng(12)
+    reqs = _requests(rng, 80)
+    for r in reqs:
+        r.keys = np.zeros(len(r.steps), dtype=np.int64)
+    pool = FakeGroupPool(steps_per_tick=500, slots=100, max_systems=10**6)
+    rep = S.SolveServer(pool, group_systems=9).run(reqs, arrivals=np.zeros(len(reqs)))
+    assert rep.completed == 80 and sorted(pool.collected) == list(range(80))
+    assert len(pool.groups) > 1
+    for g in pool.groups:
+        assert len(g) == 1 or sum(len(reqs[k].steps) for k in g) <= 9
+    # queue order is kept within each adjoint flag
+    for flag in (False, True):
+        order = [k for g in pool.groups for k in g if reqs[k].adjoint == flag]
+        assert order == sorted(order)
+
+
+class _CopyingStream:
+    """A request stream that builds a new object on every access, as
+    RequestStream does."""
+
+    def __init__(self, reqs):
+        self.reqs = reqs
+
+    def __len__(self):
+        return len(self.reqs)
+
+    def __getitem__(self, k):
+        import copy
+        return copy.copy(self.reqs[k])
+
+
+def test_latency_is_measured_from_each_arrival_with_a_copying_stream():
+    rng = np.random.default_rng(13)
+    base = _requests(rng, 30)
+    for r in base:
+        r.keys = np.zeros(len(r.steps), dtype=np.int64)
+        r.steps = np.minimum(r.steps, 400)
+    arrivals = 0.02 * np.arange(30)
+    for group in (True, False):
+        pool = FakeGroupPool(steps_per_tick=500, slots=100, max_systems=10**6)
+        rep = S.SolveServer(pool, group=group).run(_CopyingStream(base), arrivals=arrivals)
+        assert rep.completed == 30
+        # each request is served within a few ticks of its own arrival (the
+        # last arrives at 0.58 s)
+        assert max(rep.latencies_ms) < 250.0, (group, max(rep.latencies_ms))
