@@ -518,6 +518,8 @@ def serving_section(args) -> dict:
     sec = {"workload": f"Poisson {args.serving_rps:g} req/s for {args.serving_s:g} s, 1-256 "
                        f"systems per request (DOF 1..64, steps log-uniform [1e3, 1e5]), 70 % "
                        f"forward+adjoint, FP64, 200 GB HBM budget",
+           "admission": "queued requests in groups of at most 4,096 systems per adjoint flag, "
+                        "one pool job per group",
            "offered_rps": args.serving_rps, "arrived": rep.arrived, "admitted": rep.admitted,
            "completed": rep.completed, "completed_frac": rep.completed / max(rep.arrived, 1),
            "backlog": rep.backlog, "p50_ms": rep.p50_ms, "p99_ms": rep.p99_ms,
