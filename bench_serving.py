@@ -51,6 +51,10 @@ def main():
     ap.add_argument("--fresh-frac", type=float, default=0.0,
                     help="fraction of requests carrying fresh, unregistered systems")
     ap.add_argument("--fresh-systems", type=int, default=2048)
+    ap.add_argument("--fresh-structures", choices=["new", "registered"], default="new",
+                    help="fresh requests' sparsity: structures the pool never registered "
+                         "(tiles) or the registered ones with coefficients drawn afresh "
+                         "(lane modules)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--group-systems", type=int, default=4096,
                     help="most systems in one admitted group")
@@ -66,7 +70,8 @@ def main():
     n_signals = 32
     bank = S.signal_bank(rng, n_signals, a.tmax, steps_per_sample=10)
     interp = linear_interp_matrix(10)
-    fresh = S.FreshSystems(a.fresh_systems) if a.fresh_frac > 0 else None
+    fresh = (S.FreshSystems(a.fresh_systems, like=lib if a.fresh_structures == "registered"
+                            else None) if a.fresh_frac > 0 else None)
 
     def new_pool(steps_per_tick):
         pool = S.Pool(lanes=a.lanes, slot_config=1, checkpoint_every=256,
@@ -99,6 +104,7 @@ def main():
                           "system_steps_per_s": rep.system_steps_per_s, "ticks": rep.ticks,
                           "gpu_busy_frac": rep.gpu_busy_frac, "wall_s": rep.duration_s,
                           "fwd_adj_fraction": 0.7, "fresh_fraction": a.fresh_frac,
+                          "fresh_structures": a.fresh_structures if a.fresh_frac > 0 else None,
                           "grouped": not a.no_group, "group_systems": a.group_systems,
                           "pool_at_stop": end_stats,
                           "refused_groups": refusals,

@@ -244,6 +244,11 @@ int pasif_pool_set_bank(pasif_pool* pool, const double* excitations, uint32_t n_
 int pasif_pool_job_create(pasif_pool* pool, pasif_driver** job, const double* excitations,
                           uint32_t n_exc, uint32_t exc_len, uint32_t sample_rate,
                           uint32_t numsteps);
+/* Packs and admits a loaded job.  A system of a job on the pool's signal
+ * bank whose sparsity structure a lane module of the pool runs (a
+ * registered structure, any coefficient values) runs on that module; the
+ * job's other systems, and every system of a job with its own excitations,
+ * run on tiles. */
 int pasif_pool_submit(pasif_pool* pool, pasif_driver* job, int adjoint);
 int pasif_pool_tick(pasif_pool* pool, int wait, uint32_t* active_tiles);
 int pasif_pool_wait(pasif_pool* pool, double* kernel_ms);

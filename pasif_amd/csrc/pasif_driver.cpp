@@ -436,6 +436,9 @@ struct pasif_pool {
   std::vector<Module> mods;
   bool mods_built = false;
   std::map<std::vector<uint32_t>, int32_t> key_lane;
+  // forward structure alone -> a class whose forward side it is (forward-only
+  // jobs carrying their own arrays)
+  std::map<std::vector<uint32_t>, int32_t> fkey_lane;
   std::vector<int32_t> reg_lane;
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
@@ -4111,9 +4114,9 @@ void pool_build_modules(pasif_pool* P) {
   const double eps = std::numeric_limits<R>::epsilon();
   std::map<std::vector<uint32_t>, uint32_t> cls_of_key;
   std::vector<LaneShape> all;
-  std::vector<std::vector<uint32_t>> keys;
+  std::vector<std::vector<uint32_t>> keys, fkeys;
   std::vector<uint32_t> count;
-  std::vector<uint32_t> key;
+  std::vector<uint32_t> key, fkey;
   for (const auto& e : P->reg) {
     LaneShape sh;
     sh.f = lane_side(e.f, 0, false, eps);
@@ -4121,6 +4124,7 @@ void pool_build_modules(pasif_pool* P) {
     if (e.has_adj) sh.a = lane_side(e.a, 0, true, eps);
     key.clear();
     shape_key(sh.f, key);
+    const size_t nf = key.size();
     key.push_back(e.has_adj ? 1u : 0u);
     if (e.has_adj) shape_key(sh.a, key);
     auto it = cls_of_key.find(key);
@@ -4128,6 +4132,7 @@ void pool_build_modules(pasif_pool* P) {
       it = cls_of_key.emplace(key, uint32_t(all.size())).first;
       all.push_back(std::move(sh));
       keys.push_back(key);
+      fkeys.emplace_back(key.begin(), key.begin() + long(nf));
       count.push_back(0);
     }
     count[it->second] += 1;
@@ -4240,8 +4245,10 @@ void pool_build_modules(pasif_pool* P) {
       P->err = err;
       throw Fail{PASIF_ERR_DEVICE};
     }
-    for (size_t k = 0; k < order.size(); ++k)
+    for (size_t k = 0; k < order.size(); ++k) {
       P->key_lane[keys[order[k]]] = lane_code(m, uint32_t(k));
+      P->fkey_lane.emplace(fkeys[order[k]], lane_code(m, uint32_t(k)));
+    }
   }
   P->reg_lane.assign(P->reg.size(), -1);
   size_t id = 0;
@@ -4445,6 +4452,10 @@ void pool_tick_impl(pasif_pool* P) {
 
 }  // namespace
 
+namespace {
+void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint);
+}  // namespace
+
 extern "C" {
 
 int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile_lanes,
@@ -4568,6 +4579,7 @@ int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc
 int pasif_pool_submit(pasif_pool* P, pasif_driver* d, int adjoint) {
   return pool_guarded(P, [&] {
     POOL_REQUIRE(d && d->pool == P, PASIF_ERR_ARGUMENT, "job does not belong to this pool");
+    pool_match_lanes(P, d, adjoint != 0);
     admit_job(P, d, adjoint);
   });
 }
@@ -4601,6 +4613,7 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
       // registered structure now, so that no request waits for a compilation
       P->mods_built = false;
       P->key_lane.clear();
+      P->fkey_lane.clear();
       if (P->precision == 64)
         pool_build_modules<double>(P);
       else
@@ -4630,6 +4643,48 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
 namespace {
 // One request of registered systems: validated, concatenated, packed and
 // admitted (throws Fail; PASIF_ERR_MEMORY when slots or budget refuse it).
+// A job carrying its own arrays: each system whose structure a pool lane
+// module already runs (the same sparsity, any coefficient values) joins that
+// module; the others, and every system of a job with its own excitations
+// (the modules read the pool's bank), run on tiles.
+void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
+  d->lane_cls.clear();
+  if (!d->sys_ready || (adjoint && !d->adj_ready) || d->ext_exc != P->bank.p || P->reg.empty())
+    return;
+  if (!P->mods_built) {
+    if (P->precision == 64)
+      pool_build_modules<double>(P);
+    else
+      pool_build_modules<float>(P);
+  }
+  if (P->key_lane.empty()) return;
+  const double eps = P->precision == 64 ? std::numeric_limits<double>::epsilon()
+                                        : double(std::numeric_limits<float>::epsilon());
+  const uint32_t S = uint32_t(d->fwd.size.size());
+  std::vector<int32_t> cls(S, -1);
+  std::vector<uint32_t> key;
+  bool any = false;
+  for (uint32_t s = 0; s < S; ++s) {
+    const RowTerms f = slice_block(d->fwd, s, false);
+    key.clear();
+    shape_key(lane_side(f, 0, false, eps), key);
+    const std::map<std::vector<uint32_t>, int32_t>* m = &P->fkey_lane;
+    if (adjoint) {
+      RowTerms a = slice_block(d->adj, s, true);
+      a.nset = f.n;
+      key.push_back(1u);
+      shape_key(lane_side(a, 0, true, eps), key);
+      m = &P->key_lane;
+    }
+    const auto it = m->find(key);
+    if (it != m->end()) {
+      cls[s] = it->second;
+      any = true;
+    }
+  }
+  if (any) d->lane_cls = std::move(cls);
+}
+
 pasif_driver* pool_submit_one(pasif_pool* P, uint32_t n, const uint32_t* ids,
                               const int32_t* exc_of_system, const uint32_t* steps_of_system,
                               int adjoint) {

@@ -160,21 +160,28 @@ class SystemLibrary:
 
 
 def _cat_csr(parts, offs):
+    """Block-diagonal concatenation of CSR parts (data, indices, indptr), the
+    column indices of part k shifted by offs[k]."""
     data = np.concatenate([p[0] for p in parts])
-    ind = np.concatenate([p[1] + o for p, o in zip(parts, offs)])
-    nnz = np.cumsum([0] + [len(p[0]) for p in parts])
-    ptr = np.concatenate([[0]] + [p[2][1:] + b for p, b in zip(parts, nnz[:-1])])
+    nnz = np.fromiter((len(p[0]) for p in parts), dtype=np.int64, count=len(parts))
+    ind = np.concatenate([p[1] for p in parts]).astype(np.int64, copy=False)
+    ind = ind + np.repeat(np.asarray(offs, dtype=np.int64), nnz)
+    rows = np.fromiter((len(p[2]) - 1 for p in parts), dtype=np.int64, count=len(parts))
+    base = np.concatenate([[0], np.cumsum(nnz)[:-1]])
+    ptr = np.concatenate([[0]] + [p[2][1:] for p in parts]).astype(np.int64, copy=False)
+    ptr[1:] += np.repeat(base, rows)
     return data, ind, ptr
 
 
 def _cat_tens(parts, offs_a, offs_b, n_a_axes):
     """indices: the first n_a_axes axes offset by offs_a, the rest by offs_b"""
-    idx, val = [], []
-    for (ix, v), oa, ob in zip(parts, offs_a, offs_b):
-        o = np.array([oa] * n_a_axes + [ob] * (ix.shape[1] - n_a_axes), dtype=np.int64)
-        idx.append(ix + o[None, :])
-        val.append(v)
-    return np.concatenate(idx, axis=0), np.concatenate(val)
+    cnt = np.fromiter((len(v) for _, v in parts), dtype=np.int64, count=len(parts))
+    idx = np.concatenate([ix for ix, _ in parts], axis=0).astype(np.int64)
+    val = np.concatenate([v for _, v in parts])
+    if idx.shape[0]:
+        idx[:, :n_a_axes] += np.repeat(np.asarray(offs_a, dtype=np.int64), cnt)[:, None]
+        idx[:, n_a_axes:] += np.repeat(np.asarray(offs_b, dtype=np.int64), cnt)[:, None]
+    return idx, val
 
 
 @dataclass
@@ -293,15 +300,25 @@ class FreshSystems:
     DOF size drawn with the heterogeneous probabilities, coefficients drawn
     afresh (K scale, Gamma, Lambda) and its sparsity one of ``variants``
     structure variants of its size (``workloads.structure_variant``).
-    Requests of them carry the full arrays; the pool packs them into tiles."""
+    Requests of them carry the full arrays; the pool packs them into tiles.
 
-    def __init__(self, n: int, seed: int = 9, variants: int = 12, adjoint: bool = True):
+    ``like``: a :class:`SystemLibrary` whose sparsity structures the systems
+    keep (its DOF sizes, the base structure of each), with coefficients
+    drawn afresh: the pool recognises the structures and runs such systems on
+    its lane modules."""
+
+    def __init__(self, n: int, seed: int = 9, variants: int = 12, adjoint: bool = True,
+                 like: Optional["SystemLibrary"] = None):
         rng = np.random.default_rng(seed)
+        choices = np.asarray(W.DOF_CHOICES)
         probs = np.asarray(W.DOF_PROBS, dtype=np.float64)
-        self.dofs = rng.choice(np.asarray(W.DOF_CHOICES), size=n, p=probs / probs.sum())
+        if like is not None:
+            keep = np.isin(choices, np.array(sorted(like.entries)))
+            choices, probs = choices[keep], probs[keep]
+        self.dofs = rng.choice(choices, size=n, p=probs / probs.sum())
         self.systems: List[SystemArrays] = []
         for d in self.dofs.tolist():
-            v = int(rng.integers(0, W.variant_count(int(d), variants)))
+            v = 0 if like is not None else int(rng.integers(0, W.variant_count(int(d), variants)))
             s = rng.uniform(0.9, 1.1)
             gv = rng.normal(0.0, 0.05, 2 * d)
             lv = rng.uniform(0.0, 1.0, d)

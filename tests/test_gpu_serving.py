@@ -485,3 +485,67 @@ def test_fresh_request_alone_equals_mix_with_registered_and_matches_oracle():
     rep = S.SolveServer(pool).run(stream, arrivals=arr, max_wall_s=120)
     pool.close()
     assert rep.completed == rep.arrived == 24 and rep.backlog == 0
+
+
+def test_fresh_coefficients_on_registered_structures_run_on_lane_modules():
+    """A request carrying its own arrays whose sparsity structures the pool
+    has registered (coefficients drawn afresh) runs on the pool's lane
+    modules; its results match the oracle and are bitwise the same alone and
+    beside requests of unregistered structures (tiles) and registered ids."""
+    print("[gpu case] fresh coefficients on registered structures", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(49)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    same = S.FreshSystems(12, seed=8, like=lib)
+    new = S.FreshSystems(24, seed=5)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=150, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        return pool
+
+    fs = S.FreshRequestStream(same, 6, 79, 6, tmin=200, tmax=1200, max_systems=10,
+                              adjoint_frac=0.5)
+    reqs = [fs[k] for k in range(6)]
+    assert any(r.adjoint for r in reqs) and not all(r.adjoint for r in reqs)
+    alone = {}
+    for r in reqs:
+        pool = new_pool()
+        job = pool.submit(r)
+        assert job._kernelInfo(r.adjoint)["kernel"] == "lane", r.rid
+        while not pool.finished(job):
+            pool.tick(wait=True)
+        alone[r.rid] = pool.collect(job, r)
+        pool.close()
+        oq, og = _oracle(r, bank, 1000, interp)
+        assert rel_err(alone[r.rid][0], oq) < TOL64, r.rid
+        if r.adjoint:
+            assert rel_err(alone[r.rid][1], og) < TOL64, r.rid
+    other = S.FreshRequestStream(new, 2, 80, 6, tmin=200, tmax=1200, max_systems=10,
+                                 adjoint_frac=1.0)
+    ids = [S.draw_request(lib, rng, 200 + i, 6, tmin=200, tmax=1200, max_systems=12)
+           for i in range(2)]
+    pool = new_pool()
+    jobs = [(pool.submit(other[0]), other[0]), (pool.submit_ids(ids[0]), ids[0])]
+    jobs += [(pool.submit(r), r) for r in reqs]
+    jobs += [(pool.submit(other[1]), other[1]), (pool.submit_ids(ids[1]), ids[1])]
+    assert all(j is not None for j, _ in jobs)
+    got = {}
+    while jobs:
+        pool.tick(wait=True)
+        keep = []
+        for j, r in jobs:
+            if pool.finished(j):
+                got[id(r)] = pool.collect(j, r)
+            else:
+                keep.append((j, r))
+        jobs = keep
+    pool.close()
+    for r in reqs:
+        q, g = got[id(r)]
+        np.testing.assert_array_equal(q, alone[r.rid][0])
+        if r.adjoint:
+            np.testing.assert_array_equal(g, alone[r.rid][1])
