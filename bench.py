@@ -26,7 +26,8 @@ Beside it, in the same line:
   forward and forward + adjoint, persistent vs plain, parity on a stratified
   sample;
 * ``serving``: Poisson requests at 1,000 per second for 60 s against the
-  continuous-batching pool, p50 / p99 latency, backlog, system-steps/s.
+  continuous-batching pool, p50 / p99 latency, backlog, system-steps/s, and
+  the overload rows (5,000 and 20,000 per second over 20 s each).
 
 The parity metric is ``oracle.parity.rel_err``: per output
 ``|g - o| / max(|o|, 1e-4 max_s |o|)``, the floor over the same system's
@@ -526,6 +527,25 @@ def serving_section(args) -> dict:
            "p99_all_arrived_ms": rep.p99_all_ms, "system_steps_per_s": rep.system_steps_per_s,
            "gpu_busy_frac": rep.gpu_busy_frac, "admit_ms_mean": rep.admit_ms_mean,
            "wall_s": rep.duration_s}
+    # overload rows: the same stream at higher rates over shorter windows
+    # (the pool saturates; latency is then the FIFO's wait)
+    over = []
+    for li, load in enumerate(args.serving_overload):
+        arr = S.poisson_arrivals(rng, load, args.serving_overload_s)
+        reqs = S.RequestStream(lib, len(arr), 4001 + li, n_signals, tmin=1e3, tmax=1e5,
+                               max_systems=256)
+        pool = new_pool(5000)
+        rep = S.SolveServer(pool).run(reqs, horizon_s=args.serving_overload_s, drain=False,
+                                      max_wall_s=args.serving_overload_s + 15, arrivals=arr)
+        pool.close()
+        over.append({"offered_rps": load, "window_s": args.serving_overload_s,
+                     "arrived": rep.arrived, "admitted": rep.admitted,
+                     "completed": rep.completed, "backlog": rep.backlog,
+                     "p50_ms": rep.p50_ms, "p99_ms": rep.p99_ms,
+                     "p99_all_arrived_ms": rep.p99_all_ms,
+                     "system_steps_per_s": rep.system_steps_per_s,
+                     "gpu_busy_frac": rep.gpu_busy_frac})
+    sec["overload"] = over
     rates = []
     for k in range(2):
         r = S.draw_request(lib, rng, 10**6 + k, n_signals, tmin=1e3, tmax=3e3, max_systems=256)
@@ -564,6 +584,8 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=100_000)
     ap.add_argument("--serving-rps", type=float, default=1000.0)
     ap.add_argument("--serving-s", type=float, default=60.0)
+    ap.add_argument("--serving-overload", type=float, nargs="*", default=[5000.0, 20000.0])
+    ap.add_argument("--serving-overload-s", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: systems are independent and the path needs no "
