@@ -58,6 +58,11 @@ def main():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--group-systems", type=int, default=4096,
                     help="most systems in one admitted group")
+    ap.add_argument("--compile-new", action="store_true",
+                    help="compile structures no lane module runs into further modules in "
+                         "the background (pasif_pool_set_compile)")
+    ap.add_argument("--warm-rerun", action="store_true",
+                    help="with --compile-new: a second window on the same pool")
     ap.add_argument("--no-group", action="store_true",
                     help="admit one pool job per request instead of one per admission "
                          "batch and adjoint flag")
@@ -79,22 +84,33 @@ def main():
         pool.set_bank(bank, 1000, interp)
         pool.register_library(lib)
         pool.set_slots(a.slots)
+        if a.compile_new:
+            pool.set_compile(True)
         return pool
 
-    for li, load in enumerate(a.loads):
+    def window(pool, load, li, state):
         arr = S.poisson_arrivals(rng, load, a.duration)
         reqs = S.RequestStream(lib, len(arr), a.seed * 1000 + li, n_signals, tmin=a.tmin,
-                              tmax=a.tmax, max_systems=a.max_systems)
+                               tmax=a.tmax, max_systems=a.max_systems)
         if fresh is not None:
             reqs = S.MixedRequests(reqs, S.FreshRequestStream(
                 fresh, len(arr), a.seed * 1000 + li + 500, n_signals, tmin=a.tmin, tmax=a.tmax,
                 max_systems=a.max_systems), a.fresh_frac, seed=a.seed + li)
-        pool = new_pool(a.steps_per_tick)
-        rep = S.SolveServer(pool, group=not a.no_group, group_systems=a.group_systems).run(reqs, horizon_s=a.duration, drain=False,
-                                      max_wall_s=a.max_wall, arrivals=arr)
+        rep = S.SolveServer(pool, group=not a.no_group, group_systems=a.group_systems).run(
+            reqs, horizon_s=a.duration, drain=False, max_wall_s=a.max_wall, arrivals=arr)
         end_stats = pool.stats()
+        if a.compile_new:
+            t0 = time.perf_counter()
+            while True:
+                built, failed, queued = pool.compile_wait(1)
+                print(f"[serving] modules compiled {built}, failed {failed}, "
+                      f"structures queued {queued}", file=sys.stderr, flush=True)
+                if not queued:
+                    break
+            end_stats["modules_built"], end_stats["builds_failed"] = built, failed
+            end_stats["compile_wait_s"] = time.perf_counter() - t0
         refusals = dict(pool.refusals)
-        pool.close()
+        pool.refusals.clear()
         print(json.dumps({"metric": "serving latency", "offered_rps": load,
                           "duration_s": a.duration, "arrived": rep.arrived,
                           "admitted": rep.admitted, "completed": rep.completed,
@@ -105,10 +121,19 @@ def main():
                           "gpu_busy_frac": rep.gpu_busy_frac, "wall_s": rep.duration_s,
                           "fwd_adj_fraction": 0.7, "fresh_fraction": a.fresh_frac,
                           "fresh_structures": a.fresh_structures if a.fresh_frac > 0 else None,
+                          "compile_new": state,
                           "grouped": not a.no_group, "group_systems": a.group_systems,
                           "pool_at_stop": end_stats,
                           "refused_groups": refusals,
                           "dtype": "f64"}), flush=True)
+
+    for li, load in enumerate(a.loads):
+        pool = new_pool(a.steps_per_tick)
+        window(pool, load, li, "cold start" if a.compile_new else None)
+        if a.compile_new and a.warm_rerun:
+            # the same pool again: the structures of the first window compiled
+            window(pool, load, li + 100, "after the first window")
+        pool.close()
 
     # the plain kernel run per request: one launch per step, each request alone
     rates = []

@@ -271,6 +271,18 @@ int pasif_pool_stats(const pasif_pool* pool, uint32_t* jobs, double* used_bytes,
 int pasif_pool_set_slots(pasif_pool* pool, uint32_t max_tiles);
 /* slots held now (tiles and lane workgroups still launched) and the cap */
 int pasif_pool_slots(const pasif_pool* pool, uint32_t* used, uint32_t* max_tiles);
+/* on != 0: structures of jobs carrying their own arrays (on the signal bank)
+ * that no lane module runs are compiled into further lane modules on a
+ * thread of the pool's own, a few per module; a finished module is taken
+ * in by the next submit or tick, and later jobs of those structures run on
+ * it (jobs admitted before stay on tiles).  Off by default. */
+int pasif_pool_set_compile(pasif_pool* pool, int on);
+/* Waits for at most max_builds module compilations (0: until every queued
+ * structure has its module or failed to compile; those stay on tiles), each
+ * taken in when done: modules built and failed so far, structures still
+ * queued or compiling. */
+int pasif_pool_compile_wait(pasif_pool* pool, uint32_t max_builds, uint32_t* modules,
+                            uint32_t* failed, uint32_t* queued);
 /* Register every system of `src` (a handle, host-only or not, whose system
  * -- and, optionally, adjoint -- was allocated) with the pool: ids
  * first_id .. first_id + count - 1 in the handle's system order.  Requests

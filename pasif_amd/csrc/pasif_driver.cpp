@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -440,8 +441,32 @@ struct pasif_pool {
   // jobs carrying their own arrays)
   std::map<std::vector<uint32_t>, int32_t> fkey_lane;
   std::vector<int32_t> reg_lane;
-  hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  // Structures of array-carrying jobs that no module runs (compile_new set):
+  // queued, then compiled into further single-wave modules on a thread of
+  // their own, a batch at a time; a finished batch is installed by the next
+  // submit or tick, and later jobs of those structures run on it.
+  struct NewClass {
+    LaneShape sh;
+    std::vector<uint32_t> key, fkey;
+  };
+  struct Build {
+    std::thread th;
+    std::atomic<bool> done{false};
+    bool ok = false;
+    std::string src, err;
+    std::vector<char> code;
+    Module M;
+    std::vector<NewClass> cls;
+  };
+  bool compile_new = false;
+  std::vector<NewClass> pending;
+  std::set<std::vector<uint32_t>> queued;  // keys pending, compiling or tried
+  std::unique_ptr<Build> build;
+  uint32_t built_modules = 0, failed_builds = 0;
+  // one stream (and join event) per lane module
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
   // pinned staging of batched collects (grows, freed with the pool)
   unsigned char* pinned = nullptr;
   size_t pinned_bytes = 0;
@@ -4103,6 +4128,63 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
   P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false, tiles});
 }
 
+// How a class runs on the pool's lane modules: 1 = in one wave (its group
+// size chosen here: the shortest step among the sizes within `slack` x the
+// least work, default 2, since a tick lasts `budget` steps of the slowest
+// class and a request waits for its slowest system; PASIF_POOL_WORK_SLACK=1
+// takes the least work, throughput over latency), w >= 2 = over the w waves
+// of a workgroup, 0 = on tiles.
+uint32_t pool_class_fit(LaneShape& sh, int bits) {
+  const auto opt = group_options(sh, bits);
+  if (!opt.empty()) {
+    double slack = 2.0;
+    if (const char* e = std::getenv("PASIF_POOL_WORK_SLACK")) {
+      const double v = std::strtod(e, nullptr);
+      if (v >= 1.0 && v <= 16.0) slack = v;
+    }
+    double wmin = 0;
+    for (size_t k = 0; k < opt.size(); ++k) {
+      const double wk = opt[k].second * opt[k].first;
+      if (k == 0 || wk < wmin) wmin = wk;
+    }
+    size_t b = opt.size();
+    for (size_t k = 0; k < opt.size(); ++k)
+      if (opt[k].second * opt[k].first <= slack * wmin &&
+          (b == opt.size() || opt[k].second < opt[b].second))
+        b = k;
+    (void)lane_group(sh, opt[b].first);
+    return 1;
+  }
+  for (uint32_t w = 2; w <= kLaneMaxWaves; ++w)
+    if (lane_partition(sh, w) <= kLaneMaxWaveLive) return w;
+  return 0;
+}
+
+// Layout of a single-wave pool module: a fixed LDS share per wave (eight
+// waves per CU resident) sets each class's checkpoint interval; registers
+// for at most four waves per SIMD.
+LaneLayout pool_single_layout(std::vector<LaneShape>& shapes, uint32_t Kreq, size_t el) {
+  LaneLayout lay;
+  uint32_t live = 0;
+  bool adj = false;
+  lay.seg = Kreq;
+  for (LaneShape& sh : shapes) {
+    live = std::max(live, lane_live(sh));
+    if (!sh.adjoint) continue;
+    adj = true;
+    const uint32_t G = sh.group > 1 ? sh.group : 1u;
+    const size_t stage = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G) * el;
+    const size_t step = size_t(lane_seg_rows(sh)) * kLaneWave * el;
+    const size_t b = std::min<size_t>(kLaneMaxLds, std::max<size_t>(kMaxLdsBytes / 8, stage + 6 * step));
+    const size_t fit = b > stage ? (b - stage) / step : 0;
+    sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
+    lay.seg = std::min(lay.seg, sh.seg);
+  }
+  if (!adj) lay.seg = 1;
+  lay.occ = std::min<uint32_t>(4, std::max<uint32_t>(1, 512 / (live * uint32_t(el * 8 / 32) + 48)));
+  return lay;
+}
+
 // Lane modules over the registered structures: module 0 runs the classes
 // that fit one lane or a group of lanes (each at its cheapest group size:
 // requests arrive continuously, so throughput, not one request's chain of
@@ -4139,35 +4221,7 @@ void pool_build_modules(pasif_pool* P) {
   }
   std::vector<uint32_t> need(all.size(), 0);
   const int bits = int(sizeof(R) * 8);
-  for (size_t c = 0; c < all.size(); ++c) {
-    const auto opt = group_options(all[c], bits);
-    if (!opt.empty()) {
-      // the shortest step among the sizes within `slack` x the least work
-      // (default 2): a tick lasts `budget` steps of the slowest class, and
-      // a request waits for its slowest system; PASIF_POOL_WORK_SLACK=1
-      // takes the least work (throughput over latency)
-      double slack = 2.0;
-      if (const char* e = std::getenv("PASIF_POOL_WORK_SLACK")) {
-        const double v = std::strtod(e, nullptr);
-        if (v >= 1.0 && v <= 16.0) slack = v;
-      }
-      double wmin = 0;
-      for (size_t k = 0; k < opt.size(); ++k) {
-        const double wk = opt[k].second * opt[k].first;
-        if (k == 0 || wk < wmin) wmin = wk;
-      }
-      size_t b = opt.size();
-      for (size_t k = 0; k < opt.size(); ++k)
-        if (opt[k].second * opt[k].first <= slack * wmin &&
-            (b == opt.size() || opt[k].second < opt[b].second))
-          b = k;
-      (void)lane_group(all[c], opt[b].first);
-      need[c] = 1;
-    } else {
-      for (uint32_t w = 2; w <= kLaneMaxWaves && !need[c]; ++w)
-        if (lane_partition(all[c], w) <= kLaneMaxWaveLive) need[c] = w;
-    }
-  }
+  for (size_t c = 0; c < all.size(); ++c) need[c] = pool_class_fit(all[c], bits);
   P->mods.clear();
   P->mods.resize(2);
   const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
@@ -4188,23 +4242,7 @@ void pool_build_modules(pasif_pool* P) {
     for (const LaneShape& sh : M.shapes) adj = adj || sh.adjoint;
     LaneLayout lay;
     if (m == 0) {
-      // a fixed LDS share per wave (eight waves per CU resident) sets each
-      // class's checkpoint interval; registers for at most four waves per SIMD
-      uint32_t live = 0;
-      lay.seg = Kreq;
-      for (LaneShape& sh : M.shapes) {
-        live = std::max(live, lane_live(sh));
-        if (!sh.adjoint) continue;
-        const uint32_t G = sh.group > 1 ? sh.group : 1u;
-        const size_t stage = size_t(kLaneStage + (G > 1 ? 3 * kLaneChunk : 0)) * (kLaneWave / G) * el;
-        const size_t step = size_t(lane_seg_rows(sh)) * kLaneWave * el;
-        const size_t b = std::min<size_t>(kLaneMaxLds, std::max<size_t>(kMaxLdsBytes / 8, stage + 6 * step));
-        const size_t fit = b > stage ? (b - stage) / step : 0;
-        sh.seg = uint32_t(std::max<size_t>(1, std::min<size_t>(Kreq, fit > 0 ? fit - 1 : 0)));
-        lay.seg = std::min(lay.seg, sh.seg);
-      }
-      if (!adj) lay.seg = 1;
-      lay.occ = std::min<uint32_t>(4, std::max<uint32_t>(1, 512 / (live * uint32_t(bits / 32) + 48)));
+      lay = pool_single_layout(M.shapes, Kreq, el);
     } else {
       for (LaneShape& sh : M.shapes) (void)lane_partition(sh, waves);
       lay.waves = waves;
@@ -4454,6 +4492,10 @@ void pool_tick_impl(pasif_pool* P) {
 
 namespace {
 void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint);
+void pool_install_build(pasif_pool* P, bool wait);
+template <typename R>
+void pool_start_build(pasif_pool* P);
+void pool_compile_step(pasif_pool* P);
 }  // namespace
 
 extern "C" {
@@ -4485,8 +4527,10 @@ int pasif_pool_create(pasif_pool** out, int gpu_id, int precision, uint32_t tile
     POOL_HIP(hipEventCreate(&P->ev1));
     POOL_HIP(hipEventCreateWithFlags(&P->uploaded, hipEventDisableTiming));
     for (int m = 0; m < 2; ++m) {
-      POOL_HIP(hipStreamCreateWithFlags(&P->aux[m], hipStreamNonBlocking));
-      POOL_HIP(hipEventCreateWithFlags(&P->join[m], hipEventDisableTiming));
+      P->aux.push_back(nullptr);
+      P->join.push_back(nullptr);
+      POOL_HIP(hipStreamCreateWithFlags(&P->aux.back(), hipStreamNonBlocking));
+      POOL_HIP(hipEventCreateWithFlags(&P->join.back(), hipEventDisableTiming));
     }
     POOL_HIP(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
     P->precision = precision;
@@ -4579,13 +4623,38 @@ int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc
 int pasif_pool_submit(pasif_pool* P, pasif_driver* d, int adjoint) {
   return pool_guarded(P, [&] {
     POOL_REQUIRE(d && d->pool == P, PASIF_ERR_ARGUMENT, "job does not belong to this pool");
+    pool_install_build(P, false);
     pool_match_lanes(P, d, adjoint != 0);
     admit_job(P, d, adjoint);
+    if (P->precision == 64)
+      pool_start_build<double>(P);
+    else
+      pool_start_build<float>(P);
   });
 }
 
 int pasif_pool_set_slots(pasif_pool* P, uint32_t max_tiles) {
   return pool_guarded(P, [&] { P->max_tiles = max_tiles; });
+}
+
+int pasif_pool_set_compile(pasif_pool* P, int on) {
+  return pool_guarded(P, [&] { P->compile_new = on != 0; });
+}
+
+int pasif_pool_compile_wait(pasif_pool* P, uint32_t max_builds, uint32_t* modules,
+                            uint32_t* failed, uint32_t* queued) {
+  return pool_guarded(P, [&] {
+    pool_compile_step(P);  // starts a build if structures wait and none runs
+    for (uint32_t k = 0; P->build && (max_builds == 0 || k < max_builds); ++k)
+      pool_install_build(P, true);
+    if (modules) *modules = P->built_modules;
+    if (failed) *failed = P->failed_builds;
+    if (queued) {
+      size_t n = P->pending.size();
+      if (P->build) n += P->build->cls.size();
+      *queued = uint32_t(n);
+    }
+  });
 }
 
 int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_id,
@@ -4614,6 +4683,7 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
       P->mods_built = false;
       P->key_lane.clear();
       P->fkey_lane.clear();
+      P->queued.clear();
       if (P->precision == 64)
         pool_build_modules<double>(P);
       else
@@ -4646,18 +4716,23 @@ namespace {
 // A job carrying its own arrays: each system whose structure a pool lane
 // module already runs (the same sparsity, any coefficient values) joins that
 // module; the others, and every system of a job with its own excitations
-// (the modules read the pool's bank), run on tiles.
+// (the modules read the pool's bank), run on tiles.  With compile_new set,
+// the structures of the others that fit one wave are queued for a module
+// of their own.
 void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
   d->lane_cls.clear();
-  if (!d->sys_ready || (adjoint && !d->adj_ready) || d->ext_exc != P->bank.p || P->reg.empty())
+  if (!d->sys_ready || (adjoint && !d->adj_ready) || d->ext_exc != P->bank.p)
     return;
+  // the registered modules come first (indices 0 and 1): modules compiled
+  // for new structures are appended after them
   if (!P->mods_built) {
     if (P->precision == 64)
       pool_build_modules<double>(P);
     else
       pool_build_modules<float>(P);
   }
-  if (P->key_lane.empty()) return;
+  if (P->key_lane.empty() && !P->compile_new) return;
+  const int bits = P->precision == 64 ? 64 : 32;
   const double eps = P->precision == 64 ? std::numeric_limits<double>::epsilon()
                                         : double(std::numeric_limits<float>::epsilon());
   const uint32_t S = uint32_t(d->fwd.size.size());
@@ -4666,23 +4741,119 @@ void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
   bool any = false;
   for (uint32_t s = 0; s < S; ++s) {
     const RowTerms f = slice_block(d->fwd, s, false);
+    LaneShape sh;
+    sh.f = lane_side(f, 0, false, eps);
     key.clear();
-    shape_key(lane_side(f, 0, false, eps), key);
+    shape_key(sh.f, key);
+    const size_t nf = key.size();
+    key.push_back(adjoint ? 1u : 0u);
     const std::map<std::vector<uint32_t>, int32_t>* m = &P->fkey_lane;
     if (adjoint) {
       RowTerms a = slice_block(d->adj, s, true);
       a.nset = f.n;
-      key.push_back(1u);
-      shape_key(lane_side(a, 0, true, eps), key);
+      sh.adjoint = true;
+      sh.a = lane_side(a, 0, true, eps);
+      shape_key(sh.a, key);
       m = &P->key_lane;
     }
-    const auto it = m->find(key);
+    const auto it = adjoint ? m->find(key) : m->find(std::vector<uint32_t>(key.begin(),
+                                                                           key.begin() + long(nf)));
     if (it != m->end()) {
       cls[s] = it->second;
       any = true;
+    } else if (P->compile_new && P->queued.insert(key).second) {
+      if (pool_class_fit(sh, bits) == 1) {
+        pasif_pool::NewClass c;
+        c.sh = std::move(sh);
+        c.fkey.assign(key.begin(), key.begin() + long(nf));
+        c.key = key;
+        P->pending.push_back(std::move(c));
+      }
     }
   }
   if (any) d->lane_cls = std::move(cls);
+}
+
+// Classes per background build: a module's compile time grows with its
+// classes (about 2 s each), and a structure waits for its whole batch.
+constexpr size_t kPoolBuildClasses = 4;
+
+// Start compiling the next batch of queued structures (none running).
+template <typename R>
+void pool_start_build(pasif_pool* P) {
+  if (P->build || P->pending.empty()) return;
+  std::string err;
+  if (!lane_jit_flags_ok(&err)) {
+    P->pending.clear();
+    ++P->failed_builds;
+    return;
+  }
+  std::unique_ptr<pasif_pool::Build> B;
+  while (!B && !P->pending.empty()) {
+    B.reset(new pasif_pool::Build());
+    const size_t take = std::min(P->pending.size(), kPoolBuildClasses);
+    B->cls.assign(std::make_move_iterator(P->pending.begin()),
+                  std::make_move_iterator(P->pending.begin() + long(take)));
+    P->pending.erase(P->pending.begin(), P->pending.begin() + long(take));
+    for (const auto& c : B->cls) B->M.shapes.push_back(c.sh);
+    const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
+    B->M.lay = pool_single_layout(B->M.shapes, Kreq, sizeof(R));
+    B->M.lds = lane_lds_bytes(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
+    if (B->M.lds > kLaneMaxLds) {
+      ++P->failed_builds;  // these structures stay on tiles
+      B.reset();
+    }
+  }
+  if (!B) return;
+  B->src = lane_source(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
+  pasif_pool::Build* b = B.get();
+  B->th = std::thread([b] {
+    b->ok = lane_compile_code(b->src, false, &b->code, &b->err);
+    b->done.store(true, std::memory_order_release);
+  });
+  P->build = std::move(B);
+}
+
+// Install a finished build as a further module (wait: join a running one
+// first), then start the next batch.
+void pool_install_build(pasif_pool* P, bool wait) {
+  if (!P->build) return;
+  if (!wait && !P->build->done.load(std::memory_order_acquire)) return;
+  P->build->th.join();
+  std::unique_ptr<pasif_pool::Build> B = std::move(P->build);
+  bool ok = B->ok;
+  if (ok) {
+    POOL_HIP(hipSetDevice(P->dev));
+    std::string err;
+    ok = lane_load_code(B->src, false, P->dev, B->code, &B->M.kern, &err);
+    if (ok) {
+      P->aux.push_back(nullptr);
+      P->join.push_back(nullptr);
+      POOL_HIP(hipStreamCreateWithFlags(&P->aux.back(), hipStreamNonBlocking));
+      POOL_HIP(hipEventCreateWithFlags(&P->join.back(), hipEventDisableTiming));
+      const int m = int(P->mods.size());
+      P->mods.push_back(std::move(B->M));
+      for (size_t k = 0; k < B->cls.size(); ++k) {
+        const int32_t code = lane_code(m, uint32_t(k));
+        P->key_lane[B->cls[k].key] = code;
+        P->fkey_lane.emplace(B->cls[k].fkey, code);
+      }
+      ++P->built_modules;
+    }
+  }
+  if (!ok) ++P->failed_builds;
+  if (P->precision == 64)
+    pool_start_build<double>(P);
+  else
+    pool_start_build<float>(P);
+}
+
+void pool_compile_step(pasif_pool* P) {
+  pool_install_build(P, false);
+  if (P->precision == 64)
+    pool_start_build<double>(P);
+  else
+    pool_start_build<float>(P);
 }
 
 pasif_driver* pool_submit_one(pasif_pool* P, uint32_t n, const uint32_t* ids,
@@ -4796,6 +4967,7 @@ int pasif_pool_tick(pasif_pool* P, int wait, uint32_t* active_tiles) {
     POOL_REQUIRE(!P->in_flight, PASIF_ERR_STATE, "a tick is already in flight");
     POOL_HIP(hipSetDevice(P->dev));
     pool_reap(P);
+    pool_compile_step(P);
     if (P->precision == 64)
       pool_tick_impl<double>(P);
     else
@@ -5002,6 +5174,7 @@ int pasif_pool_slots(const pasif_pool* P, uint32_t* used, uint32_t* max_tiles) {
 
 int pasif_pool_destroy(pasif_pool* P) {
   if (!P) return PASIF_OK;
+  if (P->build && P->build->th.joinable()) P->build->th.join();
   if (P->timing && P->tm_n)
     fprintf(stderr,
             "[pasif] admission of %llu requests, ms per request: concatenation %.3f, "
@@ -5026,7 +5199,7 @@ int pasif_pool_destroy(pasif_pool* P) {
   P->arena.reset();
   if (P->pinned) (void)hipHostFree(P->pinned);
   P->pinned = nullptr;
-  for (int m = 0; m < 2; ++m) {
+  for (size_t m = 0; m < P->aux.size(); ++m) {
     if (P->aux[m]) {
       (void)hipStreamSynchronize(P->aux[m]);
       (void)hipStreamDestroy(P->aux[m]);

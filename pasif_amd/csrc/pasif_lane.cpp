@@ -2477,4 +2477,55 @@ bool lane_compile(const std::string& src, int dev, LaneKernel* out, std::string*
   return lane_compile_many({src}, {ilp}, dev, {out}, err);
 }
 
+namespace {
+uint64_t jit_key(const std::string& src, bool ilp, std::string* err, bool* ok) {
+  std::vector<std::string> optv;
+  *ok = jit_options(ilp, &optv, err);
+  std::string opts;
+  for (const std::string& x : optv) opts += x + " ";
+  return fnv1a(src + opts);
+}
+}  // namespace
+
+bool lane_compile_code(const std::string& src, bool ilp, std::vector<char>* code,
+                       std::string* err) {
+  std::lock_guard<std::mutex> lk(g_jit_mu);
+  bool ok = false;
+  const uint64_t key = jit_key(src, ilp, err, &ok);
+  if (!ok) return false;
+  code->clear();
+  read_cached(jit_cache_path(key), code);
+  if (!code->empty()) return true;
+  if (!rtc_compile(src, code, err, ilp)) return false;
+  write_cached(jit_cache_path(key), *code);
+  return true;
+}
+
+bool lane_load_code(const std::string& src, bool ilp, int dev, const std::vector<char>& code,
+                    LaneKernel* out, std::string* err) {
+  std::lock_guard<std::mutex> lk(g_jit_mu);
+  bool ok = false;
+  const uint64_t key = jit_key(src, ilp, err, &ok);
+  if (!ok) return false;
+  auto it = g_jit.find({dev, key});
+  if (it != g_jit.end()) {
+    *out = it->second;
+    return true;
+  }
+  LaneKernel k;
+  hipError_t e = hipModuleLoadData(&k.mod, code.data());
+  if (e != hipSuccess) {
+    *err = std::string("loading the lane kernel module failed: ") + hipGetErrorName(e);
+    return false;
+  }
+  e = hipModuleGetFunction(&k.fn, k.mod, "pasif_lane_kernel");
+  if (e != hipSuccess) {
+    *err = std::string("lane kernel not found in its module: ") + hipGetErrorName(e);
+    return false;
+  }
+  g_jit[{dev, key}] = k;
+  *out = k;
+  return true;
+}
+
 }  // namespace pasif

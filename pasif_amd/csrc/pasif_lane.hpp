@@ -225,5 +225,13 @@ bool lane_compile_only(const std::string& src, size_t* code_bytes, std::string* 
                        bool ilp = false);
 // checks the caller's PASIF_JIT_FLAGS against the accepted options
 bool lane_jit_flags_ok(std::string* err);
+// Two halves of lane_compile for a caller that compiles on a thread of its
+// own: the code object of `src` (the disk cache, else hipRTC; no device
+// needed; serialised with the library's other compilations), then its
+// module loaded on `dev` (from the thread that owns the device's work).
+bool lane_compile_code(const std::string& src, bool ilp, std::vector<char>* code,
+                       std::string* err);
+bool lane_load_code(const std::string& src, bool ilp, int dev, const std::vector<char>& code,
+                    LaneKernel* out, std::string* err);
 
 }  // namespace pasif

@@ -53,6 +53,8 @@ _POOL_SIGS = {
                                      C.POINTER(C.c_size_t)]),
     "pasif_pool_stats": (C.c_int, [C.c_void_p, _u32p, _f64p, _f64p, _u32p]),
     "pasif_pool_slots": (C.c_int, [C.c_void_p, _u32p, _u32p]),
+    "pasif_pool_set_compile": (C.c_int, [C.c_void_p, C.c_int]),
+    "pasif_pool_compile_wait": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _u32p, _u32p]),
     "pasif_pool_set_slots": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pasif_pool_register": (C.c_int, [C.c_void_p, C.c_void_p, _u32p, _u32p]),
     "pasif_pool_submit_systems": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _i32p, _u32p,
@@ -380,8 +382,9 @@ class Pool:
         # slots and HBM bytes a registered system took in admitted groups, by
         # adjoint flag (running estimates: a group predicted not to fit is
         # not packed)
-        self._slots_per_system = {False: None, True: None}
-        self._bytes_per_system = {False: None, True: None}
+        # keyed by (registered ids, adjoint flag)
+        self._slots_per_system = {}
+        self._bytes_per_system = {}
         self.hbm_budget = float(hbm_budget)
         h = C.c_void_p()
         rc = self.lib.pasif_pool_create(C.byref(h), gpu, precision, lanes, slot_config,
@@ -544,6 +547,21 @@ class Pool:
         self._chk(rc)
         return out
 
+    def set_compile(self, on: bool = True):
+        """Compile the structures of array-carrying requests that no lane
+        module runs into further modules on a background thread (later
+        requests of those structures run on them)."""
+        self._chk(self.lib.pasif_pool_set_compile(self.h, int(bool(on))))
+
+    def compile_wait(self, max_builds: int = 0):
+        """Block until every queued structure is compiled (``max_builds`` > 0:
+        at most that many module compilations); returns (modules built,
+        builds failed, structures still queued)."""
+        m, f, q = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+        self._chk(self.lib.pasif_pool_compile_wait(self.h, int(max_builds), C.byref(m),
+                                                   C.byref(f), C.byref(q)))
+        return m.value, f.value, q.value
+
     def slots(self):
         """(slots held now, slot cap): tiles and lane workgroups ticks still
         launch; cap 0 = none."""
@@ -558,13 +576,13 @@ class Pool:
         group was admitted; at least one into an empty pool)."""
         if not reqs:
             return 0
-        flag = bool(reqs[0].adjoint)
+        flag = (reqs[0].keys is not None, bool(reqs[0].adjoint))
         used, cap = self.slots()
         free = float("inf")
-        per = self._slots_per_system[flag]
+        per = self._slots_per_system.get(flag)
         if cap and per:
             free = (cap - used) / (1.05 * per)
-        per_b = self._bytes_per_system[flag]
+        per_b = self._bytes_per_system.get(flag)
         if per_b:
             free = min(free, (self.hbm_budget - self.stats()["used_bytes"]) / (1.05 * per_b))
         n, tot = 0, 0
@@ -576,32 +594,51 @@ class Pool:
         return max(n, 1) if used == 0 else n
 
     def submit_group(self, reqs: Sequence[Request]):
-        """Admit requests of registered systems (one adjoint flag) as one job:
-        the packer sorts all their systems by step count together, so a wave
-        holds systems of similar length instead of one request's spread.  None
+        """Admit requests of one kind (registered ids, or carrying their own
+        arrays) and one adjoint flag as one job: the packer sorts all their
+        systems by step count together and fills each structure class's
+        waves across requests, instead of one request's log-uniform spread
+        (and, for array requests, a few systems per class) per wave.  None
         when the slots or the HBM budget do not take them all."""
         used0 = self.slots()[0]
         bytes0 = self.stats()["used_bytes"]
-        ids = np.ascontiguousarray(self.ids[np.concatenate([r.keys for r in reqs])],
-                                   dtype=np.uint32)
         exc = np.ascontiguousarray(np.concatenate([r.exc_ids for r in reqs]), dtype=np.int32)
         steps = np.ascontiguousarray(np.concatenate([r.steps for r in reqs]), dtype=np.uint32)
-        j = C.c_void_p()
-        rc = self.lib.pasif_pool_submit_systems(
-            self.h, len(ids), ids.ctypes.data_as(_u32p), exc.ctypes.data_as(_i32p),
-            steps.ctypes.data_as(_u32p), int(reqs[0].adjoint), C.byref(j))
-        if rc == 4:
-            self.refusals[self.lib.pasif_pool_last_error(self.h).decode().split(":")[0]] += 1
-            return None
-        self._chk(rc)
-        flag = bool(reqs[0].adjoint)
+        adjoint = bool(reqs[0].adjoint)
+        if reqs[0].keys is None:
+            d = self._job()
+            sizes, sizes_a = self._load_systems(d, [x for r in reqs for x in r.systems], adjoint)
+            d._setSystemsLayout(sizes_fwd=sizes, sizes_adj=sizes_a, exc_of_system=exc,
+                                steps_of_system=steps)
+            d._allocateSystemOnDevice()
+            if adjoint:
+                d._allocateAdjointOnDevice()
+            rc = self.lib.pasif_pool_submit(self.h, d._h, int(adjoint))
+            if rc == 4:
+                self.refusals[self.lib.pasif_pool_last_error(self.h).decode().split(":")[0]] += 1
+                self.lib.pasif_destroy(d._h)
+                d._h = None
+                return None
+            self._chk(rc)
+        else:
+            ids = np.ascontiguousarray(self.ids[np.concatenate([r.keys for r in reqs])],
+                                       dtype=np.uint32)
+            j = C.c_void_p()
+            rc = self.lib.pasif_pool_submit_systems(
+                self.h, len(ids), ids.ctypes.data_as(_u32p), exc.ctypes.data_as(_i32p),
+                steps.ctypes.data_as(_u32p), int(adjoint), C.byref(j))
+            if rc == 4:
+                self.refusals[self.lib.pasif_pool_last_error(self.h).decode().split(":")[0]] += 1
+                return None
+            self._chk(rc)
+            d = object.__new__(_Driver)
+            d._pl = self.lib
+            d._h = j
+        flag = (reqs[0].keys is not None, adjoint)
         for est, now, before in ((self._slots_per_system, self.slots()[0], used0),
                                  (self._bytes_per_system, self.stats()["used_bytes"], bytes0)):
-            per = (now - before) / len(ids)
-            est[flag] = per if est[flag] is None else 0.7 * est[flag] + 0.3 * per
-        d = object.__new__(_Driver)
-        d._pl = self.lib
-        d._h = j
+            per = (now - before) / len(steps)
+            est[flag] = per if est.get(flag) is None else 0.7 * est[flag] + 0.3 * per
         return d
 
     def collect_groups(self, jobs: Sequence, groups: Sequence[Sequence[Request]]):
@@ -791,6 +828,7 @@ class SolveServer:
 
         batched = by_id and hasattr(self.pool, "submit_batch")
         grouped = batched and self.group and hasattr(self.pool, "submit_group")
+        # (array-carrying requests are grouped too, among themselves)
         # due / pending entries are (job, [requests]): one request per job, or
         # a group of them admitted as one job
 
@@ -823,30 +861,35 @@ class SolveServer:
             nonlocal admitted, blocked, admit_s
             if blocked or not queue:
                 return False
-            if reqs[queue[0]].keys is None:     # carries its arrays: one submit
+            first = reqs[queue[0]]
+            if first.keys is None and not grouped:    # carries its arrays: one submit
                 return admit_one()
             ta = time.perf_counter()
             # the queued requests, each fetched once (a stream may build a new
             # object per access)
             take, obj = [], {}
             for k in range(min(limit, len(queue))):
-                r = reqs[queue[k]]
-                if r.keys is None:
+                r = first if k == 0 else reqs[queue[k]]
+                if r.keys is None and not grouped:
                     break
                 r.arrival = float(arr[queue[k]])
                 take.append(queue[k])
                 obj[queue[k]] = r
             entries = []
             if grouped:
-                # per adjoint flag, groups of consecutive queued requests of
-                # at most group_systems systems (a single larger request is a
-                # group of its own), cut to what the pool is expected to take;
-                # a group the pool refuses is halved until it fits, and the
-                # flag's admission stops at the first cut
+                # per kind (registered ids / own arrays) and adjoint flag,
+                # groups of consecutive queued requests of at most
+                # group_systems systems (a single larger request is a group of
+                # its own), cut to what the pool is expected to take; a group
+                # the pool refuses is halved until it fits, and the admission
+                # of that kind and flag stops at the first cut.  Array requests
+                # cost host time to assemble: one group of them per call, so
+                # the loop gets back to launching ticks.
                 got = set()
                 fit = getattr(self.pool, "group_fit", None)
-                for flag in (False, True):
-                    members = [k for k in take if bool(obj[k].adjoint) == flag]
+                for by_id, flag in ((True, False), (True, True), (False, False), (False, True)):
+                    members = [k for k in take if bool(obj[k].adjoint) == flag and
+                               (obj[k].keys is not None) == by_id]
                     stop = False
                     while members and not stop:
                         n, tot = 0, 0
@@ -872,6 +915,7 @@ class SolveServer:
                         entries.append((job, [obj[k] for k in chunk]))
                         got.update(chunk)
                         members = members[len(chunk):]
+                        stop = stop or not by_id
                 for _ in take:
                     queue.popleft()
                 for k in reversed(take):

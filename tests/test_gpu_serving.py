@@ -549,3 +549,105 @@ def test_fresh_coefficients_on_registered_structures_run_on_lane_modules():
         np.testing.assert_array_equal(q, alone[r.rid][0])
         if r.adjoint:
             np.testing.assert_array_equal(g, alone[r.rid][1])
+
+
+def test_new_structures_compiled_in_the_background_then_run_on_lanes():
+    """With set_compile, structures of array-carrying requests that no module
+    runs are compiled into further lane modules in the background: requests
+    admitted before run on tiles, the same requests admitted after the
+    compile run on the new modules; both match the oracle."""
+    print("[gpu case] background compilation of new structures", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(50)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    new = S.FreshSystems(40, seed=5)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+    pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=150, hbm_budget=4e9)
+    pool.set_bank(bank, 1000, interp)
+    pool.register_library(lib)
+    pool.set_compile(True)
+    # systems of at most 8 DOF: their classes fit one wave
+    small = [k for k, d in enumerate(new.dofs.tolist()) if d <= 8]
+    assert len(small) >= 6
+    reqs = []
+    for i, adj in enumerate((False, True, True)):
+        pick = rng.choice(small, 4, replace=False)
+        reqs.append(S.Request(900 + i, [new.systems[k] for k in pick], rng.integers(0, 6, 4),
+                              rng.integers(200, 1200, 4), adj))
+
+    def run(kind_ok):
+        jobs = []
+        for r in reqs:
+            j = pool.submit(r)
+            assert j is not None
+            assert kind_ok(j._kernelInfo(r.adjoint)["kernel"]), r.rid
+            jobs.append((j, r))
+        out = {}
+        while jobs:
+            pool.tick(wait=True)
+            keep = []
+            for j, r in jobs:
+                if pool.finished(j):
+                    out[r.rid] = pool.collect(j, r)
+                else:
+                    keep.append((j, r))
+            jobs = keep
+        return out
+
+    before = run(lambda k: k in ("tile", "mixed", "lane"))
+    built, failed, queued = pool.compile_wait()
+    assert built >= 1 and failed == 0 and queued == 0, (built, failed, queued)
+    after = run(lambda k: k == "lane")
+    pool.close()
+    for r in reqs:
+        oq, og = _oracle(r, bank, 1000, interp)
+        for res in (before, after):
+            q, g = res[r.rid]
+            assert rel_err(q, oq) < TOL64, r.rid
+            if r.adjoint:
+                assert rel_err(g, og) < TOL64, r.rid
+
+
+def test_group_of_array_requests_equals_each_alone():
+    """Array-carrying requests admitted as one job (``submit_group``: fresh
+    coefficients of registered structures on lane modules, new structures on
+    tiles) get results bitwise equal to each request submitted alone, and
+    the oracle's."""
+    print("[gpu case] group of array-carrying requests", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(51)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    same = S.FreshSystems(12, seed=9, like=lib)
+    new = S.FreshSystems(24, seed=6)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+
+    def new_pool():
+        pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=150, hbm_budget=4e9)
+        pool.set_bank(bank, 1000, interp)
+        pool.register_library(lib)
+        return pool
+
+    a = S.FreshRequestStream(same, 3, 82, 6, tmin=200, tmax=1200, max_systems=8,
+                             adjoint_frac=1.0)
+    b = S.FreshRequestStream(new, 3, 83, 6, tmin=200, tmax=1200, max_systems=8,
+                             adjoint_frac=1.0)
+    reqs = [a[0], b[0], a[1], b[1], a[2], b[2]]
+    for i, r in enumerate(reqs):
+        r.rid = 700 + i
+    pool = new_pool()
+    job = pool.submit_group(reqs)
+    assert job is not None
+    while not pool.finished(job):
+        pool.tick(wait=True)
+    pool.collect_groups([job], [reqs])
+    pool.close()
+    for r in reqs:
+        solo = new_pool()
+        q, g = _run_alone(solo, r)
+        solo.close()
+        np.testing.assert_array_equal(r.result, q)
+        np.testing.assert_array_equal(r.grad, g)
+        oq, og = _oracle(r, bank, 1000, interp)
+        assert rel_err(q, oq) < TOL64 and rel_err(g, og) < TOL64, r.rid

@@ -159,9 +159,9 @@ def test_server_admits_id_batches_and_array_requests_from_one_stream():
 
 
 class FakeGroupPool(FakeBatchPool):
-    """FakeBatchPool that takes id requests as groups (one job per group):
-    a group is refused when the systems resident with it would pass
-    ``max_systems``."""
+    """FakeBatchPool that takes requests as groups (one job per group, id
+    requests and array-carrying ones never mixed): a group is refused when
+    the systems resident with it would pass ``max_systems``."""
 
     def __init__(self, steps_per_tick, slots, max_systems):
         super().__init__(steps_per_tick, slots)
@@ -170,7 +170,7 @@ class FakeGroupPool(FakeBatchPool):
 
     def submit_group(self, reqs):
         assert len({bool(r.adjoint) for r in reqs}) == 1
-        assert all(r.keys is not None for r in reqs)
+        assert len({r.keys is None for r in reqs}) == 1
         resident = sum(sum(len(r.steps) for r in j.req) for j in self.jobs
                        if isinstance(j.req, list))
         if len(self.jobs) >= self.slots or \
@@ -193,7 +193,7 @@ class FakeGroupPool(FakeBatchPool):
                 self.collected.append(r.rid)
 
 
-def test_server_groups_id_requests_by_adjoint_flag_and_halves_refused_groups():
+def test_server_groups_requests_by_kind_and_adjoint_flag_and_halves_refused_groups():
     rng = np.random.default_rng(11)
     reqs = _requests(rng, 120)
     fresh = set(int(k) for k in rng.choice(120, 9, replace=False))
@@ -202,14 +202,15 @@ def test_server_groups_id_requests_by_adjoint_flag_and_halves_refused_groups():
     pool = FakeGroupPool(steps_per_tick=500, slots=6, max_systems=40)
     rep = S.SolveServer(pool).run(reqs, arrivals=np.zeros(len(reqs)))
     assert rep.completed == rep.admitted == 120 and rep.backlog == 0
-    assert sorted(pool.single) == sorted(fresh)
+    assert pool.single == []
     grouped = [k for g in pool.groups for k in g]
-    assert sorted(grouped) == sorted(set(range(120)) - fresh)
+    assert sorted(grouped) == list(range(120))
     assert sorted(pool.collected) == list(range(120))
-    # a group never passes the system cap and holds one adjoint flag
+    # a group never passes the system cap and holds one kind and adjoint flag
     for g in pool.groups:
         assert sum(len(reqs[k].steps) for k in g) <= 40
         assert len({reqs[k].adjoint for k in g}) == 1
+        assert len({k in fresh for k in g}) == 1
     assert any(len(g) > 1 for g in pool.groups)
     for r in reqs:
         if r.rid not in fresh:
@@ -221,6 +222,7 @@ def test_server_groups_id_requests_by_adjoint_flag_and_halves_refused_groups():
     rep2 = S.SolveServer(pool2, group=False).run(reqs, arrivals=np.zeros(len(reqs)))
     assert rep2.completed == 120 and pool2.groups == []
     assert sorted(pool2.batched) == sorted(set(range(120)) - fresh)
+    assert sorted(pool2.single) == sorted(fresh)
 
 
 def test_groups_stay_within_group_systems():
