@@ -459,6 +459,9 @@ struct pasif_pool {
     std::vector<NewClass> cls;
   };
   bool compile_new = false;
+  // raw structure of an array job's system -> its class key (and the forward
+  // part's length): repeated structures skip the lane-side analysis
+  std::map<std::vector<uint32_t>, std::pair<std::vector<uint32_t>, uint32_t>> raw_key;
   std::vector<NewClass> pending;
   std::set<std::vector<uint32_t>> queued;  // keys pending, compiling or tried
   std::unique_ptr<Build> build;
@@ -4713,6 +4716,34 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
 namespace {
 // One request of registered systems: validated, concatenated, packed and
 // admitted (throws Fail; PASIF_ERR_MEMORY when slots or budget refuse it).
+// The raw structure of system s of a side: per term kind, each row's term
+// count, each term's factor indices relative to the system (and its
+// set-point block) and whether its value survives the eps drop, then the
+// forced rows.  Equal raw structures give equal lane classes.
+void raw_structure(const RowTerms& rt, uint32_t s, bool adjoint, double eps,
+                   std::vector<uint32_t>& fp) {
+  const uint32_t off = rt.off[s], n = rt.size[s];
+  const uint32_t xo = adjoint ? rt.xoff[s] : off;
+  fp.push_back(n);
+  for (int k = 0; k < 4; ++k) {
+    const int nfac = k + 1;
+    const uint32_t b = rt.ptr[k][off], e = rt.ptr[k][off + n];
+    for (uint32_t r = 0; r < n; ++r) fp.push_back(rt.ptr[k][off + r + 1] - rt.ptr[k][off + r]);
+    for (uint32_t t = b; t < e; ++t) {
+      const uint32_t* ix = rt.idx[k].data() + size_t(t) * nfac;
+      fp.push_back(ix[0] - off);
+      for (int a = 1; a < nfac; ++a) fp.push_back(ix[a] - xo);
+      fp.push_back(std::fabs(rt.val[k][t]) > eps ? 1u : 0u);
+    }
+  }
+  if (!adjoint)
+    for (uint32_t r = 0; r < n; ++r)
+      if (std::fabs(rt.force[off + r]) > eps) fp.push_back(0x80000000u | r);
+}
+
+// raw structures remembered (the map starts over beyond this)
+constexpr size_t kPoolRawKeys = 1u << 16;
+
 // A job carrying its own arrays: each system whose structure a pool lane
 // module already runs (the same sparsity, any coefficient values) joins that
 // module; the others, and every system of a job with its own excitations
@@ -4739,34 +4770,54 @@ void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
   std::vector<int32_t> cls(S, -1);
   std::vector<uint32_t> key;
   bool any = false;
+  std::vector<uint32_t> fp;
   for (uint32_t s = 0; s < S; ++s) {
-    const RowTerms f = slice_block(d->fwd, s, false);
+    // the raw structure first: a structure seen before skips the lane-side
+    // analysis (its class key is cached)
+    fp.clear();
+    raw_structure(d->fwd, s, false, eps, fp);
+    fp.push_back(adjoint ? 1u : 0u);
+    if (adjoint) raw_structure(d->adj, s, true, eps, fp);
+    auto hit = P->raw_key.find(fp);
     LaneShape sh;
-    sh.f = lane_side(f, 0, false, eps);
-    key.clear();
-    shape_key(sh.f, key);
-    const size_t nf = key.size();
-    key.push_back(adjoint ? 1u : 0u);
-    const std::map<std::vector<uint32_t>, int32_t>* m = &P->fkey_lane;
-    if (adjoint) {
-      RowTerms a = slice_block(d->adj, s, true);
-      a.nset = f.n;
-      sh.adjoint = true;
-      sh.a = lane_side(a, 0, true, eps);
-      shape_key(sh.a, key);
-      m = &P->key_lane;
+    bool have_shape = false;
+    if (hit == P->raw_key.end()) {
+      sh.f = lane_side(d->fwd, s, false, eps);
+      key.clear();
+      shape_key(sh.f, key);
+      const uint32_t nf = uint32_t(key.size());
+      key.push_back(adjoint ? 1u : 0u);
+      if (adjoint) {
+        sh.adjoint = true;
+        sh.a = lane_side(d->adj, s, true, eps);
+        shape_key(sh.a, key);
+      }
+      have_shape = true;
+      if (P->raw_key.size() >= kPoolRawKeys) P->raw_key.clear();
+      hit = P->raw_key.emplace(fp, std::make_pair(key, nf)).first;
     }
-    const auto it = adjoint ? m->find(key) : m->find(std::vector<uint32_t>(key.begin(),
-                                                                           key.begin() + long(nf)));
-    if (it != m->end()) {
+    const std::vector<uint32_t>& k = hit->second.first;
+    const uint32_t nf = hit->second.second;
+    const std::map<std::vector<uint32_t>, int32_t>::const_iterator it =
+        adjoint ? P->key_lane.find(k)
+                : P->fkey_lane.find(std::vector<uint32_t>(k.begin(), k.begin() + long(nf)));
+    if (it != (adjoint ? P->key_lane.end() : P->fkey_lane.end())) {
       cls[s] = it->second;
       any = true;
-    } else if (P->compile_new && P->queued.insert(key).second) {
+    } else if (P->compile_new && !P->queued.count(k)) {
+      P->queued.insert(k);
+      if (!have_shape) {
+        sh.f = lane_side(d->fwd, s, false, eps);
+        if (adjoint) {
+          sh.adjoint = true;
+          sh.a = lane_side(d->adj, s, true, eps);
+        }
+      }
       if (pool_class_fit(sh, bits) == 1) {
         pasif_pool::NewClass c;
         c.sh = std::move(sh);
-        c.fkey.assign(key.begin(), key.begin() + long(nf));
-        c.key = key;
+        c.fkey.assign(k.begin(), k.begin() + long(nf));
+        c.key = k;
         P->pending.push_back(std::move(c));
       }
     }
