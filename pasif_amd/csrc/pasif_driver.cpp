@@ -448,6 +448,7 @@ struct pasif_pool {
   struct NewClass {
     LaneShape sh;
     std::vector<uint32_t> key, fkey;
+    uint32_t waves = 1;  // 1: single-wave module; >= 2: multi-wave, that many waves
   };
   struct Build {
     std::thread th;
@@ -4188,6 +4189,40 @@ LaneLayout pool_single_layout(std::vector<LaneShape>& shapes, uint32_t Kreq, siz
   return lay;
 }
 
+// Layout of a multi-wave pool module (16 instances per workgroup over
+// `waves` waves): the longest checkpoint interval up to Kreq whose segment
+// fits the workgroup's LDS; false when none does.
+bool pool_multi_layout(std::vector<LaneShape>& shapes, uint32_t waves, uint32_t Kreq, size_t el,
+                       LaneLayout* out) {
+  LaneLayout lay;
+  bool adj = false;
+  for (LaneShape& sh : shapes) {
+    (void)lane_partition(sh, waves);
+    adj = adj || sh.adjoint;
+  }
+  lay.waves = waves;
+  lay.lw = 16;
+  lay.seg = 1;
+  if (adj) {
+    uint32_t lo = 1, hi = Kreq;
+    lay.seg = 0;
+    while (lo <= hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      LaneLayout t = lay;
+      t.seg = mid;
+      if (lane_lds_bytes(shapes, int(el * 8), t) <= kMaxLdsBytes) {
+        lay.seg = mid;
+        lo = mid + 1;
+      } else {
+        hi = mid - 1;
+      }
+    }
+  }
+  if (lay.seg == 0 || lane_lds_bytes(shapes, int(el * 8), lay) > kMaxLdsBytes) return false;
+  *out = lay;
+  return true;
+}
+
 // Lane modules over the registered structures: module 0 runs the classes
 // that fit one lane or a group of lanes (each at its cheapest group size:
 // requests arrive continuously, so throughput, not one request's chain of
@@ -4241,35 +4276,12 @@ void pool_build_modules(pasif_pool* P) {
     if (order.empty()) continue;
     auto& M = P->mods[size_t(m)];
     for (uint32_t c : order) M.shapes.push_back(all[c]);
-    bool adj = false;
-    for (const LaneShape& sh : M.shapes) adj = adj || sh.adjoint;
     LaneLayout lay;
     if (m == 0) {
       lay = pool_single_layout(M.shapes, Kreq, el);
-    } else {
-      for (LaneShape& sh : M.shapes) (void)lane_partition(sh, waves);
-      lay.waves = waves;
-      lay.lw = 16;
-      lay.seg = 1;
-      if (adj) {
-        uint32_t lo = 1, hi = Kreq;
-        lay.seg = 0;
-        while (lo <= hi) {
-          const uint32_t mid = lo + (hi - lo) / 2;
-          LaneLayout t = lay;
-          t.seg = mid;
-          if (lane_lds_bytes(M.shapes, int(el * 8), t) <= kMaxLdsBytes) {
-            lay.seg = mid;
-            lo = mid + 1;
-          } else {
-            hi = mid - 1;
-          }
-        }
-      }
-      if (lay.seg == 0 || lane_lds_bytes(M.shapes, int(el * 8), lay) > kMaxLdsBytes) {
-        M.shapes.clear();  // does not fit: these classes stay on tiles
-        continue;
-      }
+    } else if (!pool_multi_layout(M.shapes, waves, Kreq, el, &lay)) {
+      M.shapes.clear();  // does not fit: these classes stay on tiles
+      continue;
     }
     M.lay = lay;
     M.lds = lane_lds_bytes(M.shapes, int(el * 8), lay);
@@ -4813,11 +4825,13 @@ void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
           sh.a = lane_side(d->adj, s, true, eps);
         }
       }
-      if (pool_class_fit(sh, bits) == 1) {
+      const uint32_t waves = pool_class_fit(sh, bits);
+      if (waves >= 1) {
         pasif_pool::NewClass c;
         c.sh = std::move(sh);
         c.fkey.assign(k.begin(), k.begin() + long(nf));
         c.key = k;
+        c.waves = waves;
         P->pending.push_back(std::move(c));
       }
     }
@@ -4840,17 +4854,30 @@ void pool_start_build(pasif_pool* P) {
     return;
   }
   std::unique_ptr<pasif_pool::Build> B;
+  const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
   while (!B && !P->pending.empty()) {
     B.reset(new pasif_pool::Build());
-    const size_t take = std::min(P->pending.size(), kPoolBuildClasses);
-    B->cls.assign(std::make_move_iterator(P->pending.begin()),
-                  std::make_move_iterator(P->pending.begin() + long(take)));
-    P->pending.erase(P->pending.begin(), P->pending.begin() + long(take));
+    // single-wave classes a few to a module (queue order); a multi-wave
+    // class alone (its compile time grows with its waves)
+    const uint32_t waves = P->pending.front().waves;
+    std::vector<size_t> pick;
+    for (size_t i = 0; i < P->pending.size() && pick.size() < kPoolBuildClasses; ++i)
+      if ((P->pending[i].waves == 1) == (waves == 1) && (waves == 1 || pick.empty()))
+        pick.push_back(i);
+    for (size_t i : pick) B->cls.push_back(std::move(P->pending[i]));
+    for (size_t j = pick.size(); j-- > 0;) P->pending.erase(P->pending.begin() + long(pick[j]));
     for (const auto& c : B->cls) B->M.shapes.push_back(c.sh);
-    const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
-    B->M.lay = pool_single_layout(B->M.shapes, Kreq, sizeof(R));
-    B->M.lds = lane_lds_bytes(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
-    if (B->M.lds > kLaneMaxLds) {
+    bool fits = true;
+    if (waves == 1) {
+      B->M.lay = pool_single_layout(B->M.shapes, Kreq, sizeof(R));
+    } else {
+      fits = pool_multi_layout(B->M.shapes, waves, Kreq, sizeof(R), &B->M.lay);
+    }
+    if (fits) {
+      B->M.lds = lane_lds_bytes(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
+      fits = B->M.lds <= (waves > 1 ? kMaxLdsBytes : kLaneMaxLds);
+    }
+    if (!fits) {
       ++P->failed_builds;  // these structures stay on tiles
       B.reset();
     }

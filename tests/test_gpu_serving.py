@@ -651,3 +651,55 @@ def test_group_of_array_requests_equals_each_alone():
         np.testing.assert_array_equal(r.grad, g)
         oq, og = _oracle(r, bank, 1000, interp)
         assert rel_err(q, oq) < TOL64 and rel_err(g, og) < TOL64, r.rid
+
+
+def test_background_compilation_of_large_new_structures():
+    """Background compilation also takes structures of 32 and 64 DOF that no
+    registered module runs (forward-only and forward + adjoint keys compiled
+    separately); results before (tiles) and after (lane modules) match the
+    oracle."""
+    print("[gpu case] background compilation, 32 / 64 DOF new structures", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(52)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    bank = S.signal_bank(rng, 6, 3000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+    pool = S.Pool(lanes=1024, slot_config=1, steps_per_tick=150, hbm_budget=8e9)
+    pool.set_bank(bank, 1000, interp)
+    pool.register_library(lib)
+    pool.set_compile(True)
+    sysm = [S.chain_arrays(64, 1.02, rng.normal(0, 0.05, 128), rng.uniform(0, 1, 64), True,
+                           variant=3),
+            S.chain_arrays(32, 0.97, rng.normal(0, 0.05, 64), rng.uniform(0, 1, 32), True,
+                           variant=2)]
+    reqs = [S.Request(1, sysm, np.array([1, 2]), np.array([600, 900]), True),
+            S.Request(2, sysm, np.array([3, 4]), np.array([700, 500]), False)]
+
+    def run():
+        out = {}
+        for r in reqs:
+            j = pool.submit(r)
+            kind = j._kernelInfo(r.adjoint)["kernel"]
+            while not pool.finished(j):
+                pool.tick(wait=True)
+            out[r.rid] = (kind, pool.collect(j, r))
+        return out
+
+    before = run()
+    assert all(v[0] == "tile" for v in before.values()), before
+    while True:
+        built, failed, queued = pool.compile_wait(1)
+        print(f"  compiled {built}, failed {failed}, queued {queued}", flush=True)
+        if not queued:
+            break
+    assert failed == 0 and built >= 2
+    after = run()
+    assert all(v[0] == "lane" for v in after.values()), after
+    pool.close()
+    for r in reqs:
+        oq, og = _oracle(r, bank, 1000, interp)
+        for res in (before, after):
+            q, g = res[r.rid][1]
+            assert rel_err(q, oq) < TOL64, r.rid
+            if r.adjoint:
+                assert rel_err(g, og) < TOL64, r.rid
