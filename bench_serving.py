@@ -56,6 +56,11 @@ def main():
                          "(tiles) or the registered ones with coefficients drawn afresh "
                          "(lane modules)")
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, at most "
+                         "32; 0: HIP's default of 4): each pool lane module is one launch "
+                         "per tick on a stream of its own, and launches beyond the queues "
+                         "run one after another")
     ap.add_argument("--group-systems", type=int, default=4096,
                     help="most systems in one admitted group")
     ap.add_argument("--compile-new", action="store_true",
@@ -67,6 +72,9 @@ def main():
                     help="admit one pool job per request instead of one per admission "
                          "batch and adjoint flag")
     a = ap.parse_args()
+    if a.hw_queues:
+        # read when HIP initialises: set before the library is loaded
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, a.hw_queues)))
 
     from pasif_amd import serving as S
     from pasif_amd.workloads import linear_interp_matrix
@@ -122,6 +130,7 @@ def main():
                           "fwd_adj_fraction": 0.7, "fresh_fraction": a.fresh_frac,
                           "fresh_structures": a.fresh_structures if a.fresh_frac > 0 else None,
                           "compile_new": state,
+                          "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                           "grouped": not a.no_group, "group_systems": a.group_systems,
                           "pool_at_stop": end_stats,
                           "refused_groups": refusals,

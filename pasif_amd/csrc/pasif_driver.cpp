@@ -4840,8 +4840,11 @@ void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
 }
 
 // Classes per background build: a module's compile time grows with its
-// classes (about 2 s each), and a structure waits for its whole batch.
-constexpr size_t kPoolBuildClasses = 4;
+// classes (about 2 s each) and a structure waits for its whole batch, while
+// every module is one more launch per tick (launches beyond the device's
+// hardware queues run one after another, each lasting the tick's steps): a
+// build takes half the queue, between 4 and 16 classes.
+constexpr size_t kPoolBuildMin = 4, kPoolBuildMax = 16;
 
 // Start compiling the next batch of queued structures (none running).
 template <typename R>
@@ -4860,8 +4863,10 @@ void pool_start_build(pasif_pool* P) {
     // single-wave classes a few to a module (queue order); a multi-wave
     // class alone (its compile time grows with its waves)
     const uint32_t waves = P->pending.front().waves;
+    const size_t batch =
+        std::min(kPoolBuildMax, std::max(kPoolBuildMin, P->pending.size() / 2));
     std::vector<size_t> pick;
-    for (size_t i = 0; i < P->pending.size() && pick.size() < kPoolBuildClasses; ++i)
+    for (size_t i = 0; i < P->pending.size() && pick.size() < batch; ++i)
       if ((P->pending[i].waves == 1) == (waves == 1) && (waves == 1 || pick.empty()))
         pick.push_back(i);
     for (size_t i : pick) B->cls.push_back(std::move(P->pending[i]));
