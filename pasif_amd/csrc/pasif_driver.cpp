@@ -460,6 +460,7 @@ struct pasif_pool {
     std::vector<NewClass> cls;
   };
   bool compile_new = false;
+  uint32_t mods_forcing = 0;  // the forcing rule the modules were compiled for
   // raw structure of an array job's system -> its class key (and the forward
   // part's length): repeated structures skip the lane-side analysis
   std::map<std::vector<uint32_t>, std::pair<std::vector<uint32_t>, uint32_t>> raw_key;
@@ -4132,6 +4133,19 @@ void admit_job(pasif_pool* P, pasif_driver* d, int adjoint) {
   P->jobs.push_back({d, adjoint != 0, work, 0, bytes, false, tiles});
 }
 
+// The forcing rule of the pool's signal bank as the lane kernels compile it
+// (1 interpolation, 2 zero-order hold; 0 no bank yet: both).
+uint32_t pool_forcing(const pasif_pool* P) {
+  return P->bank_n ? (P->bank_w ? 1u : 2u) : 0u;
+}
+
+// Pool modules: only the bank's forcing rule, no per-step outputs (pool jobs
+// ask for final states and gradients only).
+void pool_specialise(const pasif_pool* P, LaneLayout* lay) {
+  lay->forcing = pool_forcing(P);
+  lay->full = false;
+}
+
 // How a class runs on the pool's lane modules: 1 = in one wave (its group
 // size chosen here: the shortest step among the sizes within `slack` x the
 // least work, default 2, since a tick lasts `budget` steps of the slowest
@@ -4262,6 +4276,7 @@ void pool_build_modules(pasif_pool* P) {
   for (size_t c = 0; c < all.size(); ++c) need[c] = pool_class_fit(all[c], bits);
   P->mods.clear();
   P->mods.resize(2);
+  P->mods_forcing = pool_forcing(P);
   const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
   const size_t el = sizeof(R);
   for (int m = 0; m < 2; ++m) {
@@ -4283,6 +4298,7 @@ void pool_build_modules(pasif_pool* P) {
       M.shapes.clear();  // does not fit: these classes stay on tiles
       continue;
     }
+    pool_specialise(P, &lay);
     M.lay = lay;
     M.lds = lane_lds_bytes(M.shapes, int(el * 8), lay);
     POOL_REQUIRE(M.lds <= (lay.waves > 1 ? kMaxLdsBytes : kLaneMaxLds), PASIF_ERR_UNSUPPORTED,
@@ -4632,6 +4648,19 @@ int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc
     P->bank_len = exc_len;
     P->bank_rate = sample_rate;
     P->bank_w = interp ? window : 0;
+    if (P->mods_built && P->mods_forcing != 0 && P->mods_forcing != pool_forcing(P)) {
+      // modules compiled for the other forcing rule: built again at the next
+      // submit (a build in flight compiles for the old rule: dropped)
+      if (P->build) {
+        P->build->th.join();
+        P->build.reset();
+      }
+      P->mods_built = false;
+      P->key_lane.clear();
+      P->fkey_lane.clear();
+      P->queued.clear();
+      P->pending.clear();
+    }
   });
 }
 
@@ -4879,6 +4908,7 @@ void pool_start_build(pasif_pool* P) {
       fits = pool_multi_layout(B->M.shapes, waves, Kreq, sizeof(R), &B->M.lay);
     }
     if (fits) {
+      pool_specialise(P, &B->M.lay);
       B->M.lds = lane_lds_bytes(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
       fits = B->M.lds <= (waves > 1 ? kMaxLdsBytes : kLaneMaxLds);
     }

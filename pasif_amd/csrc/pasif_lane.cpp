@@ -1033,6 +1033,7 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
         left -= stop - n;
         const bool all = n < Tmin;
         auto in_mode = [&](auto mode_t) {
+#if FORCING == 0
           if (W == 0u) {
             if (all) steps_to(Flag<true>(), mode_t, Flag<true>(), stop);
             else steps_to(Flag<false>(), mode_t, Flag<true>(), stop);
@@ -1040,11 +1041,22 @@ __device__ __forceinline__ void run(const LaneArgs& a, u32 wave, R* smem) {
             if (all) steps_to(Flag<true>(), mode_t, Flag<false>(), stop);
             else steps_to(Flag<false>(), mode_t, Flag<false>(), stop);
           }
+#elif FORCING == 2
+          if (all) steps_to(Flag<true>(), mode_t, Flag<true>(), stop);
+          else steps_to(Flag<false>(), mode_t, Flag<true>(), stop);
+#else
+          if (all) steps_to(Flag<true>(), mode_t, Flag<false>(), stop);
+          else steps_to(Flag<false>(), mode_t, Flag<false>(), stop);
+#endif
         };
         if (mode == mPlain) in_mode(Mode<mPlain>());
         else if (mode == mCk) in_mode(Mode<mCk>());
+#if FULL
         else if (mode == mRec) in_mode(Mode<mRec>());
         else in_mode(Mode<mFull>());
+#else
+        else in_mode(Mode<mRec>());
+#endif
       }
       // without per-step outputs: an instance whose last step fell in this
       // range kept its final state since (inactive steps leave it)
@@ -1461,6 +1473,7 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
         left -= stop - n;
         const bool all = n < Tmin;
         // one step is long here: per-step bookkeeping checks cost little
+#if FORCING == 0
         if (W == 0u) {
           if (all) steps_to(Flag<true>(), Flag<true>(), Flag<true>(), stop);
           else steps_to(Flag<false>(), Flag<true>(), Flag<true>(), stop);
@@ -1468,6 +1481,13 @@ __device__ __forceinline__ void run_mw(const LaneArgs& a, u32 wg, R* smem) {
           if (all) steps_to(Flag<true>(), Flag<true>(), Flag<false>(), stop);
           else steps_to(Flag<false>(), Flag<true>(), Flag<false>(), stop);
         }
+#elif FORCING == 2
+        if (all) steps_to(Flag<true>(), Flag<true>(), Flag<true>(), stop);
+        else steps_to(Flag<false>(), Flag<true>(), Flag<true>(), stop);
+#else
+        if (all) steps_to(Flag<true>(), Flag<true>(), Flag<false>(), stop);
+        else steps_to(Flag<false>(), Flag<true>(), Flag<false>(), stop);
+#endif
       }
       if (!gen && valid && fsteps > nfirst && fsteps <= nend)
 #pragma unroll
@@ -2093,7 +2113,8 @@ std::string lane_source(const std::vector<LaneShape>& shapes, int bits, const La
   if (mw) dims = mw_dims(shapes, lay.waves);
   o << "#define LW " << (mw ? lay.lw : kLaneWave) << "u\n#define NYMAX " << dims.nymax
     << "u\n#define NFMAX " << dims.nfmax << "u\n#define SEGK " << lay.seg << "u\n#define SEGN "
-    << (dims.adjoint ? size_t(lay.seg + 1) * dims.nfmax * lay.lw : 0) << "u\n#define EXACT "
+    << (dims.adjoint ? size_t(lay.seg + 1) * dims.nfmax * lay.lw : 0) << "u\n#define FORCING "
+    << lay.forcing << "\n#define FULL " << (lay.full ? 1 : 0) << "\n#define EXACT "
     << (lay.exact && !mw ? 1 : 0) << "\n";
   if (bits == 64) {
     o << "__device__ __forceinline__ R fmaR(R a, R b, R c) { return __builtin_fma(a, b, c); }\n"

@@ -181,6 +181,13 @@ struct LaneLayout {
   // single-wave adjoint kernels: the exact discrete adjoint (the recompute
   // keeps every step's stage inputs y2..y4 beside the post-step states)
   bool exact = false;
+  // step loops compiled: forcing 0 = both forcing rules (chosen at run time
+  // from the interpolation window), 1 = interpolation only, 2 = zero-order
+  // hold only; full = false leaves out the loops with per-step outputs
+  // (trajectory, window maximum), which a caller that never asks for them
+  // (the pool) does not need; smaller code compiles faster
+  uint32_t forcing = 0;
+  bool full = true;
 };
 // Values per lane and set-point row that a checkpoint segment of K steps
 // keeps: K + 1 post-step states, and three stage inputs per step for the

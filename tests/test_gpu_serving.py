@@ -703,3 +703,34 @@ def test_background_compilation_of_large_new_structures():
             assert rel_err(q, oq) < TOL64, r.rid
             if r.adjoint:
                 assert rel_err(g, og) < TOL64, r.rid
+
+
+def test_zero_order_hold_bank_on_lane_modules_matches_oracle():
+    """Lane modules compiled for a zero-order-hold bank (no interpolation
+    window), then rebuilt when the bank switches to interpolation: id requests
+    match the oracle under each rule."""
+    print("[gpu case] pool lane modules, zero-order hold then interpolation", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(53)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    bank = S.signal_bank(rng, 5, 2000, steps_per_sample=10)
+    pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=300, hbm_budget=4e9)
+    for interp in (None, linear_interp_matrix(10)):
+        pool.set_bank(bank, 1000, interp)
+        if interp is None:
+            pool.register_library(lib)
+        reqs = [S.draw_request(lib, rng, 300 + i, 5, tmin=200, tmax=1500, max_systems=8)
+                for i in range(3)]
+        jobs = [pool.submit_ids(r) for r in reqs]
+        assert all(j is not None for j in jobs)
+        for j, r in zip(jobs, reqs):
+            assert j._kernelInfo(r.adjoint)["kernel"] == "lane", r.rid
+        while not all(pool.finished(j) for j in jobs):
+            pool.tick(wait=True)
+        for j, r in zip(jobs, reqs):
+            q, g = pool.collect(j, r)
+            oq, og = _oracle(r, bank, 1000, interp)
+            assert rel_err(q, oq) < TOL64, (interp is None, r.rid)
+            if r.adjoint:
+                assert rel_err(g, og) < TOL64, (interp is None, r.rid)
+    pool.close()
