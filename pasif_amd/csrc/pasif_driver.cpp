@@ -4782,8 +4782,9 @@ void raw_structure(const RowTerms& rt, uint32_t s, bool adjoint, double eps,
       if (std::fabs(rt.force[off + r]) > eps) fp.push_back(0x80000000u | r);
 }
 
-// raw structures remembered (the map starts over beyond this)
-constexpr size_t kPoolRawKeys = 1u << 16;
+// raw structures remembered (the map starts over beyond this; an entry of a
+// 64-DOF system holds a few KB)
+constexpr size_t kPoolRawKeys = 1u << 13;
 
 // A job carrying its own arrays: each system whose structure a pool lane
 // module already runs (the same sparsity, any coefficient values) joins that
