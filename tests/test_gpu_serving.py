@@ -25,7 +25,7 @@ def _run_alone(pool, req):
     return pool.collect(job, req)
 
 
-def _oracle(req, bank, lib_rate, interp):
+def _oracle(req, bank, lib_rate, interp, dtype=np.float64):
     sy = req.systems
     sizes = np.array([s.n for s in sy])
     offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
@@ -41,7 +41,7 @@ def _oracle(req, bank, lib_rate, interp):
                       q0=np.concatenate([s.q0 for s in sy]), sizes=sizes)
     lay = O.mapped_layout(sizes, req.exc_ids, req.steps)
     F = O.Forcing(bank, interp)
-    q = O.rk4_forward(fwd, F, lib_rate, lay).q[0]
+    q = O.rk4_forward(fwd, F, lib_rate, lay, dtype=dtype).q[0]
     if not req.adjoint:
         return q, None
     sa = np.array([s.na for s in sy])
@@ -56,7 +56,8 @@ def _oracle(req, bank, lib_rate, interp):
                       psi_idx=pi, psi_val=pv, force=np.zeros(na),
                       q0=np.concatenate([s.lamT for s in sy]), n_set=n, sizes=sa)
     ast = np.repeat(req.steps, sa)[None, :]
-    g = O.adjoint_gradient(fwd, adj, F, lib_rate, lay, ast, checkpoint_every=256)[0]
+    g = O.adjoint_gradient(fwd, adj, F, lib_rate, lay, ast, checkpoint_every=256,
+                           dtype=dtype)[0]
     return q, g
 
 
@@ -733,4 +734,40 @@ def test_zero_order_hold_bank_on_lane_modules_matches_oracle():
             assert rel_err(q, oq) < TOL64, (interp is None, r.rid)
             if r.adjoint:
                 assert rel_err(g, og) < TOL64, (interp is None, r.rid)
+    pool.close()
+
+
+def test_fp32_pool_matches_the_fp32_oracle():
+    """An FP32 pool (lane modules compiled for FP32) serves registered-id
+    requests and requests carrying fresh coefficients of registered
+    structures within 3e-2 of the FP32 oracle (the FP32 oracle's own spread
+    under a one-ULP perturbation sets that bound at the full-size
+    configurations; these runs are far shorter)."""
+    print("[gpu case] FP32 pool", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(54)
+    lib = S.SystemLibrary(dofs=(1, 2, 4, 16), variants=2)
+    same = S.FreshSystems(12, seed=10, like=lib)
+    bank = S.signal_bank(rng, 5, 2000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+    pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=300, hbm_budget=4e9, precision=32)
+    pool.set_bank(bank, 1000, interp)
+    pool.register_library(lib)
+    reqs = [S.draw_request(lib, rng, 400 + i, 5, tmin=200, tmax=1500, max_systems=8)
+            for i in range(3)]
+    fs = S.FreshRequestStream(same, 2, 84, 5, tmin=200, tmax=1500, max_systems=8,
+                              adjoint_frac=1.0)
+    arrays = [fs[0], fs[1]]
+    jobs = [(pool.submit_ids(r), r) for r in reqs] + [(pool.submit(r), r) for r in arrays]
+    assert all(j is not None for j, _ in jobs)
+    for j, r in jobs:
+        assert j._kernelInfo(r.adjoint)["kernel"] == "lane", r.rid
+    while not all(pool.finished(j) for j, _ in jobs):
+        pool.tick(wait=True)
+    for j, r in jobs:
+        q, g = pool.collect(j, r)
+        oq, og = _oracle(r, bank, 1000, interp, dtype=np.float32)
+        assert rel_err(q, oq) < 3e-2, r.rid
+        if r.adjoint:
+            assert rel_err(g, og) < 3e-2, r.rid
     pool.close()
