@@ -276,8 +276,10 @@ int pasif_pool_slots(const pasif_pool* pool, uint32_t* used, uint32_t* max_tiles
  * thread of the pool's own (half the queue per module, 4 to 16 classes); a
  * finished module is taken in by the next submit or tick, and later jobs of
  * those structures run on it (jobs admitted before stay on tiles).  Each
- * module is one more launch per tick, on a stream of its own.  Off by
- * default. */
+ * module is one more launch per tick, on a stream of its own: once nothing
+ * is queued and the single-wave classes compiled so far span three or more
+ * modules, they are compiled again as one module, which later jobs use.
+ * Off by default. */
 int pasif_pool_set_compile(pasif_pool* pool, int on);
 /* Waits for at most max_builds module compilations (0: until every queued
  * structure has its module or failed to compile; those stay on tiles), each

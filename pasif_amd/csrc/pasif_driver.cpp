@@ -458,6 +458,7 @@ struct pasif_pool {
     std::vector<char> code;
     Module M;
     std::vector<NewClass> cls;
+    bool merge = false;  // the classes of earlier modules, recompiled as one
   };
   bool compile_new = false;
   uint32_t mods_forcing = 0;  // the forcing rule the modules were compiled for
@@ -466,6 +467,12 @@ struct pasif_pool {
   std::map<std::vector<uint32_t>, std::pair<std::vector<uint32_t>, uint32_t>> raw_key;
   std::vector<NewClass> pending;
   std::set<std::vector<uint32_t>> queued;  // keys pending, compiling or tried
+  // single-wave classes of background modules not merged yet, with the
+  // module each runs in: once the queue is empty and they span
+  // kPoolMergeModules modules, they are compiled again as one module (one
+  // launch per tick instead of several)
+  std::vector<NewClass> loose;
+  std::vector<int> loose_mod;
   std::unique_ptr<Build> build;
   uint32_t built_modules = 0, failed_builds = 0;
   // one stream (and join event) per lane module
@@ -4660,6 +4667,8 @@ int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc
       P->fkey_lane.clear();
       P->queued.clear();
       P->pending.clear();
+      P->loose.clear();
+      P->loose_mod.clear();
     }
   });
 }
@@ -4728,6 +4737,8 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
       P->key_lane.clear();
       P->fkey_lane.clear();
       P->queued.clear();
+      P->loose.clear();
+      P->loose_mod.clear();
       if (P->precision == 64)
         pool_build_modules<double>(P);
       else
@@ -4875,11 +4886,42 @@ void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint) {
 // hardware queues run one after another, each lasting the tick's steps): a
 // build takes half the queue, between 4 and 16 classes.
 constexpr size_t kPoolBuildMin = 4, kPoolBuildMax = 16;
+constexpr size_t kPoolMergeModules = 3;
 
 // Start compiling the next batch of queued structures (none running).
 template <typename R>
 void pool_start_build(pasif_pool* P) {
-  if (P->build || P->pending.empty()) return;
+  if (P->build) return;
+  const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
+  if (P->pending.empty()) {
+    // idle: merge the single-wave classes spread over several background
+    // modules into one (at most kLaneMaxClasses of them)
+    const std::set<int> mods(P->loose_mod.begin(), P->loose_mod.end());
+    if (mods.size() < kPoolMergeModules) return;
+    std::string err;
+    if (!lane_jit_flags_ok(&err)) return;
+    std::unique_ptr<pasif_pool::Build> B(new pasif_pool::Build());
+    B->merge = true;
+    const size_t n = std::min<size_t>(P->loose.size(), kLaneMaxClasses);
+    B->cls.assign(P->loose.begin(), P->loose.begin() + long(n));
+    for (const auto& c : B->cls) B->M.shapes.push_back(c.sh);
+    B->M.lay = pool_single_layout(B->M.shapes, Kreq, sizeof(R));
+    pool_specialise(P, &B->M.lay);
+    B->M.lds = lane_lds_bytes(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
+    if (B->M.lds > kLaneMaxLds) {  // not expected: each class fitted alone
+      P->loose.clear();
+      P->loose_mod.clear();
+      return;
+    }
+    B->src = lane_source(B->M.shapes, int(sizeof(R) * 8), B->M.lay);
+    pasif_pool::Build* b = B.get();
+    B->th = std::thread([b] {
+      b->ok = lane_compile_code(b->src, false, &b->code, &b->err);
+      b->done.store(true, std::memory_order_release);
+    });
+    P->build = std::move(B);
+    return;
+  }
   std::string err;
   if (!lane_jit_flags_ok(&err)) {
     P->pending.clear();
@@ -4887,7 +4929,6 @@ void pool_start_build(pasif_pool* P) {
     return;
   }
   std::unique_ptr<pasif_pool::Build> B;
-  const uint32_t Kreq = P->ck_every ? P->ck_every : 256;
   while (!B && !P->pending.empty()) {
     B.reset(new pasif_pool::Build());
     // single-wave classes a few to a module (queue order); a multi-wave
@@ -4946,11 +4987,23 @@ void pool_install_build(pasif_pool* P, bool wait) {
       POOL_HIP(hipStreamCreateWithFlags(&P->aux.back(), hipStreamNonBlocking));
       POOL_HIP(hipEventCreateWithFlags(&P->join.back(), hipEventDisableTiming));
       const int m = int(P->mods.size());
+      const bool single = B->M.lay.waves == 1;
       P->mods.push_back(std::move(B->M));
       for (size_t k = 0; k < B->cls.size(); ++k) {
         const int32_t code = lane_code(m, uint32_t(k));
         P->key_lane[B->cls[k].key] = code;
-        P->fkey_lane.emplace(B->cls[k].fkey, code);
+        P->fkey_lane[B->cls[k].fkey] = code;
+      }
+      if (B->merge) {
+        // later jobs of these classes run on the merged module; the modules
+        // they ran on keep only the jobs already admitted
+        P->loose.erase(P->loose.begin(), P->loose.begin() + long(B->cls.size()));
+        P->loose_mod.erase(P->loose_mod.begin(), P->loose_mod.begin() + long(B->cls.size()));
+      } else if (single) {
+        for (auto& c : B->cls) {
+          P->loose.push_back(std::move(c));
+          P->loose_mod.push_back(m);
+        }
       }
       ++P->built_modules;
     }
