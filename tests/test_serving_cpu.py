@@ -271,3 +271,32 @@ def test_latency_is_measured_from_each_arrival_with_a_copying_stream():
         # each request is served within a few ticks of its own arrival (the
         # last arrives at 0.58 s)
         assert max(rep.latencies_ms) < 250.0, (group, max(rep.latencies_ms))
+
+
+def test_group_fit_follows_free_slots_and_budget():
+    # Pool.group_fit without a device: the slot and byte figures come from
+    # stand-ins; the per-system estimates are what earlier groups took
+    pool = object.__new__(S.Pool)
+    pool._slots_per_system, pool._bytes_per_system = {}, {}
+    pool.hbm_budget = 1000.0
+    state = {"slots": 0, "bytes": 0.0}
+    pool.slots = lambda: (state["slots"], 100)
+    pool.stats = lambda: {"used_bytes": state["bytes"]}
+    reqs = [S.Request(k, [None] * 5, np.zeros(5, dtype=np.int64), np.full(5, 10), False,
+                      keys=np.zeros(5, dtype=np.int64)) for k in range(10)]
+    assert pool.group_fit(reqs) == 10            # nothing known yet: try them all
+    pool._slots_per_system[(True, False)] = 2.0  # slots per system of id groups
+    state["slots"] = 50                          # 50 free: 23.8 systems -> 4 requests
+    assert pool.group_fit(reqs) == 4
+    state["slots"] = 100                         # full, and not empty: none
+    assert pool.group_fit(reqs) == 0
+    state["slots"] = 0                           # empty: 47.6 systems -> 9 requests
+    assert pool.group_fit(reqs) == 9
+    pool._bytes_per_system[(True, False)] = 100.0    # the budget binds: 9.5 systems
+    assert pool.group_fit(reqs) == 1
+    state["bytes"] = 990.0                       # no bytes left, no slots held: at least one
+    assert pool.group_fit(reqs) == 1
+    # array requests and adjoint groups keep estimates of their own
+    arr = [S.Request(k, [None] * 5, np.zeros(5, dtype=np.int64), np.full(5, 10), True)
+           for k in range(3)]
+    assert pool.group_fit(arr) == 3
