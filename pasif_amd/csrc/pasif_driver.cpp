@@ -4529,6 +4529,21 @@ void pool_tick_impl(pasif_pool* P) {
 }  // namespace
 
 namespace {
+// Forget every lane class of the pool (the modules are about to be built
+// again): a background build in flight is waited for and dropped, queued
+// structures are forgotten (they are queued again when seen).
+void pool_forget_modules(pasif_pool* P) {
+  if (P->build) {
+    if (P->build->th.joinable()) P->build->th.join();
+    P->build.reset();
+  }
+  P->key_lane.clear();
+  P->fkey_lane.clear();
+  P->queued.clear();
+  P->pending.clear();
+  P->loose.clear();
+  P->loose_mod.clear();
+}
 void pool_match_lanes(pasif_pool* P, pasif_driver* d, bool adjoint);
 void pool_install_build(pasif_pool* P, bool wait);
 template <typename R>
@@ -4657,18 +4672,9 @@ int pasif_pool_set_bank(pasif_pool* P, const double* excitations, uint32_t n_exc
     P->bank_w = interp ? window : 0;
     if (P->mods_built && P->mods_forcing != 0 && P->mods_forcing != pool_forcing(P)) {
       // modules compiled for the other forcing rule: built again at the next
-      // submit (a build in flight compiles for the old rule: dropped)
-      if (P->build) {
-        P->build->th.join();
-        P->build.reset();
-      }
+      // submit
+      pool_forget_modules(P);
       P->mods_built = false;
-      P->key_lane.clear();
-      P->fkey_lane.clear();
-      P->queued.clear();
-      P->pending.clear();
-      P->loose.clear();
-      P->loose_mod.clear();
     }
   });
 }
@@ -4733,12 +4739,8 @@ int pasif_pool_register(pasif_pool* P, const pasif_driver* src, uint32_t* first_
     if (P->jobs.empty()) {
       // no job runs on the lane modules yet: (re)build them over every
       // registered structure now, so that no request waits for a compilation
+      pool_forget_modules(P);
       P->mods_built = false;
-      P->key_lane.clear();
-      P->fkey_lane.clear();
-      P->queued.clear();
-      P->loose.clear();
-      P->loose_mod.clear();
       if (P->precision == 64)
         pool_build_modules<double>(P);
       else
@@ -4996,9 +4998,11 @@ void pool_install_build(pasif_pool* P, bool wait) {
       }
       if (B->merge) {
         // later jobs of these classes run on the merged module; the modules
-        // they ran on keep only the jobs already admitted
-        P->loose.erase(P->loose.begin(), P->loose.begin() + long(B->cls.size()));
-        P->loose_mod.erase(P->loose_mod.begin(), P->loose_mod.begin() + long(B->cls.size()));
+        // they ran on keep only the jobs already admitted (the merge took
+        // the first classes of `loose`, which only a merge removes)
+        const size_t n = std::min(B->cls.size(), P->loose.size());
+        P->loose.erase(P->loose.begin(), P->loose.begin() + long(n));
+        P->loose_mod.erase(P->loose_mod.begin(), P->loose_mod.begin() + long(n));
       } else if (single) {
         for (auto& c : B->cls) {
           P->loose.push_back(std::move(c));
