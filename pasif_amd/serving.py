@@ -164,22 +164,28 @@ class SystemLibrary:
 def _cat_csr(parts, offs):
     """Block-diagonal concatenation of CSR parts (data, indices, indptr), the
     column indices of part k shifted by offs[k]."""
-    data = np.concatenate([p[0] for p in parts])
-    nnz = np.fromiter((len(p[0]) for p in parts), dtype=np.int64, count=len(parts))
-    ind = np.concatenate([p[1] for p in parts]).astype(np.int64, copy=False)
-    ind = ind + np.repeat(np.asarray(offs, dtype=np.int64), nnz)
-    rows = np.fromiter((len(p[2]) - 1 for p in parts), dtype=np.int64, count=len(parts))
-    base = np.concatenate([[0], np.cumsum(nnz)[:-1]])
-    ptr = np.concatenate([[0]] + [p[2][1:] for p in parts]).astype(np.int64, copy=False)
+    datas, inds, ptrs = zip(*parts)
+    data = np.concatenate(datas)
+    nnz = np.array([len(x) for x in datas], dtype=np.int64)
+    ind = np.concatenate(inds)
+    ind = (ind if ind.dtype == np.int64 else ind.astype(np.int64))
+    ind += np.repeat(np.asarray(offs, dtype=np.int64), nnz)
+    rows = np.array([len(x) - 1 for x in ptrs], dtype=np.int64)
+    base = np.cumsum(nnz) - nnz
+    ptr = np.empty(int(rows.sum()) + 1, dtype=np.int64)
+    ptr[0] = 0
+    ptr[1:] = np.concatenate([x[1:] for x in ptrs])
     ptr[1:] += np.repeat(base, rows)
     return data, ind, ptr
 
 
 def _cat_tens(parts, offs_a, offs_b, n_a_axes):
     """indices: the first n_a_axes axes offset by offs_a, the rest by offs_b"""
-    cnt = np.fromiter((len(v) for _, v in parts), dtype=np.int64, count=len(parts))
-    idx = np.concatenate([ix for ix, _ in parts], axis=0).astype(np.int64)
-    val = np.concatenate([v for _, v in parts])
+    idxs, vals = zip(*parts)
+    cnt = np.array([len(v) for v in vals], dtype=np.int64)
+    idx = np.concatenate(idxs, axis=0)
+    idx = idx if idx.dtype == np.int64 else idx.astype(np.int64)
+    val = np.concatenate(vals)
     if idx.shape[0]:
         idx[:, :n_a_axes] += np.repeat(np.asarray(offs_a, dtype=np.int64), cnt)[:, None]
         idx[:, n_a_axes:] += np.repeat(np.asarray(offs_b, dtype=np.int64), cnt)[:, None]
