@@ -771,3 +771,51 @@ def test_fp32_pool_matches_the_fp32_oracle():
         if r.adjoint:
             assert rel_err(g, og) < 3e-2, r.rid
     pool.close()
+
+
+def test_many_new_structures_compiled_merged_and_served():
+    """Many new structures at once: the background builds split them over
+    several modules and, once the queue is empty, merge those into one;
+    requests admitted after that run on lanes and match the oracle."""
+    print("[gpu case] background compilation of many structures, merge", flush=True)
+    from pasif_amd.workloads import linear_interp_matrix
+    rng = np.random.default_rng(55)
+    lib = S.SystemLibrary(dofs=(1, 2), variants=1)
+    new = S.FreshSystems(120, seed=11, variants=12)
+    bank = S.signal_bank(rng, 5, 2000, steps_per_sample=10)
+    interp = linear_interp_matrix(10)
+    pool = S.Pool(lanes=512, slot_config=1, steps_per_tick=400, hbm_budget=8e9)
+    pool.set_bank(bank, 1000, interp)
+    pool.register_library(lib)
+    pool.set_compile(True)
+    small = [k for k, d in enumerate(new.dofs.tolist()) if d <= 16]
+    reqs = []
+    for i in range(4):
+        pick = small[i::4]
+        reqs.append(S.Request(950 + i, [new.systems[k] for k in pick],
+                              rng.integers(0, 5, len(pick)), rng.integers(100, 600, len(pick)),
+                              i % 2 == 1))
+    for r in reqs:                      # queue every structure
+        j = pool.submit(r)
+        assert j is not None
+        pool.drop(j)
+    while True:
+        built, failed, queued = pool.compile_wait(1)
+        print(f"  compiled {built}, failed {failed}, queued {queued}", flush=True)
+        assert failed == 0
+        if not queued:
+            break
+    # these structures take several modules, then one more that merges them
+    assert built >= 4, built
+    jobs = [(pool.submit(r), r) for r in reqs]
+    for j, r in jobs:
+        assert j._kernelInfo(r.adjoint)["kernel"] == "lane", r.rid
+    while not all(pool.finished(j) for j, _ in jobs):
+        pool.tick(wait=True)
+    for j, r in jobs:
+        q, g = pool.collect(j, r)
+        oq, og = _oracle(r, bank, 1000, interp)
+        assert rel_err(q, oq) < TOL64, r.rid
+        if r.adjoint:
+            assert rel_err(g, og) < TOL64, r.rid
+    pool.close()
