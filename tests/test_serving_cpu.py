@@ -300,3 +300,33 @@ def test_group_fit_follows_free_slots_and_budget():
     arr = [S.Request(k, [None] * 5, np.zeros(5, dtype=np.int64), np.full(5, 10), True)
            for k in range(3)]
     assert pool.group_fit(arr) == 3
+
+
+def test_block_concatenation_matches_a_dense_block_diagonal():
+    # _cat_csr / _cat_tens against scipy's block_diag and per-part index
+    # shifts written out one by one
+    import scipy.sparse as sp
+    rng = np.random.default_rng(14)
+    mats, tens, offs = [], [], []
+    o = 0
+    for k in range(7):
+        n = int(rng.integers(1, 6))
+        m = sp.random(n, n, density=0.5, random_state=int(rng.integers(1 << 30)), format="csr")
+        mats.append(m)
+        nnz = int(rng.integers(0, 5))
+        tens.append((rng.integers(0, n, (nnz, 3)).astype(np.int64), rng.normal(size=nnz)))
+        offs.append(o)
+        o += n
+    data, ind, ptr = S._cat_csr([(m.data, m.indices.astype(np.int64), m.indptr.astype(np.int64))
+                                 for m in mats], offs)
+    got = sp.csr_matrix((data, ind, ptr), shape=(o, o)).toarray()
+    np.testing.assert_array_equal(got, sp.block_diag(mats).toarray())
+    offs_b = [2 * x + 1 for x in offs]
+    idx, val = S._cat_tens(tens, offs, offs_b, 1)
+    want_i, want_v = [], []
+    for (ix, v), a, b in zip(tens, offs, offs_b):
+        for row, x in zip(ix, v):
+            want_i.append([row[0] + a, row[1] + b, row[2] + b])
+            want_v.append(x)
+    np.testing.assert_array_equal(idx, np.array(want_i, dtype=np.int64).reshape(-1, 3))
+    np.testing.assert_array_equal(val, np.array(want_v))
